@@ -1,0 +1,151 @@
+/*
+ * luna_hip.h -- C ABI of the MI355X propagation library (libluna_hip.so).
+ *
+ * The library replaces the hot path of Luna.jl's propagation:
+ *   - the nonlinear transform callable  transform(nl, Eω, z)
+ *       TransModeAvg  src/NonlinearRHS.jl:362-373  (field and envelope grids)
+ *       TransRadial   src/NonlinearRHS.jl:466-474
+ *   - the driver + stepper  Luna.run  src/Luna.jl:358-391  →  RK45.solve_precon
+ *       src/RK45.jl:19-25, 27-89, 128-309, 343-406 (tableau src/dopri.jl:1-28)
+ *
+ * Conventions: complex arrays are interleaved (re, im) doubles, i.e. the memory
+ * layout of Julia's ComplexF64 / C's double _Complex.  Multi-column arrays are
+ * column-major with ω fastest: element (iω, ir) lives at iω + n_omega*ir.
+ * Every entry point returns an lh_status code; lh_strerror() describes it.
+ * Host pointers handed to lh_create are copied; the caller keeps ownership.
+ */
+#ifndef LUNA_HIP_H
+#define LUNA_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes.  The reference raises Julia exceptions for the same events:
+ * PPT field out of range  src/Ionisation.jl:262-265
+ * repeated step rejection src/RK45.jl:71-77
+ * bad arguments           src/Nonlinear.jl:113, src/Interface.jl:586       */
+enum {
+    LH_OK = 0,
+    LH_E_ARG = 1,          /* invalid argument / unsupported size            */
+    LH_E_HIP = 2,          /* HIP runtime error                              */
+    LH_E_PPT_RANGE = 3,    /* |E| above the PPT table maximum                */
+    LH_E_REPEAT_LIMIT = 4, /* more than repeat_limit consecutive rejections  */
+    LH_E_NONFINITE = 5,    /* non-finite state                               */
+    LH_E_NOMEM = 6,        /* device allocation failed                       */
+    LH_E_NODEVICE = 7      /* no HIP device available                        */
+};
+
+enum { LH_GRID_REAL = 0, LH_GRID_ENV = 1 };           /* Grid.RealGrid / Grid.EnvGrid */
+enum { LH_TRANS_MODEAVG = 0, LH_TRANS_RADIAL = 1 };   /* TransModeAvg / TransRadial  */
+enum { LH_KERR_NONE = 0, LH_KERR_FIELD = 1, LH_KERR_ENV = 2 };
+
+/* Everything a transform closure captures in the reference
+ * (src/NonlinearRHS.jl:323-399, 406-524; src/Nonlinear.jl:24-177, 201-333). */
+typedef struct lh_desc {
+    int32_t grid_kind;        /* LH_GRID_REAL or LH_GRID_ENV                     */
+    int32_t transform;        /* LH_TRANS_MODEAVG or LH_TRANS_RADIAL             */
+    int32_t n_omega;          /* Nω: rows of the state                           */
+    int32_t n_t;              /* coarse time samples (stepfun filter grid)       */
+    int32_t n_to;             /* oversampled time samples (RHS grid)             */
+    int32_t n_r;              /* radial columns (1 for mode-averaged)            */
+    double dt_o;              /* δto, the RHS time step                          */
+    const double *omega;      /* [n_omega] ω                                     */
+    const uint8_t *sidx;      /* [n_omega] real-frequency mask                   */
+    const double *omega_win;  /* [n_omega] ωwin                                 */
+    const double *t_win;      /* [n_t]   twin  (stepfun filter)                  */
+    const double *to_win;     /* [n_to]  towin (RHS apodisation)                 */
+    const double *linop;      /* complex [n_omega*n_r] constant linear operator  */
+    const double *norm;       /* complex [n_omega*n_r] normalisation factor:
+                                 mode-average: (pre/β)·√Aeff (applied on sidx,
+                                 then ×ωwin on sidx); radial: ωwin·(-iω)/(2·norm)
+                                 applied on every bin                            */
+    double field_div;         /* mode-average: Eto /= nlscale·√Aeff(z); radial 1 */
+    double density;           /* ρ (constant in z)                               */
+    int32_t kerr;             /* LH_KERR_*                                       */
+    int32_t plasma;           /* 1: PlasmaCumtrapz with the PPT spline           */
+    double gamma3;            /* γ3 (or χ3 for the GNLSE)                        */
+    int32_t ppt_n;            /* spline nodes                                    */
+    int32_t raman;            /* 1: RamanPolarEnv (envelope grid only)           */
+    const double *ppt_x;      /* [ppt_n] uniform node positions E                */
+    const double *ppt_y;      /* [ppt_n] log(rate)                               */
+    const double *ppt_d;      /* [ppt_n] CSpline derivatives                     */
+    double ionpot;            /* ionisation potential (J)                        */
+    double preionfrac;        /* pre-ionisation fraction                         */
+    const double *raman_h;    /* [n_to] windowed h(t) on tt = 0..n_to-1 (×δto)   */
+    const double *qdht;       /* [n_r*n_r] QDHT matrix T (symmetric)             */
+    double qdht_fwd;          /* r→k scale R/K                                   */
+    double qdht_inv;          /* k→r scale K/R                                   */
+} lh_desc;
+
+/* RK45 / Luna.run options (src/Luna.jl:358-362 defaults; src/RK45.jl:19-28). */
+typedef struct lh_rk_opts {
+    double z0;                /* 0.0                                             */
+    double zmax;              /* grid.zmax                                       */
+    double rtol;              /* 1e-6                                            */
+    double atol;              /* 1e-10                                           */
+    double safety;            /* 0.9                                             */
+    double max_dz;            /* zmax/2                                          */
+    double min_dz;            /* 0                                               */
+    double init_dz;           /* 1e-4                                            */
+    int32_t repeat_limit;     /* 10                                              */
+    int32_t save_n;           /* 201 (MemoryOutput GridCondition points)         */
+} lh_rk_opts;
+
+/* Per-run counters (src/RK45.jl:38-82). */
+typedef struct lh_stats {
+    int64_t steps;            /* step attempts                                   */
+    int64_t accepted;
+    int64_t rejected;
+    int64_t rhs_evals;
+    int32_t status;           /* lh status code of this run/instance             */
+    int32_t saved;            /* saves written                                   */
+    double z;                 /* final tn                                        */
+    double dz;                /* final dtn                                       */
+} lh_stats;
+
+typedef struct lh_ctx lh_ctx;
+
+const char *lh_strerror(int status);
+int lh_device_count(int *count);
+
+/* Create a transform context: uploads the constants of *desc to the current
+ * HIP device.  Replaces TransModeAvg(...) / TransRadial(...) construction,
+ * src/NonlinearRHS.jl:340-350, 428-456, and Luna.setup src/Luna.jl:115-152,239-279. */
+int lh_create(lh_ctx **ctx, const lh_desc *desc);
+int lh_destroy(lh_ctx *ctx);
+
+/* One RHS evaluation on host arrays: nl = transform(Eω, z).
+ * Replaces (t::TransModeAvg)(nl, Eω, z) src/NonlinearRHS.jl:362-373 and
+ * (t::TransRadial)(nl, Eω, z) src/NonlinearRHS.jl:466-474.
+ * nl, Ew: complex [n_omega*n_r].  Synchronous. */
+int lh_rhs(lh_ctx *ctx, double *nl, const double *Ew, double z);
+
+/* Same on device pointers, enqueued on stream (hipStream_t, NULL = default).
+ * Errors from inside the kernel are reported by lh_ctx_status after a sync. */
+int lh_rhs_device(lh_ctx *ctx, void *nl_dev, const void *Ew_dev, double z, void *stream);
+int lh_ctx_status(lh_ctx *ctx);
+
+/* Full propagation: Luna.run + RK45.solve_precon with the per-step ω/t window
+ * filter (src/Luna.jl:366-372) and MemoryOutput/GridCondition saves
+ * (src/Output.jl:65-81, 490-502).  The whole adaptive loop runs on the device.
+ * Ew_inout: complex [n_omega*n_r], initial field in, final state out.
+ * saves: complex [save_n][n_omega*n_r] (may be NULL); zsaves: [save_n] (may be NULL). */
+int lh_solve(lh_ctx *ctx, double *Ew_inout, const lh_rk_opts *opts,
+             double *saves, double *zsaves, lh_stats *stats);
+
+/* Batched solve: n independent instances (each its own grid, constants, step
+ * size control and status) in one persistent launch, one workgroup per
+ * instance.  Replaces the per-point loop of Scans.runscan src/Scans.jl:275-430.
+ * Ew_inout[i]: instance i's field; saves[i] may be NULL; stats[i] per instance.
+ * Returns LH_OK when the launch ran; per-instance failures are in stats[i].status. */
+int lh_batch_solve(lh_ctx *const *ctxs, int n, double *const *Ew_inout,
+                   const lh_rk_opts *opts, double *const *saves, double *const *zsaves,
+                   lh_stats *stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LUNA_HIP_H */

@@ -1,0 +1,141 @@
+"""ctypes mirror of ``include/luna_hip.h`` and the loader of ``libluna_hip.so``.
+
+The shared library is built in-tree (``luna.jl_amd/_build/libluna_hip.so``) by
+``luna.jl_amd/build.py``.  There is no CPU fallback: if the library is missing
+every entry point raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+LH_OK, LH_E_ARG, LH_E_HIP, LH_E_PPT_RANGE, LH_E_REPEAT_LIMIT, LH_E_NONFINITE, LH_E_NOMEM, LH_E_NODEVICE = range(8)
+LH_GRID_REAL, LH_GRID_ENV = 0, 1
+LH_TRANS_MODEAVG, LH_TRANS_RADIAL = 0, 1
+LH_KERR_NONE, LH_KERR_FIELD, LH_KERR_ENV = 0, 1, 2
+
+_dp = C.POINTER(C.c_double)
+_u8p = C.POINTER(C.c_uint8)
+
+
+class LhDesc(C.Structure):
+    _fields_ = [
+        ("grid_kind", C.c_int32), ("transform", C.c_int32),
+        ("n_omega", C.c_int32), ("n_t", C.c_int32), ("n_to", C.c_int32), ("n_r", C.c_int32),
+        ("dt_o", C.c_double),
+        ("omega", _dp), ("sidx", _u8p), ("omega_win", _dp), ("t_win", _dp), ("to_win", _dp),
+        ("linop", _dp), ("norm", _dp),
+        ("field_div", C.c_double), ("density", C.c_double),
+        ("kerr", C.c_int32), ("plasma", C.c_int32), ("gamma3", C.c_double),
+        ("ppt_n", C.c_int32), ("raman", C.c_int32),
+        ("ppt_x", _dp), ("ppt_y", _dp), ("ppt_d", _dp),
+        ("ionpot", C.c_double), ("preionfrac", C.c_double),
+        ("raman_h", _dp), ("qdht", _dp), ("qdht_fwd", C.c_double), ("qdht_inv", C.c_double),
+    ]
+
+
+class LhRkOpts(C.Structure):
+    _fields_ = [
+        ("z0", C.c_double), ("zmax", C.c_double), ("rtol", C.c_double), ("atol", C.c_double),
+        ("safety", C.c_double), ("max_dz", C.c_double), ("min_dz", C.c_double),
+        ("init_dz", C.c_double), ("repeat_limit", C.c_int32), ("save_n", C.c_int32),
+    ]
+
+
+class LhStats(C.Structure):
+    _fields_ = [
+        ("steps", C.c_int64), ("accepted", C.c_int64), ("rejected", C.c_int64),
+        ("rhs_evals", C.c_int64), ("status", C.c_int32), ("saved", C.c_int32),
+        ("z", C.c_double), ("dz", C.c_double),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+def _ptr(a, ctype=C.c_double):
+    if a is None:
+        return None
+    return a.ctypes.data_as(C.POINTER(ctype))
+
+
+class DescHolder:
+    """Owns the numpy arrays an ``LhDesc`` points into (keeps them alive)."""
+
+    def __init__(self, **arrays_and_scalars):
+        self._keep = []
+        d = LhDesc()
+        for name, val in arrays_and_scalars.items():
+            if isinstance(val, np.ndarray):
+                if name == "sidx":
+                    arr = np.ascontiguousarray(val, dtype=np.uint8)
+                    setattr(d, name, _ptr(arr, C.c_uint8))
+                else:
+                    if np.iscomplexobj(val):
+                        arr = np.ascontiguousarray(val, dtype=np.complex128)
+                    else:
+                        arr = np.ascontiguousarray(val, dtype=np.float64)
+                    setattr(d, name, _ptr(arr))
+                self._keep.append(arr)
+            elif val is not None:
+                setattr(d, name, val)
+        self.desc = d
+
+    @property
+    def ref(self):
+        return C.byref(self.desc)
+
+
+def rk_opts(zmax, z0=0.0, rtol=1e-6, atol=1e-10, safety=0.9, max_dz=None, min_dz=0.0,
+            init_dz=1e-4, repeat_limit=10, save_n=201):
+    """``Luna.run`` defaults, ``src/Luna.jl:358-362``."""
+    o = LhRkOpts()
+    o.z0, o.zmax, o.rtol, o.atol, o.safety = z0, zmax, rtol, atol, safety
+    o.max_dz = zmax / 2 if max_dz is None else max_dz
+    o.min_dz, o.init_dz, o.repeat_limit, o.save_n = min_dz, init_dz, repeat_limit, save_n
+    return o
+
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_build", "libluna_hip.so")
+_lib = None
+
+
+class LunaHipError(RuntimeError):
+    def __init__(self, status, where=""):
+        self.status = status
+        msg = lib().lh_strerror(status).decode() if _lib is not None else str(status)
+        super().__init__(f"{where}: {msg} (status {status})")
+
+
+def lib():
+    """Load libluna_hip.so; raises if it has not been built (no fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.isfile(LIB_PATH):
+        raise RuntimeError(f"libluna_hip.so not built: expected {LIB_PATH} "
+                           "(run __graft_entry__.build())")
+    L = C.CDLL(LIB_PATH)
+    L.lh_strerror.restype = C.c_char_p
+    L.lh_strerror.argtypes = [C.c_int]
+    L.lh_device_count.argtypes = [C.POINTER(C.c_int)]
+    L.lh_create.argtypes = [C.POINTER(C.c_void_p), C.POINTER(LhDesc)]
+    L.lh_destroy.argtypes = [C.c_void_p]
+    L.lh_rhs.argtypes = [C.c_void_p, _dp, _dp, C.c_double]
+    L.lh_rhs_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_void_p]
+    L.lh_ctx_status.argtypes = [C.c_void_p]
+    L.lh_solve.argtypes = [C.c_void_p, _dp, C.POINTER(LhRkOpts), _dp, _dp, C.POINTER(LhStats)]
+    L.lh_batch_solve.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.POINTER(_dp), C.POINTER(LhRkOpts),
+                                 C.POINTER(_dp), C.POINTER(_dp), C.POINTER(LhStats)]
+    _lib = L
+    return L
+
+
+EXPORTED = ["lh_strerror", "lh_device_count", "lh_create", "lh_destroy", "lh_rhs",
+            "lh_rhs_device", "lh_ctx_status", "lh_solve", "lh_batch_solve"]
+
+
+def check(status, where=""):
+    if status != LH_OK:
+        raise LunaHipError(status, where)

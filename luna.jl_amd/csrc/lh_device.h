@@ -1,0 +1,264 @@
+// lh_device.h -- device building blocks for the propagation kernels (gfx950).
+//
+// Complex arithmetic on double2, LDS-resident Stockham FFTs, block-wide scans
+// and reductions.  One workgroup = LH_T threads (8 wave64s) works on one
+// transform; the whole oversampled time series of one RHS lives in LDS.
+// 512 threads leave 256 registers per lane for the scan/spline phase.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#ifndef LH_T
+#define LH_T 512                // threads per workgroup
+#endif
+#define LH_NW (LH_T / 64)       // wavefronts per workgroup
+#define LH_TWN 16384            // size of the global twiddle table
+
+// ---------------------------------------------------------------- complex ops
+__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 cscal(double s, double2 a) { return make_double2(s * a.x, s * a.y); }
+__device__ __forceinline__ double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
+// a * (-i)
+__device__ __forceinline__ double2 cmul_mi(double2 a) { return make_double2(a.y, -a.x); }
+// a * (+i)
+__device__ __forceinline__ double2 cmul_pi(double2 a) { return make_double2(-a.y, a.x); }
+
+// exp(L*h) for complex L and real h, Julia's exp(::Complex) for finite input.
+__device__ __forceinline__ double2 cexp_lh(double2 L, double h) {
+    double zr = L.x * h, zi = L.y * h;
+    double er = exp(zr);
+    if (zi == 0.0) return make_double2(er, zi);
+    double s, c;
+    sincos(zi, &s, &c);
+    return make_double2(er * c, er * s);
+}
+
+// LDS padding: one spare double2 every 8 keeps radix-8 strided accesses
+// conflict-free for 16-byte lanes.
+__device__ __forceinline__ int pad(int i) { return i + (i >> 3); }
+__host__ __device__ constexpr int padded_len(int n) { return n + (n >> 3) + 1; }
+
+// ------------------------------------------------------------- register DFTs
+template <bool INV>
+__device__ __forceinline__ void dft2(double2* v) {
+    double2 a = v[0];
+    v[0] = cadd(a, v[1]);
+    v[1] = csub(a, v[1]);
+}
+
+template <bool INV>
+__device__ __forceinline__ void dft4(double2& v0, double2& v1, double2& v2, double2& v3) {
+    double2 a0 = cadd(v0, v2), a1 = csub(v0, v2), a2 = cadd(v1, v3), a3 = csub(v1, v3);
+    v0 = cadd(a0, a2);
+    v2 = csub(a0, a2);
+    if (!INV) {
+        v1 = cadd(a1, cmul_mi(a3));
+        v3 = cadd(a1, cmul_pi(a3));
+    } else {
+        v1 = cadd(a1, cmul_pi(a3));
+        v3 = cadd(a1, cmul_mi(a3));
+    }
+}
+
+template <bool INV>
+__device__ __forceinline__ void dft4v(double2* v) { dft4<INV>(v[0], v[1], v[2], v[3]); }
+
+template <bool INV>
+__device__ __forceinline__ void dft8(double2* v) {
+    const double c = 0.70710678118654752440;
+    double2 e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6];
+    double2 o0 = v[1], o1 = v[3], o2 = v[5], o3 = v[7];
+    dft4<INV>(e0, e1, e2, e3);
+    dft4<INV>(o0, o1, o2, o3);
+    double2 w1, w2, w3;
+    if (!INV) {
+        w1 = make_double2(c * (o1.x + o1.y), c * (o1.y - o1.x));
+        w2 = cmul_mi(o2);
+        w3 = make_double2(c * (o3.y - o3.x), -c * (o3.x + o3.y));
+    } else {
+        w1 = make_double2(c * (o1.x - o1.y), c * (o1.x + o1.y));
+        w2 = cmul_pi(o2);
+        w3 = make_double2(-c * (o3.x + o3.y), c * (o3.x - o3.y));
+    }
+    v[0] = cadd(e0, o0); v[4] = csub(e0, o0);
+    v[1] = cadd(e1, w1); v[5] = csub(e1, w1);
+    v[2] = cadd(e2, w2); v[6] = csub(e2, w2);
+    v[3] = cadd(e3, w3); v[7] = csub(e3, w3);
+}
+
+template <int R, bool INV>
+__device__ __forceinline__ void dft(double2* v) {
+    if constexpr (R == 2) dft2<INV>(v);
+    else if constexpr (R == 4) dft4v<INV>(v);
+    else dft8<INV>(v);
+}
+
+// ------------------------------------------------------------ Stockham FFT
+// One radix-R Stockham pass over M complex points in LDS (padded layout).
+// Active threads: TA = min(M/8, LH_T); each does M/(R*TA) butterflies.  Twiddles come
+// from the global table tw[m] = exp(-2πi m / LH_TWN).
+template <int M, int R, int NS, bool INV>
+__device__ __forceinline__ void fft_pass(double2* buf, const double2* __restrict__ tw) {
+    constexpr int TA = (M / 8 < LH_T) ? M / 8 : LH_T;
+    constexpr int NB = M / R;
+    constexpr int BPT = NB / TA;
+    constexpr int TSTRIDE = (LH_TWN / M) * (M / (NS * R));
+    const int tid = threadIdx.x;
+    double2 v[BPT][R];
+    if (tid < TA) {
+#pragma unroll
+        for (int b = 0; b < BPT; b++) {
+            int j = tid + b * TA;
+#pragma unroll
+            for (int r = 0; r < R; r++) v[b][r] = buf[pad(j + r * NB)];
+        }
+    }
+    __syncthreads();
+    if (tid < TA) {
+#pragma unroll
+        for (int b = 0; b < BPT; b++) {
+            int j = tid + b * TA;
+            int k = j & (NS - 1);
+            if constexpr (NS > 1) {
+#pragma unroll
+                for (int r = 1; r < R; r++) {
+                    double2 w = tw[r * k * TSTRIDE];
+                    if (INV) w.y = -w.y;
+                    v[b][r] = cmul(v[b][r], w);
+                }
+            }
+            dft<R, INV>(v[b]);
+            int idx = (j - k) * R + k;
+#pragma unroll
+            for (int r = 0; r < R; r++) buf[pad(idx + r * NS)] = v[b][r];
+        }
+    }
+    __syncthreads();
+}
+
+template <int M, int NS, bool INV>
+__device__ __forceinline__ void fft_passes(double2* buf, const double2* __restrict__ tw) {
+    if constexpr (NS < M) {
+        constexpr int R = (M / NS >= 8) ? 8 : (M / NS);
+        fft_pass<M, R, NS, INV>(buf, tw);
+        fft_passes<M, NS * R, INV>(buf, tw);
+    }
+}
+
+// Unnormalised complex FFT of length M held in LDS (natural order in and out).
+// INV=false: exp(-2πi jk/M) (FFTW_FORWARD); INV=true: exp(+2πi jk/M).
+// Caller must __syncthreads() before (data complete); returns synchronised.
+template <int M, bool INV>
+__device__ __forceinline__ void fft_lds(double2* buf, const double2* __restrict__ tw) {
+    fft_passes<M, 1, INV>(buf, tw);
+}
+
+// Half-length real-transform packing.  For a real series x of length 2M held
+// as z[j] = x[2j] + i x[2j+1]:
+//   c2r pre-step: X[0..M] → Z[0..M-1] with Z = A + iB,
+//     A = X[k] + conj(X[M-k]),  B = (X[k] - conj(X[M-k]))·conj(W_{2M}^k)
+//   (imag of X[0] and X[M] ignored as in FFTW's c2r), then z = IFFT_M(Z).
+template <int M>
+__device__ __forceinline__ void c2r_prepack(double2* buf, const double2* __restrict__ tw) {
+    constexpr int TS = LH_TWN / (2 * M);
+    for (int k = threadIdx.x; k <= M / 2; k += LH_T) {
+        int m = M - k;
+        double2 Xk = buf[pad(k)], Xm = buf[pad(m)];
+        if (k == 0) {
+            Xk.y = 0.0;
+            Xm.y = 0.0;
+            double2 A = cadd(Xk, cconj(Xm));
+            double2 B = csub(Xk, cconj(Xm));
+            buf[pad(0)] = cadd(A, cmul_pi(B));
+        } else if (k == M / 2) {
+            double2 A = cadd(Xk, cconj(Xk));
+            double2 B = cmul(csub(Xk, cconj(Xk)), cconj(tw[k * TS]));
+            buf[pad(k)] = cadd(A, cmul_pi(B));
+        } else {
+            double2 wk = cconj(tw[k * TS]), wm = cconj(tw[m * TS]);
+            double2 Ak = cadd(Xk, cconj(Xm));
+            double2 Bk = cmul(csub(Xk, cconj(Xm)), wk);
+            double2 Am = cadd(Xm, cconj(Xk));
+            double2 Bm = cmul(csub(Xm, cconj(Xk)), wm);
+            buf[pad(k)] = cadd(Ak, cmul_pi(Bk));
+            buf[pad(m)] = cadd(Am, cmul_pi(Bm));
+        }
+    }
+}
+
+// r2c post-step for bin k (0 <= k <= M): Z = FFT_M(z) in LDS →
+//   X[k] = E + W_{2M}^k O,  E = ½(Z[k] + conj(Z[M-k])),  O = -½ i (Z[k] - conj(Z[M-k])).
+template <int M>
+__device__ __forceinline__ double2 r2c_bin(const double2* buf, const double2* __restrict__ tw, int k) {
+    constexpr int TS = LH_TWN / (2 * M);
+    double2 zk = buf[pad(k == M ? 0 : k)];
+    double2 zc = cconj(buf[pad(k == 0 ? 0 : M - k)]);
+    double2 E = cscal(0.5, cadd(zk, zc));
+    double2 d = csub(zk, zc);
+    double2 O = make_double2(0.5 * d.y, -(0.5 * d.x));
+    double2 w = (k == M) ? make_double2(-1.0, 0.0) : tw[k * TS];
+    return cadd(E, cmul(w, O));
+}
+
+// ------------------------------------------------------------ block scans
+// Exclusive prefix sum of one value per thread over the workgroup.
+// wsum: LDS scratch of LH_NW doubles.  Returns this thread's exclusive offset.
+__device__ __forceinline__ double block_exscan(double v, double* wsum) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    double x = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        double y = __shfl_up(x, off, 64);
+        if (lane >= off) x += y;
+    }
+    double ex = __shfl_up(x, 1, 64);
+    if (lane == 0) ex = 0.0;
+    if (lane == 63) wsum[wid] = x;
+    __syncthreads();
+    if (wid == 0) {
+        double w = (lane < LH_NW) ? wsum[lane] : 0.0;
+#pragma unroll
+        for (int off = 1; off < LH_NW; off <<= 1) {
+            double y = __shfl_up(w, off, 64);
+            if (lane >= off) w += y;
+        }
+        double wex = __shfl_up(w, 1, 64);
+        if (lane == 0) wex = 0.0;
+        if (lane < LH_NW) wsum[LH_NW + lane] = wex;
+    }
+    __syncthreads();
+    double r = wsum[LH_NW + wid] + ex;
+    __syncthreads();
+    return r;
+}
+
+// Sum of three values over the workgroup (fixed tree → deterministic).
+__device__ __forceinline__ void block_sum3(double& a, double& b, double& c, double* scratch) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        a += __shfl_xor(a, off, 64);
+        b += __shfl_xor(b, off, 64);
+        c += __shfl_xor(c, off, 64);
+    }
+    if (lane == 0) {
+        scratch[3 * wid + 0] = a;
+        scratch[3 * wid + 1] = b;
+        scratch[3 * wid + 2] = c;
+    }
+    __syncthreads();
+    double sa = 0, sb = 0, sc = 0;
+    for (int w = 0; w < LH_NW; w++) {
+        sa += scratch[3 * w + 0];
+        sb += scratch[3 * w + 1];
+        sc += scratch[3 * w + 2];
+    }
+    __syncthreads();
+    a = sa;
+    b = sb;
+    c = sc;
+}

@@ -1,0 +1,72 @@
+"""Host runtime over the C ABI: device contexts, single and batched solves."""
+import ctypes as C
+
+import numpy as np
+
+from . import _capi
+
+
+def _p(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class Context:
+    """Owns one ``lh_ctx`` (device copies of a transform's constants)."""
+
+    def __init__(self, holder):
+        self.holder = holder
+        d = holder.desc
+        self.n_omega, self.n_r = d.n_omega, d.n_r
+        self.n_state = d.n_omega * d.n_r
+        L = _capi.lib()
+        h = C.c_void_p()
+        _capi.check(L.lh_create(C.byref(h), holder.ref), "lh_create")
+        self.handle = h
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h is not None and h.value:
+            try:
+                _capi.lib().lh_destroy(h)
+            except Exception:
+                pass
+            self.handle = None
+
+    def rhs(self, Ew, z=0.0):
+        Ew = np.ascontiguousarray(np.asarray(Ew, dtype=np.complex128).reshape(-1, order="F"))
+        if Ew.size != self.n_state:
+            raise ValueError("Eω has the wrong size")
+        nl = np.empty_like(Ew)
+        _capi.check(_capi.lib().lh_rhs(self.handle, _p(nl), _p(Ew), z), "lh_rhs")
+        return nl
+
+    def solve(self, Ew0, opts, want_saves=True):
+        Ew = np.ascontiguousarray(np.asarray(Ew0, dtype=np.complex128).reshape(-1, order="F")).copy()
+        saves = np.zeros((opts.save_n, self.n_state), dtype=np.complex128) if want_saves else None
+        zs = np.zeros(max(opts.save_n, 1))
+        stats = _capi.LhStats()
+        st = _capi.lib().lh_solve(self.handle, _p(Ew), C.byref(opts),
+                                  _p(saves) if want_saves else None, _p(zs), C.byref(stats))
+        _capi.check(st, "lh_solve")
+        return Ew, saves, zs, stats
+
+
+def batch_solve(contexts, Ews, opts, want_saves=True):
+    """Run every instance in one persistent launch (``lh_batch_solve``)."""
+    n = len(contexts)
+    handles = (C.c_void_p * n)(*[c.handle for c in contexts])
+    ew_arrs = [np.ascontiguousarray(np.asarray(e, dtype=np.complex128).reshape(-1, order="F")).copy()
+               for e in Ews]
+    ew_ptrs = (C.POINTER(C.c_double) * n)(*[_p(a) for a in ew_arrs])
+    if want_saves:
+        sv = [np.zeros((opts.save_n, c.n_state), dtype=np.complex128) for c in contexts]
+        sv_ptrs = (C.POINTER(C.c_double) * n)(*[_p(a) for a in sv])
+    else:
+        sv = [None] * n
+        sv_ptrs = None
+    zs = [np.zeros(max(opts.save_n, 1)) for _ in range(n)]
+    zs_ptrs = (C.POINTER(C.c_double) * n)(*[_p(a) for a in zs])
+    stats = (_capi.LhStats * n)()
+    st = _capi.lib().lh_batch_solve(handles, n, ew_ptrs, C.byref(opts), sv_ptrs, zs_ptrs, stats)
+    _capi.check(st, "lh_batch_solve")
+    return ew_arrs, sv, zs, list(stats)

@@ -1,0 +1,133 @@
+"""ctypes wrapper of the C oracle (TEST INFRASTRUCTURE ONLY).
+
+Only tests/, ``__graft_entry__.smoke`` and bench.py's ``cpu_baseline`` leg load
+this module, and only as the checker.  The product never imports it.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "_build", "libluna_oracle.so")
+_lib = None
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", HERE])
+
+
+def lib(capi):
+    """Load the oracle; ``capi`` is the package's ``_capi`` module (struct layouts)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.isfile(LIB):
+        build()
+    L = C.CDLL(LIB)
+    dp = C.POINTER(C.c_double)
+    D = C.POINTER(capi.LhDesc)
+    L.orc_fft_c2c.argtypes = [C.c_int, dp, C.c_int]
+    L.orc_rfft.argtypes = [C.c_int, dp, dp]
+    L.orc_irfft_unnorm.argtypes = [C.c_int, dp, dp]
+    L.orc_cumtrapz.argtypes = [C.c_int, dp, C.c_double, dp]
+    L.orc_cspline_eval.argtypes = [C.c_int, dp, dp, dp, C.c_int, dp, dp]
+    L.orc_ppt_rate.argtypes = [D, C.c_int, dp, dp]
+    L.orc_responses_real.argtypes = [D, dp, dp]
+    L.orc_rhs.argtypes = [D, dp, dp, C.c_double]
+    L.orc_solve.argtypes = [D, dp, C.POINTER(capi.LhRkOpts), dp, dp, C.POINTER(capi.LhStats)]
+    L.ORC_RHS_CB = C.CFUNCTYPE(C.c_int, C.c_void_p, dp, dp, C.c_double)
+    L.orc_solve_precon.argtypes = [C.c_int, dp, dp, C.c_double, C.c_double, C.c_double, C.c_double,
+                                   C.c_double, L.ORC_RHS_CB, C.c_void_p, C.c_int, dp, dp,
+                                   C.POINTER(C.c_int64)]
+    _lib = L
+    return L
+
+
+def _p(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class Oracle:
+    def __init__(self, capi):
+        self.capi = capi
+        self.L = lib(capi)
+
+    def fft(self, x, sign=-1):
+        y = np.ascontiguousarray(x, dtype=np.complex128).copy()
+        self.L.orc_fft_c2c(len(y), _p(y), sign)
+        return y
+
+    def rfft(self, x):
+        x = np.ascontiguousarray(x, dtype=float)
+        out = np.zeros(len(x) // 2 + 1, dtype=np.complex128)
+        self.L.orc_rfft(len(x), _p(x), _p(out))
+        return out
+
+    def irfft_unnorm(self, X, n):
+        X = np.ascontiguousarray(X, dtype=np.complex128)
+        out = np.zeros(n)
+        self.L.orc_irfft_unnorm(n, _p(X), _p(out))
+        return out
+
+    def cumtrapz(self, y, dx):
+        y = np.ascontiguousarray(y, dtype=float)
+        out = np.zeros_like(y)
+        self.L.orc_cumtrapz(len(y), _p(y), dx, _p(out))
+        return out
+
+    def cspline(self, x, y, D, x0):
+        x0 = np.ascontiguousarray(x0, dtype=float)
+        out = np.zeros_like(x0)
+        self.L.orc_cspline_eval(len(x), _p(np.ascontiguousarray(x)), _p(np.ascontiguousarray(y)),
+                                _p(np.ascontiguousarray(D)), len(x0), _p(x0), _p(out))
+        return out
+
+    def ppt_rate(self, holder, E):
+        E = np.ascontiguousarray(E, dtype=float)
+        out = np.zeros_like(E)
+        st = self.L.orc_ppt_rate(holder.ref, len(E), _p(E), _p(out))
+        return st, out
+
+    def responses_real(self, holder, E):
+        E = np.ascontiguousarray(E, dtype=float)
+        out = np.zeros_like(E)
+        st = self.L.orc_responses_real(holder.ref, _p(E), _p(out))
+        return st, out
+
+    def rhs(self, holder, Ew, z=0.0):
+        Ew = np.ascontiguousarray(Ew, dtype=np.complex128)
+        nl = np.zeros_like(Ew)
+        st = self.L.orc_rhs(holder.ref, _p(nl), _p(Ew), z)
+        return st, nl
+
+    def solve(self, holder, Ew0, opts, want_saves=True):
+        d = holder.desc
+        n = d.n_omega * d.n_r
+        Ew = np.ascontiguousarray(Ew0, dtype=np.complex128).copy()
+        saves = np.zeros((opts.save_n, n), dtype=np.complex128) if want_saves else None
+        zs = np.zeros(opts.save_n)
+        stats = self.capi.LhStats()
+        st = self.L.orc_solve(holder.ref, _p(Ew), C.byref(opts), _p(saves) if want_saves else None,
+                              _p(zs), C.byref(stats))
+        return st, Ew, saves, zs, stats
+
+    def solve_precon(self, f, linop, y0, t0, dt0, tmax, rtol=1e-6, atol=1e-10, outputN=201):
+        n = len(y0)
+        linop = np.ascontiguousarray(linop, dtype=np.complex128)
+        y0 = np.ascontiguousarray(y0, dtype=np.complex128)
+        tout = np.zeros(outputN)
+        yout = np.zeros((outputN, n), dtype=np.complex128)
+
+        def cb(user, out, y, t):
+            yv = np.ctypeslib.as_array(y, shape=(2 * n,)).view(np.complex128)
+            ov = np.ctypeslib.as_array(out, shape=(2 * n,)).view(np.complex128)
+            ov[:] = f(yv.copy(), t)
+            return 0
+
+        cfun = self.L.ORC_RHS_CB(cb)
+        steps = C.c_int64(0)
+        st = self.L.orc_solve_precon(n, _p(linop), _p(y0), t0, dt0, tmax, rtol, atol, cfun, None,
+                                     outputN, _p(tout), _p(yout), C.byref(steps))
+        return st, tout, yout, steps.value
