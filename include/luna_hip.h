@@ -145,6 +145,23 @@ int lh_batch_solve(lh_ctx *const *ctxs, int n, double *const *Ew_inout,
                    const lh_rk_opts *opts, double *const *saves, double *const *zsaves,
                    lh_stats *stats);
 
+/* Device-resident batch path (no counterpart in the reference; it splits
+ * lh_batch_solve so a sweep can be rerun on inputs already in HBM):
+ * lh_set_state uploads an instance's initial field once; lh_batch_solve_resident
+ * solves every instance from its resident initial field and leaves the final
+ * state and saves in device memory; lh_get_state / lh_get_saves copy them out. */
+int lh_set_state(lh_ctx *ctx, const double *Ew);
+int lh_batch_solve_resident(lh_ctx *const *ctxs, int n, const lh_rk_opts *opts, int want_saves,
+                            lh_stats *stats);
+int lh_get_state(lh_ctx *ctx, double *Ew);
+int lh_get_saves(lh_ctx *ctx, double *saves, double *zsaves);
+
+/* Measurement hook (no counterpart in the reference): HIP-event durations of
+ * the persistent launches of the last batch, one per grid-size class
+ * (Nto = 16384, 8192, 4096), with the number of instances in each.
+ * nclass must be >= 3. */
+int lh_last_launch_times(double *ms, int *n_inst, int nclass);
+
 #ifdef __cplusplus
 }
 #endif

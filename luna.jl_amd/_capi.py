@@ -128,12 +128,21 @@ def lib():
     L.lh_solve.argtypes = [C.c_void_p, _dp, C.POINTER(LhRkOpts), _dp, _dp, C.POINTER(LhStats)]
     L.lh_batch_solve.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.POINTER(_dp), C.POINTER(LhRkOpts),
                                  C.POINTER(_dp), C.POINTER(_dp), C.POINTER(LhStats)]
+    L.lh_set_state.argtypes = [C.c_void_p, _dp]
+    L.lh_get_state.argtypes = [C.c_void_p, _dp]
+    L.lh_get_saves.argtypes = [C.c_void_p, _dp, _dp]
+    L.lh_batch_solve_resident.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.POINTER(LhRkOpts), C.c_int,
+                                          C.POINTER(LhStats)]
+    L.lh_last_launch_times.argtypes = [_dp, C.POINTER(C.c_int), C.c_int]
     _lib = L
     return L
 
 
 EXPORTED = ["lh_strerror", "lh_device_count", "lh_create", "lh_destroy", "lh_rhs",
-            "lh_rhs_device", "lh_ctx_status", "lh_solve", "lh_batch_solve"]
+            "lh_rhs_device", "lh_ctx_status", "lh_solve", "lh_batch_solve", "lh_set_state",
+            "lh_batch_solve_resident", "lh_get_state", "lh_get_saves", "lh_last_launch_times"]
+
+CLASS_NTO = (16384, 8192, 4096)
 
 
 def check(status, where=""):

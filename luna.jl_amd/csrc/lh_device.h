@@ -97,16 +97,34 @@ __device__ __forceinline__ void dft(double2* v) {
     else dft8<INV>(v);
 }
 
+// ------------------------------------------------------------ twiddles
+// Two-level table in LDS: W^m = hi[m >> 7] · lo[m & 127], W = exp(-2πi/LH_TWN).
+// Filled once per workgroup from the global table (4 KB of LDS).
+struct TwLds {
+    double2 lo[128];
+    double2 hi[128];
+};
+
+__device__ __forceinline__ void tw_load(TwLds& t, const double2* __restrict__ tw) {
+    for (int i = threadIdx.x; i < 128; i += LH_T) {
+        t.lo[i] = tw[i];
+        t.hi[i] = tw[128 * i];
+    }
+}
+
+__device__ __forceinline__ double2 tw_get(const TwLds& t, int m) { return cmul(t.hi[m >> 7], t.lo[m & 127]); }
+
 // ------------------------------------------------------------ Stockham FFT
 // One radix-R Stockham pass over M complex points in LDS (padded layout).
-// Active threads: TA = min(M/8, LH_T); each does M/(R*TA) butterflies.  Twiddles come
-// from the global table tw[m] = exp(-2πi m / LH_TWN).
+// Active threads: TA = min(M/8, LH_T); each does M/(R*TA) butterflies.  The
+// twiddle w = exp(∓2πi k/(NS·R)) comes from the LDS table; its powers w^r by
+// repeated multiplication (error ≤ R ulp, far below the 1e-12 parity budget).
 template <int M, int R, int NS, bool INV>
-__device__ __forceinline__ void fft_pass(double2* buf, const double2* __restrict__ tw) {
+__device__ __forceinline__ void fft_pass(double2* buf, const TwLds& tw) {
     constexpr int TA = (M / 8 < LH_T) ? M / 8 : LH_T;
     constexpr int NB = M / R;
     constexpr int BPT = NB / TA;
-    constexpr int TSTRIDE = (LH_TWN / M) * (M / (NS * R));
+    constexpr int TSTRIDE = LH_TWN / (NS * R);
     const int tid = threadIdx.x;
     double2 v[BPT][R];
     if (tid < TA) {
@@ -124,11 +142,13 @@ __device__ __forceinline__ void fft_pass(double2* buf, const double2* __restrict
             int j = tid + b * TA;
             int k = j & (NS - 1);
             if constexpr (NS > 1) {
+                double2 w1 = tw_get(tw, k * TSTRIDE);
+                if (INV) w1.y = -w1.y;
+                double2 w = w1;
 #pragma unroll
                 for (int r = 1; r < R; r++) {
-                    double2 w = tw[r * k * TSTRIDE];
-                    if (INV) w.y = -w.y;
                     v[b][r] = cmul(v[b][r], w);
+                    if (r + 1 < R) w = cmul(w, w1);
                 }
             }
             dft<R, INV>(v[b]);
@@ -141,7 +161,7 @@ __device__ __forceinline__ void fft_pass(double2* buf, const double2* __restrict
 }
 
 template <int M, int NS, bool INV>
-__device__ __forceinline__ void fft_passes(double2* buf, const double2* __restrict__ tw) {
+__device__ __forceinline__ void fft_passes(double2* buf, const TwLds& tw) {
     if constexpr (NS < M) {
         constexpr int R = (M / NS >= 8) ? 8 : (M / NS);
         fft_pass<M, R, NS, INV>(buf, tw);
@@ -153,7 +173,7 @@ __device__ __forceinline__ void fft_passes(double2* buf, const double2* __restri
 // INV=false: exp(-2πi jk/M) (FFTW_FORWARD); INV=true: exp(+2πi jk/M).
 // Caller must __syncthreads() before (data complete); returns synchronised.
 template <int M, bool INV>
-__device__ __forceinline__ void fft_lds(double2* buf, const double2* __restrict__ tw) {
+__device__ __forceinline__ void fft_lds(double2* buf, const TwLds& tw) {
     fft_passes<M, 1, INV>(buf, tw);
 }
 
@@ -163,7 +183,7 @@ __device__ __forceinline__ void fft_lds(double2* buf, const double2* __restrict_
 //     A = X[k] + conj(X[M-k]),  B = (X[k] - conj(X[M-k]))·conj(W_{2M}^k)
 //   (imag of X[0] and X[M] ignored as in FFTW's c2r), then z = IFFT_M(Z).
 template <int M>
-__device__ __forceinline__ void c2r_prepack(double2* buf, const double2* __restrict__ tw) {
+__device__ __forceinline__ void c2r_prepack(double2* buf, const TwLds& tw) {
     constexpr int TS = LH_TWN / (2 * M);
     for (int k = threadIdx.x; k <= M / 2; k += LH_T) {
         int m = M - k;
@@ -176,10 +196,10 @@ __device__ __forceinline__ void c2r_prepack(double2* buf, const double2* __restr
             buf[pad(0)] = cadd(A, cmul_pi(B));
         } else if (k == M / 2) {
             double2 A = cadd(Xk, cconj(Xk));
-            double2 B = cmul(csub(Xk, cconj(Xk)), cconj(tw[k * TS]));
+            double2 B = cmul(csub(Xk, cconj(Xk)), cconj(tw_get(tw, k * TS)));
             buf[pad(k)] = cadd(A, cmul_pi(B));
         } else {
-            double2 wk = cconj(tw[k * TS]), wm = cconj(tw[m * TS]);
+            double2 wk = cconj(tw_get(tw, k * TS)), wm = cconj(tw_get(tw, m * TS));
             double2 Ak = cadd(Xk, cconj(Xm));
             double2 Bk = cmul(csub(Xk, cconj(Xm)), wk);
             double2 Am = cadd(Xm, cconj(Xk));
@@ -193,21 +213,21 @@ __device__ __forceinline__ void c2r_prepack(double2* buf, const double2* __restr
 // r2c post-step for bin k (0 <= k <= M): Z = FFT_M(z) in LDS →
 //   X[k] = E + W_{2M}^k O,  E = ½(Z[k] + conj(Z[M-k])),  O = -½ i (Z[k] - conj(Z[M-k])).
 template <int M>
-__device__ __forceinline__ double2 r2c_bin(const double2* buf, const double2* __restrict__ tw, int k) {
+__device__ __forceinline__ double2 r2c_bin(const double2* buf, const TwLds& tw, int k) {
     constexpr int TS = LH_TWN / (2 * M);
     double2 zk = buf[pad(k == M ? 0 : k)];
     double2 zc = cconj(buf[pad(k == 0 ? 0 : M - k)]);
     double2 E = cscal(0.5, cadd(zk, zc));
     double2 d = csub(zk, zc);
     double2 O = make_double2(0.5 * d.y, -(0.5 * d.x));
-    double2 w = (k == M) ? make_double2(-1.0, 0.0) : tw[k * TS];
+    double2 w = (k == M) ? make_double2(-1.0, 0.0) : tw_get(tw, k * TS);
     return cadd(E, cmul(w, O));
 }
 
 // ------------------------------------------------------------ block scans
-// Exclusive prefix sum of one value per thread over the workgroup.
-// wsum: LDS scratch of LH_NW doubles.  Returns this thread's exclusive offset.
-__device__ __forceinline__ double block_exscan(double v, double* wsum) {
+// Exclusive prefix sum of one value per thread over the workgroup; also
+// returns the workgroup total.  wsum: LDS scratch of 2*LH_NW+1 doubles.
+__device__ __forceinline__ double block_exscan(double v, double* wsum, double* total = nullptr) {
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     double x = v;
 #pragma unroll
@@ -229,9 +249,11 @@ __device__ __forceinline__ double block_exscan(double v, double* wsum) {
         double wex = __shfl_up(w, 1, 64);
         if (lane == 0) wex = 0.0;
         if (lane < LH_NW) wsum[LH_NW + lane] = wex;
+        if (lane == LH_NW - 1) wsum[2 * LH_NW] = w;
     }
     __syncthreads();
     double r = wsum[LH_NW + wid] + ex;
+    if (total) *total = wsum[2 * LH_NW];
     __syncthreads();
     return r;
 }

@@ -43,21 +43,21 @@ static const long long LH_MAX_STEPS = 500000;  // hard cap on step attempts per 
 struct DInst {
     int grid_kind, transform, n_omega, n_t, n_to, n_r;
     int kerr, plasma, raman, ppt_n;
-    double field_div, density, kerr_fac, ionpot, preion_p1, e_ratio;
-    double dt_o, inv_nto, inv_nt, sc_t, sc_f, ppt_emin, ppt_emax;
+    double density, kerr_fac, ionpot, preion_p1, e_ratio;
+    double dt_o, e_scale, inv_nt, sc_t, ppt_emin, ppt_emax;
     const double* omega_win;
-    const uint8_t* sidx;
     const double* t_win;
     const double* to_win;
     const double2* linop;
-    const double2* norm;
-    const double4* ppt;  // (x, y, D, 0) per node
+    const double2* outfac;  // per-bin output factor: sc_f·norm·ωwin (sidx) or sc_f
+    const double4* ppt;     // (x, y, D, 0) per spline node
     // solver state
-    double2* Y;
+    const double2* Yin;  // initial field (never written)
+    double2* Y;          // receives the final state s.yn
     double2* YN;
     double2* K[7];
-    double2* saves;      // [save_n][n_state] or null
-    const double* zgrid; // [save_n]
+    double2* saves;       // [save_n][n_state] or null
+    const double* zgrid;  // [save_n]
     lh_stats* stats;
     double z0, zmax, rtol, atol, safety, max_dz, min_dz, init_dz;
     int repeat_limit, save_n;
@@ -83,11 +83,22 @@ __constant__ double c_iC[4][7] = {
     {37.0 / 12, 0, -1000.0 / 159, 125.0 / 12, -729.0 / 106, 11.0 / 3, -4},
     {-145.0 / 128, 0, 1000.0 / 371, -375.0 / 64, 25515.0 / 6784, -55.0 / 28, 5.0 / 2}};
 
-struct Smem {
-    double2 buf[padded_len(8192)];
+// LDS of one workgroup: the padded transform buffer sized for the largest
+// transform it runs (MO complex points + Nyquist), twiddles, scan scratch.
+template <int MO>
+struct SmemT {
+    double2 buf[padded_len(MO)];
+    TwLds tw;
     double edge[LH_T];
-    double wsum[5 * LH_NW];
+    double wsum[5 * LH_NW];  // scans use [0, 2*LH_NW], block_last uses [4*LH_NW]
     int flag;
+};
+
+// Number of state bins each thread owns for an oversampled half-length MO.
+template <int MO>
+struct Bins {
+    static constexpr int N = MO / 2 + 1;  // Nω of a real grid with Nto = 2·MO, Nt = MO
+    static constexpr int NPT = (N + LH_T - 1) / LH_T;
 };
 
 // --------------------------------------------------------------------------
@@ -119,187 +130,256 @@ __device__ __forceinline__ double ppt_rate(const DInst& in, double aE, int& bad)
     return exp(v);
 }
 
-// cumtrapz over the workgroup: thread owns Q consecutive samples y[q]
-// (global index g0+q); on return y[q] holds the running integral.
-template <int Q>
-__device__ __forceinline__ void block_cumtrapz(double* y, double dx, int g0, Smem& sm) {
+// cumtrapz over the workgroup for one round of QR samples per thread
+// (global index g0+q).  carry: running integral before this round; prev0:
+// the integrand at the sample just before this round (thread 0 only).  On
+// return y[q] holds the running integral and carry the round's final value.
+template <int QR, class Smem>
+__device__ __forceinline__ void block_cumtrapz(double* y, double dx, int g0, Smem& sm, double& carry,
+                                               double prev0) {
     const int tid = threadIdx.x;
-    sm.edge[tid] = y[Q - 1];
+    sm.edge[tid] = y[QR - 1];
     __syncthreads();
-    double prev = (tid > 0) ? sm.edge[tid - 1] : 0.0;
+    double prev = (tid > 0) ? sm.edge[tid - 1] : prev0;
     double acc = 0.0;
     double last = prev;
 #pragma unroll
-    for (int q = 0; q < Q; q++) {
+    for (int q = 0; q < QR; q++) {
         double cur = y[q];
         double term = (g0 + q == 0) ? 0.0 : (0.5 * (last + cur)) * dx;
         last = cur;
         acc += term;
         y[q] = acc;
     }
-    double off = block_exscan(acc, sm.wsum);
+    double total;
+    double off = carry + block_exscan(acc, sm.wsum, &total);
 #pragma unroll
-    for (int q = 0; q < Q; q++) y[q] = off + y[q];
+    for (int q = 0; q < QR; q++) y[q] = off + y[q];
+    carry = carry + total;
+}
+
+// Last integrand value of a round (held by the last thread) → all threads.
+template <class Smem>
+__device__ __forceinline__ double block_last(double v, Smem& sm) {
+    if (threadIdx.x == LH_T - 1) sm.wsum[4 * LH_NW] = v;
+    __syncthreads();
+    double r = sm.wsum[4 * LH_NW];
+    __syncthreads();
+    return r;
+}
+
+template <class Smem>
+__device__ __forceinline__ double field_at(const Smem& sm, int g, double e_scale) {
+    double2 v = sm.buf[pad(g >> 1)];
+    return ((g & 1) ? v.y : v.x) * e_scale;
 }
 
 // Pointwise field responses on the packed real series in LDS (in place):
-// E = (c2r/Nto)/field_div; P = ρε0γ3 E³ + ρ·Plasma(E); P *= towin.
-template <int MO>
-__device__ __forceinline__ double field_sample(const Smem& sm, int tid, int q, double inv_nto, double fdiv) {
-    constexpr int S = MO / LH_T;
-    double2 v = sm.buf[pad(S * tid + (q >> 1))];
-    return (inv_nto * ((q & 1) ? v.y : v.x)) / fdiv;
-}
-
-// The packed real series stays in LDS (this thread's Q samples are only
-// touched by this thread); scan operands live in registers.
-template <int MO>
+// E = c2r/Nto/field_div; P = ρε0γ3 E³ + ρ·Plasma(E); P *= towin.
+// The series is processed in rounds of QR samples per thread (contiguous
+// chunks), with the three cumtrapz integrals carried from round to round;
+// each thread's samples are touched only by that thread.
+template <int MO, class Smem>
 __device__ __forceinline__ void pointwise_field(const DInst& in, Smem& sm) {
-    constexpr int S = MO / LH_T;
-    constexpr int Q = 2 * S;
+    constexpr int QT = 2 * MO / LH_T;          // samples per thread over all rounds
+    constexpr int QR = (QT > 16) ? 16 : QT;    // samples per thread per round
+    constexpr int NR = QT / QR;                // rounds
     const int tid = threadIdx.x;
-    const int g0 = Q * tid;
-    const double inv_nto = in.inv_nto, fdiv = in.field_div;
-    double a[Q];  // plasma polarisation (zero without plasma)
-    if (in.plasma) {
-        double jf[Q];
-        int bad = 0;
-#pragma unroll
-        for (int q = 0; q < Q; q++) {
-            double E = field_sample<MO>(sm, tid, q, inv_nto, fdiv);
-            a[q] = ppt_rate(in, fabs(E), bad);
-            jf[q] = a[q];  // rate, kept for the loss term
-        }
-        if (bad) sm.flag = bad;
-        const double dt = in.dt_o;
-        block_cumtrapz<Q>(a, dt, g0, sm);  // a = ∫rate
-        const double pp1 = in.preion_p1, er = in.e_ratio, ip = in.ionpot;
-#pragma unroll
-        for (int q = 0; q < Q; q++) {
-            double E = field_sample<MO>(sm, tid, q, inv_nto, fdiv);
-            double f = pp1 - exp(-a[q]);
-            double rate = jf[q];
-            a[q] = (f * er) * E;                                           // phase
-            jf[q] = (E != 0.0) ? ((ip * rate) * (1 - f)) / E : 0.0;  // loss current
-        }
-        block_cumtrapz<Q>(a, dt, g0, sm);  // a = J (without loss term)
-#pragma unroll
-        for (int q = 0; q < Q; q++) a[q] = a[q] + jf[q];
-        block_cumtrapz<Q>(a, dt, g0, sm);  // a = P
-    }
+    const double es = in.e_scale;
     const bool kerr = (in.kerr == LH_KERR_FIELD), plasma = in.plasma;
     const double kfac = in.kerr_fac, rho = in.density;
     const double* tw = in.to_win;
-#pragma unroll
-    for (int s = 0; s < S; s++) {
-        double2 v = sm.buf[pad(S * tid + s)];
-        double E0 = (inv_nto * v.x) / fdiv, E1 = (inv_nto * v.y) / fdiv;
-        double p0 = kerr ? kfac * (E0 * E0 * E0) : 0.0;
-        double p1 = kerr ? kfac * (E1 * E1 * E1) : 0.0;
+    const double dt = in.dt_o, pp1 = in.preion_p1, er = in.e_ratio, ip = in.ionpot;
+    double cF = 0.0, cJ = 0.0, cP = 0.0;           // carried integrals
+    double lastRate = 0.0, lastPhi = 0.0, lastJ = 0.0;
+    double Fprev = 0.0, fprev = pp1 - 1.0;         // exp(-0) = 1
+#pragma unroll 1
+    for (int r = 0; r < NR; r++) {
+        const int g0 = r * (LH_T * QR) + QR * tid;
+        double a[QR];
         if (plasma) {
-            p0 = p0 + rho * a[2 * s];
-            p1 = p1 + rho * a[2 * s + 1];
+            double jf[QR];
+            int bad = 0;
+#pragma unroll
+            for (int q = 0; q < QR; q++) {
+                double E = field_at(sm, g0 + q, es);
+                a[q] = ppt_rate(in, fabs(E), bad);
+                jf[q] = a[q];  // rate, kept for the loss term
+            }
+            if (bad) sm.flag = bad;
+            double lr = block_last(jf[QR - 1], sm);
+            block_cumtrapz<QR>(a, dt, g0, sm, cF, lastRate);  // a = ∫rate
+            lastRate = lr;
+#pragma unroll
+            for (int q = 0; q < QR; q++) {
+                double E = field_at(sm, g0 + q, es);
+                double F = a[q];
+                double f = (F == Fprev) ? fprev : pp1 - exp(-F);  // F is flat outside the pulse
+                Fprev = F;
+                fprev = f;
+                double rate = jf[q];
+                a[q] = (f * er) * E;                                                     // phase
+                jf[q] = (rate != 0.0 && E != 0.0) ? ((ip * rate) * (1 - f)) / E : 0.0;  // loss current
+            }
+            double lp = block_last(a[QR - 1], sm);
+            block_cumtrapz<QR>(a, dt, g0, sm, cJ, lastPhi);  // a = J without loss term
+            lastPhi = lp;
+#pragma unroll
+            for (int q = 0; q < QR; q++) a[q] = a[q] + jf[q];
+            double lj = block_last(a[QR - 1], sm);
+            block_cumtrapz<QR>(a, dt, g0, sm, cP, lastJ);  // a = P
+            lastJ = lj;
         }
-        sm.buf[pad(S * tid + s)] = make_double2(p0 * tw[g0 + 2 * s], p1 * tw[g0 + 2 * s + 1]);
+#pragma unroll
+        for (int s2 = 0; s2 < QR / 2; s2++) {
+            const int g = g0 + 2 * s2;
+            double2 v = sm.buf[pad(g >> 1)];
+            double E0 = v.x * es, E1 = v.y * es;
+            double p0 = kerr ? kfac * (E0 * E0 * E0) : 0.0;
+            double p1 = kerr ? kfac * (E1 * E1 * E1) : 0.0;
+            if (plasma) {
+                p0 = p0 + rho * a[2 * s2];
+                p1 = p1 + rho * a[2 * s2 + 1];
+            }
+            sm.buf[pad(g >> 1)] = make_double2(p0 * tw[g], p1 * tw[g + 1]);
+        }
     }
 }
 
 // Field-grid mode-averaged transform on LDS data.  On entry buf[0..N) holds
 // sc_t·Eω (zero to MO).  Calls epi(k, nl_k) for every output bin k < N.
-template <int MO, class Epi>
-__device__ __forceinline__ void rhs_field(const DInst& in, Smem& sm, const double2* __restrict__ tw, Epi epi) {
+template <int MO, class Smem, class Epi>
+__device__ __forceinline__ void rhs_field(const DInst& in, Smem& sm, Epi epi) {
     __syncthreads();
-    c2r_prepack<MO>(sm.buf, tw);
+    c2r_prepack<MO>(sm.buf, sm.tw);
     __syncthreads();
-    fft_lds<MO, true>(sm.buf, tw);
-    pointwise_field<MO>(in, sm);
+    fft_lds<MO, true>(sm.buf, sm.tw);
+    pointwise_field<MO, Smem>(in, sm);
+    constexpr int N = Bins<MO>::N, NPT = Bins<MO>::NPT;
+    double2 of[NPT];  // loads in flight during the forward FFT
+#pragma unroll
+    for (int i = 0; i < NPT; i++) {
+        int k = threadIdx.x + i * LH_T;
+        of[i] = (k < N) ? in.outfac[k] : make_double2(0.0, 0.0);
+    }
     __syncthreads();
-    fft_lds<MO, false>(sm.buf, tw);
-    for (int k = threadIdx.x; k < in.n_omega; k += LH_T) {
-        double2 nl = cscal(in.sc_f, r2c_bin<MO>(sm.buf, tw, k));
-        if (in.sidx[k]) {
-            nl = cmul(nl, in.norm[k]);
-            nl = cscal(in.omega_win[k], nl);
-        }
-        epi(k, nl);
+    fft_lds<MO, false>(sm.buf, sm.tw);
+#pragma unroll
+    for (int i = 0; i < NPT; i++) {
+        int k = threadIdx.x + i * LH_T;
+        if (k < N) epi(k, cmul(r2c_bin<MO>(sm.buf, sm.tw, k), of[i]));
     }
 }
 
 // Stage prologue: buf[k] = sc_t · (y + Σ_j Δz B_ij k_j)·exp(L hf); k1 first
-// propagated by h0 (evaluate!, src/RK45.jl:209-223).
-template <int MO>
+// propagated by h0 (evaluate!, src/RK45.jl:209-223).  Loops run over the
+// k-vectors outside and this thread's bins inside, so each vector's loads
+// for all bins are in flight together.
+template <int MO, class Smem>
 __device__ __forceinline__ void stage_prologue(const DInst& in, Smem& sm, const double2* Y, double2* const* K,
                                                int stage, double dt, double h0, double hf) {
-    const int N = in.n_omega;
-    double c[6];
+    constexpr int N = Bins<MO>::N, NPT = Bins<MO>::NPT;
+    const int tid = threadIdx.x;
+    double2 yb[NPT], L[NPT];
 #pragma unroll
-    for (int j = 0; j < 6; j++) c[j] = (j <= stage) ? dt * c_B[stage][j] : 0.0;
-    for (int k = threadIdx.x; k < N; k += LH_T) {
-        double2 L = in.linop[k];
-        double2 yb = Y[k];
-        if (stage == 0 && h0 != 0.0) {
-            double2 k1 = cmul(K[0][k], cexp_lh(L, h0));
-            K[0][k] = k1;
-        }
-#pragma unroll
-        for (int j = 0; j < 6; j++)
-            if (j <= stage && c_B[stage][j] != 0.0) yb = cadd(yb, cscal(c[j], K[j][k]));
-        double2 yp = (hf != 0.0) ? cmul(yb, cexp_lh(L, hf)) : yb;
-        sm.buf[pad(k)] = cscal(in.sc_t, yp);
+    for (int i = 0; i < NPT; i++) {
+        int k = tid + i * LH_T;
+        bool v = k < N;
+        yb[i] = v ? Y[k] : make_double2(0.0, 0.0);
+        L[i] = v ? in.linop[k] : make_double2(0.0, 0.0);
     }
-    for (int k = N + threadIdx.x; k <= MO; k += LH_T) sm.buf[pad(k)] = make_double2(0.0, 0.0);
+    if (stage == 0 && h0 != 0.0) {
+        double2* K0 = K[0];
+#pragma unroll
+        for (int i = 0; i < NPT; i++) {
+            int k = tid + i * LH_T;
+            if (k < N) K0[k] = cmul(K0[k], cexp_lh(L[i], h0));
+        }
+    }
+#pragma unroll 1
+    for (int j = 0; j <= stage; j++) {
+        const double cj = dt * c_B[stage][j];
+        if (cj == 0.0) continue;
+        const double2* Kj = K[j];
+#pragma unroll
+        for (int i = 0; i < NPT; i++) {
+            int k = tid + i * LH_T;
+            if (k < N) yb[i] = cadd(yb[i], cscal(cj, Kj[k]));
+        }
+    }
+    const double sc_t = in.sc_t;
+#pragma unroll
+    for (int i = 0; i < NPT; i++) {
+        int k = tid + i * LH_T;
+        if (k < N) {
+            double2 yp = (hf != 0.0) ? cmul(yb[i], cexp_lh(L[i], hf)) : yb[i];
+            sm.buf[pad(k)] = cscal(sc_t, yp);
+        }
+    }
+    for (int k = N + tid; k <= MO; k += LH_T) sm.buf[pad(k)] = make_double2(0.0, 0.0);
 }
 
 // Accepted-step window filter (stepfun, src/Luna.jl:366-370) on the coarse
 // grid: YN ← rfft(twin · irfft(ωwin · YN·exp(L h))).
-template <int MC>
-__device__ __forceinline__ void stepfun_filter(const DInst& in, Smem& sm, const double2* __restrict__ tw, double2* YN,
-                                               double h) {
-    const int N = in.n_omega;  // == MC + 1
-    for (int k = threadIdx.x; k < N; k += LH_T) {
-        double2 v = YN[k];
-        if (h != 0.0) v = cmul(v, cexp_lh(in.linop[k], h));
-        sm.buf[pad(k)] = cscal(in.omega_win[k], v);
+template <int MC, class Smem>
+__device__ __forceinline__ void stepfun_filter(const DInst& in, Smem& sm, double2* YN, double h) {
+    constexpr int N = MC + 1;
+    constexpr int NPT = (N + LH_T - 1) / LH_T;
+#pragma unroll
+    for (int i = 0; i < NPT; i++) {
+        int k = threadIdx.x + i * LH_T;
+        if (k < N) {
+            double2 v = YN[k];
+            if (h != 0.0) v = cmul(v, cexp_lh(in.linop[k], h));
+            sm.buf[pad(k)] = cscal(in.omega_win[k], v);
+        }
     }
     __syncthreads();
-    c2r_prepack<MC>(sm.buf, tw);
+    c2r_prepack<MC>(sm.buf, sm.tw);
     __syncthreads();
-    fft_lds<MC, true>(sm.buf, tw);
+    fft_lds<MC, true>(sm.buf, sm.tw);
+    const double inv_nt = in.inv_nt;
     for (int j = threadIdx.x; j < MC; j += LH_T) {
         double2 z = sm.buf[pad(j)];
-        double x0 = (in.inv_nt * z.x) * in.t_win[2 * j];
-        double x1 = (in.inv_nt * z.y) * in.t_win[2 * j + 1];
+        double x0 = (inv_nt * z.x) * in.t_win[2 * j];
+        double x1 = (inv_nt * z.y) * in.t_win[2 * j + 1];
         sm.buf[pad(j)] = make_double2(x0, x1);
     }
     __syncthreads();
-    fft_lds<MC, false>(sm.buf, tw);
-    for (int k = threadIdx.x; k < N; k += LH_T) YN[k] = r2c_bin<MC>(sm.buf, tw, k);
+    fft_lds<MC, false>(sm.buf, sm.tw);
+#pragma unroll
+    for (int i = 0; i < NPT; i++) {
+        int k = threadIdx.x + i * LH_T;
+        if (k < N) YN[k] = r2c_bin<MC>(sm.buf, sm.tw, k);
+    }
     __syncthreads();
 }
 
 // --------------------------------------------------------------------------
 // Persistent adaptive solve of one instance (Luna.run → RK45.solve_precon).
 // --------------------------------------------------------------------------
-template <int MO, int MC>
-__device__ void solve_instance(const DInst& in, Smem& sm, const double2* __restrict__ tw) {
-    const int N = in.n_omega;
+template <int MO, int MC, class Smem>
+__device__ __forceinline__ void solve_instance(const DInst& in, Smem& sm) {
+    constexpr int N = Bins<MO>::N, NPT = Bins<MO>::NPT;
     double2* K[7];
 #pragma unroll
     for (int j = 0; j < 7; j++) K[j] = in.K[j];
     double2* const Y0 = in.Y;
     double2* Y = in.Y;
     double2* YN = in.YN;
+    for (int k = threadIdx.x; k < N; k += LH_T) Y[k] = in.Yin[k];
     if (threadIdx.x == 0) sm.flag = LH_OK;
+    __syncthreads();
 
     long long steps = 0, accepted = 0, rejected = 0, rhs_evals = 0;
     int status = LH_OK, saved = 0, repeated = 0;
     double t = in.z0, tn = in.z0, dt = in.init_dz, dtn = in.init_dz, errlast = 0.0;
 
     // PreconStepper init: k1 = fbar(y0, t, t)
-    stage_prologue<MO>(in, sm, Y, K, 0, 0.0, 0.0, 0.0);
+    stage_prologue<MO, Smem>(in, sm, Y, K, 0, 0.0, 0.0, 0.0);
     {
         double2* k0 = K[0];
-        rhs_field<MO>(in, sm, tw, [&](int k, double2 nl) { k0[k] = nl; });
+        rhs_field<MO, Smem>(in, sm, [&](int k, double2 nl) { k0[k] = nl; });
     }
     rhs_evals++;
     __syncthreads();
@@ -314,13 +394,14 @@ __device__ void solve_instance(const DInst& in, Smem& sm, const double2* __restr
         double h0 = tn - t;
         dt = dtn;
         t = tn;
+#pragma unroll 1
         for (int st = 0; st < 6 && status == LH_OK; st++) {
             double t2 = t + c_nodes[st] * dt;
             double hf = t2 - t, hb = t - t2;
-            stage_prologue<MO>(in, sm, Y, K, st, dt, h0, hf);
+            stage_prologue<MO, Smem>(in, sm, Y, K, st, dt, h0, hf);
             double2* kout = K[st + 1];
             const double2* Lp = in.linop;
-            rhs_field<MO>(in, sm, tw, [&](int k, double2 nl) { kout[k] = cmul(nl, cexp_lh(Lp[k], hb)); });
+            rhs_field<MO, Smem>(in, sm, [&](int k, double2 nl) { kout[k] = cmul(nl, cexp_lh(Lp[k], hb)); });
             rhs_evals++;
             __syncthreads();
             if (sm.flag != LH_OK) status = sm.flag;
@@ -329,26 +410,40 @@ __device__ void solve_instance(const DInst& in, Smem& sm, const double2* __restr
         steps++;
         // yn = y + Δz Σ b5 k ; yerr = Σ (Δz k) e ; weaknorm (src/RK45.jl:170-181, 343-354)
         double sy = 0.0, syn = 0.0, se = 0.0;
-        double cb[7], ce[7];
+        {
+            double2 yv[NPT], yn[NPT], ye[NPT];
 #pragma unroll
-        for (int j = 0; j < 7; j++) {
-            cb[j] = dt * c_b5[j];
-            ce[j] = c_b5[j] - c_b4[j];
-        }
-        for (int k = threadIdx.x; k < N; k += LH_T) {
-            double2 yv = Y[k];
-            double2 yn = yv, ye = make_double2(0.0, 0.0);
-#pragma unroll
-            for (int j = 0; j < 7; j++) {
-                if (c_b5[j] == 0.0) continue;
-                double2 kv = K[j][k];
-                yn = cadd(yn, cscal(cb[j], kv));
-                ye = cadd(ye, cscal(ce[j], cscal(dt, kv)));
+            for (int i = 0; i < NPT; i++) {
+                int k = threadIdx.x + i * LH_T;
+                yv[i] = (k < N) ? Y[k] : make_double2(0.0, 0.0);
+                yn[i] = yv[i];
+                ye[i] = make_double2(0.0, 0.0);
             }
-            sy += yv.x * yv.x + yv.y * yv.y;
-            syn += yn.x * yn.x + yn.y * yn.y;
-            se += ye.x * ye.x + ye.y * ye.y;
-            YN[k] = yn;
+#pragma unroll 1
+            for (int j = 0; j < 7; j++) {
+                if (j == 1) continue;  // b5[2] = b4[2] = 0
+                const double cb = dt * c_b5[j], ce = c_b5[j] - c_b4[j];
+                const double2* Kj = K[j];
+#pragma unroll
+                for (int i = 0; i < NPT; i++) {
+                    int k = threadIdx.x + i * LH_T;
+                    if (k < N) {
+                        double2 kv = Kj[k];
+                        yn[i] = cadd(yn[i], cscal(cb, kv));
+                        ye[i] = cadd(ye[i], cscal(ce, cscal(dt, kv)));
+                    }
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < NPT; i++) {
+                int k = threadIdx.x + i * LH_T;
+                if (k < N) {
+                    sy += yv[i].x * yv[i].x + yv[i].y * yv[i].y;
+                    syn += yn[i].x * yn[i].x + yn[i].y * yn[i].y;
+                    se += ye[i].x * ye[i].x + ye[i].y * ye[i].y;
+                    YN[k] = yn[i];
+                }
+            }
         }
         block_sum3(sy, syn, se, sm.wsum);
         double errwt = fmax(fmax(sqrt(sy), sqrt(syn)), in.atol);
@@ -375,11 +470,11 @@ __device__ void solve_instance(const DInst& in, Smem& sm, const double2* __restr
         }
         if (ok) {
             tn = t + dt;
-            double2* tmp = K[0];
+            double2* tmp = K[0];  // FSAL: ks[1] .= ks[7] (src/RK45.jl:184-186)
             K[0] = K[6];
             K[6] = tmp;
-            stepfun_filter<MC>(in, sm, tw, YN, tn - t);
-            // GridCondition saves via dense output (quirk: ks[1] holds k7)
+            stepfun_filter<MC, Smem>(in, sm, YN, tn - t);
+            // GridCondition saves via dense output; ks[1] and ks[7] both hold k7
             while (saved < in.save_n && in.zgrid[saved] <= tn) {
                 double ti = in.zgrid[saved];
                 if (in.saves) {
@@ -426,7 +521,7 @@ __device__ void solve_instance(const DInst& in, Smem& sm, const double2* __restr
         }
         __syncthreads();
     }
-    // The final state s.yn lives in whichever buffer in.Y points to after the
+    // The final state s.yn lives in whichever buffer Y points to after the
     // swaps; copy it into the caller-visible buffer Y0.
     if (Y != Y0)
         for (int k = threadIdx.x; k < N; k += LH_T) Y0[k] = Y[k];
@@ -445,54 +540,63 @@ __device__ void solve_instance(const DInst& in, Smem& sm, const double2* __restr
     __syncthreads();
 }
 
-// Dispatch on the instance's grid sizes.
-__global__ __launch_bounds__(LH_T) void lh_solve_kernel(const DInst* insts, const double2* __restrict__ tw) {
-    __shared__ Smem sm;
-    const DInst& in = insts[blockIdx.x];
-    if (in.grid_kind == LH_GRID_REAL && in.transform == LH_TRANS_MODEAVG) {
-        if (in.n_to == 16384 && in.n_t == 8192)
-            solve_instance<8192, 4096>(in, sm, tw);
-        else if (in.n_to == 8192 && in.n_t == 4096)
-            solve_instance<4096, 2048>(in, sm, tw);
-        else if (in.n_to == 4096 && in.n_t == 2048)
-            solve_instance<2048, 1024>(in, sm, tw);
-        else
-            goto unsupported;
-        return;
-    }
-unsupported:
-    if (threadIdx.x == 0 && in.stats) {
-        lh_stats s;
-        memset(&s, 0, sizeof(s));
-        s.status = LH_E_ARG;
-        *in.stats = s;
-    }
+// One persistent kernel per grid size: one workgroup per instance.
+template <int MO, int MC>
+__global__ __launch_bounds__(LH_T) void lh_solve_kernel(const DInst* insts, const double2* __restrict__ twg) {
+    __shared__ SmemT<MO> sm;
+    tw_load(sm.tw, twg);
+    __syncthreads();
+    solve_instance<MO, MC, SmemT<MO>>(insts[blockIdx.x], sm);
 }
 
-__global__ __launch_bounds__(LH_T) void lh_rhs_kernel(const DInst* insts, const double2* __restrict__ tw,
-                                                      int* status) {
-    __shared__ Smem sm;
-    const DInst& in = insts[blockIdx.x];
-    if (threadIdx.x == 0) sm.flag = LH_OK;
-    const int N = in.n_omega;
+template <int MO, class Smem>
+__device__ __forceinline__ void rhs_once(const DInst& in, Smem& sm) {
+    constexpr int N = Bins<MO>::N;
     for (int k = threadIdx.x; k < N; k += LH_T) sm.buf[pad(k)] = cscal(in.sc_t, in.rhs_in[k]);
+    for (int k = N + threadIdx.x; k <= MO; k += LH_T) sm.buf[pad(k)] = make_double2(0, 0);
     double2* out = in.rhs_out;
-    if (in.grid_kind == LH_GRID_REAL && in.transform == LH_TRANS_MODEAVG) {
-        if (in.n_to == 16384) {
-            for (int k = N + threadIdx.x; k <= 8192; k += LH_T) sm.buf[pad(k)] = make_double2(0, 0);
-            rhs_field<8192>(in, sm, tw, [&](int k, double2 nl) { out[k] = nl; });
-        } else if (in.n_to == 8192) {
-            for (int k = N + threadIdx.x; k <= 4096; k += LH_T) sm.buf[pad(k)] = make_double2(0, 0);
-            rhs_field<4096>(in, sm, tw, [&](int k, double2 nl) { out[k] = nl; });
-        } else if (in.n_to == 4096) {
-            for (int k = N + threadIdx.x; k <= 2048; k += LH_T) sm.buf[pad(k)] = make_double2(0, 0);
-            rhs_field<2048>(in, sm, tw, [&](int k, double2 nl) { out[k] = nl; });
-        }
-    }
+    rhs_field<MO, Smem>(in, sm, [&](int k, double2 nl) { out[k] = nl; });
+}
+
+template <int MO>
+__global__ __launch_bounds__(LH_T) void lh_rhs_kernel(const DInst* insts, const double2* __restrict__ twg,
+                                                      int* status) {
+    __shared__ SmemT<MO> sm;
+    tw_load(sm.tw, twg);
+    if (threadIdx.x == 0) sm.flag = LH_OK;
+    __syncthreads();
+    rhs_once<MO, SmemT<MO>>(insts[blockIdx.x], sm);
     __syncthreads();
     if (threadIdx.x == 0) status[blockIdx.x] = sm.flag;
 }
 
+// Size classes with a compiled kernel: (Nto/2, Nt/2).
+enum { LH_CLASS_16384 = 0, LH_CLASS_8192 = 1, LH_CLASS_4096 = 2, LH_NCLASS = 3 };
+
+static int size_class(int n_to) {
+    switch (n_to) {
+        case 16384: return LH_CLASS_16384;
+        case 8192: return LH_CLASS_8192;
+        case 4096: return LH_CLASS_4096;
+        default: return -1;
+    }
+}
+
+static void launch_solve(int cls, int n, const DInst* dinst, const double2* tw, hipStream_t s) {
+    switch (cls) {
+        case LH_CLASS_16384: hipLaunchKernelGGL((lh_solve_kernel<8192, 4096>), dim3(n), dim3(LH_T), 0, s, dinst, tw); break;
+        case LH_CLASS_8192: hipLaunchKernelGGL((lh_solve_kernel<4096, 2048>), dim3(n), dim3(LH_T), 0, s, dinst, tw); break;
+        default: hipLaunchKernelGGL((lh_solve_kernel<2048, 1024>), dim3(n), dim3(LH_T), 0, s, dinst, tw); break;
+    }
+}
+
+static void launch_rhs(int cls, const DInst* dinst, const double2* tw, int* status, hipStream_t s) {
+    switch (cls) {
+        case LH_CLASS_16384: hipLaunchKernelGGL((lh_rhs_kernel<8192>), dim3(1), dim3(LH_T), 0, s, dinst, tw, status); break;
+        case LH_CLASS_8192: hipLaunchKernelGGL((lh_rhs_kernel<4096>), dim3(1), dim3(LH_T), 0, s, dinst, tw, status); break;
+        default: hipLaunchKernelGGL((lh_rhs_kernel<2048>), dim3(1), dim3(LH_T), 0, s, dinst, tw, status); break;
+    }
+}
 
 // ==========================================================================
 // Host side: contexts, constant upload, launches (C ABI, include/luna_hip.h)
@@ -508,13 +612,16 @@ struct lh_ctx {
     int n_state;
     double ppt_emin, ppt_emax;
     double* omega_win;
-    uint8_t* sidx;
     double* t_win;
     double* to_win;
     double2* linop;
-    double2* norm;
+    double2* outfac;
     double4* ppt;
     // solver buffers (allocated on first use)
+    double2* Yin;
+    int have_state;
+    int have_result;
+    int result_save_n;
     double2* Y;
     double2* YN;
     double2* K[7];
@@ -528,10 +635,15 @@ struct lh_ctx {
     double2* rhs_out;
     int* dstatus;
     int last_status;
+    int rhs_pending;
+    hipStream_t rhs_stream;
 };
 
 static std::mutex g_mu;
 static double2* g_tw[64] = {nullptr};
+// HIP-event durations of the last resident batch, per size class
+static float g_last_ms[3] = {0, 0, 0};
+static int g_last_n[3] = {0, 0, 0};
 
 #define HIPCK(x)                                                                     \
     do {                                                                             \
@@ -603,6 +715,8 @@ static int upload(T** dst, const void* src, size_t count) {
     return LH_OK;
 }
 
+static int size_class(int n_to);
+
 static bool supported(const lh_desc* d) {
     if (d->grid_kind != LH_GRID_REAL || d->transform != LH_TRANS_MODEAVG || d->n_r != 1) return false;
     if (!(d->n_to == 16384 || d->n_to == 8192 || d->n_to == 4096)) return false;
@@ -610,6 +724,16 @@ static bool supported(const lh_desc* d) {
     if (d->n_omega != d->n_t / 2 + 1) return false;
     if (d->raman || d->kerr == LH_KERR_ENV) return false;
     return true;
+}
+
+// to_time!/to_freq! length-ratio scales, src/NonlinearRHS.jl:16-53.
+static double scale_to_time(const lh_desc& d) {
+    if (d.grid_kind == LH_GRID_REAL) return (double)(d.n_to / 2) / (double)(d.n_omega - 1);
+    return (double)d.n_to / (double)d.n_omega;
+}
+static double scale_to_freq(const lh_desc& d) {
+    if (d.grid_kind == LH_GRID_REAL) return (double)(d.n_omega - 1) / (double)(d.n_to / 2);
+    return (double)d.n_omega / (double)d.n_to;
 }
 
 static DInst base_inst(const lh_ctx* c) {
@@ -626,31 +750,22 @@ static DInst base_inst(const lh_ctx* c) {
     in.plasma = d.plasma;
     in.raman = d.raman;
     in.ppt_n = d.ppt_n;
-    in.field_div = d.field_div;
     in.density = d.density;
     in.kerr_fac = d.density * EPS0 * d.gamma3;  // (ρ ε0) γ3, Nonlinear.jl:30
     in.ionpot = d.ionpot;
     in.preion_p1 = d.preionfrac + 1;
     in.e_ratio = ECHARGE * ECHARGE / MELEC;
     in.dt_o = d.dt_o;
-    in.inv_nto = 1.0 / (double)d.n_to;
+    in.e_scale = (1.0 / (double)d.n_to) / d.field_div;  // E = c2r/Nto/(nlscale·√Aeff)
     in.inv_nt = 1.0 / (double)d.n_t;
-    if (d.grid_kind == LH_GRID_REAL) {
-        int No = d.n_to / 2 + 1;
-        in.sc_t = (double)(No - 1) / (double)(d.n_omega - 1);
-        in.sc_f = (double)(d.n_omega - 1) / (double)(No - 1);
-    } else {
-        in.sc_t = (double)d.n_to / (double)d.n_omega;
-        in.sc_f = (double)d.n_omega / (double)d.n_to;
-    }
+    in.sc_t = scale_to_time(d);
     in.ppt_emin = c->ppt_emin;
     in.ppt_emax = c->ppt_emax;
     in.omega_win = c->omega_win;
-    in.sidx = c->sidx;
     in.t_win = c->t_win;
     in.to_win = c->to_win;
     in.linop = c->linop;
-    in.norm = c->norm;
+    in.outfac = c->outfac;
     in.ppt = c->ppt;
     return in;
 }
@@ -659,6 +774,7 @@ static DInst base_inst(const lh_ctx* c) {
 static int ensure_solver_buffers(lh_ctx* c, int save_n, bool want_saves) {
     const size_t NS = (size_t)c->n_state;
     if (!c->Y) {
+        HIPCK(hipMalloc(&c->Yin, sizeof(double2) * NS));
         HIPCK(hipMalloc(&c->Y, sizeof(double2) * NS));
         HIPCK(hipMalloc(&c->YN, sizeof(double2) * NS));
         for (int j = 0; j < 7; j++) HIPCK(hipMalloc(&c->K[j], sizeof(double2) * NS));
@@ -688,16 +804,18 @@ static void save_grid(const lh_rk_opts* o, std::vector<double>& g) {
         g[i] = (n == 1) ? o->z0 : (double)((a * (long double)(n - 1 - i) + b * (long double)i) / (long double)(n - 1));
 }
 
-static int prepare_instance(lh_ctx* c, const double* Ew, const lh_rk_opts* o, bool want_saves, DInst* in,
-                            lh_stats* dstats_slot) {
+static int prepare_instance(lh_ctx* c, const lh_rk_opts* o, bool want_saves, DInst* in, lh_stats* dstats_slot,
+                            hipStream_t s) {
     if (o->save_n < 0 || o->repeat_limit < 0 || !(o->rtol > 0) || !(o->zmax >= o->z0)) return LH_E_ARG;
+    if (!c->have_state) return LH_E_ARG;
     int st = ensure_solver_buffers(c, o->save_n, want_saves);
     if (st) return st;
     std::vector<double> g;
     save_grid(o, g);
-    HIPCK(hipMemcpyAsync(c->zgrid, g.data(), sizeof(double) * g.size(), hipMemcpyHostToDevice, c->stream));
-    HIPCK(hipMemcpyAsync(c->Y, Ew, sizeof(double2) * (size_t)c->n_state, hipMemcpyHostToDevice, c->stream));
+    HIPCK(hipMemcpyAsync(c->zgrid, g.data(), sizeof(double) * g.size(), hipMemcpyHostToDevice, s));
+    HIPCK(hipStreamSynchronize(s));  // g is a local buffer
     *in = base_inst(c);
+    in->Yin = c->Yin;
     in->Y = c->Y;
     in->YN = c->YN;
     for (int j = 0; j < 7; j++) in->K[j] = c->K[j];
@@ -714,17 +832,8 @@ static int prepare_instance(lh_ctx* c, const double* Ew, const lh_rk_opts* o, bo
     in->init_dz = o->init_dz;
     in->repeat_limit = o->repeat_limit;
     in->save_n = o->save_n;
-    return LH_OK;
-}
-
-static int fetch_instance(lh_ctx* c, double* Ew, const lh_rk_opts* o, double* saves, double* zsaves,
-                          hipStream_t s) {
-    HIPCK(hipMemcpyAsync(Ew, c->Y, sizeof(double2) * (size_t)c->n_state, hipMemcpyDeviceToHost, s));
-    if (saves && o->save_n > 0)
-        HIPCK(hipMemcpyAsync(saves, c->saves, sizeof(double2) * (size_t)c->n_state * o->save_n,
-                             hipMemcpyDeviceToHost, s));
-    if (zsaves && o->save_n > 0)
-        HIPCK(hipMemcpyAsync(zsaves, c->zgrid, sizeof(double) * o->save_n, hipMemcpyDeviceToHost, s));
+    c->have_result = 0;
+    c->result_save_n = want_saves ? o->save_n : 0;
     return LH_OK;
 }
 
@@ -756,7 +865,7 @@ int lh_destroy(lh_ctx* c) {
     if (!c) return LH_OK;
     hipSetDevice(c->device);
     if (c->stream) hipStreamSynchronize(c->stream);
-    void* ptrs[] = {c->omega_win, c->sidx, c->t_win, c->to_win, c->linop, c->norm, c->ppt, c->Y, c->YN,
+    void* ptrs[] = {c->omega_win, c->t_win, c->to_win, c->linop, c->outfac, c->ppt, c->Yin, c->Y, c->YN,
                     c->saves, c->zgrid, c->dstats, c->dinst, c->rhs_in, c->rhs_out, c->dstatus};
     for (void* p : ptrs)
         if (p) hipFree(p);
@@ -784,10 +893,26 @@ int lh_create(lh_ctx** out, const lh_desc* d) {
     if (!st && hipStreamCreate(&c->stream) != hipSuccess) st = LH_E_HIP;
     const size_t N = (size_t)d->n_omega, NS = (size_t)c->n_state;
     if (!st) st = upload(&c->omega_win, d->omega_win, N);
-    if (!st) st = upload(&c->sidx, d->sidx, N);
     if (!st) st = upload(&c->t_win, d->t_win, (size_t)d->n_t);
     if (!st) st = upload(&c->to_win, d->to_win, (size_t)d->n_to);
-    if (!st) st = upload(&c->norm, d->norm, NS);
+    if (!st) {
+        // Output factor per bin: to_freq! crop scale, then norm! and ×ωwin on
+        // sidx bins (mode-average, src/NonlinearRHS.jl:368-387), or the radial
+        // factor on every bin (:473).
+        const double sf = scale_to_freq(*d);
+        const double2* nf = (const double2*)d->norm;
+        std::vector<double2> of(NS);
+        for (size_t i = 0; i < NS; i++) {
+            size_t k = i % N;
+            if (d->transform == LH_TRANS_RADIAL)
+                of[i] = make_double2(sf * nf[i].x, sf * nf[i].y);
+            else if (d->sidx[k])
+                of[i] = make_double2((sf * nf[i].x) * d->omega_win[k], (sf * nf[i].y) * d->omega_win[k]);
+            else
+                of[i] = make_double2(sf, 0.0);
+        }
+        st = upload(&c->outfac, of.data(), NS);
+    }
     if (!st) {
         std::vector<double2> lin(NS, make_double2(0.0, 0.0));
         if (d->linop) memcpy(lin.data(), d->linop, sizeof(double2) * NS);
@@ -819,7 +944,18 @@ int lh_create(lh_ctx** out, const lh_desc* d) {
     return LH_OK;
 }
 
-int lh_ctx_status(lh_ctx* c) { return c ? c->last_status : LH_E_ARG; }
+int lh_ctx_status(lh_ctx* c) {
+    if (!c) return LH_E_ARG;
+    if (c->rhs_pending) {  // status of the last lh_rhs_device launch
+        int kst = LH_OK;
+        HIPCK(hipSetDevice(c->device));
+        HIPCK(hipStreamSynchronize(c->rhs_stream ? c->rhs_stream : c->stream));
+        HIPCK(hipMemcpy(&kst, c->dstatus, sizeof(int), hipMemcpyDeviceToHost));
+        c->last_status = kst;
+        c->rhs_pending = 0;
+    }
+    return c->last_status;
+}
 
 int lh_rhs_device(lh_ctx* c, void* nl_dev, const void* Ew_dev, double z, void* stream) {
     (void)z;  // constant density / Aeff / linear operator: no z dependence
@@ -833,8 +969,10 @@ int lh_rhs_device(lh_ctx* c, void* nl_dev, const void* Ew_dev, double z, void* s
     in.rhs_in = (const double2*)Ew_dev;
     in.rhs_out = (double2*)nl_dev;
     HIPCK(hipMemcpyAsync(c->dinst, &in, sizeof(DInst), hipMemcpyHostToDevice, s));
-    hipLaunchKernelGGL(lh_rhs_kernel, dim3(1), dim3(LH_T), 0, s, c->dinst, tw, c->dstatus);
+    launch_rhs(size_class(c->d.n_to), c->dinst, tw, c->dstatus, s);
     HIPCK(hipGetLastError());
+    c->rhs_pending = 1;
+    c->rhs_stream = s;
     return LH_OK;
 }
 
@@ -854,6 +992,7 @@ int lh_rhs(lh_ctx* c, double* nl, const double* Ew, double z) {
     HIPCK(hipMemcpyAsync(&kst, c->dstatus, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIPCK(hipStreamSynchronize(c->stream));
     c->last_status = kst;
+    c->rhs_pending = 0;
     return kst;
 }
 
@@ -869,48 +1008,145 @@ int lh_solve(lh_ctx* c, double* Ew, const lh_rk_opts* o, double* saves, double* 
     return sp->status;
 }
 
-int lh_batch_solve(lh_ctx* const* ctxs, int n, double* const* Ew, const lh_rk_opts* o, double* const* saves,
-                   double* const* zsaves, lh_stats* stats) {
-    if (!ctxs || n <= 0 || !Ew || !o || !stats) return LH_E_ARG;
+int lh_set_state(lh_ctx* c, const double* Ew) {
+    if (!c || !Ew) return LH_E_ARG;
+    HIPCK(hipSetDevice(c->device));
+    int st = ensure_solver_buffers(c, 0, false);
+    if (st) return st;
+    HIPCK(hipMemcpy(c->Yin, Ew, sizeof(double2) * (size_t)c->n_state, hipMemcpyHostToDevice));
+    c->have_state = 1;
+    return LH_OK;
+}
+
+int lh_get_state(lh_ctx* c, double* Ew) {
+    if (!c || !Ew || !c->have_result) return LH_E_ARG;
+    HIPCK(hipSetDevice(c->device));
+    HIPCK(hipMemcpy(Ew, c->Y, sizeof(double2) * (size_t)c->n_state, hipMemcpyDeviceToHost));
+    return LH_OK;
+}
+
+int lh_get_saves(lh_ctx* c, double* saves, double* zsaves) {
+    if (!c || !c->have_result) return LH_E_ARG;
+    HIPCK(hipSetDevice(c->device));
+    int n = c->result_save_n;
+    if (saves && n > 0)
+        HIPCK(hipMemcpy(saves, c->saves, sizeof(double2) * (size_t)c->n_state * n, hipMemcpyDeviceToHost));
+    if (zsaves && n > 0) HIPCK(hipMemcpy(zsaves, c->zgrid, sizeof(double) * n, hipMemcpyDeviceToHost));
+    return LH_OK;
+}
+
+int lh_batch_solve_resident(lh_ctx* const* ctxs, int n, const lh_rk_opts* o, int want_saves, lh_stats* stats) {
+    if (!ctxs || n <= 0 || !o || !stats) return LH_E_ARG;
     const int dev = ctxs[0]->device;
     for (int i = 0; i < n; i++)
-        if (!ctxs[i] || ctxs[i]->device != dev || !Ew[i]) return LH_E_ARG;
+        if (!ctxs[i] || ctxs[i]->device != dev || size_class(ctxs[i]->d.n_to) < 0) return LH_E_ARG;
     for (int i = 0; i < n; i++)  // the same context twice would share buffers
         for (int j = i + 1; j < n; j++)
             if (ctxs[i] == ctxs[j]) return LH_E_ARG;
     HIPCK(hipSetDevice(dev));
-    hipStream_t s = ctxs[0]->stream;
+    hipStream_t s0 = ctxs[0]->stream;
     const double2* tw;
     int st = twiddles(dev, &tw);
     if (st) return st;
+    // group instances by size class; each class is one persistent launch on
+    // its own stream so the classes run concurrently
+    int count[LH_NCLASS] = {0}, first[LH_NCLASS] = {0};
+    for (int i = 0; i < n; i++) count[size_class(ctxs[i]->d.n_to)]++;
+    for (int c = 1; c < LH_NCLASS; c++) first[c] = first[c - 1] + count[c - 1];
+    std::vector<int> pos(n);
+    {
+        int fill[LH_NCLASS] = {0};
+        for (int i = 0; i < n; i++) {
+            int c = size_class(ctxs[i]->d.n_to);
+            pos[i] = first[c] + fill[c]++;
+        }
+    }
     DInst* dinst = nullptr;
     lh_stats* dstats = nullptr;
     HIPCK(hipMalloc(&dinst, sizeof(DInst) * (size_t)n));
     HIPCK(hipMalloc(&dstats, sizeof(lh_stats) * (size_t)n));
     std::vector<DInst> h(n);
+    for (int i = 0; i < n && !st; i++) st = prepare_instance(ctxs[i], o, want_saves != 0, &h[pos[i]], dstats + pos[i], s0);
+    if (!st && hipMemcpyAsync(dinst, h.data(), sizeof(DInst) * (size_t)n, hipMemcpyHostToDevice, s0) != hipSuccess)
+        st = LH_E_HIP;
+    hipEvent_t ready = nullptr, done[LH_NCLASS] = {nullptr}, t0[LH_NCLASS] = {nullptr}, t1[LH_NCLASS] = {nullptr};
+    hipStream_t cs[LH_NCLASS] = {nullptr};
+    if (!st && hipEventCreateWithFlags(&ready, hipEventDisableTiming) != hipSuccess) st = LH_E_HIP;
+    if (!st && hipEventRecord(ready, s0) != hipSuccess) st = LH_E_HIP;
+    for (int c = 0; c < LH_NCLASS && !st; c++) {
+        if (!count[c]) continue;
+        if (hipStreamCreate(&cs[c]) != hipSuccess || hipEventCreateWithFlags(&done[c], hipEventDisableTiming) != hipSuccess ||
+            hipStreamWaitEvent(cs[c], ready, 0) != hipSuccess) {
+            st = LH_E_HIP;
+            break;
+        }
+        if (hipEventCreate(&t0[c]) != hipSuccess || hipEventCreate(&t1[c]) != hipSuccess ||
+            hipEventRecord(t0[c], cs[c]) != hipSuccess) {
+            st = LH_E_HIP;
+            break;
+        }
+        launch_solve(c, count[c], dinst + first[c], tw, cs[c]);
+        if (hipGetLastError() != hipSuccess || hipEventRecord(t1[c], cs[c]) != hipSuccess ||
+            hipEventRecord(done[c], cs[c]) != hipSuccess || hipStreamWaitEvent(s0, done[c], 0) != hipSuccess)
+            st = LH_E_HIP;
+    }
+    std::vector<lh_stats> hs(n);
+    if (!st && hipMemcpyAsync(hs.data(), dstats, sizeof(lh_stats) * (size_t)n, hipMemcpyDeviceToHost, s0) != hipSuccess)
+        st = LH_E_HIP;
+    if (hipStreamSynchronize(s0) != hipSuccess && !st) st = LH_E_HIP;
+    for (int c = 0; c < LH_NCLASS; c++) {
+        float ms = 0.0f;
+        if (!st && t0[c] && t1[c] && hipEventElapsedTime(&ms, t0[c], t1[c]) == hipSuccess) {
+            g_last_ms[c] = ms;
+            g_last_n[c] = count[c];
+        } else if (!count[c]) {
+            g_last_ms[c] = 0.0f;
+            g_last_n[c] = 0;
+        }
+        if (t0[c]) (void)hipEventDestroy(t0[c]);
+        if (t1[c]) (void)hipEventDestroy(t1[c]);
+        if (cs[c]) (void)hipStreamDestroy(cs[c]);
+        if (done[c]) (void)hipEventDestroy(done[c]);
+    }
+    if (ready) (void)hipEventDestroy(ready);
+    (void)hipFree(dinst);
+    (void)hipFree(dstats);
+    if (st) return st;
+    for (int i = 0; i < n; i++) {
+        stats[i] = hs[pos[i]];
+        ctxs[i]->last_status = stats[i].status;
+        ctxs[i]->have_result = 1;
+    }
+    return LH_OK;
+}
+
+int lh_last_launch_times(double* ms, int* n_inst, int nclass) {
+    if (!ms || !n_inst || nclass < LH_NCLASS) return LH_E_ARG;
+    for (int c = 0; c < LH_NCLASS; c++) {
+        ms[c] = g_last_ms[c];
+        n_inst[c] = g_last_n[c];
+    }
+    return LH_OK;
+}
+
+int lh_batch_solve(lh_ctx* const* ctxs, int n, double* const* Ew, const lh_rk_opts* o, double* const* saves,
+                   double* const* zsaves, lh_stats* stats) {
+    if (!ctxs || n <= 0 || !Ew || !o || !stats) return LH_E_ARG;
+    for (int i = 0; i < n; i++) {
+        if (!ctxs[i] || !Ew[i]) return LH_E_ARG;
+        int st = lh_set_state(ctxs[i], Ew[i]);
+        if (st) return st;
+    }
+    int want = 0;
+    for (int i = 0; i < n; i++)
+        if (saves && saves[i]) want = 1;
+    int st = lh_batch_solve_resident(ctxs, n, o, want, stats);
+    if (st) return st;
     for (int i = 0; i < n && !st; i++) {
-        lh_ctx* c = ctxs[i];
-        bool ws = saves && saves[i];
-        // uploads of per-instance data are ordered on the launch stream
-        hipStream_t keep = c->stream;
-        c->stream = s;
-        st = prepare_instance(c, Ew[i], o, ws, &h[i], dstats + i);
-        c->stream = keep;
+        st = lh_get_state(ctxs[i], Ew[i]);
+        if (!st && ((saves && saves[i]) || (zsaves && zsaves[i])))
+            st = lh_get_saves(ctxs[i], saves ? saves[i] : nullptr, zsaves ? zsaves[i] : nullptr);
     }
-    if (!st && hipMemcpyAsync(dinst, h.data(), sizeof(DInst) * (size_t)n, hipMemcpyHostToDevice, s) != hipSuccess)
-        st = LH_E_HIP;
-    if (!st) {
-        hipLaunchKernelGGL(lh_solve_kernel, dim3(n), dim3(LH_T), 0, s, dinst, tw);
-        if (hipGetLastError() != hipSuccess) st = LH_E_HIP;
-    }
-    if (!st && hipMemcpyAsync(stats, dstats, sizeof(lh_stats) * (size_t)n, hipMemcpyDeviceToHost, s) != hipSuccess)
-        st = LH_E_HIP;
-    for (int i = 0; i < n && !st; i++)
-        st = fetch_instance(ctxs[i], Ew[i], o, saves ? saves[i] : nullptr, zsaves ? zsaves[i] : nullptr, s);
-    if (hipStreamSynchronize(s) != hipSuccess && !st) st = LH_E_HIP;
-    hipFree(dinst);
-    hipFree(dstats);
-    for (int i = 0; i < n; i++) ctxs[i]->last_status = stats[i].status;
     return st;
 }
 

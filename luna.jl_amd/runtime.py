@@ -70,3 +70,49 @@ def batch_solve(contexts, Ews, opts, want_saves=True):
     st = _capi.lib().lh_batch_solve(handles, n, ew_ptrs, C.byref(opts), sv_ptrs, zs_ptrs, stats)
     _capi.check(st, "lh_batch_solve")
     return ew_arrs, sv, zs, list(stats)
+
+
+class ResidentBatch:
+    """A sweep whose initial fields stay in HBM: upload once, solve many times.
+
+    ``solve`` runs every instance from its resident initial field; final states
+    and saves remain on the device until ``state(i)`` / ``saves(i)`` copy them.
+    """
+
+    def __init__(self, contexts, Ews):
+        self.contexts = list(contexts)
+        L = _capi.lib()
+        for c, e in zip(self.contexts, Ews):
+            e = np.ascontiguousarray(np.asarray(e, dtype=np.complex128).reshape(-1, order="F"))
+            _capi.check(L.lh_set_state(c.handle, _p(e)), "lh_set_state")
+        n = len(self.contexts)
+        self._handles = (C.c_void_p * n)(*[c.handle for c in self.contexts])
+
+    def solve(self, opts, want_saves=True):
+        n = len(self.contexts)
+        stats = (_capi.LhStats * n)()
+        _capi.check(_capi.lib().lh_batch_solve_resident(self._handles, n, C.byref(opts), int(want_saves), stats),
+                    "lh_batch_solve_resident")
+        self.save_n = opts.save_n
+        return list(stats)
+
+    def state(self, i):
+        c = self.contexts[i]
+        out = np.empty(c.n_state, dtype=np.complex128)
+        _capi.check(_capi.lib().lh_get_state(c.handle, _p(out)), "lh_get_state")
+        return out
+
+    def saves(self, i):
+        c = self.contexts[i]
+        sv = np.empty((self.save_n, c.n_state), dtype=np.complex128)
+        zs = np.empty(self.save_n)
+        _capi.check(_capi.lib().lh_get_saves(c.handle, _p(sv), _p(zs)), "lh_get_saves")
+        return sv, zs
+
+
+def last_launch_times():
+    """{n_to: (ms, instances)} of the persistent launches of the last batch."""
+    ms = (C.c_double * 3)()
+    n = (C.c_int * 3)()
+    _capi.check(_capi.lib().lh_last_launch_times(ms, n, 3), "lh_last_launch_times")
+    return {_capi.CLASS_NTO[c]: (ms[c], n[c]) for c in range(3) if n[c] > 0}

@@ -102,3 +102,21 @@ def test_batch_equals_single(pkg, capi):
         assert stats[i].steps == st1.steps
         assert np.array_equal(outs[i], Ew1)
         assert np.array_equal(svs[i], sv1)
+
+
+def test_resident_batch_matches_host_batch(pkg, capi):
+    print("case: resident batch", flush=True)
+    from luna_jl_amd import configs, runtime
+    insts = [configs.capillary_instance(energy=e, pressure=p, trange=t, flength=0.05)
+             for e, p, t in [(120e-6, 1.2, 0.5e-12), (90e-6, 2.0, 1e-12)]]
+    ctxs = [runtime.Context(c.holder) for c in insts]
+    opts = capi.rk_opts(0.05, save_n=5)
+    outs, svs, zss, st_host = runtime.batch_solve(ctxs, [c.Ew0 for c in insts], opts)
+    rb = runtime.ResidentBatch(ctxs, [c.Ew0 for c in insts])
+    for rep in range(2):  # inputs stay resident: a rerun gives the same answer
+        st = rb.solve(opts)
+        for i in range(2):
+            assert st[i].status == 0 and st[i].steps == st_host[i].steps
+            assert np.array_equal(rb.state(i), outs[i])
+            sv, zs = rb.saves(i)
+            assert np.array_equal(sv, svs[i])
