@@ -54,7 +54,11 @@ def GaussField(lam0, tau_fwhm, energy=None, power=None, m=1):
 def SechField(lam0, tau_fwhm=None, tau_w=None, energy=None, power=None):
     if tau_w is None:
         tau_w = tau_fwhm / (2 * math.log(1 + math.sqrt(2)))
-    return PulseField(lam0, lambda t: 1 / np.cosh(t / tau_w) ** 2, energy=energy, power=power)
+    def sech2(t):
+        e = np.exp(-2 * np.abs(t / tau_w))
+        return 4 * e / (1 + e) ** 2  # sech² without overflow in cosh
+
+    return PulseField(lam0, sech2, energy=energy, power=power)
 
 
 class ShotNoise:

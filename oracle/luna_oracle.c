@@ -198,6 +198,8 @@ typedef struct {
     int n_to, n_r, n_omega;
     double *Et, *Pt, *rate, *frac, *phase, *J, *P; /* real buffers [n_to*n_r] */
     cplx *Etc, *Ptc, *Eo, *work, *E2, *hw, *tmp;    /* complex buffers */
+    int hw_ready; /* hω depends only on h(t): the reference rebuilds it every call
+                     (Nonlinear.jl:311-312); computing it once gives identical values */
 } rhs_ws;
 
 static void ws_init(rhs_ws *w, const lh_desc *d) {
@@ -219,6 +221,7 @@ static void ws_init(rhs_ws *w, const lh_desc *d) {
     w->E2 = (cplx *)calloc(2 * nto, sizeof(cplx));
     w->hw = (cplx *)calloc(2 * nto, sizeof(cplx));
     w->tmp = (cplx *)calloc(2 * nto, sizeof(cplx));
+    w->hw_ready = 0;
 }
 
 static void ws_free(rhs_ws *w) {
@@ -262,8 +265,11 @@ static void raman_env(const lh_desc *d, rhs_ws *w, const cplx *E, cplx *Pt) {
     double rho = d->density;
     for (int i = 0; i < n2; i++) w->E2[i] = 0.0;
     for (int i = 0; i < n; i++) w->E2[i] = 0.5 * (creal(E[i]) * creal(E[i]) + cimag(E[i]) * cimag(E[i]));
-    for (int i = 0; i < n2; i++) w->hw[i] = (i < n) ? d->raman_h[i] : 0.0;
-    fft_c2c(n2, w->hw, -1);
+    if (!w->hw_ready) {
+        for (int i = 0; i < n2; i++) w->hw[i] = (i < n) ? d->raman_h[i] : 0.0;
+        fft_c2c(n2, w->hw, -1);
+        w->hw_ready = 1;
+    }
     fft_c2c(n2, w->E2, -1);
     for (int i = 0; i < n2; i++) w->tmp[i] = cmul(w->hw[i], w->E2[i]) * d->dt_o;
     fft_c2c(n2, w->tmp, +1);
