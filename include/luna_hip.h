@@ -158,8 +158,8 @@ int lh_get_saves(lh_ctx *ctx, double *saves, double *zsaves);
 
 /* Measurement hook (no counterpart in the reference): HIP-event durations of
  * the persistent launches of the last batch, one per grid-size class
- * (Nto = 16384, 8192, 4096), with the number of instances in each.
- * nclass must be >= 3. */
+ * (field grid Nto = 16384, 8192, 4096; envelope grid Nto = 8192, 4096), with
+ * the number of instances in each.  nclass must be >= 5. */
 int lh_last_launch_times(double *ms, int *n_inst, int nclass);
 
 #ifdef __cplusplus

@@ -51,6 +51,11 @@ struct DInst {
     const double2* linop;
     const double2* outfac;  // per-bin output factor: sc_f·norm·ωwin (sidx) or sc_f
     const double4* ppt;     // (x, y, D, 0) per spline node
+    // envelope grid
+    double kerr34;          // ¾·ρ·ε0·γ3 (Kerr_env)
+    double inv_2n;          // 1/(2·Nto): inverse of the zero-padded Raman transform
+    const double2* hwdt;    // rfft of h(t) zero-padded to 2·Nto, times δt: bins 0..Nto
+    double2* escr;          // HBM stash of E(t) during the Raman convolution
     // solver state
     const double2* Yin;  // initial field (never written)
     double2* Y;          // receives the final state s.yn
@@ -355,12 +360,271 @@ __device__ __forceinline__ void stepfun_filter(const DInst& in, Smem& sm, double
     __syncthreads();
 }
 
+// ==========================================================================
+// Envelope grid (EnvGrid, GNLSE): complex fields, Kerr_env and RamanPolarEnv
+// ==========================================================================
+template <int NO, int NN>
+struct EnvBins {
+    static constexpr int N = NN;
+    static constexpr int NPT = (NN + LH_T - 1) / LH_T;
+};
+
+// Index of state bin k in the oversampled FFT array (copy_scale_both!,
+// src/NonlinearRHS.jl:89-95): first N/2 bins at the start, last N/2 at the end.
+template <int NO, int NN>
+__device__ __forceinline__ int env_slot(int k) {
+    return (k < NN / 2) ? k : k + (NO - NN);
+}
+
+template <int NO, int NN, class Smem>
+__device__ __forceinline__ void stage_prologue_env(const DInst& in, Smem& sm, const double2* Y, double2* const* K,
+                                                   int stage, double dt, double h0, double hf) {
+    constexpr int N = NN, NPT = EnvBins<NO, NN>::NPT;
+    const int tid = threadIdx.x;
+    double2 yb[NPT], L[NPT];
+#pragma unroll
+    for (int i = 0; i < NPT; i++) {
+        int k = tid + i * LH_T;
+        bool v = k < N;
+        yb[i] = v ? Y[k] : make_double2(0.0, 0.0);
+        L[i] = v ? in.linop[k] : make_double2(0.0, 0.0);
+    }
+    if (stage == 0 && h0 != 0.0) {
+        double2* K0 = K[0];
+#pragma unroll
+        for (int i = 0; i < NPT; i++) {
+            int k = tid + i * LH_T;
+            if (k < N) K0[k] = cmul(K0[k], cexp_lh(L[i], h0));
+        }
+    }
+#pragma unroll 1
+    for (int j = 0; j <= stage; j++) {
+        const double cj = dt * c_B[stage][j];
+        if (cj == 0.0) continue;
+        const double2* Kj = K[j];
+#pragma unroll
+        for (int i = 0; i < NPT; i++) {
+            int k = tid + i * LH_T;
+            if (k < N) yb[i] = cadd(yb[i], cscal(cj, Kj[k]));
+        }
+    }
+    const double sc_t = in.sc_t;
+#pragma unroll
+    for (int i = 0; i < NPT; i++) {
+        int k = tid + i * LH_T;
+        if (k < N) {
+            double2 yp = (hf != 0.0) ? cmul(yb[i], cexp_lh(L[i], hf)) : yb[i];
+            sm.buf[pad(env_slot<NO, NN>(k))] = cscal(sc_t, yp);
+        }
+    }
+    if constexpr (NO > NN)
+        for (int k = NN / 2 + tid; k < NO - NN / 2; k += LH_T) sm.buf[pad(k)] = make_double2(0.0, 0.0);
+}
+
+// TransModeAvg on an envelope grid with Kerr_env and RamanPolarEnv
+// (src/NonlinearRHS.jl:25-32, 47-53, 362-373, 389-399; src/Nonlinear.jl:47-49,
+// 63-73, 282-333).  The Raman convolution of ½|E|² with h(t) over the
+// zero-padded 2·Nto grid is done as a real transform: r2c of 2·Nto reals (a
+// length-Nto complex FFT), ×(hω δt), c2r — three LDS FFTs, with E(t) kept in
+// HBM meanwhile.  Calls epi(k, nl_k) for every state bin k < Nω.
+template <int NO, int NN, class Smem, class Epi>
+__device__ __forceinline__ void rhs_env(const DInst& in, Smem& sm, Epi epi) {
+    constexpr int PER = NO / LH_T;  // complex samples per thread (contiguous chunk)
+    const int tid = threadIdx.x;
+    const int i0 = PER * tid;
+    __syncthreads();
+    fft_lds<NO, true>(sm.buf, sm.tw);
+    const double es = in.e_scale;
+    if (in.raman) {
+        double2 E[PER];
+#pragma unroll
+        for (int q = 0; q < PER; q++) {
+            double2 z = sm.buf[pad(i0 + q)];
+            E[q] = make_double2(z.x * es, z.y * es);
+            in.escr[i0 + q] = E[q];
+        }
+        __syncthreads();  // every thread has read its E before the packing overwrites
+#pragma unroll
+        for (int q = 0; q < PER / 2; q++) {
+            double a0 = 0.5 * (E[2 * q].x * E[2 * q].x + E[2 * q].y * E[2 * q].y);
+            double a1 = 0.5 * (E[2 * q + 1].x * E[2 * q + 1].x + E[2 * q + 1].y * E[2 * q + 1].y);
+            sm.buf[pad(i0 / 2 + q)] = make_double2(a0, a1);        // ½|E|², reals packed in pairs
+            sm.buf[pad(NO / 2 + i0 / 2 + q)] = make_double2(0.0, 0.0);  // zero padding to 2·Nto
+        }
+        __syncthreads();
+        fft_lds<NO, false>(sm.buf, sm.tw);
+        // r2c post-step of the 2·Nto-point transform, ×hω·δt, and the c2r
+        // pre-pack of the product, pairwise (k, Nto-k) in place
+        constexpr int TS = LH_TWN / (2 * NO);
+        for (int k = tid; k <= NO / 2; k += LH_T) {
+            int m = NO - k;
+            if (k == 0) {
+                double2 Y0 = cmul(in.hwdt[0], r2c_bin<NO>(sm.buf, sm.tw, 0));
+                double2 YM = cmul(in.hwdt[NO], r2c_bin<NO>(sm.buf, sm.tw, NO));
+                Y0.y = 0.0;
+                YM.y = 0.0;
+                double2 A = cadd(Y0, cconj(YM)), B = csub(Y0, cconj(YM));
+                sm.buf[pad(0)] = cadd(A, cmul_pi(B));
+            } else if (k == NO / 2) {
+                double2 Yk = cmul(in.hwdt[k], r2c_bin<NO>(sm.buf, sm.tw, k));
+                double2 A = cadd(Yk, cconj(Yk));
+                double2 B = cmul(csub(Yk, cconj(Yk)), cconj(tw_get(sm.tw, k * TS)));
+                sm.buf[pad(k)] = cadd(A, cmul_pi(B));
+            } else {
+                double2 Yk = cmul(in.hwdt[k], r2c_bin<NO>(sm.buf, sm.tw, k));
+                double2 Ym = cmul(in.hwdt[m], r2c_bin<NO>(sm.buf, sm.tw, m));
+                double2 wk = cconj(tw_get(sm.tw, k * TS)), wm = cconj(tw_get(sm.tw, m * TS));
+                double2 Ak = cadd(Yk, cconj(Ym)), Bk = cmul(csub(Yk, cconj(Ym)), wk);
+                double2 Am = cadd(Ym, cconj(Yk)), Bm = cmul(csub(Ym, cconj(Yk)), wm);
+                sm.buf[pad(k)] = cadd(Ak, cmul_pi(Bk));
+                sm.buf[pad(m)] = cadd(Am, cmul_pi(Bm));
+            }
+        }
+        __syncthreads();
+        fft_lds<NO, true>(sm.buf, sm.tw);  // P(t) on the 2·Nto grid, packed in pairs
+        double P[PER];
+#pragma unroll
+        for (int q = 0; q < PER / 2; q++) {
+            double2 z = sm.buf[pad(i0 / 2 + q)];
+            P[2 * q] = z.x * in.inv_2n;
+            P[2 * q + 1] = z.y * in.inv_2n;
+        }
+        __syncthreads();
+        const double rho = in.density, k34 = in.kerr34;
+        const bool kerr = (in.kerr == LH_KERR_ENV);
+        const double* tw = in.to_win;
+#pragma unroll
+        for (int q = 0; q < PER; q++) {
+            double2 e = in.escr[i0 + q];
+            double2 p = make_double2(0.0, 0.0);
+            if (kerr) {
+                double s = k34 * (e.x * e.x + e.y * e.y);
+                p = make_double2(s * e.x, s * e.y);
+            }
+            double2 re = make_double2(rho * e.x, rho * e.y);
+            p = cadd(p, make_double2(re.x * P[q], re.y * P[q]));
+            sm.buf[pad(i0 + q)] = make_double2(p.x * tw[i0 + q], p.y * tw[i0 + q]);
+        }
+    } else {
+        const double k34 = in.kerr34;
+        const bool kerr = (in.kerr == LH_KERR_ENV);
+        const double* tw = in.to_win;
+#pragma unroll
+        for (int q = 0; q < PER; q++) {
+            double2 z = sm.buf[pad(i0 + q)];
+            double2 e = make_double2(z.x * es, z.y * es);
+            double2 p = make_double2(0.0, 0.0);
+            if (kerr) {
+                double s = k34 * (e.x * e.x + e.y * e.y);
+                p = make_double2(s * e.x, s * e.y);
+            }
+            sm.buf[pad(i0 + q)] = make_double2(p.x * tw[i0 + q], p.y * tw[i0 + q]);
+        }
+    }
+    constexpr int N = NN, NPT = EnvBins<NO, NN>::NPT;
+    double2 of[NPT];
+#pragma unroll
+    for (int i = 0; i < NPT; i++) {
+        int k = tid + i * LH_T;
+        of[i] = (k < N) ? in.outfac[k] : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+    fft_lds<NO, false>(sm.buf, sm.tw);
+#pragma unroll
+    for (int i = 0; i < NPT; i++) {
+        int k = tid + i * LH_T;
+        if (k < N) epi(k, cmul(sm.buf[pad(env_slot<NO, NN>(k))], of[i]));
+    }
+}
+
+// stepfun on the envelope grid (src/Luna.jl:366-370 with a c2c FT):
+// YN ← fft(twin · ifft(ωwin · YN·exp(L h))).
+template <int NT, class Smem>
+__device__ __forceinline__ void stepfun_env(const DInst& in, Smem& sm, double2* YN, double h) {
+    constexpr int NPT = (NT + LH_T - 1) / LH_T;
+#pragma unroll
+    for (int i = 0; i < NPT; i++) {
+        int k = threadIdx.x + i * LH_T;
+        if (k < NT) {
+            double2 v = YN[k];
+            if (h != 0.0) v = cmul(v, cexp_lh(in.linop[k], h));
+            sm.buf[pad(k)] = cscal(in.omega_win[k], v);
+        }
+    }
+    __syncthreads();
+    fft_lds<NT, true>(sm.buf, sm.tw);
+    const double inv_nt = in.inv_nt;
+    for (int j = threadIdx.x; j < NT; j += LH_T) {
+        double2 z = sm.buf[pad(j)];
+        double w = in.t_win[j];
+        sm.buf[pad(j)] = make_double2((inv_nt * z.x) * w, (inv_nt * z.y) * w);
+    }
+    __syncthreads();
+    fft_lds<NT, false>(sm.buf, sm.tw);
+#pragma unroll
+    for (int i = 0; i < NPT; i++) {
+        int k = threadIdx.x + i * LH_T;
+        if (k < NT) YN[k] = sm.buf[pad(k)];
+    }
+    __syncthreads();
+}
+
+// hω·δt for the Raman convolution: rfft of h(t) zero-padded to 2·Nto reals
+// (the reference's FT * h on the doubled grid, src/Nonlinear.jl:307-312).
+template <int NO>
+__global__ __launch_bounds__(LH_T) void lh_raman_hw_kernel(const double* __restrict__ h, double dt,
+                                                           const double2* __restrict__ twg, double2* hwdt) {
+    __shared__ SmemT<NO> sm;
+    tw_load(sm.tw, twg);
+    for (int j = threadIdx.x; j < NO; j += LH_T)
+        sm.buf[pad(j)] = (j < NO / 2) ? make_double2(h[2 * j], h[2 * j + 1]) : make_double2(0.0, 0.0);
+    __syncthreads();
+    fft_lds<NO, false>(sm.buf, sm.tw);
+    for (int k = threadIdx.x; k <= NO; k += LH_T) {
+        double2 X = r2c_bin<NO>(sm.buf, sm.tw, k);
+        hwdt[k] = make_double2(X.x * dt, X.y * dt);
+    }
+}
+
 // --------------------------------------------------------------------------
 // Persistent adaptive solve of one instance (Luna.run → RK45.solve_precon).
 // --------------------------------------------------------------------------
-template <int MO, int MC, class Smem>
+// KIND 0: field grid, MO = Nto/2 complex points, MC = Nt/2.
+// KIND 1: envelope grid, MO = Nto complex points, MC = Nt = Nω.
+template <int KIND, int MO, int MC>
+struct Sizes {
+    static constexpr int N = (KIND == 0) ? Bins<MO>::N : MC;
+    static constexpr int NPT = (N + LH_T - 1) / LH_T;
+};
+
+template <int KIND, int MO, int MC, class Smem>
+__device__ __forceinline__ void do_prologue(const DInst& in, Smem& sm, const double2* Y, double2* const* K, int stage,
+                                            double dt, double h0, double hf) {
+    if constexpr (KIND == 0)
+        stage_prologue<MO, Smem>(in, sm, Y, K, stage, dt, h0, hf);
+    else
+        stage_prologue_env<MO, MC, Smem>(in, sm, Y, K, stage, dt, h0, hf);
+}
+
+template <int KIND, int MO, int MC, class Smem, class Epi>
+__device__ __forceinline__ void do_rhs(const DInst& in, Smem& sm, Epi epi) {
+    if constexpr (KIND == 0)
+        rhs_field<MO, Smem>(in, sm, epi);
+    else
+        rhs_env<MO, MC, Smem>(in, sm, epi);
+}
+
+template <int KIND, int MO, int MC, class Smem>
+__device__ __forceinline__ void do_stepfun(const DInst& in, Smem& sm, double2* YN, double h) {
+    if constexpr (KIND == 0)
+        stepfun_filter<MC, Smem>(in, sm, YN, h);
+    else
+        stepfun_env<MC, Smem>(in, sm, YN, h);
+}
+
+template <int KIND, int MO, int MC, class Smem>
 __device__ __forceinline__ void solve_instance(const DInst& in, Smem& sm) {
-    constexpr int N = Bins<MO>::N, NPT = Bins<MO>::NPT;
+    constexpr int N = Sizes<KIND, MO, MC>::N, NPT = Sizes<KIND, MO, MC>::NPT;
     double2* K[7];
 #pragma unroll
     for (int j = 0; j < 7; j++) K[j] = in.K[j];
@@ -376,10 +640,10 @@ __device__ __forceinline__ void solve_instance(const DInst& in, Smem& sm) {
     double t = in.z0, tn = in.z0, dt = in.init_dz, dtn = in.init_dz, errlast = 0.0;
 
     // PreconStepper init: k1 = fbar(y0, t, t)
-    stage_prologue<MO, Smem>(in, sm, Y, K, 0, 0.0, 0.0, 0.0);
+    do_prologue<KIND, MO, MC, Smem>(in, sm, Y, K, 0, 0.0, 0.0, 0.0);
     {
         double2* k0 = K[0];
-        rhs_field<MO, Smem>(in, sm, [&](int k, double2 nl) { k0[k] = nl; });
+        do_rhs<KIND, MO, MC, Smem>(in, sm, [&](int k, double2 nl) { k0[k] = nl; });
     }
     rhs_evals++;
     __syncthreads();
@@ -398,10 +662,10 @@ __device__ __forceinline__ void solve_instance(const DInst& in, Smem& sm) {
         for (int st = 0; st < 6 && status == LH_OK; st++) {
             double t2 = t + c_nodes[st] * dt;
             double hf = t2 - t, hb = t - t2;
-            stage_prologue<MO, Smem>(in, sm, Y, K, st, dt, h0, hf);
+            do_prologue<KIND, MO, MC, Smem>(in, sm, Y, K, st, dt, h0, hf);
             double2* kout = K[st + 1];
             const double2* Lp = in.linop;
-            rhs_field<MO, Smem>(in, sm, [&](int k, double2 nl) { kout[k] = cmul(nl, cexp_lh(Lp[k], hb)); });
+            do_rhs<KIND, MO, MC, Smem>(in, sm, [&](int k, double2 nl) { kout[k] = cmul(nl, cexp_lh(Lp[k], hb)); });
             rhs_evals++;
             __syncthreads();
             if (sm.flag != LH_OK) status = sm.flag;
@@ -473,7 +737,7 @@ __device__ __forceinline__ void solve_instance(const DInst& in, Smem& sm) {
             double2* tmp = K[0];  // FSAL: ks[1] .= ks[7] (src/RK45.jl:184-186)
             K[0] = K[6];
             K[6] = tmp;
-            stepfun_filter<MC, Smem>(in, sm, YN, tn - t);
+            do_stepfun<KIND, MO, MC, Smem>(in, sm, YN, tn - t);
             // GridCondition saves via dense output; ks[1] and ks[7] both hold k7
             while (saved < in.save_n && in.zgrid[saved] <= tn) {
                 double ti = in.zgrid[saved];
@@ -541,61 +805,92 @@ __device__ __forceinline__ void solve_instance(const DInst& in, Smem& sm) {
 }
 
 // One persistent kernel per grid size: one workgroup per instance.
-template <int MO, int MC>
+template <int KIND, int MO, int MC>
 __global__ __launch_bounds__(LH_T) void lh_solve_kernel(const DInst* insts, const double2* __restrict__ twg) {
     __shared__ SmemT<MO> sm;
     tw_load(sm.tw, twg);
     __syncthreads();
-    solve_instance<MO, MC, SmemT<MO>>(insts[blockIdx.x], sm);
+    solve_instance<KIND, MO, MC, SmemT<MO>>(insts[blockIdx.x], sm);
 }
 
-template <int MO, class Smem>
+template <int KIND, int MO, int MC, class Smem>
 __device__ __forceinline__ void rhs_once(const DInst& in, Smem& sm) {
-    constexpr int N = Bins<MO>::N;
-    for (int k = threadIdx.x; k < N; k += LH_T) sm.buf[pad(k)] = cscal(in.sc_t, in.rhs_in[k]);
-    for (int k = N + threadIdx.x; k <= MO; k += LH_T) sm.buf[pad(k)] = make_double2(0, 0);
+    constexpr int N = Sizes<KIND, MO, MC>::N;
+    for (int k = threadIdx.x; k < N; k += LH_T) {
+        double2 v = cscal(in.sc_t, in.rhs_in[k]);
+        if constexpr (KIND == 0)
+            sm.buf[pad(k)] = v;
+        else
+            sm.buf[pad(env_slot<MO, MC>(k))] = v;
+    }
+    if constexpr (KIND == 0) {
+        for (int k = N + threadIdx.x; k <= MO; k += LH_T) sm.buf[pad(k)] = make_double2(0, 0);
+    } else if constexpr (MO > MC) {
+        for (int k = MC / 2 + threadIdx.x; k < MO - MC / 2; k += LH_T) sm.buf[pad(k)] = make_double2(0, 0);
+    }
     double2* out = in.rhs_out;
-    rhs_field<MO, Smem>(in, sm, [&](int k, double2 nl) { out[k] = nl; });
+    do_rhs<KIND, MO, MC, Smem>(in, sm, [&](int k, double2 nl) { out[k] = nl; });
 }
 
-template <int MO>
+template <int KIND, int MO, int MC>
 __global__ __launch_bounds__(LH_T) void lh_rhs_kernel(const DInst* insts, const double2* __restrict__ twg,
                                                       int* status) {
     __shared__ SmemT<MO> sm;
     tw_load(sm.tw, twg);
     if (threadIdx.x == 0) sm.flag = LH_OK;
     __syncthreads();
-    rhs_once<MO, SmemT<MO>>(insts[blockIdx.x], sm);
+    rhs_once<KIND, MO, MC, SmemT<MO>>(insts[blockIdx.x], sm);
     __syncthreads();
     if (threadIdx.x == 0) status[blockIdx.x] = sm.flag;
 }
 
-// Size classes with a compiled kernel: (Nto/2, Nt/2).
-enum { LH_CLASS_16384 = 0, LH_CLASS_8192 = 1, LH_CLASS_4096 = 2, LH_NCLASS = 3 };
+// Size classes with a compiled kernel.
+enum {
+    LH_CLASS_16384 = 0,    // field grid, Nto = 16384, Nt = 8192
+    LH_CLASS_8192 = 1,     // field grid, Nto = 8192, Nt = 4096
+    LH_CLASS_4096 = 2,     // field grid, Nto = 4096, Nt = 2048
+    LH_CLASS_ENV8192 = 3,  // envelope grid, Nto = Nt = 8192
+    LH_CLASS_ENV4096 = 4,  // envelope grid, Nto = Nt = 4096
+    LH_NCLASS = 5
+};
 
-static int size_class(int n_to) {
-    switch (n_to) {
-        case 16384: return LH_CLASS_16384;
-        case 8192: return LH_CLASS_8192;
-        case 4096: return LH_CLASS_4096;
-        default: return -1;
+static int size_class_of(int grid_kind, int n_to, int n_t) {
+    if (grid_kind == LH_GRID_REAL) {
+        if (n_to == 16384 && n_t == 8192) return LH_CLASS_16384;
+        if (n_to == 8192 && n_t == 4096) return LH_CLASS_8192;
+        if (n_to == 4096 && n_t == 2048) return LH_CLASS_4096;
+    } else if (grid_kind == LH_GRID_ENV) {
+        if (n_to == 8192 && n_t == 8192) return LH_CLASS_ENV8192;
+        if (n_to == 4096 && n_t == 4096) return LH_CLASS_ENV4096;
     }
+    return -1;
 }
 
 static void launch_solve(int cls, int n, const DInst* dinst, const double2* tw, hipStream_t s) {
     switch (cls) {
-        case LH_CLASS_16384: hipLaunchKernelGGL((lh_solve_kernel<8192, 4096>), dim3(n), dim3(LH_T), 0, s, dinst, tw); break;
-        case LH_CLASS_8192: hipLaunchKernelGGL((lh_solve_kernel<4096, 2048>), dim3(n), dim3(LH_T), 0, s, dinst, tw); break;
-        default: hipLaunchKernelGGL((lh_solve_kernel<2048, 1024>), dim3(n), dim3(LH_T), 0, s, dinst, tw); break;
+        case LH_CLASS_16384: hipLaunchKernelGGL((lh_solve_kernel<0, 8192, 4096>), dim3(n), dim3(LH_T), 0, s, dinst, tw); break;
+        case LH_CLASS_8192: hipLaunchKernelGGL((lh_solve_kernel<0, 4096, 2048>), dim3(n), dim3(LH_T), 0, s, dinst, tw); break;
+        case LH_CLASS_4096: hipLaunchKernelGGL((lh_solve_kernel<0, 2048, 1024>), dim3(n), dim3(LH_T), 0, s, dinst, tw); break;
+        case LH_CLASS_ENV8192: hipLaunchKernelGGL((lh_solve_kernel<1, 8192, 8192>), dim3(n), dim3(LH_T), 0, s, dinst, tw); break;
+        default: hipLaunchKernelGGL((lh_solve_kernel<1, 4096, 4096>), dim3(n), dim3(LH_T), 0, s, dinst, tw); break;
     }
 }
 
 static void launch_rhs(int cls, const DInst* dinst, const double2* tw, int* status, hipStream_t s) {
     switch (cls) {
-        case LH_CLASS_16384: hipLaunchKernelGGL((lh_rhs_kernel<8192>), dim3(1), dim3(LH_T), 0, s, dinst, tw, status); break;
-        case LH_CLASS_8192: hipLaunchKernelGGL((lh_rhs_kernel<4096>), dim3(1), dim3(LH_T), 0, s, dinst, tw, status); break;
-        default: hipLaunchKernelGGL((lh_rhs_kernel<2048>), dim3(1), dim3(LH_T), 0, s, dinst, tw, status); break;
+        case LH_CLASS_16384: hipLaunchKernelGGL((lh_rhs_kernel<0, 8192, 4096>), dim3(1), dim3(LH_T), 0, s, dinst, tw, status); break;
+        case LH_CLASS_8192: hipLaunchKernelGGL((lh_rhs_kernel<0, 4096, 2048>), dim3(1), dim3(LH_T), 0, s, dinst, tw, status); break;
+        case LH_CLASS_4096: hipLaunchKernelGGL((lh_rhs_kernel<0, 2048, 1024>), dim3(1), dim3(LH_T), 0, s, dinst, tw, status); break;
+        case LH_CLASS_ENV8192: hipLaunchKernelGGL((lh_rhs_kernel<1, 8192, 8192>), dim3(1), dim3(LH_T), 0, s, dinst, tw, status); break;
+        default: hipLaunchKernelGGL((lh_rhs_kernel<1, 4096, 4096>), dim3(1), dim3(LH_T), 0, s, dinst, tw, status); break;
     }
+}
+
+static void launch_raman_hw(int n_to, const double* h, double dt, const double2* tw, double2* hwdt, hipStream_t s) {
+    if (n_to == 8192)
+        hipLaunchKernelGGL((lh_raman_hw_kernel<8192>), dim3(1), dim3(LH_T), 0, s, h, dt, tw, hwdt);
+    else
+        hipLaunchKernelGGL((lh_raman_hw_kernel<4096>), dim3(1), dim3(LH_T), 0, s, h, dt, tw, hwdt);
 }
 
 // ==========================================================================
@@ -617,6 +912,8 @@ struct lh_ctx {
     double2* linop;
     double2* outfac;
     double4* ppt;
+    double2* hwdt;  // envelope Raman: rfft(h, 2·Nto)·δt
+    double2* escr;  // envelope Raman: E(t) stash
     // solver buffers (allocated on first use)
     double2* Yin;
     int have_state;
@@ -642,8 +939,8 @@ struct lh_ctx {
 static std::mutex g_mu;
 static double2* g_tw[64] = {nullptr};
 // HIP-event durations of the last resident batch, per size class
-static float g_last_ms[3] = {0, 0, 0};
-static int g_last_n[3] = {0, 0, 0};
+static float g_last_ms[8] = {0};
+static int g_last_n[8] = {0};
 
 #define HIPCK(x)                                                                     \
     do {                                                                             \
@@ -715,14 +1012,20 @@ static int upload(T** dst, const void* src, size_t count) {
     return LH_OK;
 }
 
-static int size_class(int n_to);
+
+static int ctx_class(const lh_ctx* c);
 
 static bool supported(const lh_desc* d) {
-    if (d->grid_kind != LH_GRID_REAL || d->transform != LH_TRANS_MODEAVG || d->n_r != 1) return false;
-    if (!(d->n_to == 16384 || d->n_to == 8192 || d->n_to == 4096)) return false;
-    if (d->n_t * 2 != d->n_to) return false;
-    if (d->n_omega != d->n_t / 2 + 1) return false;
-    if (d->raman || d->kerr == LH_KERR_ENV) return false;
+    if (d->transform != LH_TRANS_MODEAVG || d->n_r != 1) return false;
+    if (size_class_of(d->grid_kind, d->n_to, d->n_t) < 0) return false;
+    if (d->grid_kind == LH_GRID_REAL) {
+        if (d->n_omega != d->n_t / 2 + 1) return false;
+        if (d->raman || d->kerr == LH_KERR_ENV) return false;
+    } else {
+        if (d->n_omega != d->n_t) return false;
+        if (d->plasma || d->kerr == LH_KERR_FIELD) return false;
+        if (d->raman && !d->raman_h) return false;
+    }
     return true;
 }
 
@@ -735,6 +1038,8 @@ static double scale_to_freq(const lh_desc& d) {
     if (d.grid_kind == LH_GRID_REAL) return (double)(d.n_omega - 1) / (double)(d.n_to / 2);
     return (double)d.n_omega / (double)d.n_to;
 }
+
+static int ctx_class(const lh_ctx* c) { return size_class_of(c->d.grid_kind, c->d.n_to, c->d.n_t); }
 
 static DInst base_inst(const lh_ctx* c) {
     DInst in;
@@ -752,6 +1057,10 @@ static DInst base_inst(const lh_ctx* c) {
     in.ppt_n = d.ppt_n;
     in.density = d.density;
     in.kerr_fac = d.density * EPS0 * d.gamma3;  // (ρ ε0) γ3, Nonlinear.jl:30
+    in.kerr34 = 0.75 * in.kerr_fac;             // 3/4·fac, Nonlinear.jl:48
+    in.inv_2n = 1.0 / (2.0 * (double)d.n_to);
+    in.hwdt = c->hwdt;
+    in.escr = c->escr;
     in.ionpot = d.ionpot;
     in.preion_p1 = d.preionfrac + 1;
     in.e_ratio = ECHARGE * ECHARGE / MELEC;
@@ -865,7 +1174,7 @@ int lh_destroy(lh_ctx* c) {
     if (!c) return LH_OK;
     hipSetDevice(c->device);
     if (c->stream) hipStreamSynchronize(c->stream);
-    void* ptrs[] = {c->omega_win, c->t_win, c->to_win, c->linop, c->outfac, c->ppt, c->Yin, c->Y, c->YN,
+    void* ptrs[] = {c->omega_win, c->t_win, c->to_win, c->linop, c->outfac, c->ppt, c->hwdt, c->escr, c->Yin, c->Y, c->YN,
                     c->saves, c->zgrid, c->dstats, c->dinst, c->rhs_in, c->rhs_out, c->dstatus};
     for (void* p : ptrs)
         if (p) hipFree(p);
@@ -925,6 +1234,19 @@ int lh_create(lh_ctx** out, const lh_desc* d) {
         c->ppt_emax = d->ppt_x[d->ppt_n - 1];
         st = upload(&c->ppt, p.data(), p.size());
     }
+    if (!st && d->grid_kind == LH_GRID_ENV && d->raman) {
+        const double2* tw;
+        double* dh = nullptr;
+        st = twiddles(c->device, &tw);
+        if (!st) st = upload(&dh, d->raman_h, (size_t)d->n_to);
+        if (!st && hipMalloc(&c->hwdt, sizeof(double2) * (size_t)(d->n_to + 1)) != hipSuccess) st = LH_E_NOMEM;
+        if (!st && hipMalloc(&c->escr, sizeof(double2) * (size_t)d->n_to) != hipSuccess) st = LH_E_NOMEM;
+        if (!st) {
+            launch_raman_hw(d->n_to, dh, d->dt_o, tw, c->hwdt, c->stream);
+            if (hipGetLastError() != hipSuccess || hipStreamSynchronize(c->stream) != hipSuccess) st = LH_E_HIP;
+        }
+        if (dh) (void)hipFree(dh);
+    }
     // host pointers inside the copied descriptor belong to the caller
     c->d.omega = nullptr; c->d.sidx = nullptr; c->d.omega_win = nullptr; c->d.t_win = nullptr;
     c->d.to_win = nullptr; c->d.linop = nullptr; c->d.norm = nullptr; c->d.ppt_x = nullptr;
@@ -969,7 +1291,7 @@ int lh_rhs_device(lh_ctx* c, void* nl_dev, const void* Ew_dev, double z, void* s
     in.rhs_in = (const double2*)Ew_dev;
     in.rhs_out = (double2*)nl_dev;
     HIPCK(hipMemcpyAsync(c->dinst, &in, sizeof(DInst), hipMemcpyHostToDevice, s));
-    launch_rhs(size_class(c->d.n_to), c->dinst, tw, c->dstatus, s);
+    launch_rhs(ctx_class(c), c->dinst, tw, c->dstatus, s);
     HIPCK(hipGetLastError());
     c->rhs_pending = 1;
     c->rhs_stream = s;
@@ -1039,7 +1361,7 @@ int lh_batch_solve_resident(lh_ctx* const* ctxs, int n, const lh_rk_opts* o, int
     if (!ctxs || n <= 0 || !o || !stats) return LH_E_ARG;
     const int dev = ctxs[0]->device;
     for (int i = 0; i < n; i++)
-        if (!ctxs[i] || ctxs[i]->device != dev || size_class(ctxs[i]->d.n_to) < 0) return LH_E_ARG;
+        if (!ctxs[i] || ctxs[i]->device != dev || ctx_class(ctxs[i]) < 0) return LH_E_ARG;
     for (int i = 0; i < n; i++)  // the same context twice would share buffers
         for (int j = i + 1; j < n; j++)
             if (ctxs[i] == ctxs[j]) return LH_E_ARG;
@@ -1051,13 +1373,13 @@ int lh_batch_solve_resident(lh_ctx* const* ctxs, int n, const lh_rk_opts* o, int
     // group instances by size class; each class is one persistent launch on
     // its own stream so the classes run concurrently
     int count[LH_NCLASS] = {0}, first[LH_NCLASS] = {0};
-    for (int i = 0; i < n; i++) count[size_class(ctxs[i]->d.n_to)]++;
+    for (int i = 0; i < n; i++) count[ctx_class(ctxs[i])]++;
     for (int c = 1; c < LH_NCLASS; c++) first[c] = first[c - 1] + count[c - 1];
     std::vector<int> pos(n);
     {
         int fill[LH_NCLASS] = {0};
         for (int i = 0; i < n; i++) {
-            int c = size_class(ctxs[i]->d.n_to);
+            int c = ctx_class(ctxs[i]);
             pos[i] = first[c] + fill[c]++;
         }
     }
@@ -1121,7 +1443,7 @@ int lh_batch_solve_resident(lh_ctx* const* ctxs, int n, const lh_rk_opts* o, int
 }
 
 int lh_last_launch_times(double* ms, int* n_inst, int nclass) {
-    if (!ms || !n_inst || nclass < LH_NCLASS) return LH_E_ARG;
+    if (!ms || !n_inst || nclass < LH_NCLASS) return LH_E_ARG;  // LH_NCLASS = 5
     for (int c = 0; c < LH_NCLASS; c++) {
         ms[c] = g_last_ms[c];
         n_inst[c] = g_last_n[c];

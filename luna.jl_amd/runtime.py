@@ -112,7 +112,8 @@ class ResidentBatch:
 
 def last_launch_times():
     """{n_to: (ms, instances)} of the persistent launches of the last batch."""
-    ms = (C.c_double * 3)()
-    n = (C.c_int * 3)()
-    _capi.check(_capi.lib().lh_last_launch_times(ms, n, 3), "lh_last_launch_times")
-    return {_capi.CLASS_NTO[c]: (ms[c], n[c]) for c in range(3) if n[c] > 0}
+    k = len(_capi.CLASS_NAMES)
+    ms = (C.c_double * k)()
+    n = (C.c_int * k)()
+    _capi.check(_capi.lib().lh_last_launch_times(ms, n, k), "lh_last_launch_times")
+    return {_capi.CLASS_NAMES[c]: (ms[c], n[c]) for c in range(k) if n[c] > 0}

@@ -1,0 +1,90 @@
+"""GPU parity of the envelope (GNLSE) path against the oracle.
+
+TransModeAvg on an EnvGrid with Kerr_env and RamanPolarEnv
+(src/NonlinearRHS.jl:25-53, 362-399; src/Nonlinear.jl:47-73, 282-333): single
+RHS relative L2 ≤ 1e-12; propagations ≤ 1e-6 relative spectral-energy error
+with the same step counts.  The N = 4 soliton also meets the reference's own
+criterion (test/test_gnlse.jl:8-31).
+"""
+import math
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_l2(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+def spec_err(a, b, sidx):
+    pa, pb = np.abs(a[sidx]) ** 2, np.abs(b[sidx]) ** 2
+    return np.linalg.norm(pa - pb) / np.linalg.norm(pb)
+
+
+@pytest.fixture(scope="module")
+def cfg1(pkg):
+    from luna_jl_amd import configs
+    return configs.gnlse_instance()
+
+
+def test_env_rhs_raman(cfg1, orc):
+    print("case: gnlse rhs (Nt=8192, Kerr+Raman)", flush=True)
+    from luna_jl_amd import runtime
+    ctx = runtime.Context(cfg1.holder)
+    for scale in (1.0, 3.0):
+        nl = ctx.rhs(cfg1.Ew0 * scale)
+        st, ref = orc.rhs(cfg1.holder, cfg1.Ew0 * scale)
+        assert st == 0
+        assert rel_l2(nl, ref) <= 1e-12, rel_l2(nl, ref)
+
+
+def test_env_solve_short(cfg1, orc, capi):
+    print("case: gnlse solve 1 cm", flush=True)
+    from luna_jl_amd import runtime
+    ctx = runtime.Context(cfg1.holder)
+    opts = capi.rk_opts(0.01, save_n=11)
+    Ew, sv, zs, stats = ctx.solve(cfg1.Ew0, opts)
+    st, Ew_r, sv_r, zs_r, stats_r = orc.solve(cfg1.holder, cfg1.Ew0, opts)
+    assert stats.status == 0 and stats.steps == stats_r.steps
+    assert spec_err(Ew, Ew_r, cfg1.sidx) <= 1e-8
+    for k in range(11):
+        assert spec_err(sv[k], sv_r[k], cfg1.sidx) <= 1e-8
+
+
+def test_env_n4_soliton(pkg, orc, capi):
+    print("case: N=4 soliton on the GPU (Nt=4096)", flush=True)
+    from luna_jl_amd import interface, runtime
+    gamma, beta2, N, tau0, fr = 0.1, -1e-26, 4.0, 280e-15, 0.18
+    tau_fwhm = 2 * math.log(1 + math.sqrt(2)) * tau0
+    P0 = N ** 2 * abs(beta2) / ((1 - fr) * gamma * tau0 ** 2)
+    flength = math.pi * tau0 ** 2 / abs(beta2)
+    Ew, grid, linop, tr, _, _ = interface.prop_gnlse_args(
+        gamma, flength, [0.0, 0.0, beta2], lambda0=835e-9, tau_fwhm=tau_fwhm, power=P0,
+        pulseshape="sech", lambda_lims=(450e-9, 8000e-9), trange=4e-12, raman=False, shock=False,
+        fr=fr, shotnoise=False)
+    h = tr.descriptor(linop)
+    ctx = runtime.Context(h)
+    opts = capi.rk_opts(flength, save_n=201)
+    Eg, sv, zs, stats = ctx.solve(Ew, opts)
+    st, Er, sv_r, zs_r, stats_r = orc.solve(h, Ew, opts)
+    assert stats.status == 0 and stats.steps == stats_r.steps
+    assert spec_err(sv[-1], sv_r[-1], grid.sidx) <= 1e-6
+
+    def isapprox(a, b, rtol):
+        return np.linalg.norm(a - b) <= rtol * max(np.linalg.norm(a), np.linalg.norm(b))
+    assert isapprox(np.abs(sv[0]) ** 2, np.abs(sv[-1]) ** 2, rtol=1e-2)
+    assert isapprox(np.abs(np.fft.ifft(sv[0])) ** 2, np.abs(np.fft.ifft(sv[-1])) ** 2, rtol=1e-3)
+
+
+def test_env_solve_full_cfg1(cfg1, orc, capi):
+    print("case: gnlse supercontinuum 15 cm (full)", flush=True)
+    from luna_jl_amd import runtime
+    ctx = runtime.Context(cfg1.holder)
+    opts = capi.rk_opts(cfg1.zmax, save_n=201)
+    Ew, sv, zs, stats = ctx.solve(cfg1.Ew0, opts)
+    st, Ew_r, sv_r, zs_r, stats_r = orc.solve(cfg1.holder, cfg1.Ew0, opts)
+    assert stats.status == 0 and stats.saved == 201
+    assert stats.steps == stats_r.steps
+    assert spec_err(sv[-1], sv_r[-1], cfg1.sidx) <= 1e-6
