@@ -75,7 +75,9 @@ typedef struct lh_desc {
     double ionpot;            /* ionisation potential (J)                        */
     double preionfrac;        /* pre-ionisation fraction                         */
     const double *raman_h;    /* [n_to] windowed h(t) on tt = 0..n_to-1 (×δto)   */
-    const double *qdht;       /* [n_r*n_r] QDHT matrix T (symmetric)             */
+    const double *qdht;       /* [n_r*n_r] QDHT matrix T, row-major T[i*n_r+j]:
+                                 T_ij = (2/S) J0(j_i j_j/S) / J1(j_j)^2 (Hankel.jl
+                                 QDHT, not symmetric); radial transform only     */
     double qdht_fwd;          /* r→k scale R/K                                   */
     double qdht_inv;          /* k→r scale K/R                                   */
 } lh_desc;
@@ -137,8 +139,10 @@ int lh_solve(lh_ctx *ctx, double *Ew_inout, const lh_rk_opts *opts,
              double *saves, double *zsaves, lh_stats *stats);
 
 /* Batched solve: n independent instances (each its own grid, constants, step
- * size control and status) in one persistent launch, one workgroup per
- * instance.  Replaces the per-point loop of Scans.runscan src/Scans.jl:275-430.
+ * size control and status) in one persistent launch per grid class, one
+ * workgroup per instance.  Radial contexts (whose one instance fills the GPU)
+ * are solved one after another by a graph-replayed per-step launch sequence.
+ * Replaces the per-point loop of Scans.runscan src/Scans.jl:275-430.
  * Ew_inout[i]: instance i's field; saves[i] may be NULL; stats[i] per instance.
  * Returns LH_OK when the launch ran; per-instance failures are in stats[i].status. */
 int lh_batch_solve(lh_ctx *const *ctxs, int n, double *const *Ew_inout,
@@ -158,8 +162,9 @@ int lh_get_saves(lh_ctx *ctx, double *saves, double *zsaves);
 
 /* Measurement hook (no counterpart in the reference): HIP-event durations of
  * the persistent launches of the last batch, one per grid-size class
- * (field grid Nto = 16384, 8192, 4096; envelope grid Nto = 8192, 4096), with
- * the number of instances in each.  nclass must be >= 5. */
+ * (field grid Nto = 16384, 8192, 4096; envelope grid Nto = 8192, 4096; all
+ * radial contexts together), with the number of instances in each.
+ * nclass must be >= 6. */
 int lh_last_launch_times(double *ms, int *n_inst, int nclass);
 
 #ifdef __cplusplus

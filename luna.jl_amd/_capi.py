@@ -142,7 +142,7 @@ EXPORTED = ["lh_strerror", "lh_device_count", "lh_create", "lh_destroy", "lh_rhs
             "lh_rhs_device", "lh_ctx_status", "lh_solve", "lh_batch_solve", "lh_set_state",
             "lh_batch_solve_resident", "lh_get_state", "lh_get_saves", "lh_last_launch_times"]
 
-CLASS_NAMES = (16384, 8192, 4096, "env8192", "env4096")
+CLASS_NAMES = (16384, 8192, 4096, "env8192", "env4096", "radial")
 
 
 def check(status, where=""):

@@ -10,7 +10,15 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from .interface import prop_capillary_args, prop_gnlse_args
+from . import physdata as pd
+from .fields import GaussGaussField
+from .grid import RealGrid
+from .hankel import QDHT
+from .interface import prop_capillary_args, prop_gnlse_args, setup_radial
+from .ionisation import ppt_spline_cached
+from .linops import make_const_linop_radial
+from .nonlinear import Kerr_field, PlasmaCumtrapz
+from .nonlinearrhs import const_norm_radial
 
 
 @dataclass
@@ -64,3 +72,31 @@ def gnlse_instance(trange=6e-12, flength=0.15, noise_seed=0, shotnoise=True):
     holder = transform.descriptor(linop)
     return Instance(holder=holder, Ew0=np.ascontiguousarray(Ew), zmax=flength, grid=grid, linop=linop,
                     transform=transform, sidx=grid.sidx.copy(), params=dict(n_to=grid.n_to))
+
+
+def radial_instance(gas="Ar", pressure=0.5, flength=0.5, lambda0=800e-9, tau_fwhm=30e-15, energy=200e-6,
+                    w0=40e-6, propz=-0.157, R=6e-3, N=256, trange=1e-12,
+                    lambda_lims=(400e-9, 2000e-9), kerr=True, plasma=True, ppt_n=2 ** 16):
+    """Radially symmetric free-space propagation (TransRadial), as in the
+    reference's radial example: a Gaussian beam whose 1 mm waist at z = 0
+    (``propz`` before a 40 μm focus) is focused into Ar; Kerr + PPT plasma.
+    With the defaults Nt = 4096, Nto = 8192, Nω = 2049 and Nr = 256."""
+    grid = RealGrid(flength, lambda0, lambda_lims, trange)
+    q = QDHT(R, N, dim=2)
+    rho = pd.density(gas, pressure)
+    resp = []
+    if kerr:
+        resp.append(Kerr_field(pd.gamma3_gas(gas)))
+    if plasma:
+        spl = ppt_spline_cached(gas, grid.ref_lambda, N=ppt_n)
+        resp.append(PlasmaCumtrapz(grid.to, spl, pd.ionisation_potential(gas)))
+    nfun = pd.ref_index_fun(gas, pressure)
+    linop, beta1 = make_const_linop_radial(grid, q, nfun)
+    normfun = const_norm_radial(grid, q, nfun)
+    inputs = GaussGaussField(lambda0, tau_fwhm, energy, w0, propz=propz)
+    Ewk, transform = setup_radial(grid, q, rho, normfun, resp, inputs)
+    holder = transform.descriptor(linop)
+    return Instance(holder=holder, Ew0=np.asfortranarray(Ewk), zmax=flength, grid=grid, linop=linop,
+                    transform=transform, sidx=grid.sidx.copy(),
+                    params=dict(energy=energy, pressure=pressure, trange=trange, flength=flength, R=R, N=N,
+                                n_to=grid.n_to, n_omega=grid.n_omega, n_r=N, q=q))

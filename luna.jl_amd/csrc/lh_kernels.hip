@@ -281,8 +281,21 @@ __device__ __forceinline__ void rhs_field(const DInst& in, Smem& sm, Epi epi) {
 // k-vectors outside and this thread's bins inside, so each vector's loads
 // for all bins are in flight together.
 template <int MO, class Smem>
+__device__ __forceinline__ void stage_prologue_cols(const double2* __restrict__ linop, double sc_t, Smem& sm,
+                                                    const double2* Y, double2* const* K, int stage, double dt,
+                                                    double h0, double hf);
+
+template <int MO, class Smem>
 __device__ __forceinline__ void stage_prologue(const DInst& in, Smem& sm, const double2* Y, double2* const* K,
                                                int stage, double dt, double h0, double hf) {
+    stage_prologue_cols<MO, Smem>(in.linop, in.sc_t, sm, Y, K, stage, dt, h0, hf);
+}
+
+// Same on explicit per-column pointers (one column of a radial state).
+template <int MO, class Smem>
+__device__ __forceinline__ void stage_prologue_cols(const double2* __restrict__ linop, double sc_t, Smem& sm,
+                                                    const double2* Y, double2* const* K, int stage, double dt,
+                                                    double h0, double hf) {
     constexpr int N = Bins<MO>::N, NPT = Bins<MO>::NPT;
     const int tid = threadIdx.x;
     double2 yb[NPT], L[NPT];
@@ -291,7 +304,7 @@ __device__ __forceinline__ void stage_prologue(const DInst& in, Smem& sm, const 
         int k = tid + i * LH_T;
         bool v = k < N;
         yb[i] = v ? Y[k] : make_double2(0.0, 0.0);
-        L[i] = v ? in.linop[k] : make_double2(0.0, 0.0);
+        L[i] = v ? linop[k] : make_double2(0.0, 0.0);
     }
     if (stage == 0 && h0 != 0.0) {
         double2* K0 = K[0];
@@ -312,7 +325,6 @@ __device__ __forceinline__ void stage_prologue(const DInst& in, Smem& sm, const 
             if (k < N) yb[i] = cadd(yb[i], cscal(cj, Kj[k]));
         }
     }
-    const double sc_t = in.sc_t;
 #pragma unroll
     for (int i = 0; i < NPT; i++) {
         int k = tid + i * LH_T;
@@ -327,7 +339,17 @@ __device__ __forceinline__ void stage_prologue(const DInst& in, Smem& sm, const 
 // Accepted-step window filter (stepfun, src/Luna.jl:366-370) on the coarse
 // grid: YN ← rfft(twin · irfft(ωwin · YN·exp(L h))).
 template <int MC, class Smem>
+__device__ __forceinline__ void stepfun_filter_cols(const DInst& in, const double2* __restrict__ linop, Smem& sm,
+                                                    double2* YN, double h);
+
+template <int MC, class Smem>
 __device__ __forceinline__ void stepfun_filter(const DInst& in, Smem& sm, double2* YN, double h) {
+    stepfun_filter_cols<MC, Smem>(in, in.linop, sm, YN, h);
+}
+
+template <int MC, class Smem>
+__device__ __forceinline__ void stepfun_filter_cols(const DInst& in, const double2* __restrict__ linop, Smem& sm,
+                                                    double2* YN, double h) {
     constexpr int N = MC + 1;
     constexpr int NPT = (N + LH_T - 1) / LH_T;
 #pragma unroll
@@ -335,7 +357,7 @@ __device__ __forceinline__ void stepfun_filter(const DInst& in, Smem& sm, double
         int k = threadIdx.x + i * LH_T;
         if (k < N) {
             double2 v = YN[k];
-            if (h != 0.0) v = cmul(v, cexp_lh(in.linop[k], h));
+            if (h != 0.0) v = cmul(v, cexp_lh(linop[k], h));
             sm.buf[pad(k)] = cscal(in.omega_win[k], v);
         }
     }
@@ -851,7 +873,8 @@ enum {
     LH_CLASS_4096 = 2,     // field grid, Nto = 4096, Nt = 2048
     LH_CLASS_ENV8192 = 3,  // envelope grid, Nto = Nt = 8192
     LH_CLASS_ENV4096 = 4,  // envelope grid, Nto = Nt = 4096
-    LH_NCLASS = 5
+    LH_CLASS_RADIAL = 5,   // radial transform (solved context by context)
+    LH_NCLASS = 6
 };
 
 static int size_class_of(int grid_kind, int n_to, int n_t) {
@@ -892,6 +915,9 @@ static void launch_raman_hw(int n_to, const double* h, double dt, const double2*
     else
         hipLaunchKernelGGL((lh_raman_hw_kernel<4096>), dim3(1), dim3(LH_T), 0, s, h, dt, tw, hwdt);
 }
+
+// Radially symmetric transform: per-column kernels and FP64 MFMA contraction.
+#include "lh_radial.hip"
 
 // ==========================================================================
 // Host side: contexts, constant upload, launches (C ABI, include/luna_hip.h)
@@ -934,6 +960,15 @@ struct lh_ctx {
     int last_status;
     int rhs_pending;
     hipStream_t rhs_stream;
+    // radial transform
+    double* Mt;          // transposed QDHT matrix
+    double* Ek;          // [Nr][Nto] time series per k column
+    double* Er;          // [Nr][Nto] time series per r column
+    double* partial;     // [Nr][3] weaknorm partial sums
+    DInst* pw;           // pointwise parameters (e_scale = 1)
+    RadArgs* rargs;      // [0]: solver, [1]: single RHS
+    RadCtl* rctl;        // [0]: solver, [1]: single RHS
+    hipGraphExec_t rgraph;
 };
 
 static std::mutex g_mu;
@@ -1016,6 +1051,12 @@ static int upload(T** dst, const void* src, size_t count) {
 static int ctx_class(const lh_ctx* c);
 
 static bool supported(const lh_desc* d) {
+    if (d->transform == LH_TRANS_RADIAL) {
+        // field grid, Nr a multiple of the 64-wide contraction tile
+        return d->grid_kind == LH_GRID_REAL && d->n_r >= 64 && d->n_r <= 4096 && d->n_r % 64 == 0 &&
+               rad_size_ok(d->n_to) && d->n_t == d->n_to / 2 && d->n_omega == d->n_t / 2 + 1 && !d->raman &&
+               d->kerr != LH_KERR_ENV && d->qdht != nullptr;
+    }
     if (d->transform != LH_TRANS_MODEAVG || d->n_r != 1) return false;
     if (size_class_of(d->grid_kind, d->n_to, d->n_t) < 0) return false;
     if (d->grid_kind == LH_GRID_REAL) {
@@ -1039,7 +1080,10 @@ static double scale_to_freq(const lh_desc& d) {
     return (double)d.n_omega / (double)d.n_to;
 }
 
-static int ctx_class(const lh_ctx* c) { return size_class_of(c->d.grid_kind, c->d.n_to, c->d.n_t); }
+static int ctx_class(const lh_ctx* c) {
+    if (c->d.transform == LH_TRANS_RADIAL) return LH_CLASS_RADIAL;
+    return size_class_of(c->d.grid_kind, c->d.n_to, c->d.n_t);
+}
 
 static DInst base_inst(const lh_ctx* c) {
     DInst in;
@@ -1146,6 +1190,134 @@ static int prepare_instance(lh_ctx* c, const lh_rk_opts* o, bool want_saves, DIn
     return LH_OK;
 }
 
+// --------------------------------------------------------------------------
+// Radial transform: set-up, one RHS, solve
+// --------------------------------------------------------------------------
+static int rad_setup(lh_ctx* c, const lh_desc* d) {
+    const size_t nr = (size_t)d->n_r, nto = (size_t)d->n_to;
+    std::vector<double> mt(nr * nr);
+    for (size_t k = 0; k < nr; k++)
+        for (size_t n = 0; n < nr; n++) mt[k * nr + n] = d->qdht[n * nr + k];
+    int st = upload(&c->Mt, mt.data(), mt.size());
+    if (st) return st;
+    HIPCK(hipMalloc(&c->Ek, sizeof(double) * nto * nr));
+    HIPCK(hipMalloc(&c->Er, sizeof(double) * nto * nr));
+    HIPCK(hipMalloc(&c->partial, sizeof(double) * 3 * nr));
+    HIPCK(hipMalloc(&c->rargs, sizeof(RadArgs) * 2));
+    HIPCK(hipMalloc(&c->rctl, sizeof(RadCtl) * 2));
+    DInst pw = base_inst(c);
+    pw.e_scale = 1.0;  // the 1/Nto of the inverse transform is folded into the k→r contraction
+    return upload(&c->pw, &pw, 1);
+}
+
+static void rad_ctl_init(RadCtl* h, const lh_rk_opts* o) {
+    memset(h, 0, sizeof(*h));
+    if (o) {
+        h->t = h->tn = o->z0;
+        h->dt = h->dtn = o->init_dz;
+    }
+    for (int j = 0; j < 8; j++) h->perm[j] = j;
+    h->status = LH_OK;
+}
+
+static RadArgs rad_args(const lh_ctx* c, int slot) {
+    RadArgs a;
+    memset(&a, 0, sizeof(a));
+    const lh_desc& d = c->d;
+    a.n_omega = d.n_omega;
+    a.n_t = d.n_t;
+    a.n_to = d.n_to;
+    a.n_r = d.n_r;
+    a.sc_t = scale_to_time(d);
+    a.gemm_kr = d.qdht_inv / (double)d.n_to;  // ldiv!(Eto, QDHT, Eto) with irfft's 1/Nto
+    a.gemm_rk = d.qdht_fwd;                   // mul!(Pto, QDHT, Pto)
+    a.linop = c->linop;
+    a.outfac = c->outfac;
+    a.Mt = c->Mt;
+    a.pw = c->pw;
+    a.Ek = c->Ek;
+    a.Er = c->Er;
+    a.partial = c->partial;
+    a.ctl = c->rctl + slot;
+    return a;
+}
+
+// Full propagation of one radial context on stream s (buffers and save grid
+// prepared by prepare_instance).  The per-step launch sequence is captured
+// once per context as a graph and replayed until the control block says done.
+static int rad_solve(lh_ctx* c, const lh_rk_opts* o, bool want_saves, hipStream_t s, lh_stats* out) {
+    const double2* tw;
+    int st = twiddles(c->device, &tw);
+    if (st) return st;
+    const int n_to = c->d.n_to, n_r = c->d.n_r;
+    RadArgs a = rad_args(c, 0);
+    a.Y[0] = c->Y;
+    a.Y[1] = c->YN;
+    for (int j = 0; j < 7; j++) a.K[j] = c->K[j];
+    a.saves = want_saves ? c->saves : nullptr;
+    a.zgrid = c->zgrid;
+    a.zmax = o->zmax;
+    a.rtol = o->rtol;
+    a.atol = o->atol;
+    a.safety = o->safety;
+    a.max_dz = o->max_dz;
+    a.min_dz = o->min_dz;
+    a.repeat_limit = o->repeat_limit;
+    a.save_n = o->save_n;
+    RadCtl h;
+    rad_ctl_init(&h, o);
+    h.rhs_evals = 1;
+    HIPCK(hipStreamSynchronize(s));
+    HIPCK(hipMemcpy(c->rargs, &a, sizeof(a), hipMemcpyHostToDevice));
+    HIPCK(hipMemcpy(c->rctl, &h, sizeof(h), hipMemcpyHostToDevice));
+    HIPCK(hipMemcpyAsync(c->Y, c->Yin, sizeof(double2) * (size_t)c->n_state, hipMemcpyDeviceToDevice, s));
+    rad_rhs_dispatch(n_to, n_r, c->rargs, tw, -1, s);  // PreconStepper init: k1 = fbar(y0)
+    HIPCK(hipGetLastError());
+    HIPCK(hipMemcpyAsync(&h, c->rctl, sizeof(h), hipMemcpyDeviceToHost, s));
+    HIPCK(hipStreamSynchronize(s));
+    if (h.status != LH_OK) {
+        h.done = 1;
+        HIPCK(hipMemcpy(c->rctl, &h, sizeof(h), hipMemcpyHostToDevice));
+    }
+    if (!h.done && !c->rgraph) {
+        hipGraph_t g = nullptr;
+        HIPCK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        rad_step_dispatch(n_to, n_r, c->rargs, tw, s);
+        hipError_t e1 = hipGetLastError();
+        hipError_t e2 = hipStreamEndCapture(s, &g);
+        if (e1 != hipSuccess || e2 != hipSuccess || !g) {
+            if (g) (void)hipGraphDestroy(g);
+            return LH_E_HIP;
+        }
+        hipError_t e3 = hipGraphInstantiate(&c->rgraph, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e3 != hipSuccess) {
+            c->rgraph = nullptr;
+            return LH_E_HIP;
+        }
+    }
+    // replay in chunks; steps after completion return at once
+    int chunk = 4;
+    while (!h.done) {
+        for (int i = 0; i < chunk; i++) HIPCK(hipGraphLaunch(c->rgraph, s));
+        HIPCK(hipMemcpyAsync(&h, c->rctl, sizeof(h), hipMemcpyDeviceToHost, s));
+        HIPCK(hipStreamSynchronize(s));
+        if (chunk < 16) chunk *= 2;
+    }
+    if (h.ycur == 1)  // the final state s.yn lives in YN
+        HIPCK(hipMemcpyAsync(c->Y, c->YN, sizeof(double2) * (size_t)c->n_state, hipMemcpyDeviceToDevice, s));
+    HIPCK(hipStreamSynchronize(s));
+    out->steps = h.steps;
+    out->accepted = h.accepted;
+    out->rejected = h.rejected;
+    out->rhs_evals = h.rhs_evals;
+    out->status = h.status;
+    out->saved = h.saved;
+    out->z = h.tn;
+    out->dz = h.dtn;
+    return LH_OK;
+}
+
 extern "C" {
 
 const char* lh_strerror(int status) {
@@ -1180,6 +1352,10 @@ int lh_destroy(lh_ctx* c) {
         if (p) hipFree(p);
     for (int j = 0; j < 7; j++)
         if (c->K[j]) hipFree(c->K[j]);
+    void* rptrs[] = {c->Mt, c->Ek, c->Er, c->partial, c->pw, c->rargs, c->rctl};
+    for (void* p : rptrs)
+        if (p) hipFree(p);
+    if (c->rgraph) hipGraphExecDestroy(c->rgraph);
     if (c->stream) hipStreamDestroy(c->stream);
     free(c);
     return LH_OK;
@@ -1247,6 +1423,7 @@ int lh_create(lh_ctx** out, const lh_desc* d) {
         }
         if (dh) (void)hipFree(dh);
     }
+    if (!st && d->transform == LH_TRANS_RADIAL) st = rad_setup(c, d);
     // host pointers inside the copied descriptor belong to the caller
     c->d.omega = nullptr; c->d.sidx = nullptr; c->d.omega_win = nullptr; c->d.t_win = nullptr;
     c->d.to_win = nullptr; c->d.linop = nullptr; c->d.norm = nullptr; c->d.ppt_x = nullptr;
@@ -1272,7 +1449,8 @@ int lh_ctx_status(lh_ctx* c) {
         int kst = LH_OK;
         HIPCK(hipSetDevice(c->device));
         HIPCK(hipStreamSynchronize(c->rhs_stream ? c->rhs_stream : c->stream));
-        HIPCK(hipMemcpy(&kst, c->dstatus, sizeof(int), hipMemcpyDeviceToHost));
+        const int* sp = (c->d.transform == LH_TRANS_RADIAL) ? &c->rctl[1].status : c->dstatus;
+        HIPCK(hipMemcpy(&kst, sp, sizeof(int), hipMemcpyDeviceToHost));
         c->last_status = kst;
         c->rhs_pending = 0;
     }
@@ -1287,11 +1465,22 @@ int lh_rhs_device(lh_ctx* c, void* nl_dev, const void* Ew_dev, double z, void* s
     const double2* tw;
     int st = twiddles(c->device, &tw);
     if (st) return st;
-    DInst in = base_inst(c);
-    in.rhs_in = (const double2*)Ew_dev;
-    in.rhs_out = (double2*)nl_dev;
-    HIPCK(hipMemcpyAsync(c->dinst, &in, sizeof(DInst), hipMemcpyHostToDevice, s));
-    launch_rhs(ctx_class(c), c->dinst, tw, c->dstatus, s);
+    if (c->d.transform == LH_TRANS_RADIAL) {
+        RadArgs a = rad_args(c, 1);
+        a.rhs_in = (const double2*)Ew_dev;
+        a.rhs_out = (double2*)nl_dev;
+        RadCtl h;
+        rad_ctl_init(&h, nullptr);
+        HIPCK(hipMemcpyAsync(c->rargs + 1, &a, sizeof(a), hipMemcpyHostToDevice, s));
+        HIPCK(hipMemcpyAsync(c->rctl + 1, &h, sizeof(h), hipMemcpyHostToDevice, s));
+        rad_rhs_dispatch(c->d.n_to, c->d.n_r, c->rargs + 1, tw, -1, s);
+    } else {
+        DInst in = base_inst(c);
+        in.rhs_in = (const double2*)Ew_dev;
+        in.rhs_out = (double2*)nl_dev;
+        HIPCK(hipMemcpyAsync(c->dinst, &in, sizeof(DInst), hipMemcpyHostToDevice, s));
+        launch_rhs(ctx_class(c), c->dinst, tw, c->dstatus, s);
+    }
     HIPCK(hipGetLastError());
     c->rhs_pending = 1;
     c->rhs_stream = s;
@@ -1311,7 +1500,8 @@ int lh_rhs(lh_ctx* c, double* nl, const double* Ew, double z) {
     if (st) return st;
     int kst = LH_OK;
     HIPCK(hipMemcpyAsync(nl, c->rhs_out, bytes, hipMemcpyDeviceToHost, c->stream));
-    HIPCK(hipMemcpyAsync(&kst, c->dstatus, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    const int* sp = (c->d.transform == LH_TRANS_RADIAL) ? &c->rctl[1].status : c->dstatus;
+    HIPCK(hipMemcpyAsync(&kst, sp, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIPCK(hipStreamSynchronize(c->stream));
     c->last_status = kst;
     c->rhs_pending = 0;
@@ -1391,6 +1581,7 @@ int lh_batch_solve_resident(lh_ctx* const* ctxs, int n, const lh_rk_opts* o, int
     for (int i = 0; i < n && !st; i++) st = prepare_instance(ctxs[i], o, want_saves != 0, &h[pos[i]], dstats + pos[i], s0);
     if (!st && hipMemcpyAsync(dinst, h.data(), sizeof(DInst) * (size_t)n, hipMemcpyHostToDevice, s0) != hipSuccess)
         st = LH_E_HIP;
+    std::vector<lh_stats> rad_stats(n);
     hipEvent_t ready = nullptr, done[LH_NCLASS] = {nullptr}, t0[LH_NCLASS] = {nullptr}, t1[LH_NCLASS] = {nullptr};
     hipStream_t cs[LH_NCLASS] = {nullptr};
     if (!st && hipEventCreateWithFlags(&ready, hipEventDisableTiming) != hipSuccess) st = LH_E_HIP;
@@ -1407,7 +1598,15 @@ int lh_batch_solve_resident(lh_ctx* const* ctxs, int n, const lh_rk_opts* o, int
             st = LH_E_HIP;
             break;
         }
-        launch_solve(c, count[c], dinst + first[c], tw, cs[c]);
+        if (c == LH_CLASS_RADIAL) {
+            // context by context (each fills the GPU); field classes run meanwhile
+            for (int i = 0; i < n && !st; i++)
+                if (ctx_class(ctxs[i]) == LH_CLASS_RADIAL)
+                    st = rad_solve(ctxs[i], o, want_saves != 0, cs[c], &rad_stats[i]);
+            if (st) break;
+        } else {
+            launch_solve(c, count[c], dinst + first[c], tw, cs[c]);
+        }
         if (hipGetLastError() != hipSuccess || hipEventRecord(t1[c], cs[c]) != hipSuccess ||
             hipEventRecord(done[c], cs[c]) != hipSuccess || hipStreamWaitEvent(s0, done[c], 0) != hipSuccess)
             st = LH_E_HIP;
@@ -1435,7 +1634,7 @@ int lh_batch_solve_resident(lh_ctx* const* ctxs, int n, const lh_rk_opts* o, int
     (void)hipFree(dstats);
     if (st) return st;
     for (int i = 0; i < n; i++) {
-        stats[i] = hs[pos[i]];
+        stats[i] = (ctx_class(ctxs[i]) == LH_CLASS_RADIAL) ? rad_stats[i] : hs[pos[i]];
         ctxs[i]->last_status = stats[i].status;
         ctxs[i]->have_result = 1;
     }
@@ -1443,7 +1642,7 @@ int lh_batch_solve_resident(lh_ctx* const* ctxs, int n, const lh_rk_opts* o, int
 }
 
 int lh_last_launch_times(double* ms, int* n_inst, int nclass) {
-    if (!ms || !n_inst || nclass < LH_NCLASS) return LH_E_ARG;  // LH_NCLASS = 5
+    if (!ms || !n_inst || nclass < LH_NCLASS) return LH_E_ARG;  // LH_NCLASS = 6
     for (int c = 0; c < LH_NCLASS; c++) {
         ms[c] = g_last_ms[c];
         n_inst[c] = g_last_n[c];

@@ -1,0 +1,388 @@
+// lh_radial.hip -- radially symmetric propagation (TransRadial) on gfx950.
+//
+// Included by lh_kernels.hip (shares its device helpers).  The state is
+// Eω(ω, k) with Nω rows and Nr columns (column-major, ω fastest).  One RHS
+// (src/NonlinearRHS.jl:466-474) is a sequence of launches:
+//   A  per k-column: stage prologue, zero-pad, c2r (LDS)      → Ek (t, k)
+//   G  FP64 MFMA GEMM, k → r: Er = (K/R)/Nto · Ek · Mᵀ        → Er (t, r)
+//   C  per r-column: Kerr + PPT plasma scans, ×towin (LDS)    → Er (in place)
+//   G  FP64 MFMA GEMM, r → k: Ek = (R/K) · Er · Mᵀ            → Ek (t, k)
+//   E  per k-column: r2c, crop, ×output factor, ×exp(L hb)  → k_{i+1}
+// and a step appends a per-column combination with partial weaknorm sums,
+// a one-workgroup controller, the per-column window filter and the saves.
+// Every kernel reads the step scalars from a device-side control block, so
+// one step is a fixed launch sequence that is captured once as a hipGraph and
+// replayed; once the run is done every kernel returns immediately.
+//
+// QDHT (Hankel.jl, after Guizar-Sicairos & Gutiérrez-Vega, JOSA A 21, 53
+// (2004)): j_n zeros of J0, S = j_{N+1}, r_n = j_n R/S, k_n = j_n/R, K = S/R,
+// M_nm = (2/S) J0(j_n j_m/S) / J1(j_m)²;  r → k:  Y = (R/K) M A,
+// k → r:  A = (K/R) M Y  (M² = I to the accuracy of the discrete orthogonality).
+
+typedef double lh_d4 __attribute__((ext_vector_type(4)));
+
+struct RadCtl {
+    double t, tn, dt, dtn, errlast, err, h0;
+    long long steps, accepted, rejected, rhs_evals;
+    int status, repeated, saved, save_from, save_to, ok, done, ycur;
+    int perm[8];
+};
+
+struct RadArgs {
+    int n_omega, n_t, n_to, n_r;
+    double sc_t, gemm_kr, gemm_rk;
+    const double2* linop;   // [Nr][Nω]
+    const double2* outfac;  // [Nr][Nω]
+    const double* Mt;       // [Nr][Nr] transposed QDHT matrix: Mt[k*Nr + n] = M[n][k]
+    const DInst* pw;        // scalar parameters + windows + PPT table (e_scale = 1)
+    double2* Y[2];
+    double2* K[7];
+    double* Ek;             // [Nr][Nto] (t fastest)
+    double* Er;             // [Nr][Nto]
+    double* partial;        // [Nr][3]
+    double2* saves;         // [save_n][Nr][Nω] or null
+    const double* zgrid;
+    RadCtl* ctl;
+    double zmax, rtol, atol, safety, max_dz, min_dz;
+    int repeat_limit, save_n;
+    // single RHS mode: input/output columns
+    const double2* rhs_in;
+    double2* rhs_out;
+};
+
+// ---------------------------------------------------------------- A: prologue + c2r
+template <int MO>
+__global__ __launch_bounds__(LH_T) void rad_stage_a(const RadArgs* __restrict__ ap, const double2* __restrict__ twg,
+                                                    int stage) {
+    const RadArgs& a = *ap;
+    const RadCtl& ctl = *a.ctl;
+    if (ctl.done) return;
+    __shared__ SmemT<MO> sm;
+    tw_load(sm.tw, twg);
+    const int c = blockIdx.x, N = a.n_omega;
+    const size_t off = (size_t)N * c;
+    double2* Kc[7];
+    for (int j = 0; j < 7; j++) Kc[j] = a.K[ctl.perm[j]] + off;
+    if (stage < 0) {  // plain RHS of the input column (PreconStepper init / single RHS)
+        const double2* src = (a.rhs_in ? a.rhs_in : a.Y[ctl.ycur]) + off;
+        for (int k = threadIdx.x; k < N; k += LH_T) sm.buf[pad(k)] = cscal(a.sc_t, src[k]);
+        for (int k = N + threadIdx.x; k <= MO; k += LH_T) sm.buf[pad(k)] = make_double2(0.0, 0.0);
+    } else {
+        const double t = ctl.t, dt = ctl.dt;
+        const double t2 = t + c_nodes[stage] * dt;
+        const double h0 = (stage == 0) ? ctl.h0 : 0.0;
+        stage_prologue_cols<MO, SmemT<MO>>(a.linop + off, a.sc_t, sm, a.Y[ctl.ycur] + off, Kc, stage, dt, h0,
+                                           t2 - t);
+    }
+    __syncthreads();
+    c2r_prepack<MO>(sm.buf, sm.tw);
+    __syncthreads();
+    fft_lds<MO, true>(sm.buf, sm.tw);
+    double* out = a.Ek + (size_t)a.n_to * c;
+    for (int j = threadIdx.x; j < MO; j += LH_T) {
+        double2 z = sm.buf[pad(j)];
+        out[2 * j] = z.x;
+        out[2 * j + 1] = z.y;
+    }
+}
+
+// ---------------------------------------------------------------- G: FP64 MFMA GEMM
+// C[M×N] = scale · A[M×K] · B[K×N], A and C column-major (t fastest),
+// B[k][n] = Mt[k*Nr + n] = M[n][k] (row-major, so 16 lanes load 16 consecutive n).
+// Workgroup tile 64×64 (2×2 waves of 32×32, each 2×2 MFMA 16×16×4 tiles).
+// v_mfma_f64_16x16x4f64 lane layout: A lane l = A[l%16][l/16],
+// B lane l = B[l/16][l%16], D lane l reg r = D[4r + l/16][l%16].
+__global__ __launch_bounds__(256) void rad_gemm(const RadArgs* __restrict__ ap, int dir) {
+    const RadArgs& a = *ap;
+    if (a.ctl->done) return;
+    const double* __restrict__ A = dir == 0 ? a.Ek : a.Er;
+    double* __restrict__ C = dir == 0 ? a.Er : a.Ek;
+    const double* __restrict__ B = a.Mt;
+    const double scale = dir == 0 ? a.gemm_kr : a.gemm_rk;
+    const int M = a.n_to, K = a.n_r;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int m0 = blockIdx.x * 64 + (wave & 1) * 32;
+    const int n0 = blockIdx.y * 64 + (wave >> 1) * 32;
+    const int lr = lane & 15, lk = lane >> 4;
+    lh_d4 acc00 = {0, 0, 0, 0}, acc01 = acc00, acc10 = acc00, acc11 = acc00;
+    const double* Ap0 = A + (m0 + lr);
+    const double* Bp0 = B + (n0 + lr);
+    const double* Bp1 = B + (n0 + 16 + lr);
+#pragma unroll 4
+    for (int k0 = 0; k0 < K; k0 += 4) {
+        const size_t ka = (size_t)M * (k0 + lk);
+        double a0 = Ap0[ka], a1 = Ap0[ka + 16];
+        const size_t kb = (size_t)K * (k0 + lk);
+        double b0 = Bp0[kb], b1 = Bp1[kb];
+        acc00 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc00, 0, 0, 0);
+        acc01 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc01, 0, 0, 0);
+        acc10 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc10, 0, 0, 0);
+        acc11 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc11, 0, 0, 0);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        const int row = m0 + 4 * r + lk;
+        C[row + (size_t)M * (n0 + lr)] = scale * acc00[r];
+        C[row + (size_t)M * (n0 + 16 + lr)] = scale * acc01[r];
+        C[row + 16 + (size_t)M * (n0 + lr)] = scale * acc10[r];
+        C[row + 16 + (size_t)M * (n0 + 16 + lr)] = scale * acc11[r];
+    }
+}
+
+// ---------------------------------------------------------------- C: responses per r
+template <int MO>
+__global__ __launch_bounds__(LH_T) void rad_pointwise(const RadArgs* __restrict__ ap) {
+    const RadArgs& a = *ap;
+    if (a.ctl->done) return;
+    __shared__ SmemT<MO> sm;
+    if (threadIdx.x == 0) sm.flag = LH_OK;
+    double* col = a.Er + (size_t)a.n_to * blockIdx.x;
+    for (int j = threadIdx.x; j < MO; j += LH_T) sm.buf[pad(j)] = make_double2(col[2 * j], col[2 * j + 1]);
+    __syncthreads();
+    pointwise_field<MO, SmemT<MO>>(*a.pw, sm);
+    __syncthreads();
+    for (int j = threadIdx.x; j < MO; j += LH_T) {
+        double2 z = sm.buf[pad(j)];
+        col[2 * j] = z.x;
+        col[2 * j + 1] = z.y;
+    }
+    if (threadIdx.x == 0 && sm.flag != LH_OK) atomicCAS(&a.ctl->status, LH_OK, sm.flag);
+}
+
+// ---------------------------------------------------------------- E: r2c + epilogue
+template <int MO>
+__global__ __launch_bounds__(LH_T) void rad_stage_e(const RadArgs* __restrict__ ap, const double2* __restrict__ twg,
+                                                    int stage) {
+    const RadArgs& a = *ap;
+    const RadCtl& ctl = *a.ctl;
+    if (ctl.done) return;
+    __shared__ SmemT<MO> sm;
+    tw_load(sm.tw, twg);
+    const int c = blockIdx.x, N = a.n_omega;
+    const size_t off = (size_t)N * c;
+    const double* col = a.Ek + (size_t)a.n_to * c;
+    for (int j = threadIdx.x; j < MO; j += LH_T) sm.buf[pad(j)] = make_double2(col[2 * j], col[2 * j + 1]);
+    __syncthreads();
+    fft_lds<MO, false>(sm.buf, sm.tw);
+    const double2* of = a.outfac + off;
+    if (stage < 0) {
+        double2* out = (a.rhs_out ? a.rhs_out : a.K[ctl.perm[0]]) + off;
+        for (int k = threadIdx.x; k < N; k += LH_T) out[k] = cmul(r2c_bin<MO>(sm.buf, sm.tw, k), of[k]);
+    } else {
+        const double t = ctl.t, t2 = t + c_nodes[stage] * ctl.dt, hb = t - t2;
+        double2* out = a.K[ctl.perm[stage + 1]] + off;
+        const double2* L = a.linop + off;
+        for (int k = threadIdx.x; k < N; k += LH_T)
+            out[k] = cmul(cmul(r2c_bin<MO>(sm.buf, sm.tw, k), of[k]), cexp_lh(L[k], hb));
+    }
+}
+
+// ---------------------------------------------------------------- combination
+__global__ __launch_bounds__(LH_T) void rad_combine(const RadArgs* __restrict__ ap) {
+    const RadArgs& a = *ap;
+    const RadCtl& ctl = *a.ctl;
+    if (ctl.done) return;
+    __shared__ double scratch[3 * LH_NW];
+    const int c = blockIdx.x, N = a.n_omega;
+    const size_t off = (size_t)N * c;
+    const double dt = ctl.dt;
+    const double2* Y = a.Y[ctl.ycur] + off;
+    double2* YN = a.Y[ctl.ycur ^ 1] + off;
+    double sy = 0.0, syn = 0.0, se = 0.0;
+    for (int k = threadIdx.x; k < N; k += LH_T) {
+        double2 yv = Y[k], yn = yv, ye = make_double2(0.0, 0.0);
+        for (int j = 0; j < 7; j++) {
+            if (j == 1) continue;
+            double2 kv = a.K[ctl.perm[j]][off + k];
+            yn = cadd(yn, cscal(dt * c_b5[j], kv));
+            ye = cadd(ye, cscal(c_b5[j] - c_b4[j], cscal(dt, kv)));
+        }
+        sy += yv.x * yv.x + yv.y * yv.y;
+        syn += yn.x * yn.x + yn.y * yn.y;
+        se += ye.x * ye.x + ye.y * ye.y;
+        YN[k] = yn;
+    }
+    block_sum3(sy, syn, se, scratch);
+    if (threadIdx.x == 0) {
+        a.partial[3 * c + 0] = sy;
+        a.partial[3 * c + 1] = syn;
+        a.partial[3 * c + 2] = se;
+    }
+}
+
+// ---------------------------------------------------------------- controller
+// weaknorm over all columns (fixed order), stepcontrolPI!, steplims!, FSAL,
+// save bookkeeping (src/RK45.jl:167-192, 343-406; src/Output.jl:497-502).
+__global__ void rad_control(const RadArgs* __restrict__ ap) {
+    const RadArgs& a = *ap;
+    RadCtl* ctl = a.ctl;
+    if (threadIdx.x != 0 || ctl->done) return;
+    RadCtl s = *ctl;
+    s.rhs_evals += 6;
+    if (s.status != LH_OK) {
+        s.done = 1;
+        *ctl = s;
+        return;
+    }
+    double sy = 0.0, syn = 0.0, se = 0.0;
+    for (int c = 0; c < a.n_r; c++) {
+        sy += a.partial[3 * c + 0];
+        syn += a.partial[3 * c + 1];
+        se += a.partial[3 * c + 2];
+    }
+    double errwt = fmax(fmax(sqrt(sy), sqrt(syn)), a.atol);
+    double err = sqrt(se) / a.rtol / errwt;
+    bool ok = err <= 1.0;
+    const double beta1 = 3.0 / 5 / 5, beta2 = -1.0 / 5 / 5, eps = 0.8;
+    double dtn;
+    if (ok) {
+        if (s.errlast == 0.0) s.errlast = err;
+        double fac = (err == 0.0) ? 1.5 : a.safety * pow(eps / err, beta1) * pow(eps / s.errlast, beta2);
+        dtn = fac * s.dt;
+        s.errlast = err;
+    } else {
+        dtn = !isfinite(err) ? s.dt / 2 : s.dt * fmax(0.1, a.safety * pow(err, -1.0 / 5));
+    }
+    if (dtn > a.max_dz)
+        dtn = a.max_dz;
+    else if (dtn < a.min_dz) {
+        dtn = a.min_dz;
+        ok = true;
+    }
+    s.err = err;
+    s.dtn = dtn;
+    s.steps++;
+    s.ok = ok;
+    if (ok) {
+        s.tn = s.t + s.dt;
+        int tmp = s.perm[0];  // ks[1] .= ks[7]
+        s.perm[0] = s.perm[6];
+        s.perm[6] = tmp;
+        s.accepted++;
+        s.repeated = 0;
+        s.save_from = s.saved;
+        while (s.saved < a.save_n && a.zgrid[s.saved] <= s.tn) s.saved++;
+        s.save_to = s.saved;
+    } else {
+        s.rejected++;
+        s.repeated++;
+        s.save_from = s.save_to = s.saved;
+        if (s.repeated > a.repeat_limit) {
+            s.status = LH_E_REPEAT_LIMIT;
+            s.done = 1;
+        }
+    }
+    *ctl = s;
+}
+
+// ---------------------------------------------------------------- stepfun per column
+template <int MC>
+__global__ __launch_bounds__(LH_T) void rad_stepfun(const RadArgs* __restrict__ ap, const double2* __restrict__ twg) {
+    const RadArgs& a = *ap;
+    const RadCtl& ctl = *a.ctl;
+    if (ctl.done || !ctl.ok) return;
+    __shared__ SmemT<MC> sm;
+    tw_load(sm.tw, twg);
+    __syncthreads();
+    const size_t off = (size_t)a.n_omega * blockIdx.x;
+    stepfun_filter_cols<MC>(*a.pw, a.linop + off, sm, a.Y[ctl.ycur ^ 1] + off, ctl.tn - ctl.t);
+}
+
+// ---------------------------------------------------------------- saves per column
+__global__ __launch_bounds__(LH_T) void rad_saves(const RadArgs* __restrict__ ap) {
+    const RadArgs& a = *ap;
+    const RadCtl& ctl = *a.ctl;
+    if (ctl.done || !ctl.ok || !a.saves) return;
+    const int c = blockIdx.x, N = a.n_omega;
+    const size_t off = (size_t)N * c, nst = (size_t)N * a.n_r;
+    const double2* Y = a.Y[ctl.ycur] + off;
+    const double2* YN = a.Y[ctl.ycur ^ 1] + off;
+    const double t = ctl.t, tn = ctl.tn, dt = ctl.dt;
+    for (int sidx = ctl.save_from; sidx < ctl.save_to; sidx++) {
+        const double ti = a.zgrid[sidx];
+        double2* out = a.saves + nst * sidx + off;
+        if (ti == t) {
+            for (int k = threadIdx.x; k < N; k += LH_T) out[k] = Y[k];
+        } else if (ti == tn) {
+            for (int k = threadIdx.x; k < N; k += LH_T) out[k] = YN[k];
+        } else {
+            double sg = (ti - t) / dt;
+            double sp[4] = {sg, pow(sg, 2.0), pow(sg, 3.0), pow(sg, 4.0)};
+            double b[7];
+            for (int j = 0; j < 7; j++) {
+                double acc = sp[0] * c_iC[0][j];
+                for (int p = 1; p < 4; p++) acc = acc + sp[p] * c_iC[p][j];
+                b[j] = acc;
+            }
+            const double h = ti - t;
+            for (int k = threadIdx.x; k < N; k += LH_T) {
+                double2 yi = make_double2(0.0, 0.0);
+                for (int j = 0; j < 7; j++) {
+                    if (j == 1) continue;
+                    const double2* kj = a.K[ctl.perm[j == 6 ? 0 : j]] + off;
+                    yi = cadd(yi, cscal(b[j], kj[k]));
+                }
+                out[k] = cmul(cadd(Y[k], cscal(dt, yi)), cexp_lh(a.linop[off + k], h));
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------- step end
+__global__ void rad_step_end(const RadArgs* __restrict__ ap) {
+    const RadArgs& a = *ap;
+    RadCtl* ctl = a.ctl;
+    if (threadIdx.x != 0 || ctl->done) return;
+    RadCtl s = *ctl;
+    if (s.ok) s.ycur ^= 1;  // y ← yn
+    s.h0 = s.tn - s.t;      // k1 is carried to the new t (zero after a rejection)
+    s.t = s.tn;
+    s.dt = s.dtn;
+    if (!(s.tn <= a.zmax)) {
+        s.done = 1;
+    } else if (s.steps >= LH_MAX_STEPS) {
+        s.status = LH_E_MAXSTEPS_INTERNAL;
+        s.done = 1;
+    }
+    *ctl = s;
+}
+
+// Launch helpers for one RHS and one step (also used during graph capture).
+template <int MO>
+static void rad_launch_rhs(const RadArgs* dargs, const double2* tw, int n_to, int n_r, int stage, hipStream_t s) {
+    dim3 g(n_to / 64, n_r / 64);
+    hipLaunchKernelGGL((rad_stage_a<MO>), dim3(n_r), dim3(LH_T), 0, s, dargs, tw, stage);
+    hipLaunchKernelGGL(rad_gemm, g, dim3(256), 0, s, dargs, 0);
+    hipLaunchKernelGGL((rad_pointwise<MO>), dim3(n_r), dim3(LH_T), 0, s, dargs);
+    hipLaunchKernelGGL(rad_gemm, g, dim3(256), 0, s, dargs, 1);
+    hipLaunchKernelGGL((rad_stage_e<MO>), dim3(n_r), dim3(LH_T), 0, s, dargs, tw, stage);
+}
+
+template <int MO, int MC>
+static void rad_launch_step(const RadArgs* dargs, const double2* tw, int n_to, int n_r, hipStream_t s) {
+    for (int st = 0; st < 6; st++) rad_launch_rhs<MO>(dargs, tw, n_to, n_r, st, s);
+    hipLaunchKernelGGL(rad_combine, dim3(n_r), dim3(LH_T), 0, s, dargs);
+    hipLaunchKernelGGL(rad_control, dim3(1), dim3(64), 0, s, dargs);
+    hipLaunchKernelGGL((rad_stepfun<MC>), dim3(n_r), dim3(LH_T), 0, s, dargs, tw);
+    hipLaunchKernelGGL(rad_saves, dim3(n_r), dim3(LH_T), 0, s, dargs);
+    hipLaunchKernelGGL(rad_step_end, dim3(1), dim3(64), 0, s, dargs);
+}
+
+// Oversampled lengths with compiled radial kernels (Nt = Nto/2, Nω = Nt/2+1).
+static bool rad_size_ok(int n_to) { return n_to == 2048 || n_to == 4096 || n_to == 8192; }
+
+static void rad_rhs_dispatch(int n_to, int n_r, const RadArgs* a, const double2* tw, int stage, hipStream_t s) {
+    switch (n_to) {
+        case 8192: rad_launch_rhs<4096>(a, tw, n_to, n_r, stage, s); break;
+        case 4096: rad_launch_rhs<2048>(a, tw, n_to, n_r, stage, s); break;
+        default: rad_launch_rhs<1024>(a, tw, n_to, n_r, stage, s); break;
+    }
+}
+
+static void rad_step_dispatch(int n_to, int n_r, const RadArgs* a, const double2* tw, hipStream_t s) {
+    switch (n_to) {
+        case 8192: rad_launch_step<4096, 2048>(a, tw, n_to, n_r, s); break;
+        case 4096: rad_launch_step<2048, 1024>(a, tw, n_to, n_r, s); break;
+        default: rad_launch_step<1024, 512>(a, tw, n_to, n_r, s); break;
+    }
+}

@@ -84,3 +84,65 @@ def energy_t(grid, Et):
     dt = grid.t[1] - grid.t[0]
     It = np.abs(hilbert(Et)) ** 2 if grid.kind == "real" else np.abs(Et) ** 2
     return simpson_even(dt, It)
+
+
+# --------------------------------------------------------------------------
+# Radially symmetric inputs on a QDHT grid (field grid only)
+# --------------------------------------------------------------------------
+def gauss_sigma(x, sigma):
+    """``Maths.gauss(x, σ)``."""
+    return np.exp(-0.5 * (x / sigma) ** 2)
+
+
+def energy_t_radial(grid, q, Et):
+    """``energyfuncs(grid::RealGrid, q::QDHT)[1]``, ``Fields.jl:536-541``; Et is (Nt, Nr)."""
+    dt = grid.t[1] - grid.t[0]
+    Eta = hilbert(Et, axis=0)
+    tintg = simpson_even(dt, np.abs(Eta) ** 2)
+    return 2 * math.pi * pd.c * pd.eps_0 / 2 * q.integrateR(tintg)
+
+
+def energy_omega_radial(grid, q, Ew):
+    """``energyfuncs(grid::RealGrid, q::QDHT)[2]``, ``Fields.jl:543-548``; Ew is (Nω, Nr)."""
+    dw = grid.omega[1] - grid.omega[0]
+    prefac = 2 * math.pi * pd.c * pd.eps_0 / 2 * 2 * math.pi / (grid.omega[-1] ** 2)
+    wintg = simpson_even(dw, np.abs(Ew) ** 2)
+    return prefac * q.integrateK(wintg)
+
+
+def prop_radial(Ewk, z, grid, q):
+    """Free-space propagation by z in (ω, k) space, ``Fields.jl:401-406`` (in place)."""
+    k0 = grid.omega / pd.c
+    kzsq = k0[:, None] ** 2 - (q.k ** 2)[None, :]
+    kzsq[kzsq < 0] = 0
+    kz = np.sqrt(kzsq)
+    Ewk *= np.exp(-1j * z * (kz - k0[:, None]))
+    return Ewk
+
+
+class GaussGaussField:
+    """``Fields.GaussGaussField`` (``Fields.jl:343-364``) evaluated as a
+    ``SpatioTemporalField`` (``Fields.jl:366-398``) on a field grid and QDHT:
+    Gaussian in time (intensity FWHM τfwhm, super-Gaussian order m) and in r
+    (waist w0), scaled to ``energy``, transformed to (ω, k) and propagated by
+    ``propz`` from the waist."""
+
+    def __init__(self, lam0, tau_fwhm, energy, w0, phi=0.0, tau0=0.0, m=1, propz=0.0):
+        self.lam0, self.tau_fwhm, self.energy, self.w0 = lam0, tau_fwhm, energy, w0
+        self.phi, self.tau0, self.m, self.propz = phi, tau0, m, propz
+
+    def make_Etr(self, grid, q):
+        if grid.kind != "real":
+            raise NotImplementedError("radial inputs are built on the field grid")
+        t = grid.t - self.tau0
+        w0 = pd.wlfreq(self.lam0)
+        I = gauss(t, self.tau_fwhm, power=2 * self.m)[:, None] * gauss_sigma(q.r, self.w0 / 2)[None, :]
+        return np.sqrt(I) * np.cos(w0 * t + self.phi)[:, None]
+
+    def __call__(self, grid, q):
+        Etr = self.make_Etr(grid, q)
+        Etr = Etr * (math.sqrt(self.energy) / math.sqrt(energy_t_radial(grid, q, Etr)))
+        Ewk = q.forward(np.fft.rfft(Etr, axis=0))
+        if self.propz != 0.0:
+            prop_radial(Ewk, self.propz, grid, q)
+        return Ewk

@@ -18,7 +18,7 @@ from .linops import make_const_linop
 from .modes import MarcatiliMode, SimpleMode
 from .nonlinear import (CombinedRamanResponse, Kerr_env, Kerr_field, PlasmaCumtrapz, RamanPolarEnv,
                         RamanRespNormedSingleDampedOscillator)
-from .nonlinearrhs import TransModeAvg, norm_mode_average, norm_mode_average_gnlse
+from .nonlinearrhs import TransModeAvg, TransRadial, norm_mode_average, norm_mode_average_gnlse
 
 
 class MemoryOutput:
@@ -46,12 +46,24 @@ def run(Ew, grid, linop, transform, FT=None, output=None, min_dz=0.0, max_dz=Non
     Ew_out, saves, zs, stats = ctx.solve(Ew, opts)
     if output is not None:
         n = grid.n_omega
-        output.data["Eω"] = saves.reshape(saveN, n, -1).transpose(1, 2, 0).squeeze(1) if saves.ndim == 2 else saves
+        # saves[s] holds the state column-major (ω fastest): → (Nω[, Nr], saveN)
+        output.data["Eω"] = saves.reshape(saveN, -1, n).transpose(2, 1, 0).squeeze(1)
         output.data["z"] = zs[:saveN]
         output.data["grid"] = {k: getattr(grid, k) for k in ("t", "omega", "to", "omega_o", "sidx",
                                                               "omega_win", "t_win", "to_win")}
         output.data["stats"] = stats.as_dict()
     return Ew_out, stats
+
+
+def setup_radial(grid, q, densityfun, normfun, responses, inputs):
+    """``Luna.setup(grid::RealGrid, q::QDHT, densityfun, normfun, responses, inputs)``,
+    ``Luna.jl:239-258``: returns ``(Eωk, transform)``; inputs are summed in (ω, k)."""
+    inputs = inputs if isinstance(inputs, (list, tuple)) else [inputs]
+    Ewk = np.zeros((grid.n_omega, q.N), dtype=np.complex128)
+    for f in inputs:
+        Ewk = Ewk + f(grid, q)
+    transform = TransRadial(grid, q, responses, densityfun, normfun)
+    return Ewk, transform
 
 
 def prop_capillary_args(radius, flength, gas, pressure, *, lambda0, lambda_lims, trange,

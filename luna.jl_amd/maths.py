@@ -110,20 +110,21 @@ def julia_range(start, stop, length):
 
 
 def simpson_even(dx, y):
-    """NumericalIntegration.jl ``SimpsonEven`` on a uniform grid (extended Simpson rule)."""
+    """NumericalIntegration.jl ``SimpsonEven`` on a uniform grid (extended Simpson rule),
+    along the first axis."""
     y = np.asarray(y)
     ret = (17 * y[0] + 59 * y[1] + 43 * y[2] + 49 * y[3]
            + 49 * y[-4] + 43 * y[-3] + 59 * y[-2] + 17 * y[-1]) / 48
-    ret += np.sum(y[4:-4])
+    ret = ret + np.sum(y[4:-4], axis=0)
     return dx * ret
 
 
-def hilbert(x):
-    """Analytic signal via FFT, ``Maths.jl:560-569``."""
-    x = np.asarray(x, dtype=float)
-    xf = np.fft.fft(x)
-    n = len(xf)
+def hilbert(x, axis=0):
+    """Analytic signal via FFT along ``axis``, ``Maths.jl:560-569``."""
+    x = np.moveaxis(np.asarray(x, dtype=float), axis, 0)
+    xf = np.fft.fft(x, axis=0)
+    n = xf.shape[0]
     n1 = n // 2
     xf[1:n1] *= 2
     xf[n1:] = 0
-    return np.fft.ifft(xf)
+    return np.moveaxis(np.fft.ifft(xf, axis=0), 0, axis)

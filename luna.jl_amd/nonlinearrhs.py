@@ -51,6 +51,29 @@ def norm_mode_average_gnlse(grid, aeff, shock=True):
     return f
 
 
+def norm_radial(grid, q, nfun_omega, z=0.0):
+    """Real (Nω, Nr) factor of ``norm_radial``, ``NonlinearRHS.jl:497-524``:
+    kz/(μ0 ω) where kz² = (n ω/c)² - k_r² > 0, else 1 (also at ω = 0)."""
+    w = grid.omega
+    k2 = np.zeros(len(w))
+    s = grid.sidx
+    k2[s] = (np.real(nfun_omega(w[s])) * w[s] / pd.c) ** 2
+    out = np.ones((len(w), q.N))
+    kr2 = q.k ** 2
+    for ir in range(q.N):
+        bsq = k2 - kr2[ir]
+        m = (w != 0) & (bsq > 0)
+        out[m, ir] = np.sqrt(bsq[m]) / (pd.mu_0 * w[m])
+    return out
+
+
+def const_norm_radial(grid, q, nfun_lambda):
+    """``const_norm_radial(grid, q, nfun)``, ``NonlinearRHS.jl:482-493``: nfun of
+    wavelength, factor evaluated once and returned by a closure of z."""
+    out = norm_radial(grid, q, lambda w: nfun_lambda(pd.wlfreq(w)))
+    return lambda z: out
+
+
 class _TransformBase:
     kind = None
 

@@ -97,7 +97,8 @@ class Oracle:
         return st, out
 
     def rhs(self, holder, Ew, z=0.0):
-        Ew = np.ascontiguousarray(Ew, dtype=np.complex128)
+        # multi-column states are flattened column-major (ω fastest)
+        Ew = np.ascontiguousarray(np.asarray(Ew, dtype=np.complex128).reshape(-1, order="F"))
         nl = np.zeros_like(Ew)
         st = self.L.orc_rhs(holder.ref, _p(nl), _p(Ew), z)
         return st, nl
@@ -105,7 +106,7 @@ class Oracle:
     def solve(self, holder, Ew0, opts, want_saves=True):
         d = holder.desc
         n = d.n_omega * d.n_r
-        Ew = np.ascontiguousarray(Ew0, dtype=np.complex128).copy()
+        Ew = np.ascontiguousarray(np.asarray(Ew0, dtype=np.complex128).reshape(-1, order="F")).copy()
         saves = np.zeros((opts.save_n, n), dtype=np.complex128) if want_saves else None
         zs = np.zeros(opts.save_n)
         stats = self.capi.LhStats()
