@@ -35,6 +35,9 @@ def parse():
     p.add_argument("--flength", type=float, default=1.0)
     p.add_argument("--cpu-seconds", type=float, default=15.0, help="bound on the CPU baseline sample")
     p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--workload", default="scan", choices=["scan", "radial"],
+                   help="scan: batched capillary sweep (the reported metric); radial: the "
+                        "Nr=256 x Nt=4096 radial propagation, printed as its own line")
     return p.parse_args()
 
 
@@ -51,12 +54,49 @@ def rhs_flops(n_to, n_omega):
     return fft + pointwise + stage
 
 
+def radial(args):
+    """Radially symmetric propagation: Nr = 256 × Nto = 8192, Ar 0.5 bar, 0.5 m."""
+    import __graft_entry__ as ge
+    ge.load_package()
+    from luna_jl_amd import _capi, configs, runtime
+    inst = configs.radial_instance()
+    ctx = runtime.Context(inst.holder)
+    opts = _capi.rk_opts(inst.zmax, save_n=201)
+    batch = runtime.ResidentBatch([ctx], [inst.Ew0])
+    for w in range(args.warmup):
+        batch.solve(opts)
+        log(f"bench radial: warmup {w + 1}/{args.warmup} done")
+    times = []
+    for s in range(args.steps):
+        t = time.perf_counter()
+        st = batch.solve(opts)[0]
+        times.append(time.perf_counter() - t)
+        log(f"bench radial: step {s + 1}/{args.steps}: {times[-1] * 1e3:.1f} ms, {st.steps} steps")
+    dt = float(np.mean(times))
+    n_to, n_r = inst.params["n_to"], inst.params["n_r"]
+    gemm_flop = 2.0 * n_to * n_r * n_r  # one QDHT contraction
+    out = {
+        "metric": "radial propagations/hour (Nr=256 x Nt=4096, 0.5 m)",
+        "value": 3600.0 / dt, "unit": "propagations/hour", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": dt * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "radial_ar_256x4096", "n_r": n_r, "n_to": n_to, "flength_m": inst.zmax},
+        "status": st.status, "rk_steps": st.steps, "rejected": st.rejected, "rhs_evals": st.rhs_evals,
+        "rhs_evals_per_s": st.rhs_evals / dt,
+        "qdht_gflop_per_rhs": 2 * gemm_flop / 1e9,
+        "qdht_tflops_effective": 2 * gemm_flop * st.rhs_evals / dt / 1e12,
+    }
+    print(json.dumps(out))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; instances shard across GPUs by a host "
                                    "work queue (see DESIGN.md), no multi-GPU figure is produced"}))
         sys.exit(2)
+    if args.workload == "radial":
+        return radial(args)
     import __graft_entry__ as ge
     ge.load_package()
     from luna_jl_amd import _capi, configs, runtime

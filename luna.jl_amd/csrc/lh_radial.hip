@@ -99,26 +99,44 @@ __global__ __launch_bounds__(256) void rad_gemm(const RadArgs* __restrict__ ap, 
     double* __restrict__ C = dir == 0 ? a.Er : a.Ek;
     const double* __restrict__ B = a.Mt;
     const double scale = dir == 0 ? a.gemm_kr : a.gemm_rk;
-    const int M = a.n_to, K = a.n_r;
+    const int M = a.n_to, K = a.n_r;  // K is a multiple of 64
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int m0 = blockIdx.x * 64 + (wave & 1) * 32;
     const int n0 = blockIdx.y * 64 + (wave >> 1) * 32;
     const int lr = lane & 15, lk = lane >> 4;
     lh_d4 acc00 = {0, 0, 0, 0}, acc01 = acc00, acc10 = acc00, acc11 = acc00;
-    const double* Ap0 = A + (m0 + lr);
-    const double* Bp0 = B + (n0 + lr);
-    const double* Bp1 = B + (n0 + 16 + lr);
-#pragma unroll 4
-    for (int k0 = 0; k0 < K; k0 += 4) {
-        const size_t ka = (size_t)M * (k0 + lk);
-        double a0 = Ap0[ka], a1 = Ap0[ka + 16];
-        const size_t kb = (size_t)K * (k0 + lk);
-        double b0 = Bp0[kb], b1 = Bp1[kb];
-        acc00 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc00, 0, 0, 0);
-        acc01 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc01, 0, 0, 0);
-        acc10 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc10, 0, 0, 0);
-        acc11 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc11, 0, 0, 0);
+    const double* Ap = A + (m0 + lr) + (size_t)M * lk;
+    const double* Bp = B + (n0 + lr) + (size_t)K * lk;
+    // operands of 16 consecutive k (4 MFMA k-steps), double-buffered in registers
+    double ra[2][4][2], rb[2][4][2];
+#define RAD_LOAD(buf, kbase)                                       \
+    _Pragma("unroll") for (int s4 = 0; s4 < 4; s4++) {             \
+        const size_t ka = (size_t)M * ((kbase) + 4 * s4);          \
+        const size_t kb = (size_t)K * ((kbase) + 4 * s4);          \
+        ra[buf][s4][0] = Ap[ka];                                   \
+        ra[buf][s4][1] = Ap[ka + 16];                              \
+        rb[buf][s4][0] = Bp[kb];                                   \
+        rb[buf][s4][1] = Bp[kb + 16];                              \
     }
+#define RAD_MFMA(buf)                                                                                \
+    _Pragma("unroll") for (int s4 = 0; s4 < 4; s4++) {                                               \
+        acc00 = __builtin_amdgcn_mfma_f64_16x16x4f64(ra[buf][s4][0], rb[buf][s4][0], acc00, 0, 0, 0); \
+        acc01 = __builtin_amdgcn_mfma_f64_16x16x4f64(ra[buf][s4][0], rb[buf][s4][1], acc01, 0, 0, 0); \
+        acc10 = __builtin_amdgcn_mfma_f64_16x16x4f64(ra[buf][s4][1], rb[buf][s4][0], acc10, 0, 0, 0); \
+        acc11 = __builtin_amdgcn_mfma_f64_16x16x4f64(ra[buf][s4][1], rb[buf][s4][1], acc11, 0, 0, 0); \
+    }
+    RAD_LOAD(0, 0)
+#pragma unroll 1
+    for (int k0 = 0; k0 < K; k0 += 32) {
+        RAD_LOAD(1, k0 + 16)
+        RAD_MFMA(0)
+        if (k0 + 32 < K) {
+            RAD_LOAD(0, k0 + 32)
+        }
+        RAD_MFMA(1)
+    }
+#undef RAD_LOAD
+#undef RAD_MFMA
 #pragma unroll
     for (int r = 0; r < 4; r++) {
         const int row = m0 + 4 * r + lk;
@@ -213,22 +231,31 @@ __global__ __launch_bounds__(LH_T) void rad_combine(const RadArgs* __restrict__ 
 // ---------------------------------------------------------------- controller
 // weaknorm over all columns (fixed order), stepcontrolPI!, steplims!, FSAL,
 // save bookkeeping (src/RK45.jl:167-192, 343-406; src/Output.jl:497-502).
-__global__ void rad_control(const RadArgs* __restrict__ ap) {
+// One wave of 64 lanes: lane l sums columns l, l+64, ... in order, then a
+// fixed xor tree (deterministic); lane 0 runs the scalar controller.
+__global__ __launch_bounds__(64) void rad_control(const RadArgs* __restrict__ ap) {
     const RadArgs& a = *ap;
     RadCtl* ctl = a.ctl;
-    if (threadIdx.x != 0 || ctl->done) return;
+    if (ctl->done) return;
+    double sy = 0.0, syn = 0.0, se = 0.0;
+    for (int c = threadIdx.x; c < a.n_r; c += 64) {
+        sy += a.partial[3 * c + 0];
+        syn += a.partial[3 * c + 1];
+        se += a.partial[3 * c + 2];
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        sy += __shfl_xor(sy, off, 64);
+        syn += __shfl_xor(syn, off, 64);
+        se += __shfl_xor(se, off, 64);
+    }
+    if (threadIdx.x != 0) return;
     RadCtl s = *ctl;
     s.rhs_evals += 6;
     if (s.status != LH_OK) {
         s.done = 1;
         *ctl = s;
         return;
-    }
-    double sy = 0.0, syn = 0.0, se = 0.0;
-    for (int c = 0; c < a.n_r; c++) {
-        sy += a.partial[3 * c + 0];
-        syn += a.partial[3 * c + 1];
-        se += a.partial[3 * c + 2];
     }
     double errwt = fmax(fmax(sqrt(sy), sqrt(syn)), a.atol);
     double err = sqrt(se) / a.rtol / errwt;
