@@ -35,9 +35,18 @@ def parse():
     p.add_argument("--flength", type=float, default=1.0)
     p.add_argument("--cpu-seconds", type=float, default=15.0, help="bound on the CPU baseline sample")
     p.add_argument("--no-cpu", action="store_true")
-    p.add_argument("--workload", default="scan", choices=["scan", "radial"],
+    p.add_argument("--workload", default="scan", choices=["scan", "radial", "surrogate"],
                    help="scan: batched capillary sweep (the reported metric); radial: the "
-                        "Nr=256 x Nt=4096 radial propagation, printed as its own line")
+                        "Nr=256 x Nt=4096 radial propagation; surrogate: FNO training on solver "
+                        "spectra and screening of the sweep; the last two print their own line")
+    p.add_argument("--n-train", type=int, default=4096)
+    p.add_argument("--n-cal", type=int, default=512)
+    p.add_argument("--train-steps", type=int, default=8000)
+    p.add_argument("--batch", type=int, default=32)
+    p.add_argument("--bound", type=float, default=1e-2)
+    p.add_argument("--lr", type=float, default=1e-3)
+    p.add_argument("--unweighted", action="store_true", help="plain MSE on the log spectrum")
+    p.add_argument("--dump", default=None, help="write calibration (E, P, error) to this .npz")
     return p.parse_args()
 
 
@@ -89,6 +98,76 @@ def radial(args):
     print(json.dumps(out))
 
 
+def surrogate_bench(args):
+    """FNO surrogate: trained on batched-solver exit spectra of capillary scan
+    points (seed 2), calibrated on held-out points, then used to screen the
+    256-point sweep (seed 1); every sweep point is also solved to check it."""
+    import __graft_entry__ as ge
+    ge.load_package()
+    from luna_jl_amd import surrogate as sg
+    n_tr, n_cal = args.n_train, args.n_cal
+    t0 = time.perf_counter()
+    E, P, T = sg.scan_params(n_tr + n_cal, seed=2)
+    S_in, S_out, st = sg.solve_spectra(E, P, T, flength=args.flength, log=log)
+    t_data = time.perf_counter() - t0
+    log(f"bench surrogate: {n_tr + n_cal} solver runs in {t_data:.1f} s (status ok {int((st == 0).sum())})")
+    ok = st == 0
+    S_ref = float(S_out[ok].max())
+    ys = sg.Scaler(S_ref).fit(S_out[:n_tr][ok[:n_tr]])
+    xs = sg.Scaler(S_ref).fit(S_in[:n_tr])
+    X = sg.features(S_in, E, P, xs)
+    Y = ys.encode(S_out).astype(np.float32)
+    tr = np.where(ok[:n_tr])[0]
+    cal = n_tr + np.where(ok[n_tr:])[0]
+    net = sg.FNO(max_batch=max(args.batch, 64))
+    net.set_params(net.init_params(seed=0))
+    W = None if args.unweighted else sg.loss_weights(S_out[tr])
+    net.set_data(X[tr], Y[tr], W)
+    t1 = time.perf_counter()
+    losses = net.train(args.train_steps, batch=args.batch, lr=args.lr, seed=0)
+    t_train = time.perf_counter() - t1
+    log(f"bench surrogate: {args.train_steps} steps in {t_train:.1f} s, loss {losses[:10].mean():.3e} -> "
+        f"{losses[-50:].mean():.3e}")
+    pred_cal = ys.decode(net.forward(X[cal]))
+    err_cal = sg.spectral_error(pred_cal, S_out[cal])
+    screen = sg.Screen(E[cal], P[cal], err_cal, bound=args.bound)
+    if args.dump:
+        np.savez(args.dump, E=E[cal], P=P[cal], err=err_cal, loss=losses)
+    # the 256-point sweep
+    Es, Ps, Ts = sg.scan_params(256, seed=1)
+    t2 = time.perf_counter()
+    Ss_in, Ss_out, sts = sg.solve_spectra(Es, Ps, Ts, flength=args.flength)
+    t_solve_all = time.perf_counter() - t2
+    Xs = sg.features(Ss_in, Es, Ps, xs)
+    t3 = time.perf_counter()
+    pred = ys.decode(net.forward(Xs))
+    perr = screen.predicted_error(Es, Ps)
+    t_infer = time.perf_counter() - t3
+    true_err = sg.spectral_error(pred, Ss_out)
+    acc = perr <= args.bound
+    n_acc = int(acc.sum())
+    out = {
+        "metric": "surrogate screening: solver calls cut at <=1e-2 spectral error",
+        "value": 256.0 / max(1, 256 - n_acc), "unit": "x fewer solver calls", "n_gpus": 1, "steps": 1,
+        "warmup": 0, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "bf16/fp32",
+        "data": "synthetic: exit spectra from the FP64 batched solver",
+        "config": {"workload": "fno_surrogate_capillary", "n_train": len(tr), "n_cal": len(cal), "n_x": 1024,
+                   "channels": 64, "modes": 32, "layers": 4, "params": net.n_params,
+                   "train_steps": args.train_steps, "batch": args.batch, "bound": args.bound},
+        "data_gen_s": t_data, "train_s": t_train, "train_steps_per_s": args.train_steps / t_train,
+        "loss_first": float(losses[:10].mean()), "loss_last": float(losses[-50:].mean()),
+        "cal_err": {"median": float(np.median(err_cal)), "p90": float(np.quantile(err_cal, 0.9)),
+                    "max": float(err_cal.max())},
+        "screen": {"k": screen.k, "safety": screen.safety},
+        "sweep_accepted": n_acc, "sweep_fallback": 256 - n_acc,
+        "sweep_true_err_accepted_max": float(true_err[acc].max()) if n_acc else None,
+        "sweep_true_err_all": {"median": float(np.median(true_err)), "max": float(true_err.max())},
+        "sweep_accepted_over_bound": int((true_err[acc] > args.bound).sum()),
+        "sweep_solver_s": t_solve_all, "sweep_surrogate_infer_s": t_infer,
+    }
+    print(json.dumps(out))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -97,6 +176,8 @@ def main():
         sys.exit(2)
     if args.workload == "radial":
         return radial(args)
+    if args.workload == "surrogate":
+        return surrogate_bench(args)
     import __graft_entry__ as ge
     ge.load_package()
     from luna_jl_amd import _capi, configs, runtime

@@ -53,6 +53,17 @@ class LhStats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class LhFnoCfg(C.Structure):
+    """``lh_fno_cfg`` of include/luna_fno.h."""
+    _fields_ = [("n_x", C.c_int32), ("cin", C.c_int32), ("channels", C.c_int32), ("modes", C.c_int32),
+                ("layers", C.c_int32), ("hidden", C.c_int32), ("max_batch", C.c_int32)]
+
+
+class LhAdam(C.Structure):
+    _fields_ = [("lr", C.c_float), ("beta1", C.c_float), ("beta2", C.c_float), ("eps", C.c_float),
+                ("weight_decay", C.c_float)]
+
+
 def _ptr(a, ctype=C.c_double):
     if a is None:
         return None
@@ -134,13 +145,25 @@ def lib():
     L.lh_batch_solve_resident.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.POINTER(LhRkOpts), C.c_int,
                                           C.POINTER(LhStats)]
     L.lh_last_launch_times.argtypes = [_dp, C.POINTER(C.c_int), C.c_int]
+    _fp = C.POINTER(C.c_float)
+    L.lh_fno_create.argtypes = [C.POINTER(C.c_void_p), C.POINTER(LhFnoCfg)]
+    L.lh_fno_destroy.argtypes = [C.c_void_p]
+    L.lh_fno_param_count.argtypes = [C.c_void_p, C.POINTER(C.c_int64)]
+    L.lh_fno_set_params.argtypes = [C.c_void_p, _fp]
+    L.lh_fno_get_params.argtypes = [C.c_void_p, _fp]
+    L.lh_fno_forward.argtypes = [C.c_void_p, _fp, C.c_int, _fp]
+    L.lh_fno_loss_grad.argtypes = [C.c_void_p, _fp, _fp, _fp, C.c_int, _fp, _fp]
+    L.lh_fno_set_data.argtypes = [C.c_void_p, _fp, _fp, _fp, C.c_int]
+    L.lh_fno_train.argtypes = [C.c_void_p, C.POINTER(C.c_int32), C.c_int, C.c_int, C.POINTER(LhAdam), _fp, _fp]
     _lib = L
     return L
 
 
 EXPORTED = ["lh_strerror", "lh_device_count", "lh_create", "lh_destroy", "lh_rhs",
             "lh_rhs_device", "lh_ctx_status", "lh_solve", "lh_batch_solve", "lh_set_state",
-            "lh_batch_solve_resident", "lh_get_state", "lh_get_saves", "lh_last_launch_times"]
+            "lh_batch_solve_resident", "lh_get_state", "lh_get_saves", "lh_last_launch_times",
+            "lh_fno_create", "lh_fno_destroy", "lh_fno_param_count", "lh_fno_set_params",
+            "lh_fno_get_params", "lh_fno_forward", "lh_fno_loss_grad", "lh_fno_set_data", "lh_fno_train"]
 
 CLASS_NAMES = (16384, 8192, 4096, "env8192", "env4096", "radial")
 

@@ -4,7 +4,7 @@ import subprocess
 import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-SRC = os.path.join(HERE, "csrc", "lh_kernels.hip")
+SRCS = [os.path.join(HERE, "csrc", "lh_kernels.hip"), os.path.join(HERE, "csrc", "lh_fno.hip")]
 OUT_DIR = os.path.join(HERE, "_build")
 OUT = os.path.join(OUT_DIR, "libluna_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -20,11 +20,11 @@ FLAGS = [
 def build(verbose=False):
     os.makedirs(OUT_DIR, exist_ok=True)
     tmp = OUT + f".tmp{os.getpid()}"
-    cmd = [HIPCC] + FLAGS + ["-o", tmp, SRC]
+    cmd = [HIPCC] + FLAGS + ["-o", tmp] + SRCS
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         sys.stderr.write(res.stdout + res.stderr)
-        raise RuntimeError("hipcc failed for " + SRC)
+        raise RuntimeError("hipcc failed for " + " ".join(SRCS))
     if verbose:
         sys.stderr.write(res.stderr)
     os.replace(tmp, OUT)

@@ -1,4 +1,4 @@
-"""The C-ABI library loads and exports every entry point include/luna_hip.h declares."""
+"""The C-ABI library loads and exports every entry point include/*.h declares."""
 import ctypes
 import os
 import re
@@ -8,16 +8,26 @@ import pytest
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
+HEADERS = ("luna_hip.h", "luna_fno.h")
+
+
 def declared_symbols():
-    text = open(os.path.join(ROOT, "include", "luna_hip.h")).read()
-    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(lh_[a-z_0-9]+)\s*\(", text)))
+    syms = set()
+    for h in HEADERS:
+        text = open(os.path.join(ROOT, "include", h)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        syms |= set(re.findall(r"\b(lh_[a-z_0-9]+)\s*\(", text))
+    return sorted(syms)
+
+
+def test_every_header_is_listed():
+    assert sorted(os.listdir(os.path.join(ROOT, "include"))) == sorted(HEADERS)
 
 
 def test_header_declares_entry_points():
     syms = declared_symbols()
     for s in ("lh_create", "lh_destroy", "lh_rhs", "lh_rhs_device", "lh_solve", "lh_batch_solve",
-              "lh_strerror"):
+              "lh_strerror", "lh_fno_create", "lh_fno_train", "lh_fno_forward"):
         assert s in syms
 
 
@@ -25,7 +35,7 @@ def test_library_exports_all_declared(capi):
     L = capi.lib()
     for s in declared_symbols():
         assert hasattr(L, s), s
-    assert set(capi.EXPORTED) <= set(declared_symbols())
+    assert set(capi.EXPORTED) == set(declared_symbols())
 
 
 def test_strerror_and_arg_checks(capi):
@@ -53,3 +63,23 @@ def test_struct_sizes_match_header(capi):
     sizes = [int(x) for x in out]
     assert sizes == [ctypes.sizeof(capi.LhDesc), ctypes.sizeof(capi.LhRkOpts), ctypes.sizeof(capi.LhStats),
                      capi.LhDesc.qdht_inv.offset, capi.LhDesc.ppt_x.offset]
+
+
+def test_fno_structs_match_header(capi):
+    import subprocess
+    import tempfile
+    src = '#include <stdio.h>\n#include "luna_fno.h"\nint main(){printf("%zu %zu", sizeof(lh_fno_cfg), sizeof(lh_adam));}'
+    with tempfile.TemporaryDirectory() as td:
+        c = os.path.join(td, "s.c")
+        open(c, "w").write(src)
+        exe = os.path.join(td, "s")
+        subprocess.check_call(["gcc", "-I", os.path.join(ROOT, "include"), "-o", exe, c])
+        out = [int(x) for x in subprocess.check_output([exe]).decode().split()]
+    assert out == [ctypes.sizeof(capi.LhFnoCfg), ctypes.sizeof(capi.LhAdam)]
+
+
+def test_fno_arg_checks(capi):
+    L = capi.lib()
+    h = ctypes.c_void_p()
+    cfg = capi.LhFnoCfg(n_x=100, cin=4, channels=64, modes=32, layers=4, hidden=128, max_batch=4)
+    assert L.lh_fno_create(ctypes.byref(h), ctypes.byref(cfg)) == capi.LH_E_ARG  # n_x not a multiple of 64
