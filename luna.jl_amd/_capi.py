@@ -108,7 +108,8 @@ def rk_opts(zmax, z0=0.0, rtol=1e-6, atol=1e-10, safety=0.9, max_dz=None, min_dz
 
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "_build", "libluna_hip.so")
+# LUNA_HIP_LIB selects another build of the same library (A/B timing of kernel variants)
+LIB_PATH = os.environ.get("LUNA_HIP_LIB") or os.path.join(_HERE, "_build", "libluna_hip.so")
 _lib = None
 
 

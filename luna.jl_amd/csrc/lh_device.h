@@ -97,6 +97,17 @@ __device__ __forceinline__ void dft(double2* v) {
     else dft8<INV>(v);
 }
 
+// Thread index that the compiler must re-read at each use site.  Index and
+// LDS-address arithmetic derived from it then stays local to each FFT pass or
+// scan instead of being hoisted out of the solver's loops, where dozens of
+// loop-invariant addresses would otherwise be kept live (and spilled) across
+// the whole right-hand side.
+__device__ __forceinline__ int fresh_tid() {
+    int t = threadIdx.x;
+    asm volatile("" : "+v"(t));
+    return t;
+}
+
 // ------------------------------------------------------------ twiddles
 // Two-level table in LDS: W^m = hi[m >> 7] · lo[m & 127], W = exp(-2πi/LH_TWN).
 // Filled once per workgroup from the global table (4 KB of LDS).
@@ -125,7 +136,7 @@ __device__ __forceinline__ void fft_pass(double2* buf, const TwLds& tw) {
     constexpr int NB = M / R;
     constexpr int BPT = NB / TA;
     constexpr int TSTRIDE = LH_TWN / (NS * R);
-    const int tid = threadIdx.x;
+    const int tid = fresh_tid();
     double2 v[BPT][R];
     if (tid < TA) {
 #pragma unroll
@@ -185,7 +196,7 @@ __device__ __forceinline__ void fft_lds(double2* buf, const TwLds& tw) {
 template <int M>
 __device__ __forceinline__ void c2r_prepack(double2* buf, const TwLds& tw) {
     constexpr int TS = LH_TWN / (2 * M);
-    for (int k = threadIdx.x; k <= M / 2; k += LH_T) {
+    for (int k = fresh_tid(); k <= M / 2; k += LH_T) {
         int m = M - k;
         double2 Xk = buf[pad(k)], Xm = buf[pad(m)];
         if (k == 0) {

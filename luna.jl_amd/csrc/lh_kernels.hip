@@ -142,7 +142,7 @@ __device__ __forceinline__ double ppt_rate(const DInst& in, double aE, int& bad)
 template <int QR, class Smem>
 __device__ __forceinline__ void block_cumtrapz(double* y, double dx, int g0, Smem& sm, double& carry,
                                                double prev0) {
-    const int tid = threadIdx.x;
+    const int tid = fresh_tid();
     sm.edge[tid] = y[QR - 1];
     __syncthreads();
     double prev = (tid > 0) ? sm.edge[tid - 1] : prev0;
@@ -189,7 +189,7 @@ __device__ __forceinline__ void pointwise_field(const DInst& in, Smem& sm) {
     constexpr int QT = 2 * MO / LH_T;          // samples per thread over all rounds
     constexpr int QR = (QT > 16) ? 16 : QT;    // samples per thread per round
     constexpr int NR = QT / QR;                // rounds
-    const int tid = threadIdx.x;
+    const int tid = fresh_tid();
     const double es = in.e_scale;
     const bool kerr = (in.kerr == LH_KERR_FIELD), plasma = in.plasma;
     const double kfac = in.kerr_fac, rho = in.density;
@@ -297,7 +297,7 @@ __device__ __forceinline__ void stage_prologue_cols(const double2* __restrict__ 
                                                     const double2* Y, double2* const* K, int stage, double dt,
                                                     double h0, double hf) {
     constexpr int N = Bins<MO>::N, NPT = Bins<MO>::NPT;
-    const int tid = threadIdx.x;
+    const int tid = fresh_tid();
     double2 yb[NPT], L[NPT];
 #pragma unroll
     for (int i = 0; i < NPT; i++) {
@@ -402,7 +402,7 @@ template <int NO, int NN, class Smem>
 __device__ __forceinline__ void stage_prologue_env(const DInst& in, Smem& sm, const double2* Y, double2* const* K,
                                                    int stage, double dt, double h0, double hf) {
     constexpr int N = NN, NPT = EnvBins<NO, NN>::NPT;
-    const int tid = threadIdx.x;
+    const int tid = fresh_tid();
     double2 yb[NPT], L[NPT];
 #pragma unroll
     for (int i = 0; i < NPT; i++) {
@@ -452,7 +452,7 @@ __device__ __forceinline__ void stage_prologue_env(const DInst& in, Smem& sm, co
 template <int NO, int NN, class Smem, class Epi>
 __device__ __forceinline__ void rhs_env(const DInst& in, Smem& sm, Epi epi) {
     constexpr int PER = NO / LH_T;  // complex samples per thread (contiguous chunk)
-    const int tid = threadIdx.x;
+    const int tid = fresh_tid();
     const int i0 = PER * tid;
     __syncthreads();
     fft_lds<NO, true>(sm.buf, sm.tw);
