@@ -135,6 +135,63 @@ __device__ __forceinline__ double ppt_rate(const DInst& in, double aE, int& bad)
     return exp(v);
 }
 
+// The same for Q samples at once: every index first, then all table loads,
+// then the splines, so the loads of a thread's samples are in flight together
+// instead of forming one dependent load-and-wait chain per sample.  Inactive
+// samples (below the table, or out of range) load the first interval and are
+// discarded.
+template <int Q>
+__device__ __forceinline__ void ppt_rates(const DInst& in, const double* aE, double* rate, int& bad) {
+    const double xmin = in.ppt_emin, xmax = in.ppt_emax;
+    const int n = in.ppt_n;
+    int idx[Q];
+    bool act[Q];
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        const double x = aE[q];
+        const bool below = x < xmin;
+        const bool out = !below && !(x <= xmax);  // above the table, or NaN
+        if (out) bad = (x > xmax) ? LH_E_PPT_RANGE : LH_E_NONFINITE;
+        act[q] = !below && !out;
+        int i = 2;
+        if (act[q]) {
+            if (x <= xmin)
+                i = 2;
+            else if (x >= xmax)
+                i = n;
+            else
+                i = (int)ceil((x - xmin) / (xmax - xmin) * (double)(n - 1)) + 1;
+            i = min(max(i, 2), n);
+        }
+        idx[q] = i;
+    }
+    double4 pc[Q], pm[Q];
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        pc[q] = in.ppt[idx[q] - 1];
+        pm[q] = in.ppt[idx[q] - 2];
+    }
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        double r = 0.0;
+        if (act[q]) {
+            const double x = aE[q];
+            double v;
+            if (x == pc[q].x) {
+                v = pc[q].y;
+            } else if (x == pm[q].x) {
+                v = pm[q].y;
+            } else {
+                const double t = (x - pm[q].x) / (pc[q].x - pm[q].x);
+                v = (pm[q].y + pm[q].z * t + (3 * (pc[q].y - pm[q].y) - 2 * pm[q].z - pc[q].z) * (t * t) +
+                     (2 * (pm[q].y - pc[q].y) + pm[q].z + pc[q].z) * (t * t * t));
+            }
+            r = exp(v);
+        }
+        rate[q] = r;
+    }
+}
+
 // cumtrapz over the workgroup for one round of QR samples per thread
 // (global index g0+q).  carry: running integral before this round; prev0:
 // the integrand at the sample just before this round (thread 0 only).  On
@@ -205,12 +262,13 @@ __device__ __forceinline__ void pointwise_field(const DInst& in, Smem& sm) {
         if (plasma) {
             double jf[QR];
             int bad = 0;
+            constexpr int QC = (QR < 4) ? QR : 4;
 #pragma unroll
-            for (int q = 0; q < QR; q++) {
-                double E = field_at(sm, g0 + q, es);
-                a[q] = ppt_rate(in, fabs(E), bad);
-                jf[q] = a[q];  // rate, kept for the loss term
-            }
+            for (int q = 0; q < QR; q++) jf[q] = fabs(field_at(sm, g0 + q, es));
+#pragma unroll
+            for (int c = 0; c < QR; c += QC) ppt_rates<QC>(in, jf + c, a + c, bad);
+#pragma unroll
+            for (int q = 0; q < QR; q++) jf[q] = a[q];  // rate, kept for the loss term
             if (bad) sm.flag = bad;
             double lr = block_last(jf[QR - 1], sm);
             block_cumtrapz<QR>(a, dt, g0, sm, cF, lastRate);  // a = ∫rate
