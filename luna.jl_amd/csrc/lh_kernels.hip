@@ -90,12 +90,22 @@ __constant__ double c_iC[4][7] = {
 
 // LDS of one workgroup: the padded transform buffer sized for the largest
 // transform it runs (MO complex points + Nyquist), twiddles, scan scratch.
+// plist: a compacted list of the samples that need a PPT rate (the few
+// hundred samples under the pulse), so that every thread takes one or two of
+// them instead of the threads owning the pulse doing all of that work.  Used
+// up to Nto = 8192; the largest transform leaves no room for it in LDS.
+template <int MO>
+struct PList {
+    static constexpr int CAP = (MO >= 8192) ? 1 : 4096;
+};
+
 template <int MO>
 struct SmemT {
     double2 buf[padded_len(MO)];
     TwLds tw;
     double edge[LH_T];
     double wsum[5 * LH_NW];  // scans use [0, 2*LH_NW], block_last uses [4*LH_NW]
+    double plist[PList<MO>::CAP];
     int flag;
 };
 
@@ -263,10 +273,54 @@ __device__ __forceinline__ void pointwise_field(const DInst& in, Smem& sm) {
             double jf[QR];
             int bad = 0;
             constexpr int QC = (QR < 4) ? QR : 4;
+            constexpr int PCAP = PList<MO>::CAP;
 #pragma unroll
             for (int q = 0; q < QR; q++) jf[q] = fabs(field_at(sm, g0 + q, es));
+            if constexpr (MO <= 4096) {
+                // samples inside the table's range get a rate; above it (or NaN) raise the flag
+                unsigned mask = 0;
+                int cnt = 0;
 #pragma unroll
-            for (int c = 0; c < QR; c += QC) ppt_rates<QC>(in, jf + c, a + c, bad);
+                for (int q = 0; q < QR; q++) {
+                    const double x = jf[q];
+                    const bool below = x < in.ppt_emin;
+                    const bool out = !below && !(x <= in.ppt_emax);
+                    if (out) bad = (x > in.ppt_emax) ? LH_E_PPT_RANGE : LH_E_NONFINITE;
+                    if (!below && !out) {
+                        mask |= 1u << q;
+                        cnt++;
+                    }
+                }
+                double ntot;
+                const int off = (int)block_exscan((double)cnt, sm.wsum, &ntot);
+                const int total = (int)ntot;
+                if (total <= PCAP) {  // compacted: one or two samples per thread
+                    int o = off;
+#pragma unroll
+                    for (int q = 0; q < QR; q++)
+                        if ((mask >> q) & 1u) sm.plist[o++] = jf[q];
+                    __syncthreads();
+                    int unused = 0;
+                    for (int i = tid; i < total; i += LH_T) {
+                        double r;
+                        ppt_rates<1>(in, &sm.plist[i], &r, unused);
+                        sm.plist[i] = r;
+                    }
+                    __syncthreads();
+                    o = off;
+#pragma unroll
+                    for (int q = 0; q < QR; q++) {
+                        a[q] = 0.0;
+                        if ((mask >> q) & 1u) a[q] = sm.plist[o++];
+                    }
+                } else {
+#pragma unroll
+                    for (int c = 0; c < QR; c += QC) ppt_rates<QC>(in, jf + c, a + c, bad);
+                }
+            } else {
+#pragma unroll
+                for (int c = 0; c < QR; c += QC) ppt_rates<QC>(in, jf + c, a + c, bad);
+            }
 #pragma unroll
             for (int q = 0; q < QR; q++) jf[q] = a[q];  // rate, kept for the loss term
             if (bad) sm.flag = bad;
@@ -1096,6 +1150,65 @@ static int twiddles(int dev, const double2** out) {
     return LH_OK;
 }
 
+// Constants that many contexts share (a scan's PPT table, the windows of one
+// grid) are uploaded once per device and reference-counted: identical content
+// maps to one device buffer, which then stays resident in the L2/MALL caches
+// for every instance of a batch instead of one private copy each.
+struct SharedBuf {
+    int device;
+    uint64_t hash;
+    std::vector<unsigned char> host;  // content, for an exact match
+    void* dev;
+    int refs;
+};
+static std::vector<SharedBuf*> g_shared;
+
+static uint64_t fnv1a(const void* p, size_t n) {
+    const unsigned char* b = (const unsigned char*)p;
+    uint64_t h = 1469598103934665603ULL;
+    for (size_t i = 0; i < n; i++) h = (h ^ b[i]) * 1099511628211ULL;
+    return h;
+}
+
+template <class T>
+static int shared_upload(int device, T** dst, const void* src, size_t count) {
+    *dst = nullptr;
+    if (!src || count == 0) return LH_OK;
+    const size_t bytes = sizeof(T) * count;
+    const uint64_t h = fnv1a(src, bytes);
+    std::lock_guard<std::mutex> g(g_mu);
+    for (SharedBuf* e : g_shared)
+        if (e->device == device && e->hash == h && e->host.size() == bytes && !memcmp(e->host.data(), src, bytes)) {
+            e->refs++;
+            *dst = (T*)e->dev;
+            return LH_OK;
+        }
+    void* d = nullptr;
+    HIPCK(hipMalloc(&d, bytes));
+    if (hipMemcpy(d, src, bytes, hipMemcpyHostToDevice) != hipSuccess) {
+        (void)hipFree(d);
+        return LH_E_HIP;
+    }
+    SharedBuf* e = new SharedBuf{device, h, std::vector<unsigned char>((const unsigned char*)src, (const unsigned char*)src + bytes), d, 1};
+    g_shared.push_back(e);
+    *dst = (T*)d;
+    return LH_OK;
+}
+
+static void shared_release(const void* dev) {
+    if (!dev) return;
+    std::lock_guard<std::mutex> g(g_mu);
+    for (size_t i = 0; i < g_shared.size(); i++)
+        if (g_shared[i]->dev == dev) {
+            if (--g_shared[i]->refs == 0) {
+                (void)hipFree(g_shared[i]->dev);
+                delete g_shared[i];
+                g_shared.erase(g_shared.begin() + i);
+            }
+            return;
+        }
+}
+
 template <class T>
 static int upload(T** dst, const void* src, size_t count) {
     *dst = nullptr;
@@ -1404,7 +1517,9 @@ int lh_destroy(lh_ctx* c) {
     if (!c) return LH_OK;
     hipSetDevice(c->device);
     if (c->stream) hipStreamSynchronize(c->stream);
-    void* ptrs[] = {c->omega_win, c->t_win, c->to_win, c->linop, c->outfac, c->ppt, c->hwdt, c->escr, c->Yin, c->Y, c->YN,
+    for (const void* p : {(const void*)c->omega_win, (const void*)c->t_win, (const void*)c->to_win, (const void*)c->ppt})
+        shared_release(p);
+    void* ptrs[] = {c->linop, c->outfac, c->hwdt, c->escr, c->Yin, c->Y, c->YN,
                     c->saves, c->zgrid, c->dstats, c->dinst, c->rhs_in, c->rhs_out, c->dstatus};
     for (void* p : ptrs)
         if (p) hipFree(p);
@@ -1435,9 +1550,9 @@ int lh_create(lh_ctx** out, const lh_desc* d) {
     if (hipGetDevice(&c->device) != hipSuccess) st = LH_E_HIP;
     if (!st && hipStreamCreate(&c->stream) != hipSuccess) st = LH_E_HIP;
     const size_t N = (size_t)d->n_omega, NS = (size_t)c->n_state;
-    if (!st) st = upload(&c->omega_win, d->omega_win, N);
-    if (!st) st = upload(&c->t_win, d->t_win, (size_t)d->n_t);
-    if (!st) st = upload(&c->to_win, d->to_win, (size_t)d->n_to);
+    if (!st) st = shared_upload(c->device, &c->omega_win, d->omega_win, N);
+    if (!st) st = shared_upload(c->device, &c->t_win, d->t_win, (size_t)d->n_t);
+    if (!st) st = shared_upload(c->device, &c->to_win, d->to_win, (size_t)d->n_to);
     if (!st) {
         // Output factor per bin: to_freq! crop scale, then norm! and ×ωwin on
         // sidx bins (mode-average, src/NonlinearRHS.jl:368-387), or the radial
@@ -1466,7 +1581,7 @@ int lh_create(lh_ctx** out, const lh_desc* d) {
         for (int i = 0; i < d->ppt_n; i++) p[i] = make_double4(d->ppt_x[i], d->ppt_y[i], d->ppt_d[i], 0.0);
         c->ppt_emin = d->ppt_x[0];
         c->ppt_emax = d->ppt_x[d->ppt_n - 1];
-        st = upload(&c->ppt, p.data(), p.size());
+        st = shared_upload(c->device, &c->ppt, p.data(), p.size());
     }
     if (!st && d->grid_kind == LH_GRID_ENV && d->raman) {
         const double2* tw;
