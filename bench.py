@@ -35,7 +35,7 @@ def parse():
     p.add_argument("--flength", type=float, default=1.0)
     p.add_argument("--cpu-seconds", type=float, default=15.0, help="bound on the CPU baseline sample")
     p.add_argument("--no-cpu", action="store_true")
-    p.add_argument("--workload", default="scan", choices=["scan", "radial", "surrogate"],
+    p.add_argument("--workload", default="scan", choices=["scan", "radial", "surrogate", "copy"],
                    help="scan: batched capillary sweep (the reported metric); radial: the "
                         "Nr=256 x Nt=4096 radial propagation; surrogate: FNO training on solver "
                         "spectra and screening of the sweep; the last two print their own line")
@@ -61,6 +61,18 @@ def rhs_flops(n_to, n_omega):
     pointwise = 40 * n_to                          # Kerr + PPT spline + 3 cumtrapz + windows
     stage = 60 * n_omega                           # stage combination + two exp(LΔ) + crop/norm
     return fft + pointwise + stage
+
+
+def measured_traffic():
+    """L2-to-fabric bytes per launch of the dominant kernel from the committed
+    rocprofv3 FETCH_SIZE/WRITE_SIZE summary (copy-calibrated), or None."""
+    import re
+    path = os.path.join(ROOT, "profiles", "pmc_lh_solve_kernel_capillary_scan_256.md")
+    try:
+        m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", open(path).read())
+        return float(m.group(1)) if m else None
+    except OSError:
+        return None
 
 
 def radial(args):
@@ -168,6 +180,30 @@ def surrogate_bench(args):
     print(json.dumps(out))
 
 
+def copy_calibration(args):
+    """A plain device-to-device copy of 1 GiB (the runtime's blit kernel), three
+    times: the known byte count calibrates the FETCH_SIZE / WRITE_SIZE counters."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    n = 1 << 30
+    a, b = C.c_void_p(), C.c_void_p()
+    for ptr in (a, b):
+        if hip.hipMalloc(C.byref(ptr), C.c_size_t(n)) != 0:
+            raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(a, 1, C.c_size_t(n))
+    hip.hipDeviceSynchronize()
+    t = time.perf_counter()
+    for _ in range(3):
+        if hip.hipMemcpy(b, a, C.c_size_t(n), 3) != 0:  # hipMemcpyDeviceToDevice
+            raise RuntimeError("hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    dt = (time.perf_counter() - t) / 3
+    print(json.dumps({"workload": "copy_1GiB", "bytes_read": n, "bytes_written": n, "ms": dt * 1e3,
+                      "GB_per_s": 2 * n / dt / 1e9}))
+    hip.hipFree(a)
+    hip.hipFree(b)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -178,6 +214,8 @@ def main():
         return radial(args)
     if args.workload == "surrogate":
         return surrogate_bench(args)
+    if args.workload == "copy":
+        return copy_calibration(args)
     import __graft_entry__ as ge
     ge.load_package()
     from luna_jl_amd import _capi, configs, runtime
@@ -285,8 +323,9 @@ def main():
         "plain_gpu_points_per_hour": plain_pph,
         "batched_over_plain": value / plain_pph,
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": 78.6, "unit": "TFLOP/s",
-                     "frac": achieved / 78.6, "traffic": None,
-                     "kernel": "lh_solve_kernel<8192,4096> (Nto=16384 class)", "kernel_ms": kernel_ms},
+                     "frac": achieved / 78.6, "traffic": measured_traffic(),
+                     "kernel": "lh_solve_kernel<8192,4096> (Nto=16384 class)", "kernel_ms": kernel_ms,
+                     "algorithmic_bytes": sum(st.rhs_evals * 2 * c.params["n_omega"] * 16 for st, c in big)},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
