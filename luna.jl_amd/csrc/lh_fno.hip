@@ -363,9 +363,9 @@ __global__ void fno_loss_reduce(const float* __restrict__ partial, int nb, long 
     if (threadIdx.x == 0) *out = red[0] / (float)n;
 }
 
-// Column sums of X[rows][N] (bias gradients): stage 1 per 1024-row chunk,
+// Column sums of X[rows][N] (bias gradients): stage 1 per 128-row chunk,
 // stage 2 adds the chunks in order.
-static const int FNO_CS_ROWS = 1024;
+static const int FNO_CS_ROWS = 128;
 __global__ void fno_colsum_part(const float* __restrict__ X, long long rows, int N, float* __restrict__ part) {
     __shared__ float red[4][64];
     const int c = blockIdx.x * 64 + (threadIdx.x & 63), lane4 = threadIdx.x >> 6;
@@ -533,7 +533,7 @@ static int fno_gemm_launch(lh_fno* f, GemmArgs g, int ta, int tb) {
     }
     const long long tiles = (long long)g.batch * ((g.M + 63) / 64) * ((g.N + 63) / 64);
     int ksplit = 1;
-    if (tiles < 512 && g.K > 1024 && !g.bias) {
+    if (tiles < 256 && g.K >= 1024 && !g.bias) {
         long long want = 1024 / (tiles > 0 ? tiles : 1);
         long long maxk = (g.K + 511) / 512;
         ksplit = (int)(want < maxk ? want : maxk);
