@@ -264,15 +264,17 @@ class Screen:
     normalised (energy, pressure) plane.
 
     With safety=None the factor is calibrated by leave-one-out on the
-    calibration set: the smallest factor on a 0.1 grid for which no
-    calibration point that would be accepted has an error above the bound,
-    plus one grid step of margin."""
+    calibration set: the smallest factor on a 0.1 grid for which at most a
+    fraction `miss` of the calibration points that would be accepted have an
+    error above the bound, plus one grid step of margin.  (Requiring zero
+    misses lets one isolated outlier set the factor for the whole plane.)"""
 
-    def __init__(self, E_cal, P_cal, err_cal, k=8, safety=None, bound=1e-2):
+    def __init__(self, E_cal, P_cal, err_cal, k=8, safety=None, bound=1e-2, miss=0.01):
         self.pts = np.stack([normalise(E_cal, *PARAM_RANGES["energy"]),
                              normalise(P_cal, *PARAM_RANGES["pressure"])], axis=1)
         self.err = np.asarray(err_cal, dtype=float)
         self.k = k
+        self.miss = miss
         self.safety = self.calibrate(bound) if safety is None else float(safety)
 
     def _nearest(self, q, exclude_self=False):
@@ -285,7 +287,7 @@ class Screen:
         local = self.err[self._nearest(self.pts, exclude_self=True)].max(axis=1)
         for s in grid:
             acc = s * local <= bound
-            if not np.any(self.err[acc] > bound):
+            if np.sum(self.err[acc] > bound) <= self.miss * max(1, int(acc.sum())):
                 return float(s) + 0.1
         return float(grid[-1]) + 0.1
 
