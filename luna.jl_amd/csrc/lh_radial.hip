@@ -87,11 +87,15 @@ __global__ __launch_bounds__(LH_T) void rad_stage_a(const RadArgs* __restrict__ 
 }
 
 // ---------------------------------------------------------------- G: FP64 MFMA GEMM
-// C[M×N] = scale · A[M×K] · B[K×N], A and C column-major (t fastest),
-// B[k][n] = Mt[k*Nr + n] = M[n][k] (row-major, so 16 lanes load 16 consecutive n).
-// Workgroup tile 64×64 (2×2 waves of 32×32, each 2×2 MFMA 16×16×4 tiles).
-// v_mfma_f64_16x16x4f64 lane layout: A lane l = A[l%16][l/16],
-// B lane l = B[l/16][l%16], D lane l reg r = D[4r + l/16][l%16].
+// C[M×N] = scale · A[M×K] · B[K×N]; A and C column-major (t fastest, M = Nto),
+// B[k][n] = Mt[k*Nr + n] = M[n][k].  The MFMA computes the transposed tile
+// Cᵀ = Bᵀ·Aᵀ so that its column index runs along t: each 16-lane group then
+// loads and stores 16 consecutive doubles.  v_mfma_f64_16x16x4f64 lane layout
+// (measured): operand 1 lane l = X[l%16][l/16], operand 2 lane l = Y[l/16][l%16],
+// result register r of lane l = D[4r + l/16][l%16].
+// Workgroup: 2×2 waves; a wave owns 32 r-columns × 16·MT t-rows (2 × MT MFMA
+// tiles), operands of 16 consecutive k double-buffered in registers.
+template <int MT>
 __global__ __launch_bounds__(256) void rad_gemm(const RadArgs* __restrict__ ap, int dir) {
     const RadArgs& a = *ap;
     if (a.ctl->done) return;
@@ -99,32 +103,31 @@ __global__ __launch_bounds__(256) void rad_gemm(const RadArgs* __restrict__ ap, 
     double* __restrict__ C = dir == 0 ? a.Er : a.Ek;
     const double* __restrict__ B = a.Mt;
     const double scale = dir == 0 ? a.gemm_kr : a.gemm_rk;
-    const int M = a.n_to, K = a.n_r;  // K is a multiple of 64
+    const int M = a.n_to, K = a.n_r;  // K = N, a multiple of 64; M a multiple of 32·MT
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int m0 = blockIdx.x * 64 + (wave & 1) * 32;
-    const int n0 = blockIdx.y * 64 + (wave >> 1) * 32;
+    const int n0 = blockIdx.y * 64 + (wave & 1) * 32;
+    const int m0 = blockIdx.x * (32 * MT) + (wave >> 1) * (16 * MT);
     const int lr = lane & 15, lk = lane >> 4;
-    lh_d4 acc00 = {0, 0, 0, 0}, acc01 = acc00, acc10 = acc00, acc11 = acc00;
-    const double* Ap = A + (m0 + lr) + (size_t)M * lk;
+    lh_d4 acc[2][MT];
+#pragma unroll
+    for (int t = 0; t < 2; t++)
+#pragma unroll
+        for (int j = 0; j < MT; j++) acc[t][j] = (lh_d4){0, 0, 0, 0};
     const double* Bp = B + (n0 + lr) + (size_t)K * lk;
-    // operands of 16 consecutive k (4 MFMA k-steps), double-buffered in registers
-    double ra[2][4][2], rb[2][4][2];
-#define RAD_LOAD(buf, kbase)                                       \
-    _Pragma("unroll") for (int s4 = 0; s4 < 4; s4++) {             \
-        const size_t ka = (size_t)M * ((kbase) + 4 * s4);          \
-        const size_t kb = (size_t)K * ((kbase) + 4 * s4);          \
-        ra[buf][s4][0] = Ap[ka];                                   \
-        ra[buf][s4][1] = Ap[ka + 16];                              \
-        rb[buf][s4][0] = Bp[kb];                                   \
-        rb[buf][s4][1] = Bp[kb + 16];                              \
+    const double* Ap = A + (m0 + lr) + (size_t)M * lk;
+    double rb[2][4][2], ra[2][4][MT];
+#define RAD_LOAD(buf, kbase)                                          \
+    _Pragma("unroll") for (int s4 = 0; s4 < 4; s4++) {                \
+        const size_t kb = (size_t)K * ((kbase) + 4 * s4);             \
+        const size_t ka = (size_t)M * ((kbase) + 4 * s4);             \
+        _Pragma("unroll") for (int t = 0; t < 2; t++) rb[buf][s4][t] = Bp[kb + 16 * t];  \
+        _Pragma("unroll") for (int j = 0; j < MT; j++) ra[buf][s4][j] = Ap[ka + 16 * j]; \
     }
-#define RAD_MFMA(buf)                                                                                \
-    _Pragma("unroll") for (int s4 = 0; s4 < 4; s4++) {                                               \
-        acc00 = __builtin_amdgcn_mfma_f64_16x16x4f64(ra[buf][s4][0], rb[buf][s4][0], acc00, 0, 0, 0); \
-        acc01 = __builtin_amdgcn_mfma_f64_16x16x4f64(ra[buf][s4][0], rb[buf][s4][1], acc01, 0, 0, 0); \
-        acc10 = __builtin_amdgcn_mfma_f64_16x16x4f64(ra[buf][s4][1], rb[buf][s4][0], acc10, 0, 0, 0); \
-        acc11 = __builtin_amdgcn_mfma_f64_16x16x4f64(ra[buf][s4][1], rb[buf][s4][1], acc11, 0, 0, 0); \
-    }
+#define RAD_MFMA(buf)                                                                              \
+    _Pragma("unroll") for (int s4 = 0; s4 < 4; s4++)                                               \
+        _Pragma("unroll") for (int t = 0; t < 2; t++)                                              \
+            _Pragma("unroll") for (int j = 0; j < MT; j++)                                         \
+                acc[t][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(rb[buf][s4][t], ra[buf][s4][j], acc[t][j], 0, 0, 0);
     RAD_LOAD(0, 0)
 #pragma unroll 1
     for (int k0 = 0; k0 < K; k0 += 32) {
@@ -138,13 +141,15 @@ __global__ __launch_bounds__(256) void rad_gemm(const RadArgs* __restrict__ ap, 
 #undef RAD_LOAD
 #undef RAD_MFMA
 #pragma unroll
-    for (int r = 0; r < 4; r++) {
-        const int row = m0 + 4 * r + lk;
-        C[row + (size_t)M * (n0 + lr)] = scale * acc00[r];
-        C[row + (size_t)M * (n0 + 16 + lr)] = scale * acc01[r];
-        C[row + 16 + (size_t)M * (n0 + lr)] = scale * acc10[r];
-        C[row + 16 + (size_t)M * (n0 + 16 + lr)] = scale * acc11[r];
-    }
+    for (int t = 0; t < 2; t++)
+#pragma unroll
+        for (int j = 0; j < MT; j++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int n = n0 + 16 * t + 4 * r + lk;
+                const int m = m0 + 16 * j + lr;
+                C[m + (size_t)M * n] = scale * acc[t][j][r];
+            }
 }
 
 // ---------------------------------------------------------------- C: responses per r
@@ -375,13 +380,17 @@ __global__ void rad_step_end(const RadArgs* __restrict__ ap) {
 }
 
 // Launch helpers for one RHS and one step (also used during graph capture).
+#ifndef RAD_MT
+#define RAD_MT 2  // MFMA tiles per wave along t (n_to is a multiple of 32·RAD_MT); 2 measured faster than 4
+#endif
+
 template <int MO>
 static void rad_launch_rhs(const RadArgs* dargs, const double2* tw, int n_to, int n_r, int stage, hipStream_t s) {
-    dim3 g(n_to / 64, n_r / 64);
+    dim3 g(n_to / (32 * RAD_MT), n_r / 64);
     hipLaunchKernelGGL((rad_stage_a<MO>), dim3(n_r), dim3(LH_T), 0, s, dargs, tw, stage);
-    hipLaunchKernelGGL(rad_gemm, g, dim3(256), 0, s, dargs, 0);
+    hipLaunchKernelGGL((rad_gemm<RAD_MT>), g, dim3(256), 0, s, dargs, 0);
     hipLaunchKernelGGL((rad_pointwise<MO>), dim3(n_r), dim3(LH_T), 0, s, dargs);
-    hipLaunchKernelGGL(rad_gemm, g, dim3(256), 0, s, dargs, 1);
+    hipLaunchKernelGGL((rad_gemm<RAD_MT>), g, dim3(256), 0, s, dargs, 1);
     hipLaunchKernelGGL((rad_stage_e<MO>), dim3(n_r), dim3(LH_T), 0, s, dargs, tw, stage);
 }
 
