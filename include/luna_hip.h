@@ -160,6 +160,54 @@ int lh_batch_solve_resident(lh_ctx *const *ctxs, int n, const lh_rk_opts *opts, 
 int lh_get_state(lh_ctx *ctx, double *Ew);
 int lh_get_saves(lh_ctx *ctx, double *saves, double *zsaves);
 
+/* Per-step statistics of a mode-averaged propagation: the set that
+ * Stats.default builds (src/Stats.jl:476-532) and collect_stats evaluates
+ * (src/Stats.jl:454-468) from the analytic field (plan_analytic,
+ * src/Stats.jl:421-448) on every accepted step (MemoryOutput,
+ * src/Output.jl:65-67), computed on the device right after the step's window
+ * filter.  One record of LH_NSTAT doubles per accepted step:            */
+enum {
+    LH_STAT_Z = 0,                 /* z   (zdz!, src/Stats.jl:409-412)          */
+    LH_STAT_DZ = 1,                /* dz: the next step size                    */
+    LH_STAT_OMEGA0 = 2,            /* ω0: first moment of |Eω|², :18-25         */
+    LH_STAT_ENERGY = 3,            /* energy, :32-43 with Fields.energyfuncs    */
+    LH_STAT_PEAKPOWER = 4,         /* max |Et|² of the analytic field, :85-99   */
+    LH_STAT_PEAKINTENSITY = 5,     /* peakpower / Aeff, :150-154                */
+    LH_STAT_FWHM_T_MIN = 6,        /* Maths.fwhm(t, |Et|², :linear, :min), :182-199 */
+    LH_STAT_FWHM_T_MAX = 7,        /* same, minmax = :max                       */
+    LH_STAT_ELECTRONDENSITY = 8,   /* ρ·(1 - exp(-∫rate dt)), :219-241; NaN without plasma */
+    LH_STAT_PEAK_IONISATION_RATE = 9, /* max rate, same function; NaN without plasma */
+    LH_STAT_DENSITY = 10,          /* ρ (density(dfun), :325-329)               */
+    LH_NSTAT = 11
+};
+
+/* What the statistics need beyond the transform's own constants. */
+typedef struct lh_stats_desc {
+    int32_t capacity;         /* records kept per run: the first `capacity`
+                                 accepted steps                                   */
+    int32_t reserved;         /* 0                                               */
+    const double *energy_w;   /* [n_omega] energy = Σ_k energy_w[k]·|Eω_k|²: the
+                                 quadrature of Fields.energyfuncs' energy_ω
+                                 (src/Fields.jl:510-534) with its prefactor      */
+    const double *t;          /* [n_t] grid.t, the coarse time axis (fwhm_t)     */
+    double aeff;              /* effective area Aeff (constant in z)             */
+} lh_stats_desc;
+
+/* Switch per-step statistics on for later solves of ctx (sd = NULL: off).
+ * Mode-averaged contexts only (LH_E_ARG for radial ones).  Host arrays are
+ * copied. */
+int lh_set_step_stats(lh_ctx *ctx, const lh_stats_desc *sd);
+
+/* One record for a given state: the stats closure called directly,
+ * stats(Eω, z, dz) (src/Stats.jl:460-467).  Ew: complex [n_omega]; rec: [LH_NSTAT].
+ * Synchronous; requires lh_set_step_stats. */
+int lh_stats_eval(lh_ctx *ctx, const double *Ew, double z, double dz, double *rec);
+
+/* Records of the last solve of ctx: copies min(kept, capacity) records into
+ * rec[capacity][LH_NSTAT] and sets *n to that count; kept is the smaller of
+ * the run's accepted steps and the capacity given to lh_set_step_stats. */
+int lh_get_step_stats(lh_ctx *ctx, double *rec, int capacity, int *n);
+
 /* Measurement hook (no counterpart in the reference): HIP-event durations of
  * the persistent launches of the last batch, one per grid-size class
  * (field grid Nto = 16384, 8192, 4096; envelope grid Nto = 8192, 4096; all

@@ -13,6 +13,10 @@ LH_OK, LH_E_ARG, LH_E_HIP, LH_E_PPT_RANGE, LH_E_REPEAT_LIMIT, LH_E_NONFINITE, LH
 LH_GRID_REAL, LH_GRID_ENV = 0, 1
 LH_TRANS_MODEAVG, LH_TRANS_RADIAL = 0, 1
 LH_KERR_NONE, LH_KERR_FIELD, LH_KERR_ENV = 0, 1, 2
+# per-step statistics record: field order of LH_STAT_* (keys as Stats.jl names them)
+STAT_KEYS = ("z", "dz", "ω0", "energy", "peakpower", "peakintensity", "fwhm_t_min", "fwhm_t_max",
+             "electrondensity", "peak_ionisation_rate", "density")
+LH_NSTAT = len(STAT_KEYS)
 
 _dp = C.POINTER(C.c_double)
 _u8p = C.POINTER(C.c_uint8)
@@ -51,6 +55,12 @@ class LhStats(C.Structure):
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class LhStatsDesc(C.Structure):
+    """``lh_stats_desc`` of include/luna_hip.h."""
+    _fields_ = [("capacity", C.c_int32), ("reserved", C.c_int32), ("energy_w", C.POINTER(C.c_double)),
+                ("t", C.POINTER(C.c_double)), ("aeff", C.c_double)]
 
 
 class LhFnoCfg(C.Structure):
@@ -146,6 +156,9 @@ def lib():
     L.lh_batch_solve_resident.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.POINTER(LhRkOpts), C.c_int,
                                           C.POINTER(LhStats)]
     L.lh_last_launch_times.argtypes = [_dp, C.POINTER(C.c_int), C.c_int]
+    L.lh_set_step_stats.argtypes = [C.c_void_p, C.POINTER(LhStatsDesc)]
+    L.lh_stats_eval.argtypes = [C.c_void_p, _dp, C.c_double, C.c_double, _dp]
+    L.lh_get_step_stats.argtypes = [C.c_void_p, _dp, C.c_int, C.POINTER(C.c_int)]
     _fp = C.POINTER(C.c_float)
     L.lh_fno_create.argtypes = [C.POINTER(C.c_void_p), C.POINTER(LhFnoCfg)]
     L.lh_fno_destroy.argtypes = [C.c_void_p]
@@ -163,6 +176,7 @@ def lib():
 EXPORTED = ["lh_strerror", "lh_device_count", "lh_create", "lh_destroy", "lh_rhs",
             "lh_rhs_device", "lh_ctx_status", "lh_solve", "lh_batch_solve", "lh_set_state",
             "lh_batch_solve_resident", "lh_get_state", "lh_get_saves", "lh_last_launch_times",
+            "lh_set_step_stats", "lh_stats_eval", "lh_get_step_stats",
             "lh_fno_create", "lh_fno_destroy", "lh_fno_param_count", "lh_fno_set_params",
             "lh_fno_get_params", "lh_fno_forward", "lh_fno_loss_grad", "lh_fno_set_data", "lh_fno_train"]
 

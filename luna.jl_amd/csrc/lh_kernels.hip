@@ -69,6 +69,14 @@ struct DInst {
     // single-RHS I/O
     const double2* rhs_in;
     double2* rhs_out;
+    // per-step statistics (steprec null: off)
+    const double* st_omega;  // [Nω] ω
+    const double* st_ew;     // [Nω] energy quadrature weights
+    const double* st_t;      // [Nt] grid.t
+    double st_aeff;          // Aeff
+    double st_ediv;          // sqrt(ε0·c·Aeff/2)
+    double* steprec;         // [steprec_cap][LH_NSTAT]
+    int steprec_cap;
 };
 
 // Dormand–Prince tableau, src/dopri.jl:1-28 (rational literals evaluated in double).
@@ -731,6 +739,179 @@ struct Sizes {
     static constexpr int NPT = (N + LH_T - 1) / LH_T;
 };
 
+// ==========================================================================
+// Per-step statistics (Stats.default + collect_stats, src/Stats.jl:454-532)
+// ==========================================================================
+// Maths.linterpx, src/Maths.jl:248-252.
+__device__ __forceinline__ double linterpx(double x1, double x2, double y1, double y2, double val) {
+    double slope = (y2 - y1) / (x2 - x1);
+    double icpt = y1 - slope * x1;
+    return (val - icpt) / slope;
+}
+
+// One statistics record of the state Y (complex [Nω], HBM) at (z, dz).
+//   frequency domain: ω0 = Σω|E|²/Σ|E|² (Maths.moment), energy = Σ w|E|²
+//   analytic field (plan_analytic, src/Stats.jl:421-448): field grid
+//     Et = ifft([2E_0 .. 2E_{Nω-2}, 0 ..]) over Nt = 2(Nω-1) samples;
+//     envelope grid Et = ifft(E).  One inverse FFT of length Nt = MO in LDS.
+//   peakpower = max|Et|², peakintensity = peakpower/Aeff,
+//   fwhm_t via Maths.level_xings (:linear, src/Maths.jl:162-214),
+//   electrondensity (field grid with plasma): rate = PPT(|Re Et|/√(ε0cAeff/2)),
+//     ρ·(1 - exp(-cumtrapz(rate, δt)[end])), peak rate = max rate.
+// Writes rec[LH_NSTAT] from thread 0; a PPT range/NaN error sets sm.flag.
+template <int KIND, int MO, int MC, class Smem>
+__device__ __forceinline__ void step_stats(const DInst& in, Smem& sm, const double2* Y, double z, double dz,
+                                           double* rec) {
+    constexpr int N = Sizes<KIND, MO, MC>::N;
+    constexpr int NT = MO;  // analytic-signal samples = length of grid.t
+    const int tid = fresh_tid();
+    const int lane = tid & 63, wid = tid >> 6;
+    double s0 = 0.0, s1 = 0.0, se = 0.0;
+    for (int k = tid; k < NT; k += LH_T) {
+        double2 v = (k < N) ? Y[k] : make_double2(0.0, 0.0);
+        if (k < N) {
+            double a = v.x * v.x + v.y * v.y;
+            s0 += a;
+            s1 += in.st_omega[k] * a;
+            se += in.st_ew[k] * a;
+        }
+        if constexpr (KIND == 0)  // copyto_fft!: bins 0..Nω-2 doubled, the +fs/2 bin dropped
+            sm.buf[pad(k)] = (k < N - 1) ? make_double2(2.0 * v.x, 2.0 * v.y) : make_double2(0.0, 0.0);
+        else
+            sm.buf[pad(k)] = v;
+    }
+    block_sum3(s0, s1, se, sm.wsum);  // ends synchronised: the spectrum is complete
+    fft_lds<NT, true>(sm.buf, sm.tw);
+    // pass 1: P = |Et|² (kept in .x), rate (kept in .y); argmax, max/sum of rate
+    const double inv = 1.0 / (double)NT;  // FFTW's ifft normalisation
+    const bool ion = (KIND == 0) && in.plasma;
+    double pmax = -1.0, rmax = 0.0, rsum = 0.0;
+    int imax = NT, bad = LH_OK;
+    for (int j = tid; j < NT; j += LH_T) {
+        double2 e = sm.buf[pad(j)];
+        double re = e.x * inv, im = e.y * inv;
+        double p = re * re + im * im;
+        double r = 0.0;
+        if (ion) {
+            r = ppt_rate(in, fabs(re / in.st_ediv), bad);
+            rmax = fmax(rmax, r);
+            rsum += r;
+        }
+        if (p > pmax) {  // j increases within a thread: the first maximum is kept
+            pmax = p;
+            imax = j;
+        }
+        sm.buf[pad(j)] = make_double2(p, r);
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        double p2 = __shfl_xor(pmax, off, 64);
+        int i2 = __shfl_xor(imax, off, 64);
+        if (p2 > pmax || (p2 == pmax && i2 < imax)) {
+            pmax = p2;
+            imax = i2;
+        }
+        rmax = fmax(rmax, __shfl_xor(rmax, off, 64));
+        rsum += __shfl_xor(rsum, off, 64);
+    }
+    if (lane == 0) {
+        sm.wsum[4 * wid + 0] = pmax;
+        sm.wsum[4 * wid + 1] = (double)imax;
+        sm.wsum[4 * wid + 2] = rmax;
+        sm.wsum[4 * wid + 3] = rsum;
+    }
+    if (bad != LH_OK) sm.flag = bad;
+    __syncthreads();
+    pmax = sm.wsum[0];
+    imax = (int)sm.wsum[1];
+    rmax = sm.wsum[2];
+    rsum = sm.wsum[3];
+    for (int w = 1; w < LH_NW; w++) {
+        double p2 = sm.wsum[4 * w];
+        int i2 = (int)sm.wsum[4 * w + 1];
+        if (p2 > pmax || (p2 == pmax && i2 < imax)) {
+            pmax = p2;
+            imax = i2;
+        }
+        rmax = fmax(rmax, sm.wsum[4 * w + 2]);
+        rsum += sm.wsum[4 * w + 3];
+    }
+    __syncthreads();
+    // pass 2: level crossings around the maximum at val = 0.5·max (t increasing,
+    // so x < xmax ⇔ j < imax):  :min takes the last j < imax with P < val and the
+    // first j > imax with P < val; :max the first j < imax with P > val and the
+    // last j > imax with P > val.
+    const double val = 0.5 * pmax;
+    int aL = -1, aR = NT, bL = NT, bR = -1;
+    for (int j = tid; j < NT; j += LH_T) {
+        double p = sm.buf[pad(j)].x;
+        if (j < imax) {
+            if (p < val) aL = max(aL, j);
+            if (p > val) bL = min(bL, j);
+        } else if (j > imax) {
+            if (p < val) aR = min(aR, j);
+            if (p > val) bR = max(bR, j);
+        }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        aL = max(aL, __shfl_xor(aL, off, 64));
+        aR = min(aR, __shfl_xor(aR, off, 64));
+        bL = min(bL, __shfl_xor(bL, off, 64));
+        bR = max(bR, __shfl_xor(bR, off, 64));
+    }
+    if (lane == 0) {
+        sm.wsum[4 * wid + 0] = (double)aL;
+        sm.wsum[4 * wid + 1] = (double)aR;
+        sm.wsum[4 * wid + 2] = (double)bL;
+        sm.wsum[4 * wid + 3] = (double)bR;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        for (int w = 0; w < LH_NW; w++) {
+            aL = max(aL, (int)sm.wsum[4 * w + 0]);
+            aR = min(aR, (int)sm.wsum[4 * w + 1]);
+            bL = min(bL, (int)sm.wsum[4 * w + 2]);
+            bR = max(bR, (int)sm.wsum[4 * w + 3]);
+        }
+        const double* t = in.st_t;
+        const double nan = __longlong_as_double(0x7ff8000000000000LL);
+        // a missing crossing or a neighbour off the grid is the reference's
+        // caught exception: NaN
+        double fmin_ = nan, fmax_ = nan;
+        if (imax < NT && aL >= 0 && aR < NT) {
+            double left = linterpx(t[aL], t[aL + 1], sm.buf[pad(aL)].x, sm.buf[pad(aL + 1)].x, val);
+            double right = linterpx(t[aR - 1], t[aR], sm.buf[pad(aR - 1)].x, sm.buf[pad(aR)].x, val);
+            fmin_ = fabs(right - left);
+        }
+        if (imax < NT && bL >= 1 && bL < NT && bR >= 0 && bR + 1 < NT) {
+            double left = linterpx(t[bL - 1], t[bL], sm.buf[pad(bL - 1)].x, sm.buf[pad(bL)].x, val);
+            double right = linterpx(t[bR], t[bR + 1], sm.buf[pad(bR)].x, sm.buf[pad(bR + 1)].x, val);
+            fmax_ = fabs(right - left);
+        }
+        double ed = nan, pr = nan;
+        if (ion) {
+            // cumtrapz!(out, δt) end value: Σ ½(r_{j-1} + r_j)·δt
+            const double dt = t[1] - t[0];
+            double integral = (rsum - 0.5 * (sm.buf[pad(0)].y + sm.buf[pad(NT - 1)].y)) * dt;
+            ed = (1.0 - exp(-integral)) * in.density;
+            pr = rmax;
+        }
+        rec[LH_STAT_Z] = z;
+        rec[LH_STAT_DZ] = dz;
+        rec[LH_STAT_OMEGA0] = s1 / s0;
+        rec[LH_STAT_ENERGY] = se;
+        rec[LH_STAT_PEAKPOWER] = pmax;
+        rec[LH_STAT_PEAKINTENSITY] = pmax / in.st_aeff;
+        rec[LH_STAT_FWHM_T_MIN] = fmin_;
+        rec[LH_STAT_FWHM_T_MAX] = fmax_;
+        rec[LH_STAT_ELECTRONDENSITY] = ed;
+        rec[LH_STAT_PEAK_IONISATION_RATE] = pr;
+        rec[LH_STAT_DENSITY] = in.density;
+    }
+    __syncthreads();
+}
+
 template <int KIND, int MO, int MC, class Smem>
 __device__ __forceinline__ void do_prologue(const DInst& in, Smem& sm, const double2* Y, double2* const* K, int stage,
                                             double dt, double h0, double hf) {
@@ -872,6 +1053,11 @@ __device__ __forceinline__ void solve_instance(const DInst& in, Smem& sm) {
             K[0] = K[6];
             K[6] = tmp;
             do_stepfun<KIND, MO, MC, Smem>(in, sm, YN, tn - t);
+            // MemoryOutput's statistics of the filtered state (src/Output.jl:65-67)
+            if (in.steprec && accepted < in.steprec_cap) {
+                step_stats<KIND, MO, MC, Smem>(in, sm, YN, tn, dtn, in.steprec + (size_t)accepted * LH_NSTAT);
+                if (sm.flag != LH_OK) status = sm.flag;
+            }
             // GridCondition saves via dense output; ks[1] and ks[7] both hold k7
             while (saved < in.save_n && in.zgrid[saved] <= tn) {
                 double ti = in.zgrid[saved];
@@ -978,6 +1164,19 @@ __global__ __launch_bounds__(LH_T) void lh_rhs_kernel(const DInst* insts, const 
     if (threadIdx.x == 0) status[blockIdx.x] = sm.flag;
 }
 
+// Statistics of one given state (rhs_in) into steprec[0..LH_NSTAT).
+template <int KIND, int MO, int MC>
+__global__ __launch_bounds__(LH_T) void lh_stats_kernel(const DInst* insts, const double2* __restrict__ twg,
+                                                        double z, double dz, int* status) {
+    __shared__ SmemT<MO> sm;
+    tw_load(sm.tw, twg);
+    if (threadIdx.x == 0) sm.flag = LH_OK;
+    __syncthreads();
+    const DInst& in = insts[blockIdx.x];
+    step_stats<KIND, MO, MC, SmemT<MO>>(in, sm, in.rhs_in, z, dz, in.steprec);
+    if (threadIdx.x == 0) status[blockIdx.x] = sm.flag;
+}
+
 // Size classes with a compiled kernel.
 enum {
     LH_CLASS_16384 = 0,    // field grid, Nto = 16384, Nt = 8192
@@ -1021,6 +1220,17 @@ static void launch_rhs(int cls, const DInst* dinst, const double2* tw, int* stat
     }
 }
 
+static void launch_stats(int cls, const DInst* dinst, const double2* tw, double z, double dz, int* status,
+                         hipStream_t s) {
+    switch (cls) {
+        case LH_CLASS_16384: hipLaunchKernelGGL((lh_stats_kernel<0, 8192, 4096>), dim3(1), dim3(LH_T), 0, s, dinst, tw, z, dz, status); break;
+        case LH_CLASS_8192: hipLaunchKernelGGL((lh_stats_kernel<0, 4096, 2048>), dim3(1), dim3(LH_T), 0, s, dinst, tw, z, dz, status); break;
+        case LH_CLASS_4096: hipLaunchKernelGGL((lh_stats_kernel<0, 2048, 1024>), dim3(1), dim3(LH_T), 0, s, dinst, tw, z, dz, status); break;
+        case LH_CLASS_ENV8192: hipLaunchKernelGGL((lh_stats_kernel<1, 8192, 8192>), dim3(1), dim3(LH_T), 0, s, dinst, tw, z, dz, status); break;
+        default: hipLaunchKernelGGL((lh_stats_kernel<1, 4096, 4096>), dim3(1), dim3(LH_T), 0, s, dinst, tw, z, dz, status); break;
+    }
+}
+
 static void launch_raman_hw(int n_to, const double* h, double dt, const double2* tw, double2* hwdt, hipStream_t s) {
     if (n_to == 8192)
         hipLaunchKernelGGL((lh_raman_hw_kernel<8192>), dim3(1), dim3(LH_T), 0, s, h, dt, tw, hwdt);
@@ -1037,6 +1247,7 @@ static void launch_raman_hw(int n_to, const double* h, double dt, const double2*
 static const double EPS0 = 8.854187817620389e-12;  // CODATA 2014 ε0
 static const double ECHARGE = 1.6021766208e-19;
 static const double MELEC = 9.10938356e-31;
+static const double CLIGHT = 299792458.0;
 
 struct lh_ctx {
     int device;
@@ -1081,6 +1292,15 @@ struct lh_ctx {
     RadArgs* rargs;      // [0]: solver, [1]: single RHS
     RadCtl* rctl;        // [0]: solver, [1]: single RHS
     hipGraphExec_t rgraph;
+    // per-step statistics
+    double* omega;       // [Nω] ω (shared)
+    double* st_ew;       // [Nω] energy weights (shared)
+    double* st_t;        // [Nt] grid.t (shared)
+    double st_aeff, st_ediv;
+    int st_on, st_cap;
+    double* steprec;     // [st_cap][LH_NSTAT]
+    double* st_rec1;     // [LH_NSTAT], lh_stats_eval
+    int result_nrec;     // records kept by the last solve
 };
 
 static std::mutex g_mu;
@@ -1291,6 +1511,13 @@ static DInst base_inst(const lh_ctx* c) {
     in.linop = c->linop;
     in.outfac = c->outfac;
     in.ppt = c->ppt;
+    in.st_omega = c->omega;
+    in.st_ew = c->st_ew;
+    in.st_t = c->st_t;
+    in.st_aeff = c->st_aeff;
+    in.st_ediv = c->st_ediv;
+    in.steprec = c->st_on ? c->steprec : nullptr;
+    in.steprec_cap = c->st_on ? c->st_cap : 0;
     return in;
 }
 
@@ -1517,10 +1744,12 @@ int lh_destroy(lh_ctx* c) {
     if (!c) return LH_OK;
     hipSetDevice(c->device);
     if (c->stream) hipStreamSynchronize(c->stream);
-    for (const void* p : {(const void*)c->omega_win, (const void*)c->t_win, (const void*)c->to_win, (const void*)c->ppt})
+    for (const void* p : {(const void*)c->omega_win, (const void*)c->t_win, (const void*)c->to_win, (const void*)c->ppt,
+                          (const void*)c->omega, (const void*)c->st_ew, (const void*)c->st_t})
         shared_release(p);
     void* ptrs[] = {c->linop, c->outfac, c->hwdt, c->escr, c->Yin, c->Y, c->YN,
-                    c->saves, c->zgrid, c->dstats, c->dinst, c->rhs_in, c->rhs_out, c->dstatus};
+                    c->saves, c->zgrid, c->dstats, c->dinst, c->rhs_in, c->rhs_out, c->dstatus,
+                    c->steprec, c->st_rec1};
     for (void* p : ptrs)
         if (p) hipFree(p);
     for (int j = 0; j < 7; j++)
@@ -1551,6 +1780,7 @@ int lh_create(lh_ctx** out, const lh_desc* d) {
     if (!st && hipStreamCreate(&c->stream) != hipSuccess) st = LH_E_HIP;
     const size_t N = (size_t)d->n_omega, NS = (size_t)c->n_state;
     if (!st) st = shared_upload(c->device, &c->omega_win, d->omega_win, N);
+    if (!st && d->omega) st = shared_upload(c->device, &c->omega, d->omega, N);
     if (!st) st = shared_upload(c->device, &c->t_win, d->t_win, (size_t)d->n_t);
     if (!st) st = shared_upload(c->device, &c->to_win, d->to_win, (size_t)d->n_to);
     if (!st) {
@@ -1810,7 +2040,79 @@ int lh_batch_solve_resident(lh_ctx* const* ctxs, int n, const lh_rk_opts* o, int
         stats[i] = (ctx_class(ctxs[i]) == LH_CLASS_RADIAL) ? rad_stats[i] : hs[pos[i]];
         ctxs[i]->last_status = stats[i].status;
         ctxs[i]->have_result = 1;
+        ctxs[i]->result_nrec = ctxs[i]->st_on ? (int)(stats[i].accepted < ctxs[i]->st_cap ? stats[i].accepted : ctxs[i]->st_cap) : 0;
     }
+    return LH_OK;
+}
+
+int lh_set_step_stats(lh_ctx* c, const lh_stats_desc* sd) {
+    if (!c) return LH_E_ARG;
+    if (!sd) {
+        c->st_on = 0;
+        return LH_OK;
+    }
+    if (c->d.transform != LH_TRANS_MODEAVG) return LH_E_ARG;
+    HIPCK(hipSetDevice(c->device));
+    if (sd->capacity < 0 || !sd->energy_w || !sd->t || !(sd->aeff > 0) || !c->omega) return LH_E_ARG;
+    double* ew = nullptr;
+    double* tg = nullptr;
+    int st = shared_upload(c->device, &ew, sd->energy_w, (size_t)c->d.n_omega);
+    if (!st) st = shared_upload(c->device, &tg, sd->t, (size_t)c->d.n_t);
+    if (!st && !c->st_rec1 && hipMalloc(&c->st_rec1, sizeof(double) * LH_NSTAT) != hipSuccess) st = LH_E_NOMEM;
+    if (!st && sd->capacity > c->st_cap) {
+        if (c->steprec) (void)hipFree(c->steprec);
+        c->steprec = nullptr;
+        c->st_cap = 0;
+        if (hipMalloc(&c->steprec, sizeof(double) * LH_NSTAT * (size_t)sd->capacity) != hipSuccess) st = LH_E_NOMEM;
+    }
+    if (st) {
+        shared_release(ew);
+        shared_release(tg);
+        return st;
+    }
+    shared_release(c->st_ew);
+    shared_release(c->st_t);
+    c->st_ew = ew;
+    c->st_t = tg;
+    c->st_cap = sd->capacity;
+    c->st_aeff = sd->aeff;
+    c->st_ediv = sqrt(EPS0 * CLIGHT * sd->aeff / 2);  // sqrt(ε_0*c*aeff(z)/2), src/Stats.jl:236
+    c->st_on = 1;
+    c->result_nrec = 0;
+    return LH_OK;
+}
+
+int lh_stats_eval(lh_ctx* c, const double* Ew, double z, double dz, double* rec) {
+    if (!c || !Ew || !rec || !c->st_on) return LH_E_ARG;
+    HIPCK(hipSetDevice(c->device));
+    const size_t bytes = sizeof(double2) * (size_t)c->n_state;
+    if (!c->rhs_in) {
+        HIPCK(hipMalloc(&c->rhs_in, bytes));
+        HIPCK(hipMalloc(&c->rhs_out, bytes));
+    }
+    const double2* tw;
+    int st = twiddles(c->device, &tw);
+    if (st) return st;
+    DInst in = base_inst(c);
+    in.rhs_in = c->rhs_in;
+    in.steprec = c->st_rec1;
+    HIPCK(hipMemcpyAsync(c->rhs_in, Ew, bytes, hipMemcpyHostToDevice, c->stream));
+    HIPCK(hipMemcpyAsync(c->dinst, &in, sizeof(DInst), hipMemcpyHostToDevice, c->stream));
+    launch_stats(ctx_class(c), c->dinst, tw, z, dz, c->dstatus, c->stream);
+    HIPCK(hipGetLastError());
+    int kst = LH_OK;
+    HIPCK(hipMemcpyAsync(rec, c->st_rec1, sizeof(double) * LH_NSTAT, hipMemcpyDeviceToHost, c->stream));
+    HIPCK(hipMemcpyAsync(&kst, c->dstatus, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCK(hipStreamSynchronize(c->stream));
+    return kst;
+}
+
+int lh_get_step_stats(lh_ctx* c, double* rec, int capacity, int* n) {
+    if (!c || !n || capacity < 0 || (!rec && capacity > 0) || !c->have_result) return LH_E_ARG;
+    HIPCK(hipSetDevice(c->device));
+    int k = c->result_nrec < capacity ? c->result_nrec : capacity;
+    if (k > 0) HIPCK(hipMemcpy(rec, c->steprec, sizeof(double) * LH_NSTAT * (size_t)k, hipMemcpyDeviceToHost));
+    *n = k;
     return LH_OK;
 }
 

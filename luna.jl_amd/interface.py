@@ -6,10 +6,12 @@ builders ``:453-616, 773-835``; ``prop_gnlse_args`` ``:921-979``) and the driver
 window filter, saves) runs on the GPU via ``lh_solve``.
 """
 import math
+import warnings
 
 import numpy as np
 
 from . import _capi
+from . import stats as Stats
 from . import physdata as pd
 from .fields import GaussField, SechField, ShotNoise
 from .grid import EnvGrid, RealGrid
@@ -22,10 +24,13 @@ from .nonlinearrhs import TransModeAvg, TransRadial, norm_mode_average, norm_mod
 
 
 class MemoryOutput:
-    """``Output.MemoryOutput(0, zmax, saveN)`` with ``GridCondition`` saves (``Output.jl:15-141``)."""
+    """``Output.MemoryOutput(0, zmax, saveN, statsfun)`` with ``GridCondition``
+    saves (``Output.jl:15-141``); ``statsfun`` from ``stats.default`` is
+    recorded on the device at every accepted step."""
 
-    def __init__(self, tmin, tmax, saveN):
+    def __init__(self, tmin, tmax, saveN, statsfun=None):
         self.tmin, self.tmax, self.saveN = tmin, tmax, saveN
+        self.statsfun = statsfun
         self.data = {"stats": {}, "meta": {}}
 
     def __getitem__(self, k):
@@ -43,7 +48,15 @@ def run(Ew, grid, linop, transform, FT=None, output=None, min_dz=0.0, max_dz=Non
                          max_dz=grid.zmax / 2 if max_dz is None else max_dz, min_dz=min_dz,
                          init_dz=init_dz, save_n=saveN)
     ctx = transform.context(linop)
+    statsfun = getattr(output, "statsfun", None)
+    ctx.set_step_stats(statsfun)
     Ew_out, saves, zs, stats = ctx.solve(Ew, opts)
+    if statsfun is not None:
+        rec = ctx.step_stats()
+        if stats.accepted > len(rec):
+            warnings.warn(f"statistics kept for the first {len(rec)} of {stats.accepted} steps "
+                          "(raise the capacity of stats.default)", RuntimeWarning)
+        output.data["stats"] = statsfun.to_dict(rec)
     if output is not None:
         n = grid.n_omega
         # saves[s] holds the state column-major (ω fastest): → (Nω[, Nr], saveN)
@@ -51,7 +64,7 @@ def run(Ew, grid, linop, transform, FT=None, output=None, min_dz=0.0, max_dz=Non
         output.data["z"] = zs[:saveN]
         output.data["grid"] = {k: getattr(grid, k) for k in ("t", "omega", "to", "omega_o", "sidx",
                                                               "omega_win", "t_win", "to_win")}
-        output.data["stats"] = stats.as_dict()
+        output.data["meta"]["solver"] = stats.as_dict()
     return Ew_out, stats
 
 
@@ -96,7 +109,8 @@ def prop_capillary_args(radius, flength, gas, pressure, *, lambda0, lambda_lims,
     norm = norm_mode_average(grid, beta, aeff)
     transform = TransModeAvg(grid, resp, rho, norm, aeff)
     Ew = sum(f(grid) for f in inputs)
-    output = MemoryOutput(0, grid.zmax, saveN)
+    stats = Stats.default(grid, Ew, mode, linop, transform, gas=gas)
+    output = MemoryOutput(0, grid.zmax, saveN, stats)
     return Ew, grid, linop, transform, None, output
 
 
@@ -134,7 +148,8 @@ def prop_gnlse_args(gamma, flength, betas, *, lambda0, lambda_lims, trange, tau_
     norm = norm_mode_average_gnlse(grid, aeff, shock=shock)
     transform = TransModeAvg(grid, resp, 1.0, norm, aeff)
     Ew = sum(f(grid) for f in inputs)
-    output = MemoryOutput(0, grid.zmax, saveN)
+    stats = Stats.default(grid, Ew, mode, linop, transform)
+    output = MemoryOutput(0, grid.zmax, saveN, stats)
     return Ew, grid, linop, transform, None, output
 
 

@@ -92,6 +92,19 @@ def density(gas, P=1.0, T=roomtemp):
     return rho * N_A
 
 
+def pressure(gas, density, T=roomtemp):
+    """Pressure [bar] of a number density [m^-3]: the inverse of ``density``
+    (replaces ``PhysData.pressure``, ``PhysData.jl:708-711``, CoolProp), i.e.
+    P = ρm R T (1 + B ρm + C ρm²) evaluated directly."""
+    if density == 0:
+        return 0.0
+    B0, dBdT, C, T0 = _VIRIAL[_base(gas)]
+    B = B0 + dBdT * (T - T0)
+    R = k_B * N_A
+    rho = density / N_A
+    return rho * R * T * (1 + B * rho + C * rho * rho) / bar
+
+
 def dens_1bar_0degC(gas):
     return density(gas, 1.0, 273.15)
 

@@ -22,6 +22,7 @@ class Context:
         h = C.c_void_p()
         _capi.check(L.lh_create(C.byref(h), holder.ref), "lh_create")
         self.handle = h
+        self.statsfun = None
 
     def __del__(self):
         h = getattr(self, "handle", None)
@@ -49,6 +50,37 @@ class Context:
                                   _p(saves) if want_saves else None, _p(zs), C.byref(stats))
         _capi.check(st, "lh_solve")
         return Ew, saves, zs, stats
+
+
+    # -- per-step statistics (Stats.default, src/Stats.jl:476-532) ------------
+    def set_step_stats(self, statsfun):
+        """Record ``statsfun``'s values at every accepted step of later solves
+        (None: off)."""
+        L = _capi.lib()
+        if statsfun is None:
+            _capi.check(L.lh_set_step_stats(self.handle, None), "lh_set_step_stats")
+            self.statsfun = None
+            return
+        d = statsfun.desc()
+        _capi.check(L.lh_set_step_stats(self.handle, C.byref(d)), "lh_set_step_stats")
+        self.statsfun = statsfun
+
+    def stats_eval(self, Ew, z=0.0, dz=0.0):
+        """One record (``LH_NSTAT`` values) of the state Ew."""
+        Ew = np.ascontiguousarray(np.asarray(Ew, dtype=np.complex128).reshape(-1))
+        if Ew.size != self.n_state:
+            raise ValueError("Eω has the wrong size")
+        rec = np.empty(_capi.LH_NSTAT)
+        _capi.check(_capi.lib().lh_stats_eval(self.handle, _p(Ew), z, dz, _p(rec)), "lh_stats_eval")
+        return rec
+
+    def step_stats(self, capacity=None):
+        """Records [n][LH_NSTAT] of the last solve."""
+        cap = capacity if capacity is not None else self.statsfun.capacity
+        rec = np.empty((max(cap, 1), _capi.LH_NSTAT))
+        n = C.c_int(0)
+        _capi.check(_capi.lib().lh_get_step_stats(self.handle, _p(rec), cap, C.byref(n)), "lh_get_step_stats")
+        return rec[:n.value].copy()
 
 
 def batch_solve(contexts, Ews, opts, want_saves=True):

@@ -1,0 +1,110 @@
+"""Per-step statistics of a mode-averaged propagation: the drop-in for
+``Stats.default`` (``src/Stats.jl:476-532``) and the ``collect_stats`` closure
+it returns (``src/Stats.jl:454-468``).
+
+The reference evaluates these functions on the host after every accepted step
+(``MemoryOutput``, ``src/Output.jl:65-67``), including an inverse FFT for the
+analytic field and a PPT-rate pass.  Here they run on the GPU inside the
+propagation kernel, right after the step's window filter, and write one record
+of ``_capi.LH_NSTAT`` values per accepted step:
+
+  ω0 (``:18-25``), energy (``:32-43`` with ``Fields.energyfuncs``),
+  peakpower (``:85-99``), peakintensity (``:150-154``), fwhm_t_min/max
+  (``:182-199``), electrondensity and peak_ionisation_rate (``:219-241``, field
+  grid with ``PlasmaCumtrapz`` only), density (``:325-329``), z and dz
+  (``zdz!``, ``:409-412``).
+
+``pressure`` (``:335-339``) is constant in z for the constant-density
+propagations handled here and is filled on the host from the gas model.
+``zdw`` (``Modes.zdw``, a root search on the mode's dispersion) and the
+``windows``/``userfuns``/``onaxis`` options are not provided.
+"""
+import math
+
+import numpy as np
+
+from . import _capi
+from . import physdata as pd
+from .nonlinear import PlasmaCumtrapz
+
+
+def energy_weights(grid):
+    """Weights w with energy_ω(Eω) = Σ w |Eω|², ``Fields.energyfuncs``
+    (``src/Fields.jl:510-534``).
+
+    Field grid: 2π/ω[end]² · ∫|Eω|² dω with NumericalIntegration's SimpsonEven
+    (extended Simpson rule, end weights 17, 59, 43, 49 over 48).  Envelope grid:
+    2π δω/Δω² · Σ|Eω|² with Δω = Nω δω."""
+    w = np.asarray(grid.omega, dtype=float)
+    n = len(w)
+    dw = w[1] - w[0]
+    if grid.kind == "real":
+        if n < 8:
+            raise ValueError("SimpsonEven needs at least 8 points here")
+        prefac = 2 * math.pi / (w[-1] ** 2)
+        q = np.ones(n)
+        q[:4] = np.array([17, 59, 43, 49]) / 48
+        q[-4:] = np.array([49, 43, 59, 17]) / 48
+        return prefac * dw * q
+    Dw = n * dw
+    prefac = 2 * math.pi * dw / (Dw ** 2)
+    return np.full(n, prefac)
+
+
+class StatsFun:
+    """What ``Stats.default`` returns: callable as ``stats(Eω, z, dz)`` → dict,
+    and the description the propagation kernel uses to record the same values
+    at every accepted step (``desc()``)."""
+
+    def __init__(self, grid, transform, capacity=16384, constants=None):
+        if getattr(transform, "kind", None) != _capi.LH_TRANS_MODEAVG:
+            raise NotImplementedError("per-step statistics are provided for mode-averaged transforms")
+        self.grid = grid
+        self.transform = transform
+        self.capacity = int(capacity)
+        self.energy_w = np.ascontiguousarray(energy_weights(grid))
+        self.t = np.ascontiguousarray(grid.t, dtype=float)
+        self.aeff = float(transform.aeff)
+        self.plasma = grid.kind == "real" and any(isinstance(r, PlasmaCumtrapz) for r in transform.responses)
+        self.constants = dict(constants or {})
+
+    def desc(self):
+        d = _capi.LhStatsDesc()
+        d.capacity = self.capacity
+        d.energy_w = _capi._ptr(self.energy_w)
+        d.t = _capi._ptr(self.t)
+        d.aeff = self.aeff
+        self._keep = d  # the arrays stay owned by self
+        return d
+
+    def keys(self):
+        keys = list(_capi.STAT_KEYS)
+        if not self.plasma:
+            keys = [k for k in keys if k not in ("electrondensity", "peak_ionisation_rate")]
+        return keys
+
+    def to_dict(self, records):
+        """Records [n][LH_NSTAT] → {name: array over steps}, plus host constants."""
+        records = np.asarray(records, dtype=float).reshape(-1, _capi.LH_NSTAT)
+        out = {k: records[:, _capi.STAT_KEYS.index(k)].copy() for k in self.keys()}
+        for k, v in self.constants.items():
+            out[k] = np.full(len(records), v)
+        return out
+
+    def __call__(self, Ew, z, dz):
+        ctx = self.transform.context()
+        ctx.set_step_stats(self)
+        rec = ctx.stats_eval(Ew, z, dz)
+        return {k: v[0] for k, v in self.to_dict(rec[None, :]).items()}
+
+
+def default(grid, Ew, mode, linop, transform, *, windows=None, gas=None, onaxis=False, userfuns=(),
+            capacity=16384):
+    """``Stats.default(grid, Eω, mode, linop, transform; gas, windows, onaxis,
+    userfuns)``, ``src/Stats.jl:492-532``, for a mode-averaged transform."""
+    if windows or onaxis or userfuns:
+        raise NotImplementedError("windows, onaxis and userfuns statistics are not provided")
+    constants = {}
+    if gas is not None:
+        constants["pressure"] = pd.pressure(gas, transform.density)
+    return StatsFun(grid, transform, capacity=capacity, constants=constants)

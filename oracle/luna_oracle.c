@@ -21,6 +21,11 @@
  *   - RK45.solve / PreconStepper src/RK45.jl:27-89, 128-309, 343-406, dopri.jl:1-28
  *   - Luna.run stepfun filter    src/Luna.jl:358-391
  *   - MemoryOutput/GridCondition src/Output.jl:65-81, 490-502
+ *   - Stats.default / collect_stats, mode-averaged set, src/Stats.jl:18-258,
+ *     409-412, 421-468, 476-532; Fields.energyfuncs src/Fields.jl:510-534;
+ *     Maths.moment/level_xings/linterpx src/Maths.jl:89-96, 162-252;
+ *     NumericalIntegration.jl 0.3 SimpsonEven (not in the reference tree:
+ *     the extended Simpson rule h/48·(17,59,43,49,48,…,48,49,43,59,17))
  */
 #include <complex.h>
 #include <math.h>
@@ -698,9 +703,136 @@ int orc_solve_precon(int n, const double *linop, const double *y0, double t0, do
     return st;
 }
 
+/* ------------------------------------------------------------------------ */
+/* Per-step statistics                                                       */
+/* ------------------------------------------------------------------------ */
+#define ORC_C_LIGHT 299792458.0
+#define ORC_PI 3.141592653589793
+
+/* NumericalIntegration.integrate(x, y, SimpsonEven()) on a uniform grid. */
+static double simpson_even(int n, double dx, const double *y) {
+    double r = (17 * y[0] + 59 * y[1] + 43 * y[2] + 49 * y[3] + 49 * y[n - 4] + 43 * y[n - 3] + 59 * y[n - 2] +
+                17 * y[n - 1]) / 48;
+    for (int i = 4; i < n - 4; i++) r += y[i];
+    return dx * r;
+}
+
+/* Maths.linterpx, src/Maths.jl:248-252 */
+static double linterpx(double x1, double x2, double y1, double y2, double val) {
+    double slope = (y2 - y1) / (x2 - x1);
+    double icpt = y1 - slope * x1;
+    return (val - icpt) / slope;
+}
+
+/* Maths.level_xings(x, y; method=:linear, minmax), src/Maths.jl:162-214,
+ * returning fwhm = |right - left| (src/Maths.jl:128-131); NaN where the
+ * reference's try/catch returns NaN (no crossing, or a neighbour off the grid). */
+static double fwhm_linear(int n, const double *x, const double *y, int minmax_max) {
+    int maxidx = 0;
+    for (int i = 1; i < n; i++)
+        if (y[i] > y[maxidx]) maxidx = i; /* argmax: first maximum */
+    double val = 0.5 * y[maxidx];
+    double xmax = x[maxidx];
+    int lefti = -1, righti = -1;
+    if (!minmax_max) {
+        for (int i = 0; i < n; i++)
+            if (x[i] < xmax && y[i] < val) lefti = i; /* findlast */
+        for (int i = 0; i < n; i++)
+            if (x[i] > xmax && y[i] < val) { righti = i; break; } /* findfirst */
+        if (lefti < 0 || righti < 0) return NAN;
+        double left = linterpx(x[lefti], x[lefti + 1], y[lefti], y[lefti + 1], val);
+        double right = linterpx(x[righti - 1], x[righti], y[righti - 1], y[righti], val);
+        return fabs(right - left);
+    }
+    for (int i = 0; i < n; i++)
+        if (x[i] < xmax && y[i] > val) { lefti = i; break; } /* findfirst */
+    for (int i = 0; i < n; i++)
+        if (x[i] > xmax && y[i] > val) righti = i; /* findlast */
+    if (lefti < 1 || righti < 0 || righti + 1 >= n) return NAN; /* BoundsError → NaN */
+    double left = linterpx(x[lefti - 1], x[lefti], y[lefti - 1], y[lefti], val);
+    double right = linterpx(x[righti], x[righti + 1], y[righti], y[righti + 1], val);
+    return fabs(right - left);
+}
+
+/* collect_stats(Eω, z, dz) with the Stats.default functions of a mode-averaged
+ * propagation (src/Stats.jl:454-532): analytic!, then ω0, energy, peakpower,
+ * fwhm_t, density, peakintensity(aeff), electrondensity (plasma), zdz!.
+ * t: grid.t [n_t]; aeff: Aeff; rec[LH_NSTAT]. */
+static int step_stats(const lh_desc *d, const double *t, double aeff, const cplx *Ew, double z, double dz,
+                      double *rec) {
+    const int N = d->n_omega, nt = d->n_t;
+    const double *w = d->omega;
+    double *y = (double *)malloc(sizeof(double) * (size_t)(N > nt ? N : nt));
+    cplx *Et = (cplx *)calloc((size_t)nt, sizeof(cplx));
+    double *out = (double *)malloc(sizeof(double) * (size_t)nt);
+    double *frac = (double *)malloc(sizeof(double) * (size_t)nt);
+    int st = LH_OK;
+    /* ω0: Maths.moment(ω, abs2.(Eω)), src/Stats.jl:18-25, src/Maths.jl:89-96 */
+    double sxy = 0.0, sy = 0.0;
+    for (int i = 0; i < N; i++) {
+        y[i] = creal(Ew[i]) * creal(Ew[i]) + cimag(Ew[i]) * cimag(Ew[i]);
+        sxy += w[i] * y[i];
+        sy += y[i];
+    }
+    rec[LH_STAT_OMEGA0] = sxy / sy;
+    /* energy: Fields.energyfuncs energy_ω, src/Fields.jl:510-534 */
+    if (d->grid_kind == LH_GRID_REAL) {
+        double prefac = 2 * ORC_PI / (w[N - 1] * w[N - 1]);
+        rec[LH_STAT_ENERGY] = prefac * simpson_even(N, w[1] - w[0], y);
+    } else {
+        double dw = w[1] - w[0];
+        double Dw = (double)N * dw;
+        double prefac = 2 * ORC_PI * dw / (Dw * Dw);
+        double s = 0.0;
+        for (int i = 0; i < N; i++) s += y[i];
+        rec[LH_STAT_ENERGY] = prefac * s;
+    }
+    /* analytic field: plan_analytic, src/Stats.jl:421-448 */
+    if (d->grid_kind == LH_GRID_REAL) {
+        for (int i = 0; i < N - 1; i++) Et[i] = 2 * Ew[i]; /* copyto_fft! */
+    } else {
+        for (int i = 0; i < N; i++) Et[i] = Ew[i];
+    }
+    fft_c2c(nt, Et, +1);
+    double inv = 1.0 / (double)nt;
+    for (int i = 0; i < nt; i++) Et[i] = inv * creal(Et[i]) + I * (inv * cimag(Et[i]));
+    /* peakpower, fwhm_t, peakintensity: src/Stats.jl:85-99, 150-154, 182-199 */
+    for (int i = 0; i < nt; i++) y[i] = creal(Et[i]) * creal(Et[i]) + cimag(Et[i]) * cimag(Et[i]);
+    double pmax = y[0];
+    for (int i = 1; i < nt; i++)
+        if (y[i] > pmax) pmax = y[i];
+    rec[LH_STAT_PEAKPOWER] = pmax;
+    rec[LH_STAT_FWHM_T_MIN] = fwhm_linear(nt, t, y, 0);
+    rec[LH_STAT_FWHM_T_MAX] = fwhm_linear(nt, t, y, 1);
+    rec[LH_STAT_DENSITY] = d->density;
+    rec[LH_STAT_PEAKINTENSITY] = pmax / aeff;
+    /* electrondensity(grid, ionrate, dfun, aeff), src/Stats.jl:219-241 */
+    rec[LH_STAT_ELECTRONDENSITY] = NAN;
+    rec[LH_STAT_PEAK_IONISATION_RATE] = NAN;
+    if (d->grid_kind == LH_GRID_REAL && d->plasma) {
+        double ediv = sqrt(NLSCALE_EPS0 * ORC_C_LIGHT * aeff / 2);
+        for (int i = 0; i < nt; i++) y[i] = creal(Et[i]) / ediv; /* real(Eto ./ ediv) */
+        st = ppt_rate(d, nt, y, out);
+        if (st == LH_OK) {
+            double ratemax = out[0];
+            for (int i = 1; i < nt; i++)
+                if (out[i] > ratemax) ratemax = out[i];
+            cumtrapz(nt, out, t[1] - t[0], frac);
+            double fend = 1 - exp(-frac[nt - 1]);
+            rec[LH_STAT_ELECTRONDENSITY] = fend * d->density;
+            rec[LH_STAT_PEAK_IONISATION_RATE] = ratemax;
+        }
+    }
+    rec[LH_STAT_Z] = z;
+    rec[LH_STAT_DZ] = dz;
+    free(y); free(Et); free(out); free(frac);
+    return st;
+}
+
 /* Luna.run + solve_precon + stepfun filter + GridCondition saves. */
-int orc_solve(const lh_desc *d, double *Ew_inout, const lh_rk_opts *o, double *saves, double *zsaves,
-              lh_stats *stats) {
+static int solve_impl(const lh_desc *d, double *Ew_inout, const lh_rk_opts *o, double *saves, double *zsaves,
+                      lh_stats *stats, const double *tgrid, double aeff, double *rec, int cap, int *nrec) {
+    if (nrec) *nrec = 0;
     rhs_user u;
     u.d = d;
     ws_init(&u.w, d);
@@ -749,6 +881,11 @@ int orc_solve(const lh_desc *d, double *Ew_inout, const lh_rk_opts *o, double *s
                     memcpy(col, Etc, sizeof(cplx) * (size_t)N);
                 }
             }
+            /* MemoryOutput: statistics of the filtered state, src/Output.jl:65-67 */
+            if (rec && stats->accepted <= cap) {
+                st = step_stats(d, tgrid, aeff, s.yn, s.tn, s.dtn, rec + (size_t)(stats->accepted - 1) * LH_NSTAT);
+                *nrec = (int)stats->accepted;
+            }
             /* MemoryOutput + GridCondition: src/Output.jl:65-81, 497-502 */
             while (saved < o->save_n && grid[saved] <= s.tn) {
                 if (saves) interpolate(&s, grid[saved], (cplx *)saves + (size_t)saved * n);
@@ -772,4 +909,28 @@ int orc_solve(const lh_desc *d, double *Ew_inout, const lh_rk_opts *o, double *s
     stepper_free(&s);
     ws_free(&u.w);
     return st;
+}
+
+int orc_solve(const lh_desc *d, double *Ew_inout, const lh_rk_opts *o, double *saves, double *zsaves,
+              lh_stats *stats) {
+    return solve_impl(d, Ew_inout, o, saves, zsaves, stats, NULL, 0.0, NULL, 0, NULL);
+}
+
+/* Same with per-step statistics: rec[cap][LH_NSTAT] receives the records of
+ * the first cap accepted steps, *nrec their count. */
+int orc_solve_stats(const lh_desc *d, double *Ew_inout, const lh_rk_opts *o, double *saves, double *zsaves,
+                    lh_stats *stats, const double *t, double aeff, double *rec, int cap, int *nrec) {
+    return solve_impl(d, Ew_inout, o, saves, zsaves, stats, t, aeff, rec, cap, nrec);
+}
+
+/* One record of a given state (the stats closure called directly). */
+int orc_step_stats(const lh_desc *d, const double *t, double aeff, const double *Ew, double z, double dz,
+                   double *rec) {
+    return step_stats(d, t, aeff, (const cplx *)Ew, z, dz, rec);
+}
+
+/* Maths.fwhm(x, y; method=:linear, minmax), src/Maths.jl:128-131. */
+int orc_fwhm(int n, const double *x, const double *y, int minmax_max, double *out) {
+    *out = fwhm_linear(n, x, y, minmax_max);
+    return LH_OK;
 }

@@ -37,6 +37,10 @@ def lib(capi):
     L.orc_responses_real.argtypes = [D, dp, dp]
     L.orc_rhs.argtypes = [D, dp, dp, C.c_double]
     L.orc_solve.argtypes = [D, dp, C.POINTER(capi.LhRkOpts), dp, dp, C.POINTER(capi.LhStats)]
+    L.orc_step_stats.argtypes = [D, dp, C.c_double, dp, C.c_double, C.c_double, dp]
+    L.orc_solve_stats.argtypes = [D, dp, C.POINTER(capi.LhRkOpts), dp, dp, C.POINTER(capi.LhStats), dp,
+                                  C.c_double, dp, C.c_int, C.POINTER(C.c_int)]
+    L.orc_fwhm.argtypes = [C.c_int, dp, dp, C.c_int, dp]
     L.ORC_RHS_CB = C.CFUNCTYPE(C.c_int, C.c_void_p, dp, dp, C.c_double)
     L.orc_solve_precon.argtypes = [C.c_int, dp, dp, C.c_double, C.c_double, C.c_double, C.c_double,
                                    C.c_double, L.ORC_RHS_CB, C.c_void_p, C.c_int, dp, dp,
@@ -113,6 +117,36 @@ class Oracle:
         st = self.L.orc_solve(holder.ref, _p(Ew), C.byref(opts), _p(saves) if want_saves else None,
                               _p(zs), C.byref(stats))
         return st, Ew, saves, zs, stats
+
+    def fwhm(self, x, y, minmax="min"):
+        x = np.ascontiguousarray(x, dtype=float)
+        y = np.ascontiguousarray(y, dtype=float)
+        out = C.c_double(0.0)
+        self.L.orc_fwhm(len(x), _p(x), _p(y), int(minmax == "max"), C.byref(out))
+        return out.value
+
+    def step_stats(self, holder, t, aeff, Ew, z=0.0, dz=0.0):
+        """One statistics record (``capi.LH_NSTAT`` values) of the state Ew."""
+        Ew = np.ascontiguousarray(Ew, dtype=np.complex128)
+        t = np.ascontiguousarray(t, dtype=float)
+        rec = np.zeros(self.capi.LH_NSTAT)
+        st = self.L.orc_step_stats(holder.ref, _p(t), float(aeff), _p(Ew), z, dz, _p(rec))
+        return st, rec
+
+    def solve_stats(self, holder, Ew0, opts, t, aeff, cap=4096, want_saves=False):
+        """``solve`` that also returns the per-step statistics records [n][LH_NSTAT]."""
+        d = holder.desc
+        n = d.n_omega * d.n_r
+        Ew = np.ascontiguousarray(np.asarray(Ew0, dtype=np.complex128).reshape(-1, order="F")).copy()
+        saves = np.zeros((opts.save_n, n), dtype=np.complex128) if want_saves else None
+        zs = np.zeros(max(opts.save_n, 1))
+        stats = self.capi.LhStats()
+        t = np.ascontiguousarray(t, dtype=float)
+        rec = np.zeros((cap, self.capi.LH_NSTAT))
+        nrec = C.c_int(0)
+        st = self.L.orc_solve_stats(holder.ref, _p(Ew), C.byref(opts), _p(saves) if want_saves else None,
+                                    _p(zs), C.byref(stats), _p(t), float(aeff), _p(rec), cap, C.byref(nrec))
+        return st, Ew, stats, rec[:nrec.value]
 
     def solve_precon(self, f, linop, y0, t0, dt0, tmax, rtol=1e-6, atol=1e-10, outputN=201):
         n = len(y0)

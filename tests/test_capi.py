@@ -52,8 +52,9 @@ def test_struct_sizes_match_header(capi):
     # layout of the descriptor as a C compiler sees it
     import subprocess
     import tempfile
-    src = '#include <stdio.h>\n#include <stddef.h>\n#include "luna_hip.h"\nint main(){printf("%zu %zu %zu %zu %zu",' \
-          'sizeof(lh_desc), sizeof(lh_rk_opts), sizeof(lh_stats), offsetof(lh_desc, qdht_inv), offsetof(lh_desc, ppt_x));}'
+    src = '#include <stdio.h>\n#include <stddef.h>\n#include "luna_hip.h"\nint main(){printf("%zu %zu %zu %zu %zu %zu %zu %d",' \
+          'sizeof(lh_desc), sizeof(lh_rk_opts), sizeof(lh_stats), offsetof(lh_desc, qdht_inv), offsetof(lh_desc, ppt_x),' \
+          'sizeof(lh_stats_desc), offsetof(lh_stats_desc, aeff), (int)LH_NSTAT);}'
     with tempfile.TemporaryDirectory() as td:
         c = os.path.join(td, "s.c")
         open(c, "w").write(src)
@@ -62,7 +63,17 @@ def test_struct_sizes_match_header(capi):
         out = subprocess.check_output([exe]).decode().split()
     sizes = [int(x) for x in out]
     assert sizes == [ctypes.sizeof(capi.LhDesc), ctypes.sizeof(capi.LhRkOpts), ctypes.sizeof(capi.LhStats),
-                     capi.LhDesc.qdht_inv.offset, capi.LhDesc.ppt_x.offset]
+                     capi.LhDesc.qdht_inv.offset, capi.LhDesc.ppt_x.offset,
+                     ctypes.sizeof(capi.LhStatsDesc), capi.LhStatsDesc.aeff.offset, capi.LH_NSTAT]
+
+
+def test_stat_keys_follow_header_order(capi):
+    import re
+    text = open(os.path.join(ROOT, "include", "luna_hip.h")).read()
+    names = re.findall(r"LH_STAT_([A-Z0-9_]+)\s*=\s*(\d+)", text)
+    assert [int(v) for _, v in names] == list(range(capi.LH_NSTAT))
+    expect = {"OMEGA0": "ω0"}
+    assert [expect.get(n, n.lower()) for n, _ in names] == list(capi.STAT_KEYS)
 
 
 def test_fno_structs_match_header(capi):

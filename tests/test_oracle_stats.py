@@ -1,0 +1,135 @@
+"""The oracle's per-step statistics (Stats.default for a mode-averaged
+propagation, src/Stats.jl:18-258, 409-532) pinned by known answers.
+
+  * Maths.fwhm: the reference's own cases (test/test_maths.jl:121-137): a
+    Gaussian and a double Gaussian (:min → one peak, :max → both).
+  * energy: Parseval — the input pulse is scaled to its energy in the time
+    domain (|hilbert(Et)|², SimpsonEven over t); the statistic integrates |Eω|²
+    over ω with the energyfuncs prefactor.  A dropped prefactor or wrong end
+    weights fails.
+  * ω0, peakpower, fwhm_t of a 10 fs Gaussian pulse: closed forms
+    (centre frequency, P0 = U/(τ√(π/4ln2)), FWHM = τ).
+  * electrondensity: (a) a constant (DC) field, whose rate is constant:
+    ρ(1 - exp(-r (Nt-1) δt)) — this fixes the doubled DC bin of the analytic
+    field, the √(ε0 c Aeff/2) scaling and the N-1 trapezoid intervals;
+    (b) the pulse, against numpy's FFT and trapezoid rule with the PPT rate.
+"""
+import math
+
+import numpy as np
+import pytest
+
+
+def gauss_fwhm(x, fwhm, x0=0.0):
+    return np.exp(-4 * math.log(2) * ((x - x0) / fwhm) ** 2)
+
+
+def test_fwhm_reference_cases(orc):
+    x = np.linspace(-1, 1, 2048)
+    y = gauss_fwhm(x, 0.2)
+    assert orc.fwhm(x, y) == pytest.approx(0.2, rel=1e-4)
+    assert orc.fwhm(x, y, "max") == pytest.approx(0.2, rel=1e-4)
+    fw, sep = 0.1, 0.5
+    y2 = gauss_fwhm(x, fw, -sep / 2) + gauss_fwhm(x, fw, sep / 2)
+    assert orc.fwhm(x, y2, "min") == pytest.approx(fw, rel=1e-4)
+    assert orc.fwhm(x, y2, "max") == pytest.approx(fw + sep, rel=1e-4)
+
+
+def test_fwhm_no_crossing_is_nan(orc):
+    x = np.linspace(-1, 1, 256)
+    assert math.isnan(orc.fwhm(x, np.ones(256)))
+    # maximum at the edge: no left crossing
+    assert math.isnan(orc.fwhm(x, np.exp(-((x + 1) / 0.1) ** 2)))
+    # :max needs a neighbour beyond the last point above half maximum
+    y = np.where(x > 0.5, 1.0, 0.0)
+    assert math.isnan(orc.fwhm(x, y, "max"))
+
+
+@pytest.fixture(scope="module")
+def small(pkg):
+    from luna_jl_amd import configs
+    return configs.capillary_instance(energy=150e-6, pressure=2.0, trange=0.5e-12)
+
+
+def test_stats_of_gaussian_pulse(small, orc, capi):
+    from luna_jl_amd import physdata as pd
+    g = small.grid
+    st, rec = orc.step_stats(small.holder, g.t, small.transform.aeff, small.Ew0, 0.25, 1e-3)
+    assert st == 0
+    r = dict(zip(capi.STAT_KEYS, rec))
+    assert r["z"] == 0.25 and r["dz"] == 1e-3
+    assert r["energy"] == pytest.approx(150e-6, rel=1e-9)
+    assert r["ω0"] == pytest.approx(pd.wlfreq(800e-9), rel=1e-9)
+    tau = 10e-15
+    p0 = 150e-6 / (tau * math.sqrt(math.pi / (4 * math.log(2))))
+    assert r["peakpower"] == pytest.approx(p0, rel=1e-6)
+    assert r["peakintensity"] == pytest.approx(p0 / small.transform.aeff, rel=1e-6)
+    assert r["fwhm_t_min"] == pytest.approx(tau, rel=1e-3)
+    assert r["fwhm_t_max"] == pytest.approx(tau, rel=1e-3)
+    assert r["density"] == small.transform.density
+
+
+def test_electrondensity_constant_field(small, orc, capi):
+    from luna_jl_amd import physdata as pd
+    g = small.grid
+    aeff = small.transform.aeff
+    ediv = math.sqrt(pd.eps_0 * pd.c * aeff / 2)
+    nt = g.n_t
+    spl = small.transform.responses[1].ratefunc
+    E0 = 0.5 * (spl.x[0] + spl.x[-1]) * 1e-3 + spl.x[0]  # inside the table
+    Ew = np.zeros(g.n_omega, dtype=complex)
+    Ew[0] = E0 * ediv * nt / 2  # analytic field: 2·E_0/Nt, constant and real
+    st, rec = orc.step_stats(small.holder, g.t, aeff, Ew)
+    assert st == 0
+    r = dict(zip(capi.STAT_KEYS, rec))
+    st, rate = orc.ppt_rate(small.holder, np.array([E0]))
+    dt = g.t[1] - g.t[0]
+    expect = small.transform.density * (1 - math.exp(-rate[0] * (nt - 1) * dt))
+    assert r["electrondensity"] == pytest.approx(expect, rel=1e-12)
+    assert r["peak_ionisation_rate"] == pytest.approx(rate[0], rel=1e-14)
+    assert r["peakpower"] == pytest.approx((E0 * ediv) ** 2, rel=1e-12)
+    assert math.isnan(r["fwhm_t_min"]) and math.isnan(r["fwhm_t_max"])
+    assert r["ω0"] == 0.0
+
+
+def test_electrondensity_pulse_numpy(small, orc, capi):
+    from luna_jl_amd import physdata as pd
+    g = small.grid
+    aeff = small.transform.aeff
+    Ew = small.Ew0 * 3.0  # strong enough to ionise helium noticeably
+    st, rec = orc.step_stats(small.holder, g.t, aeff, Ew)
+    assert st == 0
+    r = dict(zip(capi.STAT_KEYS, rec))
+    nt = g.n_t
+    spec = np.zeros(nt, dtype=complex)
+    spec[:g.n_omega - 1] = 2 * Ew[:-1]
+    Et = np.fft.ifft(spec)
+    E = Et.real / math.sqrt(pd.eps_0 * pd.c * aeff / 2)
+    st, rate = orc.ppt_rate(small.holder, E)
+    integral = np.trapezoid(rate, dx=g.t[1] - g.t[0])
+    assert integral > 1e-8
+    assert r["electrondensity"] == pytest.approx(small.transform.density * (1 - math.exp(-integral)), rel=1e-9)
+    assert r["peak_ionisation_rate"] == pytest.approx(rate.max(), rel=1e-9)
+    assert r["peakpower"] == pytest.approx(np.max(np.abs(Et) ** 2), rel=1e-12)
+
+
+def test_stats_envelope_grid(orc, capi, pkg):
+    from luna_jl_amd import configs
+    cfg1 = configs.gnlse_instance(shotnoise=False)  # sech pulse, 50 fs FWHM, 10 kW peak
+    g = cfg1.grid
+    st, rec = orc.step_stats(cfg1.holder, g.t, cfg1.transform.aeff, cfg1.Ew0)
+    assert st == 0
+    r = dict(zip(capi.STAT_KEYS, rec))
+    assert r["peakpower"] == pytest.approx(1e4, rel=1e-12)
+    tau_w = 50e-15 / (2 * math.log(1 + math.sqrt(2)))
+    assert r["energy"] == pytest.approx(2 * 1e4 * tau_w, rel=1e-8)  # ∫ P0 sech²(t/τw) dt
+    assert r["fwhm_t_min"] == pytest.approx(50e-15, rel=1e-3)
+    assert math.isnan(r["electrondensity"])
+
+
+def test_host_energy_weights_match_oracle(small, orc, capi):
+    from luna_jl_amd import stats
+    w = stats.energy_weights(small.grid)
+    st, rec = orc.step_stats(small.holder, small.grid.t, small.transform.aeff, small.Ew0)
+    e = np.sum(w * np.abs(small.Ew0) ** 2)
+    assert e == pytest.approx(rec[capi.STAT_KEYS.index("energy")], rel=1e-13)

@@ -1,0 +1,56 @@
+"""Host side of the per-step statistics (no GPU): the record layout, the keys
+Stats.default produces (src/Stats.jl:492-532) and the pressure constant."""
+import numpy as np
+import pytest
+
+
+@pytest.fixture(scope="module")
+def small(pkg):
+    from luna_jl_amd import configs
+    return configs.capillary_instance(energy=150e-6, pressure=2.0, trange=0.5e-12, ppt_n=2 ** 12)
+
+
+def test_keys_follow_the_reference(small, capi):
+    from luna_jl_amd import stats
+    sf = stats.default(small.grid, small.Ew0, None, small.linop, small.transform, gas="He")
+    rec = np.arange(2 * capi.LH_NSTAT, dtype=float).reshape(2, capi.LH_NSTAT)
+    d = sf.to_dict(rec)
+    for k in ("ω0", "energy", "peakpower", "fwhm_t_min", "fwhm_t_max", "density", "pressure",
+              "peakintensity", "electrondensity", "peak_ionisation_rate", "z", "dz"):
+        assert len(d[k]) == 2, k
+    assert d["z"][1] == rec[1, 0] and d["dz"][0] == rec[0, 1]
+    assert d["pressure"][0] == pytest.approx(2.0, rel=1e-12)
+
+
+def test_no_plasma_no_ionisation_keys(pkg, capi):
+    from luna_jl_amd import configs, stats
+    cfg1 = configs.gnlse_instance(shotnoise=False)
+    sf = stats.default(cfg1.grid, cfg1.Ew0, None, cfg1.linop, cfg1.transform)
+    d = sf.to_dict(np.zeros((3, capi.LH_NSTAT)))
+    assert "electrondensity" not in d and "peak_ionisation_rate" not in d
+    assert "pressure" not in d
+
+
+def test_unsupported_options_raise(small):
+    from luna_jl_amd import stats
+    with pytest.raises(NotImplementedError):
+        stats.default(small.grid, small.Ew0, None, small.linop, small.transform, windows=[(200e-9, 300e-9)])
+    with pytest.raises(NotImplementedError):
+        stats.default(small.grid, small.Ew0, None, small.linop, small.transform, onaxis=True)
+
+
+def test_pressure_inverts_density():
+    from luna_jl_amd import physdata as pd
+    for gas in ("He", "Ar", "Ne"):
+        for P in (0.1, 1.0, 3.0, 20.0):
+            assert pd.pressure(gas, pd.density(gas, P)) == pytest.approx(P, rel=1e-12)
+    assert pd.pressure("He", 0.0) == 0.0
+
+
+def test_energy_weights_envelope_is_rectangle_rule(pkg):
+    from luna_jl_amd import configs, stats
+    g = configs.gnlse_instance(shotnoise=False).grid
+    w = stats.energy_weights(g)
+    dt = g.t[1] - g.t[0]
+    # Parseval for the unnormalised DFT: Σ|X|²·δt/N = Σ|x|²·δt
+    assert np.allclose(w, dt / len(g.t), rtol=1e-12)
