@@ -35,6 +35,8 @@ def parse():
     p.add_argument("--flength", type=float, default=1.0)
     p.add_argument("--cpu-seconds", type=float, default=15.0, help="bound on the CPU baseline sample")
     p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--stats", default="on", choices=["on", "off"],
+                   help="record Stats.default's per-step statistics (as prop_capillary does)")
     p.add_argument("--workload", default="scan", choices=["scan", "radial", "surrogate", "copy"],
                    help="scan: batched capillary sweep (the reported metric); radial: the "
                         "Nr=256 x Nt=4096 radial propagation; surrogate: FNO training on solver "
@@ -69,7 +71,11 @@ _CPU_JOB = {}
 def _cpu_solve(i):
     """One full propagation of sweep point i on the CPU oracle (worker process)."""
     inst, opts, orc = _CPU_JOB["insts"][i], _CPU_JOB["opts"], _CPU_JOB["orc"]
-    st, Ew, sv, zs, sr = orc.solve(inst.holder, inst.Ew0, opts)
+    if _CPU_JOB["stats"]:
+        st, Ew, sr, rec = orc.solve_stats(inst.holder, inst.Ew0, opts, inst.grid.t, inst.transform.aeff,
+                                          want_saves=True)
+    else:
+        st, Ew, sv, zs, sr = orc.solve(inst.holder, inst.Ew0, opts)
     return st, int(sr.steps)
 
 
@@ -84,7 +90,7 @@ def cpu_baseline(insts, args):
     n = len(insts)
     sample = sorted({(k * n) // (2 * cores) for k in range(2 * cores)})
     _CPU_JOB.update(insts=insts, opts=_capi.rk_opts(args.flength, save_n=201),
-                    orc=ge.load_oracle().Oracle(_capi))
+                    orc=ge.load_oracle().Oracle(_capi), stats=args.stats == "on")
     t = time.perf_counter()
     with mp.get_context("fork").Pool(cores) as pool:
         res = pool.map(_cpu_solve, sample, chunksize=1)
@@ -93,7 +99,9 @@ def cpu_baseline(insts, args):
     log(f"bench: CPU oracle {len(sample)} points on {cores} processes in {wall:.2f} s")
     return {"value": ok * 3600.0 / wall, "unit": "scan points/hour", "cores": cores, "kind": "port",
             "sample": f"{len(sample)} of the {n} sweep points (indices spread over the sweep, both grid classes), "
-                      f"full 1 m propagations with 201 saves, one worker process per core: {wall:.2f} s wall"}
+                      f"full 1 m propagations with 201 saves"
+                      f"{' and per-step statistics' if args.stats == 'on' else ''}, "
+                      f"one worker process per core: {wall:.2f} s wall"}
 
 
 def measured_traffic():
@@ -264,6 +272,10 @@ def main():
     # processes are forked from it)
     cpu = None if args.no_cpu else cpu_baseline(insts, args)
     ctxs = [runtime.Context(c.holder) for c in insts]
+    if args.stats == "on":  # Stats.default's per-step records, as prop_capillary collects them
+        from luna_jl_amd import stats as lstats
+        for ctx, c in zip(ctxs, insts):
+            ctx.set_step_stats(lstats.StatsFun(c.grid, c.transform, capacity=4096))
     opts = _capi.rk_opts(args.flength, save_n=201)
     batch = runtime.ResidentBatch(ctxs, [c.Ew0 for c in insts])  # inputs resident in HBM
 
@@ -278,6 +290,16 @@ def main():
         kms.append(runtime.last_launch_times())
         log(f"bench: step {s + 1}/{args.steps}: {times[-1] * 1e3:.1f} ms, kernels {kms[-1]}")
     dt = float(np.mean(times))
+    stats_off_ms = None
+    if args.stats == "on":  # the same sweep without the per-step statistics
+        for ctx in ctxs:
+            ctx.set_step_stats(None)
+        t = time.perf_counter()
+        batch.solve(opts)
+        stats_off_ms = (time.perf_counter() - t) * 1e3
+        log(f"bench: sweep without statistics {stats_off_ms:.1f} ms")
+        for ctx, c in zip(ctxs, insts):
+            ctx.set_step_stats(lstats.StatsFun(c.grid, c.transform, capacity=4096))
     ok = [st.status == 0 for st in stats]
     n_ok = int(sum(ok))
     rhs_total = int(sum(st.rhs_evals for st in stats))
@@ -332,7 +354,9 @@ def main():
         "dtype": "f64",
         "data": "synthetic (seeded scan points; no external data)",
         "config": {"workload": f"capillary_scan_{n}", "instances": n, "flength_m": args.flength,
-                   "n_to": [8192, 16384], "rtol": 1e-6, "saves": 201},
+                   "n_to": [8192, 16384], "rtol": 1e-6, "saves": 201,
+                   "per_step_stats": args.stats == "on"},
+        "sweep_ms_without_stats": stats_off_ms,
         "rhs_evals_per_s": rhs_total / dt,
         "host_inclusive_ms_per_step": host_dt * 1e3,
         "step_attempts_total": steps_total,

@@ -456,20 +456,29 @@ __device__ __forceinline__ void stage_prologue_cols(const double2* __restrict__ 
     for (int k = N + tid; k <= MO; k += LH_T) sm.buf[pad(k)] = make_double2(0.0, 0.0);
 }
 
+// Statistics of the spectrum held in LDS (defined with the statistics below).
+template <int KIND, int NT, class Smem>
+__device__ __forceinline__ void stats_time(const DInst& in, Smem& sm, double s0, double s1, double se, double z,
+                                           double dz, double* rec);
+
 // Accepted-step window filter (stepfun, src/Luna.jl:366-370) on the coarse
-// grid: YN ← rfft(twin · irfft(ωwin · YN·exp(L h))).
-template <int MC, class Smem>
+// grid: YN ← rfft(twin · irfft(ωwin · YN·exp(L h))).  With STATS and a
+// record pointer, the statistics of the filtered YN follow from the values
+// still in registers (no read-back of YN).
+template <int MC, class Smem, bool STATS = false>
 __device__ __forceinline__ void stepfun_filter_cols(const DInst& in, const double2* __restrict__ linop, Smem& sm,
-                                                    double2* YN, double h);
+                                                    double2* YN, double h, double* rec = nullptr, double z = 0.0,
+                                                    double dz = 0.0);
 
 template <int MC, class Smem>
-__device__ __forceinline__ void stepfun_filter(const DInst& in, Smem& sm, double2* YN, double h) {
-    stepfun_filter_cols<MC, Smem>(in, in.linop, sm, YN, h);
+__device__ __forceinline__ void stepfun_filter(const DInst& in, Smem& sm, double2* YN, double h, double* rec,
+                                               double z, double dz) {
+    stepfun_filter_cols<MC, Smem, true>(in, in.linop, sm, YN, h, rec, z, dz);
 }
 
-template <int MC, class Smem>
+template <int MC, class Smem, bool STATS>
 __device__ __forceinline__ void stepfun_filter_cols(const DInst& in, const double2* __restrict__ linop, Smem& sm,
-                                                    double2* YN, double h) {
+                                                    double2* YN, double h, double* rec, double z, double dz) {
     constexpr int N = MC + 1;
     constexpr int NPT = (N + LH_T - 1) / LH_T;
 #pragma unroll
@@ -494,6 +503,39 @@ __device__ __forceinline__ void stepfun_filter_cols(const DInst& in, const doubl
     }
     __syncthreads();
     fft_lds<MC, false>(sm.buf, sm.tw);
+    if constexpr (STATS) {
+        if (rec) {
+            // analytic spectrum (copyto_fft!, src/Stats.jl:441-448): bins
+            // 0..Nω-2 doubled, the +fs/2 bin dropped, zero up to Nt = 2·MC
+            double2 yv[NPT];
+            double wv[NPT], ev[NPT];
+#pragma unroll
+            for (int i = 0; i < NPT; i++) {
+                int k = threadIdx.x + i * LH_T;
+                wv[i] = (k < N) ? in.st_omega[k] : 0.0;
+                ev[i] = (k < N) ? in.st_ew[k] : 0.0;
+                yv[i] = make_double2(0.0, 0.0);
+                if (k < N) {
+                    yv[i] = r2c_bin<MC>(sm.buf, sm.tw, k);
+                    YN[k] = yv[i];
+                }
+            }
+            __syncthreads();  // every bin read before the buffer is reused
+            double s0 = 0.0, s1 = 0.0, se = 0.0;
+#pragma unroll
+            for (int i = 0; i < NPT; i++) {
+                int k = threadIdx.x + i * LH_T;
+                double a = yv[i].x * yv[i].x + yv[i].y * yv[i].y;
+                s0 += a;
+                s1 += wv[i] * a;
+                se += ev[i] * a;
+                if (k < N - 1) sm.buf[pad(k)] = make_double2(2.0 * yv[i].x, 2.0 * yv[i].y);
+            }
+            for (int k = N - 1 + threadIdx.x; k < 2 * MC; k += LH_T) sm.buf[pad(k)] = make_double2(0.0, 0.0);
+            stats_time<0, 2 * MC, Smem>(in, sm, s0, s1, se, z, dz, rec);
+            return;
+        }
+    }
 #pragma unroll
     for (int i = 0; i < NPT; i++) {
         int k = threadIdx.x + i * LH_T;
@@ -682,7 +724,8 @@ __device__ __forceinline__ void rhs_env(const DInst& in, Smem& sm, Epi epi) {
 // stepfun on the envelope grid (src/Luna.jl:366-370 with a c2c FT):
 // YN ← fft(twin · ifft(ωwin · YN·exp(L h))).
 template <int NT, class Smem>
-__device__ __forceinline__ void stepfun_env(const DInst& in, Smem& sm, double2* YN, double h) {
+__device__ __forceinline__ void stepfun_env(const DInst& in, Smem& sm, double2* YN, double h, double* rec, double z,
+                                            double dz) {
     constexpr int NPT = (NT + LH_T - 1) / LH_T;
 #pragma unroll
     for (int i = 0; i < NPT; i++) {
@@ -703,12 +746,25 @@ __device__ __forceinline__ void stepfun_env(const DInst& in, Smem& sm, double2* 
     }
     __syncthreads();
     fft_lds<NT, false>(sm.buf, sm.tw);
+    double s0 = 0.0, s1 = 0.0, se = 0.0;
 #pragma unroll
     for (int i = 0; i < NPT; i++) {
         int k = threadIdx.x + i * LH_T;
-        if (k < NT) YN[k] = sm.buf[pad(k)];
+        if (k < NT) {
+            double2 v = sm.buf[pad(k)];
+            YN[k] = v;
+            if (rec) {  // the envelope's analytic field is ifft(Eω): the spectrum stays in place
+                double a = v.x * v.x + v.y * v.y;
+                s0 += a;
+                s1 += in.st_omega[k] * a;
+                se += in.st_ew[k] * a;
+            }
+        }
     }
-    __syncthreads();
+    if (rec)
+        stats_time<1, NT, Smem>(in, sm, s0, s1, se, z, dz, rec);
+    else
+        __syncthreads();
 }
 
 // hω·δt for the Raman convolution: rfft of h(t) zero-padded to 2·Nto reals
@@ -765,7 +821,6 @@ __device__ __forceinline__ void step_stats(const DInst& in, Smem& sm, const doub
     constexpr int N = Sizes<KIND, MO, MC>::N;
     constexpr int NT = MO;  // analytic-signal samples = length of grid.t
     const int tid = fresh_tid();
-    const int lane = tid & 63, wid = tid >> 6;
     double s0 = 0.0, s1 = 0.0, se = 0.0;
     for (int k = tid; k < NT; k += LH_T) {
         double2 v = (k < N) ? Y[k] : make_double2(0.0, 0.0);
@@ -780,28 +835,57 @@ __device__ __forceinline__ void step_stats(const DInst& in, Smem& sm, const doub
         else
             sm.buf[pad(k)] = v;
     }
+    stats_time<KIND, NT, Smem>(in, sm, s0, s1, se, z, dz, rec);
+}
+
+// The analytic spectrum is in LDS (not yet synchronised) and each thread holds
+// its partial sums Σ|E|², Σω|E|², Σw|E|²: the record of the state.
+template <int KIND, int NT, class Smem>
+__device__ __forceinline__ void stats_time(const DInst& in, Smem& sm, double s0, double s1, double se, double z,
+                                           double dz, double* rec) {
+    const int tid = fresh_tid();
+    const int lane = tid & 63, wid = tid >> 6;
     block_sum3(s0, s1, se, sm.wsum);  // ends synchronised: the spectrum is complete
     fft_lds<NT, true>(sm.buf, sm.tw);
-    // pass 1: P = |Et|² (kept in .x), rate (kept in .y); argmax, max/sum of rate
+    // pass 1: P = |Et|² (kept in .x), rate (kept in .y); argmax, max/sum of
+    // rate.  Sample j = tid + m·LH_T; rates in batches of QC (spline-table
+    // loads issued together).
+    constexpr int MPT = NT / LH_T, QC = 4;
+    static_assert(NT % LH_T == 0 && MPT % QC == 0, "stats sample split");
     const double inv = 1.0 / (double)NT;  // FFTW's ifft normalisation
     const bool ion = (KIND == 0) && in.plasma;
     double pmax = -1.0, rmax = 0.0, rsum = 0.0;
     int imax = NT, bad = LH_OK;
-    for (int j = tid; j < NT; j += LH_T) {
-        double2 e = sm.buf[pad(j)];
-        double re = e.x * inv, im = e.y * inv;
-        double p = re * re + im * im;
-        double r = 0.0;
-        if (ion) {
-            r = ppt_rate(in, fabs(re / in.st_ediv), bad);
-            rmax = fmax(rmax, r);
-            rsum += r;
+#pragma unroll 1
+    for (int c = 0; c < MPT; c += QC) {
+        double p[QC], aE[QC], r[QC];
+#pragma unroll
+        for (int q = 0; q < QC; q++) {
+            const double2 e = sm.buf[pad(tid + (c + q) * LH_T)];
+            const double re = e.x * inv, im = e.y * inv;
+            p[q] = re * re + im * im;
+            aE[q] = fabs(re / in.st_ediv);
+            r[q] = 0.0;
         }
-        if (p > pmax) {  // j increases within a thread: the first maximum is kept
-            pmax = p;
-            imax = j;
+        bool need = false;  // any sample at or above the table start (or NaN)
+#pragma unroll
+        for (int q = 0; q < QC; q++) need |= !(aE[q] < in.ppt_emin);
+        if (ion && need) {  // most of the window is below it: those waves skip the table
+            ppt_rates<QC>(in, aE, r, bad);
+#pragma unroll
+            for (int q = 0; q < QC; q++) {
+                rmax = fmax(rmax, r[q]);
+                rsum += r[q];
+            }
         }
-        sm.buf[pad(j)] = make_double2(p, r);
+#pragma unroll
+        for (int q = 0; q < QC; q++) {
+            if (p[q] > pmax) {  // j increases within a thread: the first maximum is kept
+                pmax = p[q];
+                imax = tid + (c + q) * LH_T;
+            }
+            sm.buf[pad(tid + (c + q) * LH_T)] = make_double2(p[q], r[q]);
+        }
     }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
@@ -843,8 +927,10 @@ __device__ __forceinline__ void step_stats(const DInst& in, Smem& sm, const doub
     // last j > imax with P > val.
     const double val = 0.5 * pmax;
     int aL = -1, aR = NT, bL = NT, bR = -1;
-    for (int j = tid; j < NT; j += LH_T) {
-        double p = sm.buf[pad(j)].x;
+#pragma unroll
+    for (int m = 0; m < MPT; m++) {
+        const int j = tid + m * LH_T;
+        const double p = sm.buf[pad(j)].x;
         if (j < imax) {
             if (p < val) aL = max(aL, j);
             if (p > val) bL = min(bL, j);
@@ -930,11 +1016,12 @@ __device__ __forceinline__ void do_rhs(const DInst& in, Smem& sm, Epi epi) {
 }
 
 template <int KIND, int MO, int MC, class Smem>
-__device__ __forceinline__ void do_stepfun(const DInst& in, Smem& sm, double2* YN, double h) {
+__device__ __forceinline__ void do_stepfun(const DInst& in, Smem& sm, double2* YN, double h, double* rec, double z,
+                                           double dz) {
     if constexpr (KIND == 0)
-        stepfun_filter<MC, Smem>(in, sm, YN, h);
+        stepfun_filter<MC, Smem>(in, sm, YN, h, rec, z, dz);
     else
-        stepfun_env<MC, Smem>(in, sm, YN, h);
+        stepfun_env<MC, Smem>(in, sm, YN, h, rec, z, dz);
 }
 
 template <int KIND, int MO, int MC, class Smem>
@@ -1052,12 +1139,12 @@ __device__ __forceinline__ void solve_instance(const DInst& in, Smem& sm) {
             double2* tmp = K[0];  // FSAL: ks[1] .= ks[7] (src/RK45.jl:184-186)
             K[0] = K[6];
             K[6] = tmp;
-            do_stepfun<KIND, MO, MC, Smem>(in, sm, YN, tn - t);
-            // MemoryOutput's statistics of the filtered state (src/Output.jl:65-67)
-            if (in.steprec && accepted < in.steprec_cap) {
-                step_stats<KIND, MO, MC, Smem>(in, sm, YN, tn, dtn, in.steprec + (size_t)accepted * LH_NSTAT);
-                if (sm.flag != LH_OK) status = sm.flag;
-            }
+            // stepfun, then MemoryOutput's statistics of the filtered state
+            // (src/Output.jl:65-67) from the same LDS pass
+            double* rec = (in.steprec && accepted < in.steprec_cap) ? in.steprec + (size_t)accepted * LH_NSTAT
+                                                                      : nullptr;
+            do_stepfun<KIND, MO, MC, Smem>(in, sm, YN, tn - t, rec, tn, dtn);
+            if (rec && sm.flag != LH_OK) status = sm.flag;
             // GridCondition saves via dense output; ks[1] and ks[7] both hold k7
             while (saved < in.save_n && in.zgrid[saved] <= tn) {
                 double ti = in.zgrid[saved];
