@@ -160,6 +160,40 @@ int lh_batch_solve_resident(lh_ctx *const *ctxs, int n, const lh_rk_opts *opts, 
 int lh_get_state(lh_ctx *ctx, double *Ew);
 int lh_get_saves(lh_ctx *ctx, double *saves, double *zsaves);
 
+/* z-dependent gas fill of a constant-radius capillary (pressure gradient),
+ * mode-averaged field grid: Capillary.gradient (src/Capillary.jl:306-340)
+ * gives p(z) and ρ(z) = densityspline(p(z)) (src/PhysData.jl:719-723), the
+ * core index √(1 + γ(λ)ρ(z)); the linear operator is then re-evaluated at
+ * every new z, as LinearOps.make_linop's linop!(out, z) (src/LinearOps.jl:
+ * 337-358) inside make_prop!(linop!) (src/RK45.jl:283-296), and the RHS uses
+ * ρ(z) and norm_mode_average's pre/β(ω, z)·√Aeff (src/NonlinearRHS.jl:375-387). */
+typedef struct lh_gradient_desc {
+    int32_t n_points;         /* pressure-profile nodes (>= 2)                   */
+    int32_t two_point;        /* 1: gradient(gas, L, p0, p1), Z = {0, L};
+                                 0: gradient(gas, Z, P), the multi-point form    */
+    const double *Z;          /* [n_points] positions (m), increasing            */
+    const double *P;          /* [n_points] pressures (bar)                      */
+    int32_t dens_n;           /* density spline nodes (uniform in pressure)      */
+    int32_t model;            /* Marcatili model: 0 :full, 1 :reduced            */
+    const double *dens_p;     /* [dens_n] range(Pmin, Pmax, dens_n)              */
+    const double *dens_rho;   /* [dens_n] density at dens_p                      */
+    const double *dens_d;     /* [dens_n] CSpline derivatives                    */
+    int32_t loss;             /* 1: complex neff (MarcatiliMode loss = true)     */
+    int32_t reserved;         /* 0                                               */
+    const double *gamma;      /* [n_omega] sellmeier_gas γ(λ) per particle       */
+    const double *nwg;        /* complex [n_omega] neff_wg(mode, ω), z-independent */
+    const double *pre;        /* complex [n_omega] norm_mode_average's pre(ω)    */
+    double sqrt_aeff;         /* √Aeff                                           */
+    double fd_h;              /* β1 stencil step h                               */
+    const double *fd;         /* [9][4] β1 = dβ/dω at ω0 by the 8th-order central
+                                 stencil: per point ω_j = ω0 + (j-4)h, γ(ω_j),
+                                 neff_wg(ω_j) (re, im)                            */
+} lh_gradient_desc;
+
+/* Attach a gradient fill to a mode-averaged field-grid context (g = NULL:
+ * constant fill again).  Host arrays are copied. */
+int lh_set_gradient(lh_ctx *ctx, const lh_gradient_desc *g);
+
 /* Per-step statistics of a mode-averaged propagation: the set that
  * Stats.default builds (src/Stats.jl:476-532) and collect_stats evaluates
  * (src/Stats.jl:454-468) from the analytic field (plan_analytic,

@@ -57,6 +57,14 @@ class LhStats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class LhGradientDesc(C.Structure):
+    """``lh_gradient_desc`` of include/luna_hip.h."""
+    _fields_ = [("n_points", C.c_int32), ("two_point", C.c_int32), ("Z", _dp), ("P", _dp),
+                ("dens_n", C.c_int32), ("model", C.c_int32), ("dens_p", _dp), ("dens_rho", _dp),
+                ("dens_d", _dp), ("loss", C.c_int32), ("reserved", C.c_int32), ("gamma", _dp),
+                ("nwg", _dp), ("pre", _dp), ("sqrt_aeff", C.c_double), ("fd_h", C.c_double), ("fd", _dp)]
+
+
 class LhStatsDesc(C.Structure):
     """``lh_stats_desc`` of include/luna_hip.h."""
     _fields_ = [("capacity", C.c_int32), ("reserved", C.c_int32), ("energy_w", C.POINTER(C.c_double)),
@@ -157,6 +165,7 @@ def lib():
                                           C.POINTER(LhStats)]
     L.lh_last_launch_times.argtypes = [_dp, C.POINTER(C.c_int), C.c_int]
     L.lh_set_step_stats.argtypes = [C.c_void_p, C.POINTER(LhStatsDesc)]
+    L.lh_set_gradient.argtypes = [C.c_void_p, C.POINTER(LhGradientDesc)]
     L.lh_stats_eval.argtypes = [C.c_void_p, _dp, C.c_double, C.c_double, _dp]
     L.lh_get_step_stats.argtypes = [C.c_void_p, _dp, C.c_int, C.POINTER(C.c_int)]
     _fp = C.POINTER(C.c_float)
@@ -176,7 +185,7 @@ def lib():
 EXPORTED = ["lh_strerror", "lh_device_count", "lh_create", "lh_destroy", "lh_rhs",
             "lh_rhs_device", "lh_ctx_status", "lh_solve", "lh_batch_solve", "lh_set_state",
             "lh_batch_solve_resident", "lh_get_state", "lh_get_saves", "lh_last_launch_times",
-            "lh_set_step_stats", "lh_stats_eval", "lh_get_step_stats",
+            "lh_set_step_stats", "lh_stats_eval", "lh_get_step_stats", "lh_set_gradient",
             "lh_fno_create", "lh_fno_destroy", "lh_fno_param_count", "lh_fno_set_params",
             "lh_fno_get_params", "lh_fno_forward", "lh_fno_loss_grad", "lh_fno_set_data", "lh_fno_train"]
 

@@ -1,0 +1,112 @@
+"""Gas-filled capillary with a pressure gradient (host-side set-up).
+
+Restates ``Capillary.gradient`` (``src/Capillary.jl:306-340``): the pressure
+profile p(z) of a two-point fill (``gradient(gas, L, p0, p1)``) or of a
+multi-point fill (``gradient(gas, Z, P)``), the density ρ(z) =
+``densityspline``(p(z)) (``src/PhysData.jl:719-723``) and the core index
+√(1 + γ(λ)ρ(z)).  ``gradient_spec`` packs what the propagation kernels need to
+re-evaluate the linear operator of ``LinearOps.make_linop``
+(``src/LinearOps.jl:337-358``) and the normalisation of ``norm_mode_average``
+(``src/NonlinearRHS.jl:375-387``) at every new z: γ(λ) and the waveguide
+index ``neff_wg`` per frequency bin (neither depends on z for a constant core
+radius, as in ``neff_β_grid``, ``src/Capillary.jl:345-360``), the density
+spline, and the β1 stencil around ω0.
+"""
+import math
+
+import numpy as np
+
+from . import _capi
+from . import physdata as pd
+from .maths import cspline_eval
+from .modes import _FD8
+
+
+class Gradient:
+    """``Capillary.gradient(gas, L, p0, p1)`` (``two_point``) or ``gradient(gas, Z, P)``."""
+
+    def __init__(self, gas, Z, P, T=pd.roomtemp, two_point=False):
+        self.gas, self.T = gas, T
+        self.Z = np.ascontiguousarray(Z, dtype=float)
+        self.P = np.ascontiguousarray(P, dtype=float)
+        self.two_point = bool(two_point)
+        if len(self.Z) != len(self.P) or len(self.Z) < 2 or np.any(np.diff(self.Z) <= 0):
+            raise ValueError("Z must increase and match P")
+        if self.two_point and (len(self.Z) != 2 or self.Z[0] != 0.0):
+            raise ValueError("a two-point gradient runs from z = 0 to L")
+        lo, hi = float(self.P.min()), float(self.P.max())
+        self.dens_p, self.dens_rho, self.dens_d = pd.density_spline(gas, hi, 0.0 if lo == hi else lo, T=T)
+
+    @classmethod
+    def linear(cls, gas, L, p0, p1, T=pd.roomtemp):
+        return cls(gas, [0.0, L], [p0, p1], T=T, two_point=True)
+
+    def pressure(self, z):
+        Z, P = self.Z, self.P
+        if self.two_point:
+            L, p0, p1 = Z[1], P[0], P[1]
+            return p1 if z > L else p0 if z <= 0 else math.sqrt(p0 * p0 + z / L * (p1 * p1 - p0 * p0))
+        if z <= Z[0]:
+            return float(P[0])
+        if z >= Z[-1]:
+            return float(P[-1])
+        i = int(np.nonzero(Z < z)[0][-1])
+        return math.sqrt(P[i] * P[i] + (z - Z[i]) / (Z[i + 1] - Z[i]) * (P[i + 1] * P[i + 1] - P[i] * P[i]))
+
+    def density(self, z):
+        return float(cspline_eval(self.dens_p, self.dens_rho, self.dens_d, self.pressure(z)))
+
+
+def gradient_spec(grid, mode, gradient, ref_lambda, shock=True):
+    """The arrays of ``lh_gradient_desc`` for a Marcatili mode of constant radius
+    (``mode`` supplies the radius, cladding, model and loss switch; its own fill
+    pressure is not used)."""
+    if grid.kind != "real":
+        raise NotImplementedError("gradient fills are provided on the field grid")
+    w = grid.omega
+    s = grid.sidx
+    gamma_fun = pd.sellmeier_gas(gradient.gas)
+    gamma = np.zeros(len(w))
+    gamma[s] = gamma_fun(pd.wlfreq(w[s]) * 1e6)
+    nwg = np.zeros(len(w), dtype=np.complex128)
+    nwg[s] = mode.neff_wg(w[s])
+    shockterm = w * w if shock else w * pd.wlfreq(grid.ref_lambda)
+    from .nonlinearrhs import NLSCALE
+    pre = np.zeros(len(w), dtype=np.complex128)
+    pre[s] = 1j * (-(((shockterm[s]) / 4) / NLSCALE) / pd.c)
+    w0 = pd.wlfreq(ref_lambda)
+    h = abs(w0) * 1e-3
+    wj = w0 + h * np.arange(-4, 5)
+    fd = np.zeros((9, 4))
+    fd[:, 0] = wj
+    fd[:, 1] = gamma_fun(pd.wlfreq(wj) * 1e6)
+    nw = mode.neff_wg(wj)
+    fd[:, 2], fd[:, 3] = nw.real, nw.imag
+    return GradientSpec(gradient, mode, gamma, nwg, pre, math.sqrt(mode.Aeff()), h, fd)
+
+
+class GradientSpec:
+    """Owns the arrays an ``LhGradientDesc`` points into."""
+
+    def __init__(self, gradient, mode, gamma, nwg, pre, sqrt_aeff, fd_h, fd):
+        self.gradient = gradient
+        self.model = 0 if mode.model == "full" else 1
+        self.loss = 1 if mode.loss else 0
+        self.gamma = np.ascontiguousarray(gamma, dtype=float)
+        self.nwg = np.ascontiguousarray(nwg, dtype=np.complex128)
+        self.pre = np.ascontiguousarray(pre, dtype=np.complex128)
+        self.sqrt_aeff = float(sqrt_aeff)
+        self.fd_h = float(fd_h)
+        self.fd = np.ascontiguousarray(fd, dtype=float)
+        g = gradient
+        self.desc = _capi.LhGradientDesc(
+            n_points=len(g.Z), two_point=int(g.two_point), Z=_capi._ptr(g.Z), P=_capi._ptr(g.P),
+            dens_n=len(g.dens_p), model=self.model, dens_p=_capi._ptr(g.dens_p),
+            dens_rho=_capi._ptr(g.dens_rho), dens_d=_capi._ptr(g.dens_d), loss=self.loss, reserved=0,
+            gamma=_capi._ptr(self.gamma), nwg=_capi._ptr(self.nwg.view(np.float64)),
+            pre=_capi._ptr(self.pre.view(np.float64)), sqrt_aeff=self.sqrt_aeff, fd_h=self.fd_h,
+            fd=_capi._ptr(self.fd))
+
+    @property
+    def ref(self):
+        return _capi.C.byref(self.desc)

@@ -77,6 +77,21 @@ struct DInst {
     double st_ediv;          // sqrt(ε0·c·Aeff/2)
     double* steprec;         // [steprec_cap][LH_NSTAT]
     int steprec_cap;
+    // z-dependent fill (pressure gradient); zg_np = 0: constant fill
+    int zg_np, zg_two, zg_model, zg_loss, zd_n;
+    double sc_f;             // to_freq! crop scale (Nω-1)/(No-1)
+    double kerr_gamma3;      // γ3 of Kerr_field (0 without Kerr)
+    double zsqrt_aeff, zfd_h;
+    const double* zg_Z;      // [zg_np] profile positions
+    const double* zg_P;      // [zg_np] profile pressures
+    const double4* zd;       // [zd_n] density spline (P, ρ, D, 0)
+    const double* zgam;      // [Nω] γ(λ)
+    const double2* znwg;     // [Nω] neff_wg
+    const double2* zpre;     // [Nω] norm prefactor
+    const unsigned char* zsidx;  // [Nω]
+    const double* zfd;       // [9][4] β1 stencil (ω, γ, nwg)
+    double2* zL;             // [Nω] linop at the last refresh (= linop)
+    double2* zOF;            // [Nω] output factor at the last refresh (= outfac)
 };
 
 // Dormand–Prince tableau, src/dopri.jl:1-28 (rational literals evaluated in double).
@@ -114,6 +129,7 @@ struct SmemT {
     double edge[LH_T];
     double wsum[5 * LH_NW];  // scans use [0, 2*LH_NW], block_last uses [4*LH_NW]
     double plist[PList<MO>::CAP];
+    double zs[2];  // gradient fill: ρ(z), β1(z) from thread 0
     int flag;
 };
 
@@ -125,32 +141,25 @@ struct Bins {
 };
 
 // --------------------------------------------------------------------------
-// PPT rate: exp(CSpline(|E|)) with the uniform index rule; |E| above the
-// table raises the range flag (src/Ionisation.jl:256-269).
+// Maths.CSpline evaluation on uniform nodes (x, y, D) with the uniform index
+// rule (src/Maths.jl:771-775, 840-848).  The PPT table (log rate) and the
+// density-vs-pressure spline of a gradient fill use it.
 // --------------------------------------------------------------------------
-__device__ __forceinline__ double ppt_rate(const DInst& in, double aE, int& bad) {
-    if (aE < in.ppt_emin) return 0.0;
-    if (!(aE <= in.ppt_emax)) {  // also catches NaN: no table index is formed
-        bad = (aE > in.ppt_emax) ? LH_E_PPT_RANGE : LH_E_NONFINITE;
-        return 0.0;
-    }
-    const double xmin = in.ppt_emin, xmax = in.ppt_emax;
-    const int n = in.ppt_n;
+__device__ __forceinline__ double cspline_uniform(const double4* nodes, int n, double xmin, double xmax, double x) {
     int i;
-    if (aE <= xmin)
+    if (x <= xmin)
         i = 2;
-    else if (aE >= xmax)
+    else if (x >= xmax)
         i = n;
     else
-        i = (int)ceil((aE - xmin) / (xmax - xmin) * (double)(n - 1)) + 1;
+        i = (int)ceil((x - xmin) / (xmax - xmin) * (double)(n - 1)) + 1;
     i = min(max(i, 2), n);
-    double4 pc = in.ppt[i - 1], pm = in.ppt[i - 2];
-    if (aE == pc.x) return exp(pc.y);
-    if (aE == pm.x) return exp(pm.y);
-    double t = (aE - pm.x) / (pc.x - pm.x);
-    double v = (pm.y + pm.z * t + (3 * (pc.y - pm.y) - 2 * pm.z - pc.z) * (t * t) +
-                (2 * (pm.y - pc.y) + pm.z + pc.z) * (t * t * t));
-    return exp(v);
+    const double4 pc = nodes[i - 1], pm = nodes[i - 2];
+    if (x == pc.x) return pc.y;
+    if (x == pm.x) return pm.y;
+    const double t = (x - pm.x) / (pc.x - pm.x);
+    return (pm.y + pm.z * t + (3 * (pc.y - pm.y) - 2 * pm.z - pc.z) * (t * t) +
+            (2 * (pm.y - pc.y) + pm.z + pc.z) * (t * t * t));
 }
 
 // The same for Q samples at once: every index first, then all table loads,
@@ -260,14 +269,13 @@ __device__ __forceinline__ double field_at(const Smem& sm, int g, double e_scale
 // chunks), with the three cumtrapz integrals carried from round to round;
 // each thread's samples are touched only by that thread.
 template <int MO, class Smem>
-__device__ __forceinline__ void pointwise_field(const DInst& in, Smem& sm) {
+__device__ __forceinline__ void pointwise_field(const DInst& in, Smem& sm, double rho, double kfac) {
     constexpr int QT = 2 * MO / LH_T;          // samples per thread over all rounds
     constexpr int QR = (QT > 16) ? 16 : QT;    // samples per thread per round
     constexpr int NR = QT / QR;                // rounds
     const int tid = fresh_tid();
     const double es = in.e_scale;
     const bool kerr = (in.kerr == LH_KERR_FIELD), plasma = in.plasma;
-    const double kfac = in.kerr_fac, rho = in.density;
     const double* tw = in.to_win;
     const double dt = in.dt_o, pp1 = in.preion_p1, er = in.e_ratio, ip = in.ionpot;
     double cF = 0.0, cJ = 0.0, cP = 0.0;           // carried integrals
@@ -374,12 +382,12 @@ __device__ __forceinline__ void pointwise_field(const DInst& in, Smem& sm) {
 // Field-grid mode-averaged transform on LDS data.  On entry buf[0..N) holds
 // sc_t·Eω (zero to MO).  Calls epi(k, nl_k) for every output bin k < N.
 template <int MO, class Smem, class Epi>
-__device__ __forceinline__ void rhs_field(const DInst& in, Smem& sm, Epi epi) {
+__device__ __forceinline__ void rhs_field(const DInst& in, Smem& sm, Epi epi, double rho, double kfac) {
     __syncthreads();
     c2r_prepack<MO>(sm.buf, sm.tw);
     __syncthreads();
     fft_lds<MO, true>(sm.buf, sm.tw);
-    pointwise_field<MO, Smem>(in, sm);
+    pointwise_field<MO, Smem>(in, sm, rho, kfac);
     constexpr int N = Bins<MO>::N, NPT = Bins<MO>::NPT;
     double2 of[NPT];  // loads in flight during the forward FFT
 #pragma unroll
@@ -459,7 +467,7 @@ __device__ __forceinline__ void stage_prologue_cols(const double2* __restrict__ 
 // Statistics of the spectrum held in LDS (defined with the statistics below).
 template <int KIND, int NT, class Smem>
 __device__ __forceinline__ void stats_time(const DInst& in, Smem& sm, double s0, double s1, double se, double z,
-                                           double dz, double* rec);
+                                           double dz, double rho, double* rec);
 
 // Accepted-step window filter (stepfun, src/Luna.jl:366-370) on the coarse
 // grid: YN ← rfft(twin · irfft(ωwin · YN·exp(L h))).  With STATS and a
@@ -468,17 +476,18 @@ __device__ __forceinline__ void stats_time(const DInst& in, Smem& sm, double s0,
 template <int MC, class Smem, bool STATS = false>
 __device__ __forceinline__ void stepfun_filter_cols(const DInst& in, const double2* __restrict__ linop, Smem& sm,
                                                     double2* YN, double h, double* rec = nullptr, double z = 0.0,
-                                                    double dz = 0.0);
+                                                    double dz = 0.0, double rho = 0.0);
 
 template <int MC, class Smem>
 __device__ __forceinline__ void stepfun_filter(const DInst& in, Smem& sm, double2* YN, double h, double* rec,
-                                               double z, double dz) {
-    stepfun_filter_cols<MC, Smem, true>(in, in.linop, sm, YN, h, rec, z, dz);
+                                               double z, double dz, double rho) {
+    stepfun_filter_cols<MC, Smem, true>(in, in.linop, sm, YN, h, rec, z, dz, rho);
 }
 
 template <int MC, class Smem, bool STATS>
 __device__ __forceinline__ void stepfun_filter_cols(const DInst& in, const double2* __restrict__ linop, Smem& sm,
-                                                    double2* YN, double h, double* rec, double z, double dz) {
+                                                    double2* YN, double h, double* rec, double z, double dz,
+                                                    double rho) {
     constexpr int N = MC + 1;
     constexpr int NPT = (N + LH_T - 1) / LH_T;
 #pragma unroll
@@ -532,7 +541,7 @@ __device__ __forceinline__ void stepfun_filter_cols(const DInst& in, const doubl
                 if (k < N - 1) sm.buf[pad(k)] = make_double2(2.0 * yv[i].x, 2.0 * yv[i].y);
             }
             for (int k = N - 1 + threadIdx.x; k < 2 * MC; k += LH_T) sm.buf[pad(k)] = make_double2(0.0, 0.0);
-            stats_time<0, 2 * MC, Smem>(in, sm, s0, s1, se, z, dz, rec);
+            stats_time<0, 2 * MC, Smem>(in, sm, s0, s1, se, z, dz, rho, rec);
             return;
         }
     }
@@ -725,7 +734,7 @@ __device__ __forceinline__ void rhs_env(const DInst& in, Smem& sm, Epi epi) {
 // YN ← fft(twin · ifft(ωwin · YN·exp(L h))).
 template <int NT, class Smem>
 __device__ __forceinline__ void stepfun_env(const DInst& in, Smem& sm, double2* YN, double h, double* rec, double z,
-                                            double dz) {
+                                            double dz, double rho) {
     constexpr int NPT = (NT + LH_T - 1) / LH_T;
 #pragma unroll
     for (int i = 0; i < NPT; i++) {
@@ -762,7 +771,7 @@ __device__ __forceinline__ void stepfun_env(const DInst& in, Smem& sm, double2* 
         }
     }
     if (rec)
-        stats_time<1, NT, Smem>(in, sm, s0, s1, se, z, dz, rec);
+        stats_time<1, NT, Smem>(in, sm, s0, s1, se, z, dz, rho, rec);
     else
         __syncthreads();
 }
@@ -796,6 +805,104 @@ struct Sizes {
 };
 
 // ==========================================================================
+// z-dependent gas fill (Capillary.gradient + LinearOps.make_linop)
+// ==========================================================================
+#define LH_EPS0 8.854187817620389e-12  // CODATA 2014 ε0
+#define LH_CLIGHT 299792458.0
+
+// 8th-order central first-derivative weights at offsets -4..4 (times 1/h)
+__constant__ double c_fd8[9] = {1.0 / 280, -4.0 / 105, 1.0 / 5, -4.0 / 5, 0.0, 4.0 / 5, -1.0 / 5, 4.0 / 105, -1.0 / 280};
+
+// Pressure profile p(z) of Capillary.gradient, src/Capillary.jl:306-340.
+__device__ __forceinline__ double zg_pressure(const DInst& in, double z) {
+    const double* Z = in.zg_Z;
+    const double* P = in.zg_P;
+    const int n = in.zg_np;
+    if (in.zg_two) {
+        const double L = Z[1], p0 = P[0], p1 = P[1];
+        if (z > L) return p1;
+        if (z <= 0) return p0;
+        return sqrt(p0 * p0 + z / L * (p1 * p1 - p0 * p0));
+    }
+    if (z <= Z[0]) return P[0];
+    if (z >= Z[n - 1]) return P[n - 1];
+    int i = 0;
+    for (int j = 0; j < n; j++)
+        if (Z[j] < z) i = j;  // findlast(x -> x < z, Z)
+    return sqrt(P[i] * P[i] + (z - Z[i]) / (Z[i + 1] - Z[i]) * (P[i + 1] * P[i + 1] - P[i] * P[i]));
+}
+
+// Principal square root of x + iy.
+__device__ __forceinline__ double2 csqrt_lh(double x, double y) {
+    if (x == 0.0 && y == 0.0) return make_double2(0.0, y);
+    const double t = sqrt((fabs(x) + hypot(x, y)) * 0.5);
+    if (x >= 0.0) return make_double2(t, y / (2.0 * t));
+    return make_double2(fabs(y) / (2.0 * t), copysign(t, y));
+}
+
+// neff(mode, εco, nwg) of a Marcatili mode, src/Capillary.jl:216-234.
+__device__ __forceinline__ double2 zg_neff(const DInst& in, double eco, double2 nwg) {
+    double2 n;
+    if (in.zg_model == 0)
+        n = csqrt_lh(eco - nwg.x, -nwg.y);
+    else
+        n = make_double2(1 + (eco - 1) / 2 - nwg.x, -nwg.y);
+    if (!in.zg_loss) n.y = 0.0;
+    return n;
+}
+
+// Re-evaluate the z-dependent constants at z (make_prop!(linop!) evaluates
+// linop!(out, t2) whenever t2 changes): ρ(z) and β1(z) on thread 0, then per
+// bin the operator -i(ω/c·conj_clamp(neff) - ω β1) (src/LinearOps.jl:337-358,
+// 254) and the output factor sc_f·pre/β·√Aeff·ωwin (β = ω/c·Re neff),
+// written to in.zL / in.zOF.  Returns ρ(z) and the Kerr factor ρ ε0 γ3.
+template <class Smem>
+__device__ __forceinline__ void zg_refresh(const DInst& in, Smem& sm, double z, double& rho, double& kfac) {
+    const int tid = fresh_tid();
+    if (tid == 0) {
+        const double p = zg_pressure(in, z);
+        const double r = cspline_uniform(in.zd, in.zd_n, in.zd[0].x, in.zd[in.zd_n - 1].x, p);
+        // β1 = Modes.dispersion(mode, 1, ω0, z): β(ω) = ω/c·Re neff(mode, ω; z)
+        // with neff's own clamp (src/Capillary.jl:162-186)
+        double acc = 0.0;
+        for (int j = 0; j < 9; j++) {
+            const double* f = in.zfd + 4 * j;
+            const double sc = sqrt(1 + f[1] * r);
+            double nr = zg_neff(in, sc * sc, make_double2(f[2], f[3])).x;
+            if (in.zg_model == 0 && nr < 1e-3) nr = 1e-3;
+            acc += c_fd8[j] * (f[0] / LH_CLIGHT * nr);
+        }
+        sm.zs[0] = r;
+        sm.zs[1] = acc / in.zfd_h;
+    }
+    __syncthreads();
+    rho = sm.zs[0];
+    const double b1 = sm.zs[1];
+    kfac = rho * LH_EPS0 * in.kerr_gamma3;
+    for (int k = tid; k < in.n_omega; k += LH_T) {
+        double2 L = make_double2(0.0, 0.0), of = make_double2(in.sc_f, 0.0);
+        if (in.zsidx[k]) {
+            const double w = in.st_omega[k];
+            const double sc = sqrt(1 + in.zgam[k] * rho);  // coren(ω; z)^2
+            const double2 n = zg_neff(in, sc * sc, in.znwg[k]);
+            const double ncr = (n.x < 1e-3) ? 1e-3 : n.x;  // conj_clamp
+            const double lim = 3000 * LH_CLIGHT / w;
+            const double nci = -((n.y < 0.0) ? 0.0 : (n.y > lim ? lim : n.y));
+            const double wc = w / LH_CLIGHT;
+            const double zr = wc * ncr - w * b1, zi = wc * nci;
+            L = make_double2(zi, -zr);
+            const double beta = wc * n.x;
+            const double2 pre = in.zpre[k];
+            const double nfx = pre.x / beta * in.zsqrt_aeff, nfy = pre.y / beta * in.zsqrt_aeff;
+            of = make_double2((in.sc_f * nfx) * in.omega_win[k], (in.sc_f * nfy) * in.omega_win[k]);
+        }
+        in.zL[k] = L;
+        in.zOF[k] = of;
+    }
+    __syncthreads();
+}
+
+// ==========================================================================
 // Per-step statistics (Stats.default + collect_stats, src/Stats.jl:454-532)
 // ==========================================================================
 // Maths.linterpx, src/Maths.jl:248-252.
@@ -817,7 +924,7 @@ __device__ __forceinline__ double linterpx(double x1, double x2, double y1, doub
 // Writes rec[LH_NSTAT] from thread 0; a PPT range/NaN error sets sm.flag.
 template <int KIND, int MO, int MC, class Smem>
 __device__ __forceinline__ void step_stats(const DInst& in, Smem& sm, const double2* Y, double z, double dz,
-                                           double* rec) {
+                                           double rho, double* rec) {
     constexpr int N = Sizes<KIND, MO, MC>::N;
     constexpr int NT = MO;  // analytic-signal samples = length of grid.t
     const int tid = fresh_tid();
@@ -835,14 +942,14 @@ __device__ __forceinline__ void step_stats(const DInst& in, Smem& sm, const doub
         else
             sm.buf[pad(k)] = v;
     }
-    stats_time<KIND, NT, Smem>(in, sm, s0, s1, se, z, dz, rec);
+    stats_time<KIND, NT, Smem>(in, sm, s0, s1, se, z, dz, rho, rec);
 }
 
 // The analytic spectrum is in LDS (not yet synchronised) and each thread holds
 // its partial sums Σ|E|², Σω|E|², Σw|E|²: the record of the state.
 template <int KIND, int NT, class Smem>
 __device__ __forceinline__ void stats_time(const DInst& in, Smem& sm, double s0, double s1, double se, double z,
-                                           double dz, double* rec) {
+                                           double dz, double rho, double* rec) {
     const int tid = fresh_tid();
     const int lane = tid & 63, wid = tid >> 6;
     block_sum3(s0, s1, se, sm.wsum);  // ends synchronised: the spectrum is complete
@@ -980,7 +1087,7 @@ __device__ __forceinline__ void stats_time(const DInst& in, Smem& sm, double s0,
             // cumtrapz!(out, δt) end value: Σ ½(r_{j-1} + r_j)·δt
             const double dt = t[1] - t[0];
             double integral = (rsum - 0.5 * (sm.buf[pad(0)].y + sm.buf[pad(NT - 1)].y)) * dt;
-            ed = (1.0 - exp(-integral)) * in.density;
+            ed = (1.0 - exp(-integral)) * rho;
             pr = rmax;
         }
         rec[LH_STAT_Z] = z;
@@ -993,7 +1100,7 @@ __device__ __forceinline__ void stats_time(const DInst& in, Smem& sm, double s0,
         rec[LH_STAT_FWHM_T_MAX] = fmax_;
         rec[LH_STAT_ELECTRONDENSITY] = ed;
         rec[LH_STAT_PEAK_IONISATION_RATE] = pr;
-        rec[LH_STAT_DENSITY] = in.density;
+        rec[LH_STAT_DENSITY] = rho;
     }
     __syncthreads();
 }
@@ -1008,20 +1115,20 @@ __device__ __forceinline__ void do_prologue(const DInst& in, Smem& sm, const dou
 }
 
 template <int KIND, int MO, int MC, class Smem, class Epi>
-__device__ __forceinline__ void do_rhs(const DInst& in, Smem& sm, Epi epi) {
+__device__ __forceinline__ void do_rhs(const DInst& in, Smem& sm, Epi epi, double rho, double kfac) {
     if constexpr (KIND == 0)
-        rhs_field<MO, Smem>(in, sm, epi);
+        rhs_field<MO, Smem>(in, sm, epi, rho, kfac);
     else
         rhs_env<MO, MC, Smem>(in, sm, epi);
 }
 
 template <int KIND, int MO, int MC, class Smem>
 __device__ __forceinline__ void do_stepfun(const DInst& in, Smem& sm, double2* YN, double h, double* rec, double z,
-                                           double dz) {
+                                           double dz, double rho) {
     if constexpr (KIND == 0)
-        stepfun_filter<MC, Smem>(in, sm, YN, h, rec, z, dz);
+        stepfun_filter<MC, Smem>(in, sm, YN, h, rec, z, dz, rho);
     else
-        stepfun_env<MC, Smem>(in, sm, YN, h, rec, z, dz);
+        stepfun_env<MC, Smem>(in, sm, YN, h, rec, z, dz, rho);
 }
 
 template <int KIND, int MO, int MC, class Smem>
@@ -1040,12 +1147,17 @@ __device__ __forceinline__ void solve_instance(const DInst& in, Smem& sm) {
     long long steps = 0, accepted = 0, rejected = 0, rhs_evals = 0;
     int status = LH_OK, saved = 0, repeated = 0;
     double t = in.z0, tn = in.z0, dt = in.init_dz, dtn = in.init_dz, errlast = 0.0;
+    // density and Kerr factor of the current z (constant unless a gradient fill
+    // re-evaluates them, with the operator, at every new z)
+    double rho = in.density, kfac = in.kerr_fac;
+    const bool zdep = (KIND == 0) && in.zg_np > 0;
+    if (zdep) zg_refresh(in, sm, in.z0, rho, kfac);
 
     // PreconStepper init: k1 = fbar(y0, t, t)
     do_prologue<KIND, MO, MC, Smem>(in, sm, Y, K, 0, 0.0, 0.0, 0.0);
     {
         double2* k0 = K[0];
-        do_rhs<KIND, MO, MC, Smem>(in, sm, [&](int k, double2 nl) { k0[k] = nl; });
+        do_rhs<KIND, MO, MC, Smem>(in, sm, [&](int k, double2 nl) { k0[k] = nl; }, rho, kfac);
     }
     rhs_evals++;
     __syncthreads();
@@ -1060,14 +1172,27 @@ __device__ __forceinline__ void solve_instance(const DInst& in, Smem& sm) {
         double h0 = tn - t;
         dt = dtn;
         t = tn;
+        if (zdep) {
+            // prop!(ks[1], t, tn) with the operator at tn; each stage then
+            // re-evaluates it at its own t2
+            zg_refresh(in, sm, t, rho, kfac);
+            if (h0 != 0.0) {
+                double2* K0 = K[0];
+                for (int k = threadIdx.x; k < N; k += LH_T) K0[k] = cmul(K0[k], cexp_lh(in.linop[k], h0));
+                __syncthreads();
+            }
+            h0 = 0.0;
+        }
 #pragma unroll 1
         for (int st = 0; st < 6 && status == LH_OK; st++) {
             double t2 = t + c_nodes[st] * dt;
             double hf = t2 - t, hb = t - t2;
+            if (zdep && st != 5) zg_refresh(in, sm, t2, rho, kfac);  // stages 5 and 6 share t + dt
             do_prologue<KIND, MO, MC, Smem>(in, sm, Y, K, st, dt, h0, hf);
             double2* kout = K[st + 1];
             const double2* Lp = in.linop;
-            do_rhs<KIND, MO, MC, Smem>(in, sm, [&](int k, double2 nl) { kout[k] = cmul(nl, cexp_lh(Lp[k], hb)); });
+            do_rhs<KIND, MO, MC, Smem>(in, sm, [&](int k, double2 nl) { kout[k] = cmul(nl, cexp_lh(Lp[k], hb)); },
+                                       rho, kfac);
             rhs_evals++;
             __syncthreads();
             if (sm.flag != LH_OK) status = sm.flag;
@@ -1143,11 +1268,12 @@ __device__ __forceinline__ void solve_instance(const DInst& in, Smem& sm) {
             // (src/Output.jl:65-67) from the same LDS pass
             double* rec = (in.steprec && accepted < in.steprec_cap) ? in.steprec + (size_t)accepted * LH_NSTAT
                                                                       : nullptr;
-            do_stepfun<KIND, MO, MC, Smem>(in, sm, YN, tn - t, rec, tn, dtn);
+            do_stepfun<KIND, MO, MC, Smem>(in, sm, YN, tn - t, rec, tn, dtn, rho);
             if (rec && sm.flag != LH_OK) status = sm.flag;
             // GridCondition saves via dense output; ks[1] and ks[7] both hold k7
             while (saved < in.save_n && in.zgrid[saved] <= tn) {
                 double ti = in.zgrid[saved];
+                if (zdep && in.saves && ti != t && ti != tn) zg_refresh(in, sm, ti, rho, kfac);  // prop!(out, t, ti)
                 if (in.saves) {
                     double2* out = in.saves + (size_t)saved * N;
                     if (ti == t) {
@@ -1223,6 +1349,8 @@ __global__ __launch_bounds__(LH_T) void lh_solve_kernel(const DInst* insts, cons
 template <int KIND, int MO, int MC, class Smem>
 __device__ __forceinline__ void rhs_once(const DInst& in, Smem& sm) {
     constexpr int N = Sizes<KIND, MO, MC>::N;
+    double rho = in.density, kfac = in.kerr_fac;
+    if (KIND == 0 && in.zg_np > 0) zg_refresh(in, sm, in.z0, rho, kfac);  // the call's z
     for (int k = threadIdx.x; k < N; k += LH_T) {
         double2 v = cscal(in.sc_t, in.rhs_in[k]);
         if constexpr (KIND == 0)
@@ -1236,7 +1364,7 @@ __device__ __forceinline__ void rhs_once(const DInst& in, Smem& sm) {
         for (int k = MC / 2 + threadIdx.x; k < MO - MC / 2; k += LH_T) sm.buf[pad(k)] = make_double2(0, 0);
     }
     double2* out = in.rhs_out;
-    do_rhs<KIND, MO, MC, Smem>(in, sm, [&](int k, double2 nl) { out[k] = nl; });
+    do_rhs<KIND, MO, MC, Smem>(in, sm, [&](int k, double2 nl) { out[k] = nl; }, rho, kfac);
 }
 
 template <int KIND, int MO, int MC>
@@ -1260,7 +1388,9 @@ __global__ __launch_bounds__(LH_T) void lh_stats_kernel(const DInst* insts, cons
     if (threadIdx.x == 0) sm.flag = LH_OK;
     __syncthreads();
     const DInst& in = insts[blockIdx.x];
-    step_stats<KIND, MO, MC, SmemT<MO>>(in, sm, in.rhs_in, z, dz, in.steprec);
+    double rho = in.density, kfac = in.kerr_fac;
+    if (KIND == 0 && in.zg_np > 0) zg_refresh(in, sm, z, rho, kfac);
+    step_stats<KIND, MO, MC, SmemT<MO>>(in, sm, in.rhs_in, z, dz, rho, in.steprec);
     if (threadIdx.x == 0) status[blockIdx.x] = sm.flag;
 }
 
@@ -1388,6 +1518,14 @@ struct lh_ctx {
     double* steprec;     // [st_cap][LH_NSTAT]
     double* st_rec1;     // [LH_NSTAT], lh_stats_eval
     int result_nrec;     // records kept by the last solve
+    // gradient fill
+    int zg_on, zg_np, zg_two, zg_model, zg_loss, zd_n;
+    double zsqrt_aeff, zfd_h;
+    double *zg_Z, *zg_P, *zfd, *zgam;  // shared
+    double4* zd;                       // shared
+    double2 *znwg, *zpre;              // shared
+    unsigned char* zsidx;              // shared
+    double2 *zL, *zOF;                 // per context
 };
 
 static std::mutex g_mu;
@@ -1605,6 +1743,29 @@ static DInst base_inst(const lh_ctx* c) {
     in.st_ediv = c->st_ediv;
     in.steprec = c->st_on ? c->steprec : nullptr;
     in.steprec_cap = c->st_on ? c->st_cap : 0;
+    in.sc_f = scale_to_freq(d);
+    in.kerr_gamma3 = (d.kerr == LH_KERR_FIELD) ? d.gamma3 : 0.0;
+    if (c->zg_on) {
+        in.zg_np = c->zg_np;
+        in.zg_two = c->zg_two;
+        in.zg_model = c->zg_model;
+        in.zg_loss = c->zg_loss;
+        in.zd_n = c->zd_n;
+        in.zsqrt_aeff = c->zsqrt_aeff;
+        in.zfd_h = c->zfd_h;
+        in.zg_Z = c->zg_Z;
+        in.zg_P = c->zg_P;
+        in.zd = c->zd;
+        in.zgam = c->zgam;
+        in.znwg = c->znwg;
+        in.zpre = c->zpre;
+        in.zsidx = c->zsidx;
+        in.zfd = c->zfd;
+        in.zL = c->zL;
+        in.zOF = c->zOF;
+        in.linop = c->zL;  // refreshed in place at every new z
+        in.outfac = c->zOF;
+    }
     return in;
 }
 
@@ -1832,11 +1993,13 @@ int lh_destroy(lh_ctx* c) {
     hipSetDevice(c->device);
     if (c->stream) hipStreamSynchronize(c->stream);
     for (const void* p : {(const void*)c->omega_win, (const void*)c->t_win, (const void*)c->to_win, (const void*)c->ppt,
-                          (const void*)c->omega, (const void*)c->st_ew, (const void*)c->st_t})
+                          (const void*)c->omega, (const void*)c->st_ew, (const void*)c->st_t, (const void*)c->zg_Z,
+                          (const void*)c->zg_P, (const void*)c->zfd, (const void*)c->zgam, (const void*)c->zd,
+                          (const void*)c->znwg, (const void*)c->zpre, (const void*)c->zsidx})
         shared_release(p);
     void* ptrs[] = {c->linop, c->outfac, c->hwdt, c->escr, c->Yin, c->Y, c->YN,
                     c->saves, c->zgrid, c->dstats, c->dinst, c->rhs_in, c->rhs_out, c->dstatus,
-                    c->steprec, c->st_rec1};
+                    c->steprec, c->st_rec1, c->zL, c->zOF};
     for (void* p : ptrs)
         if (p) hipFree(p);
     for (int j = 0; j < 7; j++)
@@ -1868,6 +2031,8 @@ int lh_create(lh_ctx** out, const lh_desc* d) {
     const size_t N = (size_t)d->n_omega, NS = (size_t)c->n_state;
     if (!st) st = shared_upload(c->device, &c->omega_win, d->omega_win, N);
     if (!st && d->omega) st = shared_upload(c->device, &c->omega, d->omega, N);
+    if (!st && d->transform == LH_TRANS_MODEAVG)
+        st = shared_upload(c->device, &c->zsidx, d->sidx, N);
     if (!st) st = shared_upload(c->device, &c->t_win, d->t_win, (size_t)d->n_t);
     if (!st) st = shared_upload(c->device, &c->to_win, d->to_win, (size_t)d->n_to);
     if (!st) {
@@ -1948,7 +2113,6 @@ int lh_ctx_status(lh_ctx* c) {
 }
 
 int lh_rhs_device(lh_ctx* c, void* nl_dev, const void* Ew_dev, double z, void* stream) {
-    (void)z;  // constant density / Aeff / linear operator: no z dependence
     if (!c || !nl_dev || !Ew_dev) return LH_E_ARG;
     HIPCK(hipSetDevice(c->device));
     hipStream_t s = stream ? (hipStream_t)stream : c->stream;
@@ -1968,6 +2132,7 @@ int lh_rhs_device(lh_ctx* c, void* nl_dev, const void* Ew_dev, double z, void* s
         DInst in = base_inst(c);
         in.rhs_in = (const double2*)Ew_dev;
         in.rhs_out = (double2*)nl_dev;
+        in.z0 = z;  // a gradient fill evaluates its constants at z
         HIPCK(hipMemcpyAsync(c->dinst, &in, sizeof(DInst), hipMemcpyHostToDevice, s));
         launch_rhs(ctx_class(c), c->dinst, tw, c->dstatus, s);
     }
@@ -2166,6 +2331,62 @@ int lh_set_step_stats(lh_ctx* c, const lh_stats_desc* sd) {
     c->st_ediv = sqrt(EPS0 * CLIGHT * sd->aeff / 2);  // sqrt(ε_0*c*aeff(z)/2), src/Stats.jl:236
     c->st_on = 1;
     c->result_nrec = 0;
+    return LH_OK;
+}
+
+static void zg_release(lh_ctx* c) {
+    for (const void* p : {(const void*)c->zg_Z, (const void*)c->zg_P, (const void*)c->zfd, (const void*)c->zgam,
+                          (const void*)c->zd, (const void*)c->znwg, (const void*)c->zpre})
+        shared_release(p);
+    c->zg_Z = c->zg_P = c->zfd = c->zgam = nullptr;
+    c->zd = nullptr;
+    c->znwg = c->zpre = nullptr;
+}
+
+int lh_set_gradient(lh_ctx* c, const lh_gradient_desc* g) {
+    if (!c) return LH_E_ARG;
+    HIPCK(hipSetDevice(c->device));
+    if (!g) {
+        c->zg_on = 0;
+        zg_release(c);
+        return LH_OK;
+    }
+    const lh_desc& d = c->d;
+    if (d.transform != LH_TRANS_MODEAVG || d.grid_kind != LH_GRID_REAL || !c->omega || !c->zsidx) return LH_E_ARG;
+    if (g->n_points < 2 || !g->Z || !g->P || g->dens_n < 4 || !g->dens_p || !g->dens_rho || !g->dens_d ||
+        !g->gamma || !g->nwg || !g->pre || !g->fd || !(g->fd_h > 0) || !(g->sqrt_aeff > 0) ||
+        (g->model != 0 && g->model != 1) || (g->two_point && g->n_points != 2))
+        return LH_E_ARG;
+    for (int i = 0; i + 1 < g->n_points; i++)
+        if (!(g->Z[i] < g->Z[i + 1])) return LH_E_ARG;
+    if (g->two_point && !(g->Z[0] == 0.0)) return LH_E_ARG;
+    const size_t N = (size_t)d.n_omega;
+    std::vector<double4> spl((size_t)g->dens_n);
+    for (int i = 0; i < g->dens_n; i++) spl[i] = make_double4(g->dens_p[i], g->dens_rho[i], g->dens_d[i], 0.0);
+    zg_release(c);
+    int st = LH_OK;
+    if (!st) st = shared_upload(c->device, &c->zg_Z, g->Z, (size_t)g->n_points);
+    if (!st) st = shared_upload(c->device, &c->zg_P, g->P, (size_t)g->n_points);
+    if (!st) st = shared_upload(c->device, &c->zd, spl.data(), spl.size());
+    if (!st) st = shared_upload(c->device, &c->zgam, g->gamma, N);
+    if (!st) st = shared_upload(c->device, &c->znwg, g->nwg, N);
+    if (!st) st = shared_upload(c->device, &c->zpre, g->pre, N);
+    if (!st) st = shared_upload(c->device, &c->zfd, g->fd, (size_t)36);
+    if (!st && !c->zL && hipMalloc(&c->zL, sizeof(double2) * N) != hipSuccess) st = LH_E_NOMEM;
+    if (!st && !c->zOF && hipMalloc(&c->zOF, sizeof(double2) * N) != hipSuccess) st = LH_E_NOMEM;
+    if (st) {
+        zg_release(c);
+        c->zg_on = 0;
+        return st;
+    }
+    c->zg_np = g->n_points;
+    c->zg_two = g->two_point;
+    c->zg_model = g->model;
+    c->zg_loss = g->loss;
+    c->zd_n = g->dens_n;
+    c->zsqrt_aeff = g->sqrt_aeff;
+    c->zfd_h = g->fd_h;
+    c->zg_on = 1;
     return LH_OK;
 }
 

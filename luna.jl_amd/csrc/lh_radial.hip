@@ -162,7 +162,7 @@ __global__ __launch_bounds__(LH_T) void rad_pointwise(const RadArgs* __restrict_
     double* col = a.Er + (size_t)a.n_to * blockIdx.x;
     for (int j = threadIdx.x; j < MO; j += LH_T) sm.buf[pad(j)] = make_double2(col[2 * j], col[2 * j + 1]);
     __syncthreads();
-    pointwise_field<MO, SmemT<MO>>(*a.pw, sm);
+    pointwise_field<MO, SmemT<MO>>(*a.pw, sm, a.pw->density, a.pw->kerr_fac);
     __syncthreads();
     for (int j = threadIdx.x; j < MO; j += LH_T) {
         double2 z = sm.buf[pad(j)];

@@ -13,6 +13,7 @@ import numpy as np
 from . import _capi
 from . import stats as Stats
 from . import physdata as pd
+from .capillary import Gradient, gradient_spec
 from .fields import GaussField, SechField, ShotNoise
 from .grid import EnvGrid, RealGrid
 from .ionisation import ppt_spline_cached
@@ -86,12 +87,26 @@ def prop_capillary_args(radius, flength, gas, pressure, *, lambda0, lambda_lims,
                         ppt_N=2 ** 16, density=None):
     """Returns ``(Eω, grid, linop, transform, FT, output)`` like ``Interface.jl:360-401``.
 
-    ``density`` overrides the number density (the reference obtains it from
-    CoolProp, which is not available here)."""
+    ``pressure``: a number (constant fill), ``(p0, p1)`` (a gradient from z = 0
+    to ``flength``, ``Capillary.gradient(gas, L, p0, p1)``) or ``(Z, P)`` (a
+    multi-point gradient), as ``makemode_s``/``makedensity`` accept
+    (``Interface.jl:491-532``).  ``density`` overrides the number density of a
+    constant fill (the reference obtains it from CoolProp, which is not
+    available here)."""
     plasma = True if plasma is None else plasma
     grid = RealGrid(flength, lambda0, lambda_lims, trange)
+    grad = None
+    if isinstance(pressure, (tuple, list)):
+        if np.ndim(pressure[0]) == 0:
+            grad = Gradient.linear(gas, flength, float(pressure[0]), float(pressure[1]), T=temperature)
+        else:
+            grad = Gradient(gas, pressure[0], pressure[1], T=temperature)
+        pressure = float(grad.pressure(0.0))
     mode = MarcatiliMode(radius, gas, pressure, T=temperature, model=model, loss=loss)
-    rho = pd.density(gas, pressure, temperature) if density is None else density
+    if grad is not None:
+        rho = grad.density(0.0)
+    else:
+        rho = pd.density(gas, pressure, temperature) if density is None else density
     resp = []
     if kerr:
         resp.append(Kerr_field(pd.gamma3_gas(gas)))
@@ -107,7 +122,8 @@ def prop_capillary_args(radius, flength, gas, pressure, *, lambda0, lambda_lims,
     linop, beta, beta1 = make_const_linop(grid, mode, grid.ref_lambda)
     aeff = mode.Aeff()
     norm = norm_mode_average(grid, beta, aeff)
-    transform = TransModeAvg(grid, resp, rho, norm, aeff)
+    spec = gradient_spec(grid, mode, grad, grid.ref_lambda) if grad is not None else None
+    transform = TransModeAvg(grid, resp, rho, norm, aeff, gradient=spec)
     Ew = sum(f(grid) for f in inputs)
     stats = Stats.default(grid, Ew, mode, linop, transform, gas=gas)
     output = MemoryOutput(0, grid.zmax, saveN, stats)

@@ -81,6 +81,28 @@ class MarcatiliMode:
             return val + 1j * (pd.c ** 3 * self.unm ** 2) / (a ** 3 * omega ** 3) * vn
         return np.real(val)
 
+    def vn(self, omega):
+        lam = pd.wlfreq(np.asarray(omega, dtype=float))
+        eps_cl = self._cladn(lam) ** 2
+        if self.kind == "HE":
+            return (eps_cl + 1) / (2 * np.sqrt(eps_cl - 1 + 0j))
+        if self.kind == "TE":
+            return 1 / np.sqrt(eps_cl - 1 + 0j)
+        return eps_cl / np.sqrt(eps_cl - 1 + 0j)
+
+    def neff_wg(self, omega):
+        """Waveguide part of the effective index, ``neff_wg`` (``Capillary.jl:188-214``):
+        neff = sqrt(εco - nwg) (:full) or 1 + (εco - 1)/2 - nwg (:reduced)."""
+        omega = np.asarray(omega, dtype=float)
+        vn, a = self.vn(omega), self.a
+        if self.model == "full":
+            k = omega / pd.c
+            return (self.unm / (k * a)) ** 2 * (1 - 1j * vn / (k * a)) ** 2
+        val = pd.c ** 2 * self.unm ** 2 / (2 * omega ** 2 * a ** 2) + 0j
+        if self.loss:
+            val = val + 1j * (pd.c ** 3 * self.unm ** 2) / (a ** 3 * omega ** 3) * vn
+        return val
+
     def beta(self, omega):
         return np.asarray(omega) / pd.c * np.real(self.neff(omega))
 

@@ -146,10 +146,19 @@ class TransModeAvg(_TransformBase):
     kind = _capi.LH_TRANS_MODEAVG
     n_r = 1
 
-    def __init__(self, grid, responses, densityfun, norm, aeff):
+    def __init__(self, grid, responses, densityfun, norm, aeff, gradient=None):
+        """``gradient``: a ``capillary.GradientSpec`` for a z-dependent fill, whose
+        density, operator and normalisation the kernels re-evaluate at every z
+        (``densityfun`` and ``norm`` then give the z = 0 values)."""
         super().__init__(grid, responses, densityfun)
         self.norm = np.asarray(norm, dtype=np.complex128)
         self.aeff = _aeff_value(aeff)
+        self.gradient = gradient
+
+    def descriptor(self, linop=None):
+        holder = super().descriptor(linop)
+        holder.gradient = self.gradient
+        return holder
 
     def norm_array(self):
         return self.norm

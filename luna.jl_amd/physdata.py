@@ -105,6 +105,15 @@ def pressure(gas, density, T=roomtemp):
     return rho * R * T * (1 + B * rho + C * rho * rho) / bar
 
 
+def density_spline(gas, Pmax, Pmin=0.0, N=2 ** 10, T=roomtemp):
+    """Nodes (P, ρ, D) of ``densityspline`` (``PhysData.jl:719-723``): a CSpline
+    of the density over N uniform pressures range(Pmin, Pmax, N)."""
+    from .maths import cspline_derivatives, julia_range
+    P = julia_range(Pmin, Pmax, N)
+    rho = np.array([density(gas, float(p), T) for p in P])
+    return P, rho, cspline_derivatives(rho)
+
+
 def dens_1bar_0degC(gas):
     return density(gas, 1.0, 273.15)
 

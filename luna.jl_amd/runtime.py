@@ -22,6 +22,9 @@ class Context:
         h = C.c_void_p()
         _capi.check(L.lh_create(C.byref(h), holder.ref), "lh_create")
         self.handle = h
+        grad = getattr(holder, "gradient", None)
+        if grad is not None:  # z-dependent fill (pressure gradient)
+            _capi.check(L.lh_set_gradient(h, grad.ref), "lh_set_gradient")
         self.statsfun = None
 
     def __del__(self):

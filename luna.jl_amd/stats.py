@@ -14,8 +14,8 @@ of ``_capi.LH_NSTAT`` values per accepted step:
   grid with ``PlasmaCumtrapz`` only), density (``:325-329``), z and dz
   (``zdz!``, ``:409-412``).
 
-``pressure`` (``:335-339``) is constant in z for the constant-density
-propagations handled here and is filled on the host from the gas model.
+``pressure`` (``:335-339``, PhysData.pressure of the density) is filled on the
+host from each record's density with the gas model.
 ``zdw`` (``Modes.zdw``, a root search on the mode's dispersion) and the
 ``windows``/``userfuns``/``onaxis`` options are not provided.
 """
@@ -56,7 +56,7 @@ class StatsFun:
     and the description the propagation kernel uses to record the same values
     at every accepted step (``desc()``)."""
 
-    def __init__(self, grid, transform, capacity=16384, constants=None):
+    def __init__(self, grid, transform, capacity=16384, gas=None):
         if getattr(transform, "kind", None) != _capi.LH_TRANS_MODEAVG:
             raise NotImplementedError("per-step statistics are provided for mode-averaged transforms")
         self.grid = grid
@@ -66,7 +66,7 @@ class StatsFun:
         self.t = np.ascontiguousarray(grid.t, dtype=float)
         self.aeff = float(transform.aeff)
         self.plasma = grid.kind == "real" and any(isinstance(r, PlasmaCumtrapz) for r in transform.responses)
-        self.constants = dict(constants or {})
+        self.gas = gas
 
     def desc(self):
         d = _capi.LhStatsDesc()
@@ -84,11 +84,12 @@ class StatsFun:
         return keys
 
     def to_dict(self, records):
-        """Records [n][LH_NSTAT] → {name: array over steps}, plus host constants."""
+        """Records [n][LH_NSTAT] → {name: array over steps} (plus ``pressure``
+        from the recorded density when a gas is given)."""
         records = np.asarray(records, dtype=float).reshape(-1, _capi.LH_NSTAT)
         out = {k: records[:, _capi.STAT_KEYS.index(k)].copy() for k in self.keys()}
-        for k, v in self.constants.items():
-            out[k] = np.full(len(records), v)
+        if self.gas is not None:
+            out["pressure"] = np.array([pd.pressure(self.gas, r) for r in out["density"]])
         return out
 
     def __call__(self, Ew, z, dz):
@@ -104,7 +105,4 @@ def default(grid, Ew, mode, linop, transform, *, windows=None, gas=None, onaxis=
     userfuns)``, ``src/Stats.jl:492-532``, for a mode-averaged transform."""
     if windows or onaxis or userfuns:
         raise NotImplementedError("windows, onaxis and userfuns statistics are not provided")
-    constants = {}
-    if gas is not None:
-        constants["pressure"] = pd.pressure(gas, transform.density)
-    return StatsFun(grid, transform, capacity=capacity, constants=constants)
+    return StatsFun(grid, transform, capacity=capacity, gas=gas)

@@ -21,6 +21,10 @@
  *   - RK45.solve / PreconStepper src/RK45.jl:27-89, 128-309, 343-406, dopri.jl:1-28
  *   - Luna.run stepfun filter    src/Luna.jl:358-391
  *   - MemoryOutput/GridCondition src/Output.jl:65-81, 490-502
+ *   - gradient fill: Capillary.gradient src/Capillary.jl:306-340, densityspline
+ *     src/PhysData.jl:719-723, neff(mode, εco, nwg) src/Capillary.jl:216-234,
+ *     make_linop src/LinearOps.jl:337-358 with conj_clamp :254, make_prop!(linop!)
+ *     src/RK45.jl:283-296, norm_mode_average src/NonlinearRHS.jl:375-387
  *   - Stats.default / collect_stats, mode-averaged set, src/Stats.jl:18-258,
  *     409-412, 421-468, 476-532; Fields.energyfuncs src/Fields.jl:510-534;
  *     Maths.moment/level_xings/linterpx src/Maths.jl:89-96, 162-252;
@@ -415,9 +419,15 @@ static const double interpC_[4][7] = {
 
 typedef int (*orc_rhs_fn)(void *user, cplx *out, const cplx *y, double t);
 
+typedef void (*orc_linop_fn)(void *user, double t, cplx *L);
+
 typedef struct {
     int n;
     const cplx *linop;
+    orc_linop_fn linop_at; /* non-constant operator: linop!(out, t) (NULL: constant) */
+    void *luser;
+    cplx *Lz;
+    double lastt2;
     orc_rhs_fn f;
     void *user;
     cplx *y, *yn, *yi, *yerr, *ybar, *ks[7];
@@ -426,11 +436,19 @@ typedef struct {
     int64_t rhs_evals;
 } stepper;
 
-/* make_prop!(linop::AbstractArray), src/RK45.jl:271-281: y .*= exp(linop*h) */
+/* make_prop!(linop::AbstractArray), src/RK45.jl:271-281: y .*= exp(linop*h);
+ * make_prop!(linop!), :283-296: the operator is evaluated at t2 (the later
+ * time, also when propagating backwards) whenever t2 changes. */
 static void prop(stepper *s, cplx *y, double t1, double t2, int bwd) {
     double h = bwd ? (t1 - t2) : (t2 - t1);
+    const cplx *Lv = s->linop;
+    if (s->linop_at) {
+        if (s->lastt2 != t2) s->linop_at(s->luser, t2, s->Lz);
+        s->lastt2 = t2;
+        Lv = s->Lz;
+    }
     for (int i = 0; i < s->n; i++) {
-        cplx L = s->linop[i];
+        cplx L = Lv[i];
         double zr = creal(L) * h, zi = cimag(L) * h;
         cplx e;
         if (zi == 0.0)
@@ -492,10 +510,14 @@ static void stepcontrol_pi(stepper *s) {
 
 static int stepper_init(stepper *s, int n, const cplx *linop, orc_rhs_fn f, void *user, const cplx *y0,
                         double t, double dt, double rtol, double atol, double safety, double max_dt,
-                        double min_dt) {
+                        double min_dt, orc_linop_fn linop_at, void *luser) {
     memset(s, 0, sizeof(*s));
     s->n = n;
     s->linop = linop;
+    s->linop_at = linop_at;
+    s->luser = luser;
+    s->lastt2 = -INFINITY; /* typemin(Float64) */
+    if (linop_at) s->Lz = (cplx *)calloc((size_t)n, sizeof(cplx));
     s->f = f;
     s->user = user;
     size_t b = sizeof(cplx) * (size_t)n;
@@ -511,7 +533,7 @@ static int stepper_init(stepper *s, int n, const cplx *linop, orc_rhs_fn f, void
 }
 
 static void stepper_free(stepper *s) {
-    free(s->y); free(s->yn); free(s->yi); free(s->yerr); free(s->ybar);
+    free(s->y); free(s->yn); free(s->yi); free(s->yerr); free(s->ybar); free(s->Lz);
     for (int i = 0; i < 7; i++) free(s->ks[i]);
 }
 
@@ -590,12 +612,25 @@ typedef struct {
     const lh_desc *d;
     rhs_ws w;
     int status;
+    const lh_gradient_desc *g; /* gradient fill or NULL */
+    cplx *nf;                  /* [n_omega] norm at the RHS's z */
 } rhs_user;
 
+static void grad_constants(const lh_desc *d, const lh_gradient_desc *g, double z, cplx *L, cplx *nf, double *rho);
+
+/* t(nl, Eω, z): with a gradient fill, ρ(z) and norm!(nl, z) at the call's z. */
 static int rhs_cb(void *user, cplx *out, const cplx *y, double t) {
-    (void)t;
     rhs_user *u = (rhs_user *)user;
-    return rhs_eval(u->d, &u->w, out, y);
+    if (!u->g) return rhs_eval(u->d, &u->w, out, y);
+    lh_desc dz = *u->d;
+    grad_constants(u->d, u->g, t, NULL, u->nf, &dz.density);
+    dz.norm = (const double *)u->nf;
+    return rhs_eval(&dz, &u->w, out, y);
+}
+
+static void grad_linop_cb(void *user, double t, cplx *L) {
+    rhs_user *u = (rhs_user *)user;
+    grad_constants(u->d, u->g, t, L, NULL, NULL);
 }
 
 int orc_fft_c2c(int n, double *inout, int sign) {
@@ -668,7 +703,7 @@ int orc_solve_precon(int n, const double *linop, const double *y0, double t0, do
     py_user pu = {f, user};
     stepper s;
     int st = stepper_init(&s, n, (const cplx *)linop, py_rhs, &pu, (const cplx *)y0, t0, dt0, rtol, atol, 0.9,
-                          INFINITY, 0.0);
+                          INFINITY, 0.0, NULL, NULL);
     if (st) return st;
     for (int i = 0; i < outputN; i++) {
         /* range(t, stop=tmax, length=outputN) */
@@ -829,19 +864,111 @@ static int step_stats(const lh_desc *d, const double *t, double aeff, const cplx
     return st;
 }
 
+/* ------------------------------------------------------------------------ */
+/* Gradient fill                                                             */
+/* ------------------------------------------------------------------------ */
+/* p(z) of Capillary.gradient, src/Capillary.jl:306-340 */
+static double grad_pressure(const lh_gradient_desc *g, double z) {
+    const double *Z = g->Z, *P = g->P;
+    int n = g->n_points;
+    if (g->two_point) {
+        double L = Z[1], p0 = P[0], p1 = P[1];
+        return z > L ? p1 : z <= 0 ? p0 : sqrt(p0 * p0 + z / L * (p1 * p1 - p0 * p0));
+    }
+    if (z <= Z[0]) return P[0];
+    if (z >= Z[n - 1]) return P[n - 1];
+    int i = 0;
+    for (int j = 0; j < n; j++)
+        if (Z[j] < z) i = j; /* findlast(x -> x < z, Z) */
+    return sqrt(P[i] * P[i] + (z - Z[i]) / (Z[i + 1] - Z[i]) * (P[i + 1] * P[i + 1] - P[i] * P[i]));
+}
+
+/* principal square root of x + iy */
+static cplx csqrt_orc(double x, double y) {
+    if (x == 0.0 && y == 0.0) return 0.0 + I * y;
+    double t = sqrt((fabs(x) + hypot(x, y)) * 0.5);
+    if (x >= 0.0) return t + I * (y / (2.0 * t));
+    return fabs(y) / (2.0 * t) + I * copysign(t, y);
+}
+
+/* neff(mode, εco, nwg), src/Capillary.jl:216-234 */
+static cplx grad_neff(const lh_gradient_desc *g, double eco, double nwr, double nwi) {
+    cplx n;
+    if (g->model == 0)
+        n = csqrt_orc(eco - nwr, -nwi);
+    else
+        n = (1 + (eco - 1) / 2 - nwr) + I * (-nwi);
+    if (!g->loss) n = creal(n);
+    return n;
+}
+
+static const double FD8_[9] = {1.0 / 280, -4.0 / 105, 1.0 / 5, -4.0 / 5, 0.0, 4.0 / 5, -1.0 / 5, 4.0 / 105, -1.0 / 280};
+
+/* The z-dependent constants at z: ρ(z) = densityspline(p(z)); the operator
+ * linop!(out, z) of make_linop (src/LinearOps.jl:337-358):
+ *   out = -im*(ω/c*conj_clamp(neff(εco(z), nwg), ω) - ω*β1(z)) on sidx, 0 elsewhere,
+ *   β1(z) = dβ/dω at ω0 with β = ω/c*real(neff(mode, ω; z)) (8th-order stencil);
+ * and norm_mode_average's factor pre/β(ω, z)*sqrt(aeff) (src/NonlinearRHS.jl:375-387).
+ * L, nf, rho may be NULL. */
+static void grad_constants(const lh_desc *d, const lh_gradient_desc *g, double z, cplx *L, cplx *nf, double *rho) {
+    double r = cspline_eval(g->dens_n, g->dens_p, g->dens_rho, g->dens_d, grad_pressure(g, z));
+    if (rho) *rho = r;
+    int N = d->n_omega;
+    const cplx *nwg = (const cplx *)g->nwg, *pre = (const cplx *)g->pre;
+    if (L) {
+        double acc = 0.0;
+        for (int j = 0; j < 9; j++) {
+            const double *f = g->fd + 4 * j;
+            double co = sqrt(1 + f[1] * r);
+            double nr = creal(grad_neff(g, co * co, f[2], f[3]));
+            if (g->model == 0 && nr < 1e-3) nr = 1e-3; /* neff(mode, ω; z)'s clamp */
+            acc += FD8_[j] * (f[0] / ORC_C_LIGHT * nr);
+        }
+        double b1 = acc / g->fd_h;
+        for (int i = 0; i < N; i++) {
+            L[i] = 0.0;
+            if (!d->sidx[i]) continue;
+            double w = d->omega[i];
+            double co = sqrt(1 + g->gamma[i] * r); /* coren(ω; z) */
+            cplx n = grad_neff(g, co * co, creal(nwg[i]), cimag(nwg[i]));
+            /* conj_clamp, src/LinearOps.jl:254 */
+            double lim = 3000 * ORC_C_LIGHT / w;
+            double ncr = creal(n) < 1e-3 ? 1e-3 : creal(n);
+            double nci = cimag(n) < 0 ? 0.0 : (cimag(n) > lim ? lim : cimag(n));
+            double wc = w / ORC_C_LIGHT;
+            double ar = wc * ncr - w * b1, ai = wc * (-nci);
+            L[i] = ai + I * (-ar); /* -im*(a) */
+        }
+    }
+    if (nf) {
+        for (int i = 0; i < N; i++) {
+            nf[i] = 1.0;
+            if (!d->sidx[i]) continue;
+            double w = d->omega[i];
+            double co = sqrt(1 + g->gamma[i] * r);
+            double beta = w / ORC_C_LIGHT * creal(grad_neff(g, co * co, creal(nwg[i]), cimag(nwg[i])));
+            nf[i] = (creal(pre[i]) / beta * g->sqrt_aeff) + I * (cimag(pre[i]) / beta * g->sqrt_aeff);
+        }
+    }
+}
+
 /* Luna.run + solve_precon + stepfun filter + GridCondition saves. */
 static int solve_impl(const lh_desc *d, double *Ew_inout, const lh_rk_opts *o, double *saves, double *zsaves,
-                      lh_stats *stats, const double *tgrid, double aeff, double *rec, int cap, int *nrec) {
+                      lh_stats *stats, const double *tgrid, double aeff, double *rec, int cap, int *nrec,
+                      const lh_gradient_desc *g) {
     if (nrec) *nrec = 0;
     rhs_user u;
+    memset(&u, 0, sizeof(u));
     u.d = d;
+    u.g = g;
     ws_init(&u.w, d);
     int N = d->n_omega, nr = d->n_r, n = N * nr;
+    if (g) u.nf = (cplx *)malloc(sizeof(cplx) * (size_t)N);
     cplx *lin = (cplx *)d->linop;
     stepper s;
     memset(stats, 0, sizeof(*stats));
     int st = stepper_init(&s, n, lin, rhs_cb, &u, (const cplx *)Ew_inout, o->z0, o->init_dz, o->rtol, o->atol,
-                          o->safety, o->max_dz, o->min_dz);
+                          o->safety, o->max_dz, o->min_dz, g ? grad_linop_cb : NULL, &u);
     int nt = d->n_t;
     double *Et = (double *)malloc(sizeof(double) * (size_t)nt);
     cplx *Etc = (cplx *)malloc(sizeof(cplx) * (size_t)nt);
@@ -883,7 +1010,9 @@ static int solve_impl(const lh_desc *d, double *Ew_inout, const lh_rk_opts *o, d
             }
             /* MemoryOutput: statistics of the filtered state, src/Output.jl:65-67 */
             if (rec && stats->accepted <= cap) {
-                st = step_stats(d, tgrid, aeff, s.yn, s.tn, s.dtn, rec + (size_t)(stats->accepted - 1) * LH_NSTAT);
+                lh_desc dz = *d; /* density(z) at tn */
+                if (g) grad_constants(d, g, s.tn, NULL, NULL, &dz.density);
+                st = step_stats(&dz, tgrid, aeff, s.yn, s.tn, s.dtn, rec + (size_t)(stats->accepted - 1) * LH_NSTAT);
                 *nrec = (int)stats->accepted;
             }
             /* MemoryOutput + GridCondition: src/Output.jl:65-81, 497-502 */
@@ -905,7 +1034,7 @@ static int solve_impl(const lh_desc *d, double *Ew_inout, const lh_rk_opts *o, d
     stats->saved = saved;
     stats->z = s.tn;
     stats->dz = s.dtn;
-    free(Et); free(Etc); free(work); free(grid);
+    free(Et); free(Etc); free(work); free(grid); free(u.nf);
     stepper_free(&s);
     ws_free(&u.w);
     return st;
@@ -913,14 +1042,40 @@ static int solve_impl(const lh_desc *d, double *Ew_inout, const lh_rk_opts *o, d
 
 int orc_solve(const lh_desc *d, double *Ew_inout, const lh_rk_opts *o, double *saves, double *zsaves,
               lh_stats *stats) {
-    return solve_impl(d, Ew_inout, o, saves, zsaves, stats, NULL, 0.0, NULL, 0, NULL);
+    return solve_impl(d, Ew_inout, o, saves, zsaves, stats, NULL, 0.0, NULL, 0, NULL, NULL);
+}
+
+/* Propagation with a gradient fill (make_linop + make_prop!(linop!)). */
+int orc_solve_grad(const lh_desc *d, const lh_gradient_desc *g, double *Ew_inout, const lh_rk_opts *o,
+                   double *saves, double *zsaves, lh_stats *stats) {
+    return solve_impl(d, Ew_inout, o, saves, zsaves, stats, NULL, 0.0, NULL, 0, NULL, g);
+}
+
+/* One RHS t(nl, Eω, z) with a gradient fill. */
+int orc_rhs_grad(const lh_desc *d, const lh_gradient_desc *g, double *nl, const double *Ew, double z) {
+    rhs_user u;
+    memset(&u, 0, sizeof(u));
+    u.d = d;
+    u.g = g;
+    ws_init(&u.w, d);
+    u.nf = (cplx *)malloc(sizeof(cplx) * (size_t)d->n_omega);
+    int st = rhs_cb(&u, (cplx *)nl, (const cplx *)Ew, z);
+    free(u.nf);
+    ws_free(&u.w);
+    return st;
+}
+
+/* The gradient fill's constants at z: operator, norm factor, density. */
+int orc_grad_constants(const lh_desc *d, const lh_gradient_desc *g, double z, double *L, double *nf, double *rho) {
+    grad_constants(d, g, z, (cplx *)L, (cplx *)nf, rho);
+    return LH_OK;
 }
 
 /* Same with per-step statistics: rec[cap][LH_NSTAT] receives the records of
  * the first cap accepted steps, *nrec their count. */
 int orc_solve_stats(const lh_desc *d, double *Ew_inout, const lh_rk_opts *o, double *saves, double *zsaves,
                     lh_stats *stats, const double *t, double aeff, double *rec, int cap, int *nrec) {
-    return solve_impl(d, Ew_inout, o, saves, zsaves, stats, t, aeff, rec, cap, nrec);
+    return solve_impl(d, Ew_inout, o, saves, zsaves, stats, t, aeff, rec, cap, nrec, NULL);
 }
 
 /* One record of a given state (the stats closure called directly). */

@@ -41,6 +41,10 @@ def lib(capi):
     L.orc_solve_stats.argtypes = [D, dp, C.POINTER(capi.LhRkOpts), dp, dp, C.POINTER(capi.LhStats), dp,
                                   C.c_double, dp, C.c_int, C.POINTER(C.c_int)]
     L.orc_fwhm.argtypes = [C.c_int, dp, dp, C.c_int, dp]
+    G = C.POINTER(capi.LhGradientDesc)
+    L.orc_rhs_grad.argtypes = [D, G, dp, dp, C.c_double]
+    L.orc_solve_grad.argtypes = [D, G, dp, C.POINTER(capi.LhRkOpts), dp, dp, C.POINTER(capi.LhStats)]
+    L.orc_grad_constants.argtypes = [D, G, C.c_double, dp, dp, dp]
     L.ORC_RHS_CB = C.CFUNCTYPE(C.c_int, C.c_void_p, dp, dp, C.c_double)
     L.orc_solve_precon.argtypes = [C.c_int, dp, dp, C.c_double, C.c_double, C.c_double, C.c_double,
                                    C.c_double, L.ORC_RHS_CB, C.c_void_p, C.c_int, dp, dp,
@@ -116,6 +120,32 @@ class Oracle:
         stats = self.capi.LhStats()
         st = self.L.orc_solve(holder.ref, _p(Ew), C.byref(opts), _p(saves) if want_saves else None,
                               _p(zs), C.byref(stats))
+        return st, Ew, saves, zs, stats
+
+    # -- gradient fill ------------------------------------------------------
+    def grad_constants(self, holder, spec, z):
+        """(operator L(ω, z), norm factor, ρ(z)) of a gradient fill."""
+        n = holder.desc.n_omega
+        Lz = np.zeros(n, dtype=np.complex128)
+        nf = np.zeros(n, dtype=np.complex128)
+        rho = C.c_double(0.0)
+        self.L.orc_grad_constants(holder.ref, spec.ref, z, _p(Lz), _p(nf), C.byref(rho))
+        return Lz, nf, rho.value
+
+    def rhs_grad(self, holder, spec, Ew, z):
+        Ew = np.ascontiguousarray(Ew, dtype=np.complex128)
+        nl = np.zeros_like(Ew)
+        st = self.L.orc_rhs_grad(holder.ref, spec.ref, _p(nl), _p(Ew), z)
+        return st, nl
+
+    def solve_grad(self, holder, spec, Ew0, opts, want_saves=True):
+        n = holder.desc.n_omega
+        Ew = np.ascontiguousarray(Ew0, dtype=np.complex128).copy()
+        saves = np.zeros((opts.save_n, n), dtype=np.complex128) if want_saves else None
+        zs = np.zeros(max(opts.save_n, 1))
+        stats = self.capi.LhStats()
+        st = self.L.orc_solve_grad(holder.ref, spec.ref, _p(Ew), C.byref(opts), _p(saves) if want_saves else None,
+                                   _p(zs), C.byref(stats))
         return st, Ew, saves, zs, stats
 
     def fwhm(self, x, y, minmax="min"):

@@ -1,0 +1,108 @@
+"""The oracle's gradient fill (Capillary.gradient + make_linop + make_prop!(linop!),
+src/Capillary.jl:306-340, src/LinearOps.jl:337-358, src/RK45.jl:283-296)
+pinned against independent host code and closed forms:
+
+  * p(z) of the two-point and multi-point profiles (closed forms, continuity)
+    and ρ(z) = density(p(z)) (spline nodes exact, elsewhere 1e-9);
+  * the operator and normalisation at z against the constant-fill operator of
+    the host mode model (modes.py, pinned to the reference's capillary
+    golden values) at the pressure p(z): a wrong sign, a missing clamp or a
+    transposed real/imaginary part fails;
+  * one RHS at z equals the constant-fill RHS at p(z);
+  * a gradient with p0 = p1 reproduces the constant-fill propagation;
+  * a linear propagation (no Kerr, no plasma: every k is zero, so every step
+    is accepted with dt growing by 1.5) equals E0 · Π exp(L(t_{k+1}) dt_k):
+    the operator is taken at the later time of each step, not the earlier.
+"""
+import math
+
+import numpy as np
+import pytest
+
+KW = dict(lambda0=800e-9, lambda_lims=(150e-9, 4e-6), trange=0.5e-12, tau_fwhm=10e-15, energy=150e-6,
+          shotnoise=False, ppt_N=2 ** 12)
+
+
+def setup(pressure, flength=0.2, **kw):
+    from luna_jl_amd import interface
+    args = dict(KW, **kw)
+    Ew, grid, linop, tr, _, out = interface.prop_capillary_args(125e-6, flength, "He", pressure, **args)
+    return Ew, grid, linop, tr, tr.descriptor(linop)
+
+
+def test_pressure_profiles(pkg):
+    from luna_jl_amd import capillary, physdata as pd
+    g = capillary.Gradient.linear("He", 1.0, 0.5, 2.0)
+    assert g.pressure(-0.1) == 0.5 and g.pressure(0.0) == 0.5 and g.pressure(1.5) == 2.0
+    assert g.pressure(0.36) == pytest.approx(math.sqrt(0.25 + 0.36 * (4 - 0.25)), rel=1e-15)
+    assert g.density(0.0) == pd.density("He", 0.5)      # spline nodes are exact
+    assert g.density(2.0) == pd.density("He", 2.0)
+    assert g.density(0.5) == pytest.approx(pd.density("He", g.pressure(0.5)), rel=1e-9)
+    m = capillary.Gradient("Ar", [0.0, 0.05, 0.1], [0.5, 2.0, 1.0])
+    assert m.pressure(0.05) == 2.0 and m.pressure(0.1) == 1.0 and m.pressure(0.2) == 1.0
+    assert m.pressure(0.025) == pytest.approx(math.sqrt(0.25 + 0.5 * (4 - 0.25)), rel=1e-15)
+    assert m.pressure(0.075) == pytest.approx(math.sqrt(4 + 0.5 * (1 - 4)), rel=1e-15)
+    assert abs(m.pressure(0.05 - 1e-12) - 2.0) < 1e-9 and abs(m.pressure(0.05 + 1e-12) - 2.0) < 1e-9
+
+
+def test_constants_match_constant_fill(orc, pkg):
+    from luna_jl_amd import linops, modes, nonlinearrhs, physdata as pd
+    Ew, grid, linop, tr, h = setup((0.5, 2.0), flength=1.0)
+    s = grid.sidx
+    for z in (0.0, 0.3, 0.77, 1.0):
+        Lz, nf, rho = orc.grad_constants(h, tr.gradient, z)
+        p = tr.gradient.gradient.pressure(z)
+        m = modes.MarcatiliMode(125e-6, "He", p)
+        Lc, beta, b1 = linops.make_const_linop(grid, m, 800e-9)
+        nfc = nonlinearrhs.norm_mode_average(grid, beta, m.Aeff())
+        assert rho == pytest.approx(pd.density("He", p), rel=1e-9)
+        # Im L = -(β - β1 ω) cancels to ~1e2 of ~1e7: compare norm-wise and in absolute phase
+        assert np.linalg.norm(Lz[s] - Lc[s]) / np.linalg.norm(Lc[s]) < 1e-7
+        assert np.max(np.abs(Lz[s] - Lc[s])) < 1e-5  # rad/m
+        assert np.all(Lz[~s] == 0)
+        assert np.max(np.abs(nf[s] - nfc[s]) / np.abs(nfc[s])) < 1e-12
+
+
+def test_rhs_matches_constant_fill(orc, pkg):
+    Ew, grid, linop, tr, h = setup((0.5, 2.0))
+    for z in (0.0, 0.07, 0.2):
+        _, _, lc, trc, hc = setup(tr.gradient.gradient.pressure(z))
+        st, nl = orc.rhs_grad(h, tr.gradient, Ew * 2.0, z)
+        st2, ref = orc.rhs(hc, Ew * 2.0, z)
+        assert st == st2 == 0
+        assert np.linalg.norm(nl - ref) / np.linalg.norm(ref) < 1e-12
+
+
+def test_flat_gradient_is_constant_fill(orc, capi, pkg):
+    Ewg, grid, lg, trg, hg = setup((1.3, 1.3), flength=0.05)
+    Ewc, _, lc, trc, hc = setup(1.3, flength=0.05)
+    opts = capi.rk_opts(0.05, save_n=3)
+    st, Eg, sv, zs, sg = orc.solve_grad(hg, trg.gradient, Ewg, opts)
+    st2, Ec, sv2, zs2, sc = orc.solve(hc, Ewc, opts)
+    assert st == st2 == 0 and sg.steps == sc.steps
+    s = grid.sidx
+    pa, pb = np.abs(Eg[s]) ** 2, np.abs(Ec[s]) ** 2
+    assert np.linalg.norm(pa - pb) / np.linalg.norm(pb) < 1e-8
+
+
+def test_linear_propagation_uses_later_time(orc, capi, pkg):
+    L = 0.2
+    Ew, grid, linop, tr, h = setup((0.5, 3.0), flength=L, plasma=False, kerr=False)
+    opts = capi.rk_opts(L, save_n=2)
+    st, Eend, sv, zs, stats = orc.solve_grad(h, tr.gradient, Ew, opts)
+    assert st == 0 and stats.rejected == 0
+    # the step sequence of a zero right-hand side: err = 0 → dt·1.5, capped at L/2
+    t, dt, steps = 0.0, opts.init_dz, []
+    while t <= L:
+        steps.append((t + dt, dt, t))
+        t, dt = t + dt, min(1.5 * dt, opts.max_dz)
+    assert len(steps) == stats.accepted
+    later = np.array(Ew, dtype=complex)
+    earlier = np.array(Ew, dtype=complex)
+    for tn, d, t0 in steps:
+        later = later * np.exp(orc.grad_constants(h, tr.gradient, tn)[0] * d) * grid.omega_win
+        earlier = earlier * np.exp(orc.grad_constants(h, tr.gradient, t0)[0] * d) * grid.omega_win
+    big = np.abs(Ew) > 1e-3 * np.abs(Ew).max()
+    err = np.linalg.norm((Eend - later)[big]) / np.linalg.norm(later[big])
+    assert err < 1e-9
+    assert np.linalg.norm((Eend - earlier)[big]) / np.linalg.norm(earlier[big]) > 1e-3

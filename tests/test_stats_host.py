@@ -14,6 +14,7 @@ def test_keys_follow_the_reference(small, capi):
     from luna_jl_amd import stats
     sf = stats.default(small.grid, small.Ew0, None, small.linop, small.transform, gas="He")
     rec = np.arange(2 * capi.LH_NSTAT, dtype=float).reshape(2, capi.LH_NSTAT)
+    rec[:, capi.STAT_KEYS.index("density")] = small.transform.density  # PhysData.pressure of this density
     d = sf.to_dict(rec)
     for k in ("ω0", "energy", "peakpower", "fwhm_t_min", "fwhm_t_max", "density", "pressure",
               "peakintensity", "electrondensity", "peak_ionisation_rate", "z", "dz"):
