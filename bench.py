@@ -37,7 +37,7 @@ def parse():
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--stats", default="on", choices=["on", "off"],
                    help="record Stats.default's per-step statistics (as prop_capillary does)")
-    p.add_argument("--workload", default="scan", choices=["scan", "radial", "surrogate", "copy"],
+    p.add_argument("--workload", default="scan", choices=["scan", "radial", "surrogate", "copy", "ppt"],
                    help="scan: batched capillary sweep (the reported metric); radial: the "
                         "Nr=256 x Nt=4096 radial propagation; surrogate: FNO training on solver "
                         "spectra and screening of the sweep; the last two print their own line")
@@ -221,6 +221,35 @@ def surrogate_bench(args):
     print(json.dumps(out))
 
 
+def ppt_tables(args):
+    """PPT rate tables (2^16 field values, makePPTcache) built by the GPU kernel
+    and by the host loop, for He (φ_0 only) and Ar (m = ±1 terms)."""
+    import __graft_entry__ as ge
+    ge.load_package()
+    from luna_jl_amd import ionisation
+    rows = {}
+    for gas in ("He", "Ar"):
+        tm = {}
+        t = time.perf_counter()
+        E, r_dev = ionisation.make_ppt_cache(gas, 800e-9, device=True)
+        dev_s = time.perf_counter() - t
+        ionisation.ppt_device_table(*[ionisation.ppt_options(gas)[k] for k in ("ionpot",)], 800e-9,
+                                    ionisation.ppt_options(gas)["Z"], ionisation.ppt_options(gas)["l"], E,
+                                    dalpha=ionisation.ppt_options(gas)["dalpha"],
+                                    alpha_ion=ionisation.ppt_options(gas)["alpha_ion"], timing=tm)
+        t = time.perf_counter()
+        E, r_host = ionisation.make_ppt_cache(gas, 800e-9)
+        host_s = time.perf_counter() - t
+        m = r_host > 1e-300
+        rows[gas] = {"gpu_call_s": dev_s, "gpu_kernel_ms": tm["kernel_ms"], "host_s": host_s,
+                     "max_rel_diff": float(np.max(np.abs(r_dev[m] - r_host[m]) / r_host[m]))}
+        log(f"bench ppt {gas}: {rows[gas]}")
+    print(json.dumps({"metric": "PPT table build time (2^16 points, 800 nm)", "value": rows["Ar"]["host_s"] /
+                      rows["Ar"]["gpu_call_s"], "unit": "x faster than the host loop (Ar)", "n_gpus": 1,
+                      "higher_is_better": True, "dtype": "f64", "config": {"workload": "ppt_table_2e16"},
+                      "tables": rows}))
+
+
 def copy_calibration(args):
     """A plain device-to-device copy of 1 GiB (the runtime's blit kernel), three
     times: the known byte count calibrates the FETCH_SIZE / WRITE_SIZE counters."""
@@ -257,6 +286,8 @@ def main():
         return surrogate_bench(args)
     if args.workload == "copy":
         return copy_calibration(args)
+    if args.workload == "ppt":
+        return ppt_tables(args)
     import __graft_entry__ as ge
     ge.load_package()
     from luna_jl_amd import _capi, configs, runtime

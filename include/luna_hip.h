@@ -194,6 +194,30 @@ typedef struct lh_gradient_desc {
  * constant fill again).  Host arrays are copied. */
 int lh_set_gradient(lh_ctx *ctx, const lh_gradient_desc *g);
 
+/* PPT ionisation-rate table (set-up of PlasmaCumtrapz): ionrate_fun_PPT
+ * (src/Ionisation.jl:317-390, φ :403-429) at n field values on the GPU, as
+ * makePPTcache tabulates it (src/Ionisation.jl:185-200) before makePPTaccel
+ * fits its log-rate spline.  One thread per field value. */
+typedef struct lh_ppt_desc {
+    double ionpot;            /* ionisation potential (J)                        */
+    double lambda0;           /* wavelength (m)                                   */
+    double Z;                 /* ion charge (PhysData.quantum_numbers)            */
+    int32_t l;                /* orbital angular momentum: 0 or 1                 */
+    int32_t msum;             /* 1: sum over m = -l..l (default); 0: m = 0 only   */
+    double dalpha;            /* polarisability difference Δα (Stark shift), or 0 */
+    double alpha_ion_au;      /* ion polarisability (a.u., dipole correction), or 0 */
+    double sum_tol;           /* converge_series rtol (1e-6)                      */
+    double occupancy;         /* 2                                                */
+    int32_t cycle_average;    /* 0                                                */
+    int32_t reserved;         /* 0                                                */
+    double cnl;               /* Cnl, or 0: 2^2ns/(ns Γ(ns+ls+1) Γ(ns-ls))        */
+    double c, au_energy, au_time, au_efield;  /* the constants PhysData uses      */
+} lh_ppt_desc;
+
+/* rate[i] = PPT rate at E[i] > 0 (synchronous, on the current device);
+ * ms (may be NULL) receives the kernel's duration. */
+int lh_ppt_table(const lh_ppt_desc *p, int n, const double *E, double *rate, double *ms);
+
 /* Per-step statistics of a mode-averaged propagation: the set that
  * Stats.default builds (src/Stats.jl:476-532) and collect_stats evaluates
  * (src/Stats.jl:454-468) from the analytic field (plan_analytic,

@@ -65,6 +65,15 @@ class LhGradientDesc(C.Structure):
                 ("nwg", _dp), ("pre", _dp), ("sqrt_aeff", C.c_double), ("fd_h", C.c_double), ("fd", _dp)]
 
 
+class LhPptDesc(C.Structure):
+    """``lh_ppt_desc`` of include/luna_hip.h."""
+    _fields_ = [("ionpot", C.c_double), ("lambda0", C.c_double), ("Z", C.c_double), ("l", C.c_int32),
+                ("msum", C.c_int32), ("dalpha", C.c_double), ("alpha_ion_au", C.c_double),
+                ("sum_tol", C.c_double), ("occupancy", C.c_double), ("cycle_average", C.c_int32),
+                ("reserved", C.c_int32), ("cnl", C.c_double), ("c", C.c_double), ("au_energy", C.c_double),
+                ("au_time", C.c_double), ("au_efield", C.c_double)]
+
+
 class LhStatsDesc(C.Structure):
     """``lh_stats_desc`` of include/luna_hip.h."""
     _fields_ = [("capacity", C.c_int32), ("reserved", C.c_int32), ("energy_w", C.POINTER(C.c_double)),
@@ -166,6 +175,7 @@ def lib():
     L.lh_last_launch_times.argtypes = [_dp, C.POINTER(C.c_int), C.c_int]
     L.lh_set_step_stats.argtypes = [C.c_void_p, C.POINTER(LhStatsDesc)]
     L.lh_set_gradient.argtypes = [C.c_void_p, C.POINTER(LhGradientDesc)]
+    L.lh_ppt_table.argtypes = [C.POINTER(LhPptDesc), C.c_int, _dp, _dp, _dp]
     L.lh_stats_eval.argtypes = [C.c_void_p, _dp, C.c_double, C.c_double, _dp]
     L.lh_get_step_stats.argtypes = [C.c_void_p, _dp, C.c_int, C.POINTER(C.c_int)]
     _fp = C.POINTER(C.c_float)
@@ -185,7 +195,7 @@ def lib():
 EXPORTED = ["lh_strerror", "lh_device_count", "lh_create", "lh_destroy", "lh_rhs",
             "lh_rhs_device", "lh_ctx_status", "lh_solve", "lh_batch_solve", "lh_set_state",
             "lh_batch_solve_resident", "lh_get_state", "lh_get_saves", "lh_last_launch_times",
-            "lh_set_step_stats", "lh_stats_eval", "lh_get_step_stats", "lh_set_gradient",
+            "lh_set_step_stats", "lh_stats_eval", "lh_get_step_stats", "lh_set_gradient", "lh_ppt_table",
             "lh_fno_create", "lh_fno_destroy", "lh_fno_param_count", "lh_fno_set_params",
             "lh_fno_get_params", "lh_fno_forward", "lh_fno_loss_grad", "lh_fno_set_data", "lh_fno_train"]
 

@@ -4,7 +4,7 @@ import subprocess
 import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-SRCS = [os.path.join(HERE, "csrc", "lh_kernels.hip"), os.path.join(HERE, "csrc", "lh_fno.hip")]
+SRCS = [os.path.join(HERE, "csrc", n) for n in ("lh_kernels.hip", "lh_fno.hip", "lh_ppt.hip")]
 OUT_DIR = os.path.join(HERE, "_build")
 OUT = os.path.join(OUT_DIR, "libluna_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

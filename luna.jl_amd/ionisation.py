@@ -219,15 +219,37 @@ def ionrate_ppt_vec(ionpot, lam0, Z, l, E, sum_tol=1e-6, dalpha=0.0, alpha_ion=0
     return ret / pd.au_time
 
 
-def make_ppt_cache(gas, lam0, N=2 ** 16, Emax=None, stark_shift=True, dipole_corr=True, **kw):
-    """(E, rate) table, ``makePPTcache`` ``Ionisation.jl:185-200``."""
+def ppt_device_table(ionpot, lam0, Z, l, E, sum_tol=1e-6, dalpha=0.0, alpha_ion=0.0, occupancy=2,
+                     Cnl=None, timing=None):
+    """The same rates computed on the GPU (``lh_ppt_table``: one thread per
+    field value, each running the converge_series loop itself)."""
+    import ctypes as C
+    from . import _capi
+    E = np.ascontiguousarray(E, dtype=float)
+    d = _capi.LhPptDesc(ionpot=float(ionpot), lambda0=float(lam0), Z=float(Z), l=int(l), msum=1,
+                        dalpha=float(dalpha or 0.0), alpha_ion_au=float(alpha_ion or 0.0) / pd.au_polarisability,
+                        sum_tol=float(sum_tol), occupancy=float(occupancy), cycle_average=0, reserved=0,
+                        cnl=float(Cnl) if Cnl is not None else 0.0, c=pd.c, au_energy=pd.au_energy,
+                        au_time=pd.au_time, au_efield=pd.au_Efield)
+    rate = np.empty_like(E)
+    ms = C.c_double(0.0)
+    _capi.check(_capi.lib().lh_ppt_table(C.byref(d), len(E), _capi._ptr(E), _capi._ptr(rate), C.byref(ms)),
+                "lh_ppt_table")
+    if timing is not None:
+        timing["kernel_ms"] = ms.value
+    return rate
+
+
+def make_ppt_cache(gas, lam0, N=2 ** 16, Emax=None, stark_shift=True, dipole_corr=True, device=False, **kw):
+    """(E, rate) table, ``makePPTcache`` ``Ionisation.jl:185-200``; ``device``:
+    computed by the GPU kernel instead of the host loop."""
     o = ppt_options(gas, stark_shift, dipole_corr)
     if Emax is None:
         Emax = 2 * barrier_suppression(o["ionpot"], o["Z"])
     Emin = Emax / 5000
     E = julia_range(Emin, Emax, N)
-    rate = ionrate_ppt_vec(o["ionpot"], lam0, o["Z"], o["l"], E, dalpha=o["dalpha"],
-                           alpha_ion=o["alpha_ion"], **kw)
+    f = ppt_device_table if device else ionrate_ppt_vec
+    rate = f(o["ionpot"], lam0, o["Z"], o["l"], E, dalpha=o["dalpha"], alpha_ion=o["alpha_ion"], **kw)
     return E, rate
 
 
@@ -259,7 +281,7 @@ class PPTSpline:
 _PPT_CACHE = {}
 
 
-def ppt_spline_cached(gas, lam0, N=2 ** 16, cachedir=None, **kw):
+def ppt_spline_cached(gas, lam0, N=2 ** 16, cachedir=None, device=False, **kw):
     """``ionrate_fun!_PPTcached`` (``Ionisation.jl:142-175``): table cached on disk (.npz)."""
     import hashlib
     import os
@@ -274,7 +296,7 @@ def ppt_spline_cached(gas, lam0, N=2 ** 16, cachedir=None, **kw):
         dat = np.load(path)
         E, rate = dat["E"], dat["rate"]
     else:
-        E, rate = make_ppt_cache(gas, lam0, N=N, **kw)
+        E, rate = make_ppt_cache(gas, lam0, N=N, device=device, **kw)
         try:
             os.makedirs(cachedir, exist_ok=True)
             tmp = path + f".{os.getpid()}.tmp.npz"
