@@ -90,11 +90,42 @@ __device__ __forceinline__ void dft8(double2* v) {
     v[3] = cadd(e3, w3); v[7] = csub(e3, w3);
 }
 
+// 16 points as 4 × 4: DFT-4 over n1 for each n2 (x[4 n1 + n2]), the twiddles
+// W16^(n2 k1), DFT-4 over n2 for each k1, then X[k1 + 4 k2] into natural order.
+template <bool INV>
+__device__ __forceinline__ void dft16(double2* v) {
+    const double c1 = 0.92387953251128675613, s1 = 0.38268343236508977173, h = 0.70710678118654752440;
+    const double sg = INV ? 1.0 : -1.0;  // W16^m = cos(πm/8) + sg·i sin(πm/8)
+#pragma unroll
+    for (int n2 = 0; n2 < 4; n2++) dft4<INV>(v[n2], v[n2 + 4], v[n2 + 8], v[n2 + 12]);
+    // v[n2 + 4 k1] holds Y[n2][k1]
+    v[5] = cmul(v[5], make_double2(c1, sg * s1));    // m = 1
+    v[9] = cmul(v[9], make_double2(h, sg * h));      // m = 2
+    v[13] = cmul(v[13], make_double2(s1, sg * c1));  // m = 3
+    v[6] = cmul(v[6], make_double2(h, sg * h));      // m = 2
+    v[10] = INV ? cmul_pi(v[10]) : cmul_mi(v[10]);   // m = 4
+    v[14] = cmul(v[14], make_double2(-h, sg * h));   // m = 6
+    v[7] = cmul(v[7], make_double2(s1, sg * c1));    // m = 3
+    v[11] = cmul(v[11], make_double2(-h, sg * h));   // m = 6
+    v[15] = cmul(v[15], make_double2(-c1, -sg * s1));  // m = 9
+#pragma unroll
+    for (int k1 = 0; k1 < 4; k1++) dft4<INV>(v[4 * k1], v[4 * k1 + 1], v[4 * k1 + 2], v[4 * k1 + 3]);
+    // v[4 k1 + k2] holds X[k1 + 4 k2]
+    double2 o[16];
+#pragma unroll
+    for (int k1 = 0; k1 < 4; k1++)
+#pragma unroll
+        for (int k2 = 0; k2 < 4; k2++) o[k1 + 4 * k2] = v[4 * k1 + k2];
+#pragma unroll
+    for (int k = 0; k < 16; k++) v[k] = o[k];
+}
+
 template <int R, bool INV>
 __device__ __forceinline__ void dft(double2* v) {
     if constexpr (R == 2) dft2<INV>(v);
     else if constexpr (R == 4) dft4v<INV>(v);
-    else dft8<INV>(v);
+    else if constexpr (R == 8) dft8<INV>(v);
+    else dft16<INV>(v);
 }
 
 // Thread index that the compiler must re-read at each use site.  Index and
@@ -127,7 +158,8 @@ __device__ __forceinline__ double2 tw_get(const TwLds& t, int m) { return cmul(t
 
 // ------------------------------------------------------------ Stockham FFT
 // One radix-R Stockham pass over M complex points in LDS (padded layout).
-// Active threads: TA = min(M/8, LH_T); each does M/(R*TA) butterflies.  The
+// Active threads: TA = min(M/8, LH_T); each does M/(R*TA) butterflies (one for a
+// radix-16 pass of 8192 points).  The
 // twiddle w = exp(∓2πi k/(NS·R)) comes from the LDS table; its powers w^r by
 // repeated multiplication (error ≤ R ulp, far below the 1e-12 parity budget).
 template <int M, int R, int NS, bool INV>
@@ -135,6 +167,7 @@ __device__ __forceinline__ void fft_pass(double2* buf, const TwLds& tw) {
     constexpr int TA = (M / 8 < LH_T) ? M / 8 : LH_T;
     constexpr int NB = M / R;
     constexpr int BPT = NB / TA;
+    static_assert(BPT >= 1 && NB % TA == 0, "FFT pass split");
     constexpr int TSTRIDE = LH_TWN / (NS * R);
     const int tid = fresh_tid();
     double2 v[BPT][R];
@@ -171,10 +204,18 @@ __device__ __forceinline__ void fft_pass(double2* buf, const TwLds& tw) {
     __syncthreads();
 }
 
+// Radix choice per pass: 8, except that transforms of 8192 points and more
+// take radix 16 after the first pass (8·16·16·4 for 8192: four LDS passes
+// instead of five, each still one butterfly per thread).
+template <int M, int NS>
+struct PassRadix {
+    static constexpr int R = (M >= 8192 && NS > 1 && M / NS >= 16) ? 16 : ((M / NS >= 8) ? 8 : (M / NS));
+};
+
 template <int M, int NS, bool INV>
 __device__ __forceinline__ void fft_passes(double2* buf, const TwLds& tw) {
     if constexpr (NS < M) {
-        constexpr int R = (M / NS >= 8) ? 8 : (M / NS);
+        constexpr int R = PassRadix<M, NS>::R;
         fft_pass<M, R, NS, INV>(buf, tw);
         fft_passes<M, NS * R, INV>(buf, tw);
     }
