@@ -113,13 +113,14 @@ __constant__ double c_iC[4][7] = {
 
 // LDS of one workgroup: the padded transform buffer sized for the largest
 // transform it runs (MO complex points + Nyquist), twiddles, scan scratch.
-// plist: a compacted list of the samples that need a PPT rate (the few
-// hundred samples under the pulse), so that every thread takes one or two of
-// them instead of the threads owning the pulse doing all of that work.  Used
-// up to Nto = 8192; the largest transform leaves no room for it in LDS.
+// plist: the rates of the samples that need one (the few hundred under the
+// pulse), so that every thread takes one or two of them instead of the
+// threads owning the pulse doing all of that work: a compacted list up to
+// Nto = 8192, a window [lo, hi] of the round for Nto = 16384, whose 156 KB of
+// buffers leave 7 KB (896 samples) for it.
 template <int MO>
 struct PList {
-    static constexpr int CAP = (MO >= 8192) ? 1 : 4096;
+    static constexpr int CAP = (MO >= 8192) ? 896 : 4096;
 };
 
 template <int MO>
@@ -257,6 +258,26 @@ __device__ __forceinline__ double block_last(double v, Smem& sm) {
     return r;
 }
 
+// Workgroup min of lo and max of hi (wsum: 2·LH_NW doubles of scratch).
+__device__ __forceinline__ void block_minmax(int& lo, int& hi, double* wsum) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        lo = min(lo, __shfl_xor(lo, off, 64));
+        hi = max(hi, __shfl_xor(hi, off, 64));
+    }
+    if (lane == 0) {
+        wsum[2 * wid] = (double)lo;
+        wsum[2 * wid + 1] = (double)hi;
+    }
+    __syncthreads();
+    for (int w = 0; w < LH_NW; w++) {
+        lo = min(lo, (int)wsum[2 * w]);
+        hi = max(hi, (int)wsum[2 * w + 1]);
+    }
+    __syncthreads();
+}
+
 template <class Smem>
 __device__ __forceinline__ double field_at(const Smem& sm, int g, double e_scale) {
     double2 v = sm.buf[pad(g >> 1)];
@@ -334,8 +355,37 @@ __device__ __forceinline__ void pointwise_field(const DInst& in, Smem& sm, doubl
                     for (int c = 0; c < QR; c += QC) ppt_rates<QC>(in, jf + c, a + c, bad);
                 }
             } else {
+                // The samples that need a rate (the few hundred under the
+                // pulse) lie in a window [lo, hi] of the round; when it fits the
+                // LDS list, all threads take its samples in turn (one or two
+                // each) instead of the ~16 threads that own them doing all of it.
+                int lo = 0x7fffffff, hi = -1;
 #pragma unroll
-                for (int c = 0; c < QR; c += QC) ppt_rates<QC>(in, jf + c, a + c, bad);
+                for (int q = 0; q < QR; q++)
+                    if (!(jf[q] < in.ppt_emin)) {
+                        lo = min(lo, g0 + q);
+                        hi = max(hi, g0 + q);
+                    }
+                block_minmax(lo, hi, sm.wsum);
+                if (hi < lo) {
+#pragma unroll
+                    for (int q = 0; q < QR; q++) a[q] = 0.0;
+                } else if (hi - lo < PCAP) {
+                    for (int g = lo + tid; g <= hi; g += LH_T) {
+                        double ae = fabs(field_at(sm, g, es)), rr = 0.0;
+                        if (!(ae < in.ppt_emin)) ppt_rates<1>(in, &ae, &rr, bad);
+                        sm.plist[g - lo] = rr;
+                    }
+                    __syncthreads();
+#pragma unroll
+                    for (int q = 0; q < QR; q++) {
+                        const int g = g0 + q;
+                        a[q] = (g >= lo && g <= hi) ? sm.plist[g - lo] : 0.0;
+                    }
+                } else {
+#pragma unroll
+                    for (int c = 0; c < QR; c += QC) ppt_rates<QC>(in, jf + c, a + c, bad);
+                }
             }
 #pragma unroll
             for (int q = 0; q < QR; q++) jf[q] = a[q];  // rate, kept for the loss term
