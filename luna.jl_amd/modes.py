@@ -18,11 +18,36 @@ from . import physdata as pd
 _FD8 = np.array([1 / 280, -4 / 105, 1 / 5, -4 / 5, 0.0, 4 / 5, -1 / 5, 4 / 105, -1 / 280])
 
 
+# 8th-order central second-derivative weights at offsets -4..4 (times 1/h²)
+_FD8_2 = np.array([-1 / 560, 8 / 315, -1 / 5, 8 / 5, -205 / 72, 8 / 5, -1 / 5, 8 / 315, -1 / 560])
+
+
 def derivative1(f, x, rel_h=1e-3):
     h = abs(x) * rel_h if x != 0 else rel_h
     pts = x + h * np.arange(-4, 5)
     vals = np.array([f(p) for p in pts], dtype=float)
     return float(np.dot(_FD8, vals) / h)
+
+
+def beta2(mode, omega, rel_h=1e-3):
+    """GVD β2 = d²β/dω² at ω (``Modes.dispersion(mode, 2, ω)``, ``src/Modes.jl:188-200``),
+    by the 8th-order central stencil."""
+    h = abs(omega) * rel_h
+    pts = omega + h * np.arange(-4, 5)
+    return float(np.dot(_FD8_2, mode.beta(pts)) / (h * h))
+
+
+def zdw(mode, lam_min=100e-9, lam_max=3000e-9):
+    """Zero-dispersion wavelength, ``Modes.zdw(m; λmin, λmax)`` (``src/Modes.jl:207-216``):
+    the root of β2(ω) in (2πc/λmax, 2πc/λmin), NaN when β2 does not change sign
+    there (the reference returns missing, which the statistics turn into NaN)."""
+    from scipy.optimize import brentq
+    lo, hi = 2 * math.pi * pd.c / lam_max, 2 * math.pi * pd.c / lam_min
+    flo, fhi = beta2(mode, lo), beta2(mode, hi)
+    if not (np.isfinite(flo) and np.isfinite(fhi)) or flo * fhi > 0:
+        return float("nan")
+    w0 = brentq(lambda w: beta2(mode, w), lo, hi, xtol=1e-12 * hi, rtol=1e-15, maxiter=200)
+    return 2 * math.pi * pd.c / w0
 
 
 class MarcatiliMode:

@@ -15,8 +15,9 @@ of ``_capi.LH_NSTAT`` values per accepted step:
   (``zdz!``, ``:409-412``).
 
 ``pressure`` (``:335-339``, PhysData.pressure of the density) is filled on the
-host from each record's density with the gas model.
-``zdw`` (``Modes.zdw``, a root search on the mode's dispersion) and the
+host from each record's density with the gas model, and ``zdw`` (``zdw_linop``,
+``:571-582``: ``Modes.zdw``, a root search on the mode's dispersion) from the
+mode: once for a constant fill, at each record's z for a gradient fill.  The
 ``windows``/``userfuns``/``onaxis`` options are not provided.
 """
 import math
@@ -56,7 +57,7 @@ class StatsFun:
     and the description the propagation kernel uses to record the same values
     at every accepted step (``desc()``)."""
 
-    def __init__(self, grid, transform, capacity=16384, gas=None):
+    def __init__(self, grid, transform, capacity=16384, gas=None, zdw=None):
         if getattr(transform, "kind", None) != _capi.LH_TRANS_MODEAVG:
             raise NotImplementedError("per-step statistics are provided for mode-averaged transforms")
         self.grid = grid
@@ -67,6 +68,7 @@ class StatsFun:
         self.aeff = float(transform.aeff)
         self.plasma = grid.kind == "real" and any(isinstance(r, PlasmaCumtrapz) for r in transform.responses)
         self.gas = gas
+        self.zdw = zdw  # None, a number, or a function of z
 
     def desc(self):
         d = _capi.LhStatsDesc()
@@ -90,6 +92,10 @@ class StatsFun:
         out = {k: records[:, _capi.STAT_KEYS.index(k)].copy() for k in self.keys()}
         if self.gas is not None:
             out["pressure"] = np.array([pd.pressure(self.gas, r) for r in out["density"]])
+        if callable(self.zdw):
+            out["zdw"] = np.array([self.zdw(z) for z in out["z"]])
+        elif self.zdw is not None:
+            out["zdw"] = np.full(len(records), float(self.zdw))
         return out
 
     def __call__(self, Ew, z, dz):
@@ -105,4 +111,14 @@ def default(grid, Ew, mode, linop, transform, *, windows=None, gas=None, onaxis=
     userfuns)``, ``src/Stats.jl:492-532``, for a mode-averaged transform."""
     if windows or onaxis or userfuns:
         raise NotImplementedError("windows, onaxis and userfuns statistics are not provided")
-    return StatsFun(grid, transform, capacity=capacity, gas=gas)
+    zdw = None
+    if mode is not None and hasattr(mode, "beta"):
+        from .modes import MarcatiliMode, zdw as mode_zdw
+        spec = getattr(transform, "gradient", None)
+        if spec is not None and isinstance(mode, MarcatiliMode):
+            g = spec.gradient  # the mode at the fill pressure of each z
+            zdw = lambda z: mode_zdw(MarcatiliMode(mode.a, g.gas, g.pressure(z), n=mode.n, m=mode.m,
+                                                   kind=mode.kind, T=g.T, model=mode.model, loss=mode.loss))
+        else:
+            zdw = mode_zdw(mode)
+    return StatsFun(grid, transform, capacity=capacity, gas=gas, zdw=zdw)

@@ -55,3 +55,35 @@ def test_energy_weights_envelope_is_rectangle_rule(pkg):
     dt = g.t[1] - g.t[0]
     # Parseval for the unnormalised DFT: Σ|X|²·δt/N = Σ|x|²·δt
     assert np.allclose(w, dt / len(g.t), rtol=1e-12)
+
+
+def test_zdw_reference_values(pkg):
+    """Modes.zdw of Marcatili modes against test/test_capillary.jl:101-104, 109-119
+    (the Ar case to 1e-5: the reference's 1e-7 rests on CoolProp's density)."""
+    from luna_jl_amd import modes
+    assert abs(1e9 * modes.zdw(modes.MarcatiliMode(125e-6, "HeB", 0.4, model="reduced")) - 379) < 1
+    assert abs(1e9 * modes.zdw(modes.MarcatiliMode(75e-6, "HeB", 5.9, model="reduced")) - 562) < 1
+    m = modes.MarcatiliMode(50e-6, "ArB", 2.0, model="reduced", T=294.0)
+    assert modes.zdw(m) == pytest.approx(7.289431065526978e-07, rel=1e-5)
+    from luna_jl_amd import physdata as pd
+    assert modes.beta2(m, pd.wlfreq(800e-9)) == pytest.approx(-1.68187063100098e-29, rel=1e-4)
+    # no zero crossing in the window → NaN, as the reference's missing
+    assert np.isnan(modes.zdw(m, lam_min=900e-9, lam_max=1000e-9))
+
+
+def test_stats_zdw_entries(small, pkg, capi):
+    from luna_jl_amd import interface, stats
+    Ew, grid, linop, tr, _, out = interface.prop_capillary_args(
+        125e-6, 0.2, "He", 1.0, lambda0=800e-9, lambda_lims=(150e-9, 4e-6), trange=0.5e-12,
+        tau_fwhm=10e-15, energy=100e-6, shotnoise=False, ppt_N=2 ** 12)
+    rec = np.zeros((2, capi.LH_NSTAT))
+    rec[:, capi.STAT_KEYS.index("density")] = tr.density
+    d = out.statsfun.to_dict(rec)
+    assert np.all(d["zdw"] == d["zdw"][0]) and 300e-9 < d["zdw"][0] < 700e-9
+    Ew, grid, linop, tr, _, out = interface.prop_capillary_args(
+        125e-6, 0.2, "He", (0.5, 3.0), lambda0=800e-9, lambda_lims=(150e-9, 4e-6), trange=0.5e-12,
+        tau_fwhm=10e-15, energy=100e-6, shotnoise=False, ppt_N=2 ** 12)
+    rec = np.zeros((2, capi.LH_NSTAT))
+    rec[:, 0] = [0.0, 0.2]
+    d = out.statsfun.to_dict(rec)
+    assert d["zdw"][1] > d["zdw"][0]  # more gas at the end: normal dispersion moves the ZDW to the red
