@@ -116,19 +116,19 @@ __constant__ double c_iC[4][7] = {
 // plist: the rates of the samples that need one (the few hundred under the
 // pulse), so that every thread takes one or two of them instead of the
 // threads owning the pulse doing all of that work: a compacted list up to
-// Nto = 8192, a window [lo, hi] of the round for Nto = 16384, whose 156 KB of
-// buffers leave 7 KB (896 samples) for it.
+// Nto = 8192, a window [lo, hi] of the round for Nto = 16384, whose 149 KB of
+// other buffers leave 11 KB (1408 samples) for it.
 template <int MO>
 struct PList {
-    static constexpr int CAP = (MO >= 8192) ? 896 : 4096;
+    static constexpr int CAP = (MO >= 8192) ? 1408 : 4096;
 };
 
 template <int MO>
 struct SmemT {
     double2 buf[padded_len(MO)];
     TwLds tw;
-    double edge[LH_T];
-    double wsum[5 * LH_NW];  // scans use [0, 2*LH_NW], block_last uses [4*LH_NW]
+    double wsum[5 * LH_NW];      // block_exscan, block_sum3, statistics reductions
+    double scan[2][3 * LH_NW];   // one-barrier scans and min/max, used in alternation
     double plist[PList<MO>::CAP];
     double zs[2];  // gradient fill: ρ(z), β1(z) from thread 0
     int flag;
@@ -220,46 +220,96 @@ __device__ __forceinline__ void ppt_rates(const DInst& in, const double* aE, dou
     }
 }
 
-// cumtrapz over the workgroup for one round of QR samples per thread
-// (global index g0+q).  carry: running integral before this round; prev0:
-// the integrand at the sample just before this round (thread 0 only).  On
-// return y[q] holds the running integral and carry the round's final value.
-template <int QR, class Smem>
-__device__ __forceinline__ void block_cumtrapz(double* y, double dx, int g0, Smem& sm, double& carry,
-                                               double prev0) {
+// Inclusive prefix sum of x over the 64 lanes of a wavefront.
+__device__ __forceinline__ double wave_incl_scan(double x, int lane) {
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const double y = __shfl_up(x, off, 64);
+        if (lane >= off) x += y;
+    }
+    return x;
+}
+
+// cumtrapz over the workgroup for one round of QR samples per thread (global
+// index g0+q), with a single barrier.  The trapezoid joining a thread's first
+// sample to its predecessor's last comes from a lane shuffle inside a
+// wavefront; for lane 0 of waves ≥ 1 it is added after the barrier from the
+// previous wave's last sample in LDS.  S: a scan slot of 3·LH_NW doubles (wave
+// totals, first and last samples of each wave); consecutive calls alternate
+// between the two slots, so a slot is rewritten only after a further barrier
+// has passed all of its readers.  carry: running integral before this round;
+// prev0: the integrand at the sample just before the round (thread 0 only).
+// On return y[q] holds the running integral and carry the round's final value;
+// returns the round's last integrand value.
+template <int QR>
+__device__ __forceinline__ double block_cumtrapz(double* y, double dx, int g0, double* S, double& carry,
+                                                 double prev0) {
     const int tid = fresh_tid();
-    sm.edge[tid] = y[QR - 1];
-    __syncthreads();
-    double prev = (tid > 0) ? sm.edge[tid - 1] : prev0;
+    const int lane = tid & 63, wid = tid >> 6;
+    const double first = y[0], lastv = y[QR - 1];
+    double prev = __shfl_up(lastv, 1, 64);
+    if (tid == 0) prev = prev0;
+    const bool joined = (g0 != 0) && !(lane == 0 && wid > 0);
     double acc = 0.0;
     double last = prev;
 #pragma unroll
     for (int q = 0; q < QR; q++) {
-        double cur = y[q];
-        double term = (g0 + q == 0) ? 0.0 : (0.5 * (last + cur)) * dx;
+        const double cur = y[q];
+        const double term = (q == 0) ? (joined ? (0.5 * (last + cur)) * dx : 0.0) : (0.5 * (last + cur)) * dx;
         last = cur;
         acc += term;
         y[q] = acc;
     }
-    double total;
-    double off = carry + block_exscan(acc, sm.wsum, &total);
+    const double x = wave_incl_scan(acc, lane);
+    double ex = __shfl_up(x, 1, 64);
+    if (lane == 0) ex = 0.0;
+    if (lane == 63) {
+        S[wid] = x;
+        S[2 * LH_NW + wid] = lastv;
+    }
+    if (lane == 0) S[LH_NW + wid] = first;
+    __syncthreads();
+    // lane w < LH_NW: wave w's joining trapezoid and total, scanned over those lanes
+    double Ww = 0.0, c = 0.0;
+    if (lane < LH_NW) {
+        Ww = S[lane];
+        c = (lane > 0) ? (0.5 * (S[2 * LH_NW + lane - 1] + S[LH_NW + lane])) * dx + Ww : Ww;
+    }
+#pragma unroll
+    for (int o = 1; o < LH_NW; o <<= 1) {
+        const double t = __shfl_up(c, o, 64);
+        if (lane >= o) c += t;
+    }
+    const double off = (carry + (__shfl(c, wid, 64) - __shfl(Ww, wid, 64))) + ex;
 #pragma unroll
     for (int q = 0; q < QR; q++) y[q] = off + y[q];
-    carry = carry + total;
+    carry = carry + __shfl(c, LH_NW - 1, 64);
+    return S[3 * LH_NW - 1];
 }
 
-// Last integrand value of a round (held by the last thread) → all threads.
-template <class Smem>
-__device__ __forceinline__ double block_last(double v, Smem& sm) {
-    if (threadIdx.x == LH_T - 1) sm.wsum[4 * LH_NW] = v;
+// Exclusive prefix sum of one value per thread with a single barrier (scan
+// slot S as for block_cumtrapz); also returns the workgroup total.
+__device__ __forceinline__ double block_exscan1(double v, double* S, double* total) {
+    const int tid = fresh_tid();
+    const int lane = tid & 63, wid = tid >> 6;
+    const double x = wave_incl_scan(v, lane);
+    double ex = __shfl_up(x, 1, 64);
+    if (lane == 0) ex = 0.0;
+    if (lane == 63) S[wid] = x;
     __syncthreads();
-    double r = sm.wsum[4 * LH_NW];
-    __syncthreads();
-    return r;
+    const double Ww = (lane < LH_NW) ? S[lane] : 0.0;
+    double c = Ww;
+#pragma unroll
+    for (int o = 1; o < LH_NW; o <<= 1) {
+        const double t = __shfl_up(c, o, 64);
+        if (lane >= o) c += t;
+    }
+    *total = __shfl(c, LH_NW - 1, 64);
+    return (__shfl(c, wid, 64) - __shfl(Ww, wid, 64)) + ex;
 }
 
-// Workgroup min of lo and max of hi (wsum: 2·LH_NW doubles of scratch).
-__device__ __forceinline__ void block_minmax(int& lo, int& hi, double* wsum) {
+// Workgroup min of lo and max of hi with a single barrier (scan slot S).
+__device__ __forceinline__ void block_minmax(int& lo, int& hi, double* S) {
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
@@ -267,15 +317,15 @@ __device__ __forceinline__ void block_minmax(int& lo, int& hi, double* wsum) {
         hi = max(hi, __shfl_xor(hi, off, 64));
     }
     if (lane == 0) {
-        wsum[2 * wid] = (double)lo;
-        wsum[2 * wid + 1] = (double)hi;
+        S[2 * wid] = (double)lo;
+        S[2 * wid + 1] = (double)hi;
     }
     __syncthreads();
+#pragma unroll
     for (int w = 0; w < LH_NW; w++) {
-        lo = min(lo, (int)wsum[2 * w]);
-        hi = max(hi, (int)wsum[2 * w + 1]);
+        lo = min(lo, (int)S[2 * w]);
+        hi = max(hi, (int)S[2 * w + 1]);
     }
-    __syncthreads();
 }
 
 template <class Smem>
@@ -302,6 +352,7 @@ __device__ __forceinline__ void pointwise_field(const DInst& in, Smem& sm, doubl
     double cF = 0.0, cJ = 0.0, cP = 0.0;           // carried integrals
     double lastRate = 0.0, lastPhi = 0.0, lastJ = 0.0;
     double Fprev = 0.0, fprev = pp1 - 1.0;         // exp(-0) = 1
+    int sl = 0;                                    // scan slot of the next workgroup scan
 #pragma unroll 1
     for (int r = 0; r < NR; r++) {
         const int g0 = r * (LH_T * QR) + QR * tid;
@@ -329,13 +380,14 @@ __device__ __forceinline__ void pointwise_field(const DInst& in, Smem& sm, doubl
                     }
                 }
                 double ntot;
-                const int off = (int)block_exscan((double)cnt, sm.wsum, &ntot);
+                const int off = (int)block_exscan1((double)cnt, sm.scan[sl], &ntot);
+                sl ^= 1;
                 const int total = (int)ntot;
                 if (total <= PCAP) {  // compacted: one or two samples per thread
-                    int o = off;
+                    // sample q of this thread sits at off + (set bits of mask below q)
 #pragma unroll
                     for (int q = 0; q < QR; q++)
-                        if ((mask >> q) & 1u) sm.plist[o++] = jf[q];
+                        if ((mask >> q) & 1u) sm.plist[off + __popc(mask & ((1u << q) - 1u))] = jf[q];
                     __syncthreads();
                     int unused = 0;
                     for (int i = tid; i < total; i += LH_T) {
@@ -344,12 +396,9 @@ __device__ __forceinline__ void pointwise_field(const DInst& in, Smem& sm, doubl
                         sm.plist[i] = r;
                     }
                     __syncthreads();
-                    o = off;
 #pragma unroll
-                    for (int q = 0; q < QR; q++) {
-                        a[q] = 0.0;
-                        if ((mask >> q) & 1u) a[q] = sm.plist[o++];
-                    }
+                    for (int q = 0; q < QR; q++)
+                        a[q] = ((mask >> q) & 1u) ? sm.plist[off + __popc(mask & ((1u << q) - 1u))] : 0.0;
                 } else {
 #pragma unroll
                     for (int c = 0; c < QR; c += QC) ppt_rates<QC>(in, jf + c, a + c, bad);
@@ -366,7 +415,8 @@ __device__ __forceinline__ void pointwise_field(const DInst& in, Smem& sm, doubl
                         lo = min(lo, g0 + q);
                         hi = max(hi, g0 + q);
                     }
-                block_minmax(lo, hi, sm.wsum);
+                block_minmax(lo, hi, sm.scan[sl]);
+                sl ^= 1;
                 if (hi < lo) {
 #pragma unroll
                     for (int q = 0; q < QR; q++) a[q] = 0.0;
@@ -390,9 +440,8 @@ __device__ __forceinline__ void pointwise_field(const DInst& in, Smem& sm, doubl
 #pragma unroll
             for (int q = 0; q < QR; q++) jf[q] = a[q];  // rate, kept for the loss term
             if (bad) sm.flag = bad;
-            double lr = block_last(jf[QR - 1], sm);
-            block_cumtrapz<QR>(a, dt, g0, sm, cF, lastRate);  // a = ∫rate
-            lastRate = lr;
+            lastRate = block_cumtrapz<QR>(a, dt, g0, sm.scan[sl], cF, lastRate);  // a = ∫rate
+            sl ^= 1;
 #pragma unroll
             for (int q = 0; q < QR; q++) {
                 double E = field_at(sm, g0 + q, es);
@@ -404,14 +453,12 @@ __device__ __forceinline__ void pointwise_field(const DInst& in, Smem& sm, doubl
                 a[q] = (f * er) * E;                                                     // phase
                 jf[q] = (rate != 0.0 && E != 0.0) ? ((ip * rate) * (1 - f)) / E : 0.0;  // loss current
             }
-            double lp = block_last(a[QR - 1], sm);
-            block_cumtrapz<QR>(a, dt, g0, sm, cJ, lastPhi);  // a = J without loss term
-            lastPhi = lp;
+            lastPhi = block_cumtrapz<QR>(a, dt, g0, sm.scan[sl], cJ, lastPhi);  // a = J without loss term
+            sl ^= 1;
 #pragma unroll
             for (int q = 0; q < QR; q++) a[q] = a[q] + jf[q];
-            double lj = block_last(a[QR - 1], sm);
-            block_cumtrapz<QR>(a, dt, g0, sm, cP, lastJ);  // a = P
-            lastJ = lj;
+            lastJ = block_cumtrapz<QR>(a, dt, g0, sm.scan[sl], cP, lastJ);  // a = P
+            sl ^= 1;
         }
 #pragma unroll
         for (int s2 = 0; s2 < QR / 2; s2++) {
