@@ -37,6 +37,23 @@ __device__ __forceinline__ double2 cexp_lh(double2 L, double h) {
     return make_double2(er * c, er * s);
 }
 
+// exp(L*h) as cexp_lh, and in `inv` exp(-L*h) from the same exp and sincos
+// (cos is even and sin odd, so only 1/exp(Re) stands in for exp(-Re): one
+// rounding apart).
+__device__ __forceinline__ double2 cexp_lh_inv(double2 L, double h, double2& inv) {
+    double zr = L.x * h, zi = L.y * h;
+    double er = exp(zr);
+    double ir = 1.0 / er;
+    if (zi == 0.0) {
+        inv = make_double2(ir, -zi);
+        return make_double2(er, zi);
+    }
+    double s, c;
+    sincos(zi, &s, &c);
+    inv = make_double2(ir * c, -(ir * s));
+    return make_double2(er * c, er * s);
+}
+
 // LDS padding: one spare double2 every 8 keeps radix-8 strided accesses
 // conflict-free for 16-byte lanes.
 __device__ __forceinline__ int pad(int i) { return i + (i >> 3); }

@@ -60,7 +60,8 @@ struct DInst {
     const double2* Yin;  // initial field (never written)
     double2* Y;          // receives the final state s.yn
     double2* YN;
-    double2* K[7];
+    double2* K[7];        // one block: K[j] = K[0] + j·kstride (slot 7: stage factor)
+    long long kstride;
     double2* saves;       // [save_n][n_state] or null
     const double* zgrid;  // [save_n]
     lh_stats* stats;
@@ -476,8 +477,44 @@ __device__ __forceinline__ void pointwise_field(const DInst& in, Smem& sm, doubl
     }
 }
 
+// Epilogues of a transform: called with v[i] = the RHS of bin
+// k = threadIdx.x + (i0 + i)·LH_T for a chunk of at most 9 of this thread's
+// bins (bins k ≥ n hold zero and are not written).
+struct EpiStore {  // out[k] = v
+    double2* out;
+    template <int NP>
+    __device__ __forceinline__ void operator()(const double2 (&v)[NP], int n, int i0) const {
+#pragma unroll
+        for (int i = 0; i < NP; i++) {
+            const int k = threadIdx.x + (i0 + i) * LH_T;
+            if (k < n) out[k] = v[i];
+        }
+    }
+};
+struct EpiStage {  // out[k] = v·F[k], F = exp(L·hb) stored by the stage prologue
+    double2* out;
+    const double2* F;
+    template <int NP>
+    __device__ __forceinline__ void operator()(const double2 (&v)[NP], int n, int i0) const {
+        static_assert(NP <= 9, "epilogue chunk");
+        // the chunk's factor loads are issued before its first store (out may
+        // alias F as far as the compiler knows, which would otherwise serialise them)
+        double2 f[NP];
+#pragma unroll
+        for (int i = 0; i < NP; i++) {
+            const int k = threadIdx.x + (i0 + i) * LH_T;
+            f[i] = (k < n) ? F[k] : make_double2(0.0, 0.0);
+        }
+#pragma unroll
+        for (int i = 0; i < NP; i++) {
+            const int k = threadIdx.x + (i0 + i) * LH_T;
+            if (k < n) out[k] = cmul(v[i], f[i]);
+        }
+    }
+};
+
 // Field-grid mode-averaged transform on LDS data.  On entry buf[0..N) holds
-// sc_t·Eω (zero to MO).  Calls epi(k, nl_k) for every output bin k < N.
+// sc_t·Eω (zero to MO).  Hands the output bins k < N to epi.
 template <int MO, class Smem, class Epi>
 __device__ __forceinline__ void rhs_field(const DInst& in, Smem& sm, Epi epi, double rho, double kfac) {
     __syncthreads();
@@ -494,33 +531,56 @@ __device__ __forceinline__ void rhs_field(const DInst& in, Smem& sm, Epi epi, do
     }
     __syncthreads();
     fft_lds<MO, false>(sm.buf, sm.tw);
+    static_assert(NPT <= 9, "one epilogue chunk");
+    double2 res[NPT];
 #pragma unroll
     for (int i = 0; i < NPT; i++) {
         int k = threadIdx.x + i * LH_T;
-        if (k < N) epi(k, cmul(r2c_bin<MO>(sm.buf, sm.tw, k), of[i]));
+        res[i] = (k < N) ? cmul(r2c_bin<MO>(sm.buf, sm.tw, k), of[i]) : make_double2(0.0, 0.0);
     }
+    epi(res, N, 0);
 }
+
+// The seven stage vectors k1..k7, addressed arithmetically: a loop over j
+// computes the address of k_j in registers (the index is uniform over the
+// workgroup) instead of reading a pointer array from private memory before
+// every vector's loads.
+struct KSet {
+    double2* base;     // the vectors lie in one block, `stride` elements apart; slot 7
+                       // holds the current stage's exp(L·(t − t2))
+    long long stride;
+    int s0, s6;        // slots of k1 and k7 (swapped by FSAL); k2..k6 keep slots 1..5
+    __device__ __forceinline__ double2* operator[](int j) const {
+        const int s = (j == 0) ? s0 : (j == 6) ? s6 : j;
+        return base + (long long)s * stride;
+    }
+    __device__ __forceinline__ void swap06() {  // FSAL: ks[1] .= ks[7] as a pointer swap
+        const int t = s0;
+        s0 = s6;
+        s6 = t;
+    }
+};
 
 // Stage prologue: buf[k] = sc_t · (y + Σ_j Δz B_ij k_j)·exp(L hf); k1 first
 // propagated by h0 (evaluate!, src/RK45.jl:209-223).  Loops run over the
 // k-vectors outside and this thread's bins inside, so each vector's loads
 // for all bins are in flight together.
-template <int MO, class Smem>
+template <int MO, class Smem, class KS>
 __device__ __forceinline__ void stage_prologue_cols(const double2* __restrict__ linop, double sc_t, Smem& sm,
-                                                    const double2* Y, double2* const* K, int stage, double dt,
-                                                    double h0, double hf);
+                                                    const double2* Y, const KS& K, int stage, double dt,
+                                                    double h0, double hf, double2* einv);
 
 template <int MO, class Smem>
-__device__ __forceinline__ void stage_prologue(const DInst& in, Smem& sm, const double2* Y, double2* const* K,
+__device__ __forceinline__ void stage_prologue(const DInst& in, Smem& sm, const double2* Y, const KSet& K,
                                                int stage, double dt, double h0, double hf) {
-    stage_prologue_cols<MO, Smem>(in.linop, in.sc_t, sm, Y, K, stage, dt, h0, hf);
+    stage_prologue_cols<MO, Smem, KSet>(in.linop, in.sc_t, sm, Y, K, stage, dt, h0, hf, K[7]);
 }
 
 // Same on explicit per-column pointers (one column of a radial state).
-template <int MO, class Smem>
+template <int MO, class Smem, class KS>
 __device__ __forceinline__ void stage_prologue_cols(const double2* __restrict__ linop, double sc_t, Smem& sm,
-                                                    const double2* Y, double2* const* K, int stage, double dt,
-                                                    double h0, double hf) {
+                                                    const double2* Y, const KS& K, int stage, double dt,
+                                                    double h0, double hf, double2* einv) {
     constexpr int N = Bins<MO>::N, NPT = Bins<MO>::NPT;
     const int tid = fresh_tid();
     double2 yb[NPT], L[NPT];
@@ -533,29 +593,52 @@ __device__ __forceinline__ void stage_prologue_cols(const double2* __restrict__ 
     }
     if (stage == 0 && h0 != 0.0) {
         double2* K0 = K[0];
+        constexpr int CH = (NPT > 9) ? 8 : NPT;  // a chunk's loads before its first store
+        static_assert(NPT % CH == 0, "prologue chunks");
 #pragma unroll
-        for (int i = 0; i < NPT; i++) {
-            int k = tid + i * LH_T;
-            if (k < N) K0[k] = cmul(K0[k], cexp_lh(L[i], h0));
+        for (int c = 0; c < NPT; c += CH) {
+            double2 kv[CH];
+#pragma unroll
+            for (int i = 0; i < CH; i++) {
+                int k = tid + (c + i) * LH_T;
+                kv[i] = (k < N) ? K0[k] : make_double2(0.0, 0.0);
+            }
+#pragma unroll
+            for (int i = 0; i < CH; i++) {
+                int k = tid + (c + i) * LH_T;
+                if (k < N) K0[k] = cmul(kv[i], cexp_lh(L[c + i], h0));
+            }
         }
     }
+    // two k-vectors per iteration: both vectors' loads are in flight together
+    // (the sum still runs over j in order)
 #pragma unroll 1
-    for (int j = 0; j <= stage; j++) {
-        const double cj = dt * c_B[stage][j];
-        if (cj == 0.0) continue;
-        const double2* Kj = K[j];
+    for (int j = 0; j <= stage; j += 2) {
+        const double c0 = dt * c_B[stage][j];
+        const double c1 = (j + 1 <= stage) ? dt * c_B[stage][j + 1] : 0.0;
+        const double2* K0j = K[j];
+        const double2* K1j = K[j + 1];
+        double2 a[NPT], b[NPT];
 #pragma unroll
         for (int i = 0; i < NPT; i++) {
             int k = tid + i * LH_T;
-            if (k < N) yb[i] = cadd(yb[i], cscal(cj, Kj[k]));
+            a[i] = (k < N && c0 != 0.0) ? K0j[k] : make_double2(0.0, 0.0);
+            b[i] = (k < N && c1 != 0.0) ? K1j[k] : make_double2(0.0, 0.0);
+        }
+#pragma unroll
+        for (int i = 0; i < NPT; i++) {
+            if (c0 != 0.0) yb[i] = cadd(yb[i], cscal(c0, a[i]));
+            if (c1 != 0.0) yb[i] = cadd(yb[i], cscal(c1, b[i]));
         }
     }
 #pragma unroll
     for (int i = 0; i < NPT; i++) {
         int k = tid + i * LH_T;
         if (k < N) {
-            double2 yp = (hf != 0.0) ? cmul(yb[i], cexp_lh(L[i], hf)) : yb[i];
+            double2 yp = yb[i], inv = make_double2(1.0, 0.0);
+            if (hf != 0.0) yp = cmul(yb[i], cexp_lh_inv(L[i], hf, inv));
             sm.buf[pad(k)] = cscal(sc_t, yp);
+            if (einv) einv[k] = inv;  // the stage epilogue's exp(L·(t − t2))
         }
     }
     for (int k = N + tid; k <= MO; k += LH_T) sm.buf[pad(k)] = make_double2(0.0, 0.0);
@@ -667,7 +750,7 @@ __device__ __forceinline__ int env_slot(int k) {
 }
 
 template <int NO, int NN, class Smem>
-__device__ __forceinline__ void stage_prologue_env(const DInst& in, Smem& sm, const double2* Y, double2* const* K,
+__device__ __forceinline__ void stage_prologue_env(const DInst& in, Smem& sm, const double2* Y, const KSet& K,
                                                    int stage, double dt, double h0, double hf) {
     constexpr int N = NN, NPT = EnvBins<NO, NN>::NPT;
     const int tid = fresh_tid();
@@ -681,10 +764,21 @@ __device__ __forceinline__ void stage_prologue_env(const DInst& in, Smem& sm, co
     }
     if (stage == 0 && h0 != 0.0) {
         double2* K0 = K[0];
+        constexpr int CH = (NPT > 9) ? 8 : NPT;  // a chunk's loads before its first store
+        static_assert(NPT % CH == 0, "prologue chunks");
 #pragma unroll
-        for (int i = 0; i < NPT; i++) {
-            int k = tid + i * LH_T;
-            if (k < N) K0[k] = cmul(K0[k], cexp_lh(L[i], h0));
+        for (int c = 0; c < NPT; c += CH) {
+            double2 kv[CH];
+#pragma unroll
+            for (int i = 0; i < CH; i++) {
+                int k = tid + (c + i) * LH_T;
+                kv[i] = (k < N) ? K0[k] : make_double2(0.0, 0.0);
+            }
+#pragma unroll
+            for (int i = 0; i < CH; i++) {
+                int k = tid + (c + i) * LH_T;
+                if (k < N) K0[k] = cmul(kv[i], cexp_lh(L[c + i], h0));
+            }
         }
     }
 #pragma unroll 1
@@ -703,8 +797,10 @@ __device__ __forceinline__ void stage_prologue_env(const DInst& in, Smem& sm, co
     for (int i = 0; i < NPT; i++) {
         int k = tid + i * LH_T;
         if (k < N) {
-            double2 yp = (hf != 0.0) ? cmul(yb[i], cexp_lh(L[i], hf)) : yb[i];
+            double2 yp = yb[i], inv = make_double2(1.0, 0.0);
+            if (hf != 0.0) yp = cmul(yb[i], cexp_lh_inv(L[i], hf, inv));
             sm.buf[pad(env_slot<NO, NN>(k))] = cscal(sc_t, yp);
+            K[7][k] = inv;  // the stage epilogue's exp(L·(t − t2))
         }
     }
     if constexpr (NO > NN)
@@ -820,10 +916,17 @@ __device__ __forceinline__ void rhs_env(const DInst& in, Smem& sm, Epi epi) {
     }
     __syncthreads();
     fft_lds<NO, false>(sm.buf, sm.tw);
+    constexpr int CH = (NPT > 9) ? 8 : NPT;
+    static_assert(NPT % CH == 0, "epilogue chunks");
 #pragma unroll
-    for (int i = 0; i < NPT; i++) {
-        int k = tid + i * LH_T;
-        if (k < N) epi(k, cmul(sm.buf[pad(env_slot<NO, NN>(k))], of[i]));
+    for (int c = 0; c < NPT; c += CH) {
+        double2 res[CH];
+#pragma unroll
+        for (int i = 0; i < CH; i++) {
+            int k = tid + (c + i) * LH_T;
+            res[i] = (k < N) ? cmul(sm.buf[pad(env_slot<NO, NN>(k))], of[c + i]) : make_double2(0.0, 0.0);
+        }
+        epi(res, N, c);
     }
 }
 
@@ -1203,7 +1306,7 @@ __device__ __forceinline__ void stats_time(const DInst& in, Smem& sm, double s0,
 }
 
 template <int KIND, int MO, int MC, class Smem>
-__device__ __forceinline__ void do_prologue(const DInst& in, Smem& sm, const double2* Y, double2* const* K, int stage,
+__device__ __forceinline__ void do_prologue(const DInst& in, Smem& sm, const double2* Y, const KSet& K, int stage,
                                             double dt, double h0, double hf) {
     if constexpr (KIND == 0)
         stage_prologue<MO, Smem>(in, sm, Y, K, stage, dt, h0, hf);
@@ -1231,9 +1334,7 @@ __device__ __forceinline__ void do_stepfun(const DInst& in, Smem& sm, double2* Y
 template <int KIND, int MO, int MC, class Smem>
 __device__ __forceinline__ void solve_instance(const DInst& in, Smem& sm) {
     constexpr int N = Sizes<KIND, MO, MC>::N, NPT = Sizes<KIND, MO, MC>::NPT;
-    double2* K[7];
-#pragma unroll
-    for (int j = 0; j < 7; j++) K[j] = in.K[j];
+    KSet K{in.K[0], in.kstride, 0, 6};
     double2* const Y0 = in.Y;
     double2* Y = in.Y;
     double2* YN = in.YN;
@@ -1253,8 +1354,7 @@ __device__ __forceinline__ void solve_instance(const DInst& in, Smem& sm) {
     // PreconStepper init: k1 = fbar(y0, t, t)
     do_prologue<KIND, MO, MC, Smem>(in, sm, Y, K, 0, 0.0, 0.0, 0.0);
     {
-        double2* k0 = K[0];
-        do_rhs<KIND, MO, MC, Smem>(in, sm, [&](int k, double2 nl) { k0[k] = nl; }, rho, kfac);
+        do_rhs<KIND, MO, MC, Smem>(in, sm, EpiStore{K[0]}, rho, kfac);
     }
     rhs_evals++;
     __syncthreads();
@@ -1286,10 +1386,7 @@ __device__ __forceinline__ void solve_instance(const DInst& in, Smem& sm) {
             double hf = t2 - t, hb = t - t2;
             if (zdep && st != 5) zg_refresh(in, sm, t2, rho, kfac);  // stages 5 and 6 share t + dt
             do_prologue<KIND, MO, MC, Smem>(in, sm, Y, K, st, dt, h0, hf);
-            double2* kout = K[st + 1];
-            const double2* Lp = in.linop;
-            do_rhs<KIND, MO, MC, Smem>(in, sm, [&](int k, double2 nl) { kout[k] = cmul(nl, cexp_lh(Lp[k], hb)); },
-                                       rho, kfac);
+            do_rhs<KIND, MO, MC, Smem>(in, sm, EpiStage{K[st + 1], K[7]}, rho, kfac);
             rhs_evals++;
             __syncthreads();
             if (sm.flag != LH_OK) status = sm.flag;
@@ -1358,9 +1455,7 @@ __device__ __forceinline__ void solve_instance(const DInst& in, Smem& sm) {
         }
         if (ok) {
             tn = t + dt;
-            double2* tmp = K[0];  // FSAL: ks[1] .= ks[7] (src/RK45.jl:184-186)
-            K[0] = K[6];
-            K[6] = tmp;
+            K.swap06();  // FSAL: ks[1] .= ks[7] (src/RK45.jl:184-186)
             // stepfun, then MemoryOutput's statistics of the filtered state
             // (src/Output.jl:65-67) from the same LDS pass
             double* rec = (in.steprec && accepted < in.steprec_cap) ? in.steprec + (size_t)accepted * LH_NSTAT
@@ -1379,25 +1474,42 @@ __device__ __forceinline__ void solve_instance(const DInst& in, Smem& sm) {
                         for (int k = threadIdx.x; k < N; k += LH_T) out[k] = YN[k];
                     } else {
                         double sg = (ti - t) / dt;
-                        double sp[4] = {sg, pow(sg, 2.0), pow(sg, 3.0), pow(sg, 4.0)};
-                        double b[7];
-#pragma unroll
-                        for (int j = 0; j < 7; j++) {
-                            double acc = sp[0] * c_iC[0][j];
-                            for (int p = 1; p < 4; p++) acc = acc + sp[p] * c_iC[p][j];
-                            b[j] = acc;
-                        }
+                        const double sp1 = sg, sp2 = pow(sg, 2.0), sp3 = pow(sg, 3.0), sp4 = pow(sg, 4.0);
                         double h = ti - t;
-                        for (int k = threadIdx.x; k < N; k += LH_T) {
-                            double2 yi = make_double2(0.0, 0.0);
+                        // this thread's bins in registers (chunks of at most 9),
+                        // one k-vector at a time, every load of a chunk before
+                        // its first store to out
+                        constexpr int CH = (NPT > 9) ? 8 : NPT;
+                        static_assert(NPT % CH == 0, "save chunks");
+#pragma unroll 1
+                        for (int c = 0; c < NPT; c += CH) {
+                            double2 yi[CH], yv[CH], Lv[CH];
 #pragma unroll
+                            for (int i = 0; i < CH; i++) yi[i] = make_double2(0.0, 0.0);
+#pragma unroll 1
                             for (int j = 0; j < 7; j++) {
                                 if (j == 1) continue;
                                 const double2* kj = (j == 6) ? K[0] : K[j];
-                                yi = cadd(yi, cscal(b[j], kj[k]));
+                                // weight b_j(σ) = Σ_p σ^p C[p][j] of the dense output
+                                const double bj =
+                                    ((sp1 * c_iC[0][j] + sp2 * c_iC[1][j]) + sp3 * c_iC[2][j]) + sp4 * c_iC[3][j];
+#pragma unroll
+                                for (int i = 0; i < CH; i++) {
+                                    const int k = threadIdx.x + (c + i) * LH_T;
+                                    if (k < N) yi[i] = cadd(yi[i], cscal(bj, kj[k]));
+                                }
                             }
-                            double2 o = cadd(Y[k], cscal(dt, yi));
-                            out[k] = cmul(o, cexp_lh(in.linop[k], h));
+#pragma unroll
+                            for (int i = 0; i < CH; i++) {
+                                const int k = threadIdx.x + (c + i) * LH_T;
+                                yv[i] = (k < N) ? Y[k] : make_double2(0.0, 0.0);
+                                Lv[i] = (k < N) ? in.linop[k] : make_double2(0.0, 0.0);
+                            }
+#pragma unroll
+                            for (int i = 0; i < CH; i++) {
+                                const int k = threadIdx.x + (c + i) * LH_T;
+                                if (k < N) out[k] = cmul(cadd(yv[i], cscal(dt, yi[i])), cexp_lh(Lv[i], h));
+                            }
                         }
                     }
                 }
@@ -1461,7 +1573,7 @@ __device__ __forceinline__ void rhs_once(const DInst& in, Smem& sm) {
         for (int k = MC / 2 + threadIdx.x; k < MO - MC / 2; k += LH_T) sm.buf[pad(k)] = make_double2(0, 0);
     }
     double2* out = in.rhs_out;
-    do_rhs<KIND, MO, MC, Smem>(in, sm, [&](int k, double2 nl) { out[k] = nl; }, rho, kfac);
+    do_rhs<KIND, MO, MC, Smem>(in, sm, EpiStore{out}, rho, kfac);
 }
 
 template <int KIND, int MO, int MC>
@@ -1584,7 +1696,8 @@ struct lh_ctx {
     int result_save_n;
     double2* Y;
     double2* YN;
-    double2* K[7];
+    double2* K[7];  // one allocation: K[j] = K[0] + j·kstride
+    long long kstride;
     double2* saves;
     int saves_cap;
     double* zgrid;
@@ -1873,7 +1986,12 @@ static int ensure_solver_buffers(lh_ctx* c, int save_n, bool want_saves) {
         HIPCK(hipMalloc(&c->Yin, sizeof(double2) * NS));
         HIPCK(hipMalloc(&c->Y, sizeof(double2) * NS));
         HIPCK(hipMalloc(&c->YN, sizeof(double2) * NS));
-        for (int j = 0; j < 7; j++) HIPCK(hipMalloc(&c->K[j], sizeof(double2) * NS));
+        // the seven stage vectors and the stage's exp(L·(t − t2)) (slot 7) in
+        // one block, each starting on a 256-byte boundary
+        const size_t NSp = (NS + 15) & ~(size_t)15;
+        HIPCK(hipMalloc(&c->K[0], sizeof(double2) * NSp * 8));
+        for (int j = 1; j < 7; j++) c->K[j] = c->K[0] + j * NSp;
+        c->kstride = (long long)NSp;
     }
     if (want_saves && save_n > c->saves_cap) {
         if (c->saves) HIPCK(hipFree(c->saves));
@@ -1915,6 +2033,7 @@ static int prepare_instance(lh_ctx* c, const lh_rk_opts* o, bool want_saves, DIn
     in->Y = c->Y;
     in->YN = c->YN;
     for (int j = 0; j < 7; j++) in->K[j] = c->K[j];
+    in->kstride = c->kstride;
     in->saves = want_saves ? c->saves : nullptr;
     in->zgrid = c->zgrid;
     in->stats = dstats_slot;
@@ -1997,6 +2116,7 @@ static int rad_solve(lh_ctx* c, const lh_rk_opts* o, bool want_saves, hipStream_
     a.Y[0] = c->Y;
     a.Y[1] = c->YN;
     for (int j = 0; j < 7; j++) a.K[j] = c->K[j];
+    a.kstride = c->kstride;
     a.saves = want_saves ? c->saves : nullptr;
     a.zgrid = c->zgrid;
     a.zmax = o->zmax;
@@ -2099,8 +2219,7 @@ int lh_destroy(lh_ctx* c) {
                     c->steprec, c->st_rec1, c->zL, c->zOF};
     for (void* p : ptrs)
         if (p) hipFree(p);
-    for (int j = 0; j < 7; j++)
-        if (c->K[j]) hipFree(c->K[j]);
+    if (c->K[0]) hipFree(c->K[0]);  // the block of all seven
     void* rptrs[] = {c->Mt, c->Ek, c->Er, c->partial, c->pw, c->rargs, c->rctl};
     for (void* p : rptrs)
         if (p) hipFree(p);

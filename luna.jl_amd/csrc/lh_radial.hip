@@ -28,6 +28,15 @@ struct RadCtl {
     int perm[8];
 };
 
+// Stage vector j of one column: slot perm[j] of the K block, at the column's offset.
+struct RadK {
+    double2* base;
+    long long stride;
+    const int* perm;
+    size_t off;
+    __device__ __forceinline__ double2* operator[](int j) const { return base + (long long)perm[j] * stride + off; }
+};
+
 struct RadArgs {
     int n_omega, n_t, n_to, n_r;
     double sc_t, gemm_kr, gemm_rk;
@@ -36,7 +45,8 @@ struct RadArgs {
     const double* Mt;       // [Nr][Nr] transposed QDHT matrix: Mt[k*Nr + n] = M[n][k]
     const DInst* pw;        // scalar parameters + windows + PPT table (e_scale = 1)
     double2* Y[2];
-    double2* K[7];
+    double2* K[7];          // one block: K[j] = K[0] + j·kstride
+    long long kstride;
     double* Ek;             // [Nr][Nto] (t fastest)
     double* Er;             // [Nr][Nto]
     double* partial;        // [Nr][3]
@@ -61,8 +71,7 @@ __global__ __launch_bounds__(LH_T) void rad_stage_a(const RadArgs* __restrict__ 
     tw_load(sm.tw, twg);
     const int c = blockIdx.x, N = a.n_omega;
     const size_t off = (size_t)N * c;
-    double2* Kc[7];
-    for (int j = 0; j < 7; j++) Kc[j] = a.K[ctl.perm[j]] + off;
+    const RadK Kc{a.K[0], a.kstride, ctl.perm, off};
     if (stage < 0) {  // plain RHS of the input column (PreconStepper init / single RHS)
         const double2* src = (a.rhs_in ? a.rhs_in : a.Y[ctl.ycur]) + off;
         for (int k = threadIdx.x; k < N; k += LH_T) sm.buf[pad(k)] = cscal(a.sc_t, src[k]);
@@ -71,8 +80,8 @@ __global__ __launch_bounds__(LH_T) void rad_stage_a(const RadArgs* __restrict__ 
         const double t = ctl.t, dt = ctl.dt;
         const double t2 = t + c_nodes[stage] * dt;
         const double h0 = (stage == 0) ? ctl.h0 : 0.0;
-        stage_prologue_cols<MO, SmemT<MO>>(a.linop + off, a.sc_t, sm, a.Y[ctl.ycur] + off, Kc, stage, dt, h0,
-                                           t2 - t);
+        stage_prologue_cols<MO, SmemT<MO>, RadK>(a.linop + off, a.sc_t, sm, a.Y[ctl.ycur] + off, Kc, stage, dt, h0,
+                                                 t2 - t, nullptr);
     }
     __syncthreads();
     c2r_prepack<MO>(sm.buf, sm.tw);
