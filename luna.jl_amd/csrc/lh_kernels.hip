@@ -60,7 +60,7 @@ struct DInst {
     const double2* Yin;  // initial field (never written)
     double2* Y;          // receives the final state s.yn
     double2* YN;
-    double2* K[7];        // one block: K[j] = K[0] + j·kstride (slot 7: stage factor)
+    double2* K[7];        // one block: K[j] = K[0] + j·kstride
     long long kstride;
     double2* saves;       // [save_n][n_state] or null
     const double* zgrid;  // [save_n]
@@ -491,15 +491,36 @@ struct EpiStore {  // out[k] = v
         }
     }
 };
-struct EpiStage {  // out[k] = v·F[k], F = exp(L·hb) stored by the stage prologue
+struct EpiStage {  // out[k] = v·exp(L_k·hb), the stage output of fbar
+    double2* out;
+    const double2* L;
+    double hb;
+    template <int NP>
+    __device__ __forceinline__ void operator()(const double2 (&v)[NP], int n, int i0) const {
+        static_assert(NP <= 9, "epilogue chunk");
+        // the chunk's operator loads are issued before its first store (out may
+        // alias L as far as the compiler knows, which would otherwise serialise them)
+        double2 Lk[NP];
+#pragma unroll
+        for (int i = 0; i < NP; i++) {
+            const int k = threadIdx.x + (i0 + i) * LH_T;
+            Lk[i] = (k < n) ? L[k] : make_double2(0.0, 0.0);
+        }
+#pragma unroll
+        for (int i = 0; i < NP; i++) {
+            const int k = threadIdx.x + (i0 + i) * LH_T;
+            if (k < n) out[k] = cmul(v[i], cexp_lh(Lk[i], hb));
+        }
+    }
+};
+
+struct EpiStageF {  // out[k] = v·F[k], F = exp(L·hb) stored by the envelope stage prologue
     double2* out;
     const double2* F;
     template <int NP>
     __device__ __forceinline__ void operator()(const double2 (&v)[NP], int n, int i0) const {
         static_assert(NP <= 9, "epilogue chunk");
-        // the chunk's factor loads are issued before its first store (out may
-        // alias F as far as the compiler knows, which would otherwise serialise them)
-        double2 f[NP];
+        double2 f[NP];  // loads before stores, as in EpiStage
 #pragma unroll
         for (int i = 0; i < NP; i++) {
             const int k = threadIdx.x + (i0 + i) * LH_T;
@@ -546,8 +567,8 @@ __device__ __forceinline__ void rhs_field(const DInst& in, Smem& sm, Epi epi, do
 // workgroup) instead of reading a pointer array from private memory before
 // every vector's loads.
 struct KSet {
-    double2* base;     // the vectors lie in one block, `stride` elements apart; slot 7
-                       // holds the current stage's exp(L·(t − t2))
+    double2* base;     // the vectors lie in one block, `stride` elements apart; slot 7:
+                       // the envelope stage's exp(L·(t − t2))
     long long stride;
     int s0, s6;        // slots of k1 and k7 (swapped by FSAL); k2..k6 keep slots 1..5
     __device__ __forceinline__ double2* operator[](int j) const {
@@ -568,19 +589,19 @@ struct KSet {
 template <int MO, class Smem, class KS>
 __device__ __forceinline__ void stage_prologue_cols(const double2* __restrict__ linop, double sc_t, Smem& sm,
                                                     const double2* Y, const KS& K, int stage, double dt,
-                                                    double h0, double hf, double2* einv);
+                                                    double h0, double hf);
 
 template <int MO, class Smem>
 __device__ __forceinline__ void stage_prologue(const DInst& in, Smem& sm, const double2* Y, const KSet& K,
                                                int stage, double dt, double h0, double hf) {
-    stage_prologue_cols<MO, Smem, KSet>(in.linop, in.sc_t, sm, Y, K, stage, dt, h0, hf, K[7]);
+    stage_prologue_cols<MO, Smem, KSet>(in.linop, in.sc_t, sm, Y, K, stage, dt, h0, hf);
 }
 
 // Same on explicit per-column pointers (one column of a radial state).
 template <int MO, class Smem, class KS>
 __device__ __forceinline__ void stage_prologue_cols(const double2* __restrict__ linop, double sc_t, Smem& sm,
                                                     const double2* Y, const KS& K, int stage, double dt,
-                                                    double h0, double hf, double2* einv) {
+                                                    double h0, double hf) {
     constexpr int N = Bins<MO>::N, NPT = Bins<MO>::NPT;
     const int tid = fresh_tid();
     double2 yb[NPT], L[NPT];
@@ -635,10 +656,8 @@ __device__ __forceinline__ void stage_prologue_cols(const double2* __restrict__ 
     for (int i = 0; i < NPT; i++) {
         int k = tid + i * LH_T;
         if (k < N) {
-            double2 yp = yb[i], inv = make_double2(1.0, 0.0);
-            if (hf != 0.0) yp = cmul(yb[i], cexp_lh_inv(L[i], hf, inv));
+            double2 yp = (hf != 0.0) ? cmul(yb[i], cexp_lh(L[i], hf)) : yb[i];
             sm.buf[pad(k)] = cscal(sc_t, yp);
-            if (einv) einv[k] = inv;  // the stage epilogue's exp(L·(t − t2))
         }
     }
     for (int k = N + tid; k <= MO; k += LH_T) sm.buf[pad(k)] = make_double2(0.0, 0.0);
@@ -797,10 +816,13 @@ __device__ __forceinline__ void stage_prologue_env(const DInst& in, Smem& sm, co
     for (int i = 0; i < NPT; i++) {
         int k = tid + i * LH_T;
         if (k < N) {
+            // exp(L·hf), and for the stage epilogue exp(L·(t − t2)) = exp(−L·hf)
+            // from the same exp and sincos into slot 7 (saves the envelope
+            // epilogue, with 16 bins per thread, its own transcendental pass)
             double2 yp = yb[i], inv = make_double2(1.0, 0.0);
             if (hf != 0.0) yp = cmul(yb[i], cexp_lh_inv(L[i], hf, inv));
             sm.buf[pad(env_slot<NO, NN>(k))] = cscal(sc_t, yp);
-            K[7][k] = inv;  // the stage epilogue's exp(L·(t − t2))
+            K[7][k] = inv;
         }
     }
     if constexpr (NO > NN)
@@ -1386,7 +1408,10 @@ __device__ __forceinline__ void solve_instance(const DInst& in, Smem& sm) {
             double hf = t2 - t, hb = t - t2;
             if (zdep && st != 5) zg_refresh(in, sm, t2, rho, kfac);  // stages 5 and 6 share t + dt
             do_prologue<KIND, MO, MC, Smem>(in, sm, Y, K, st, dt, h0, hf);
-            do_rhs<KIND, MO, MC, Smem>(in, sm, EpiStage{K[st + 1], K[7]}, rho, kfac);
+            if constexpr (KIND == 0)
+                do_rhs<KIND, MO, MC, Smem>(in, sm, EpiStage{K[st + 1], in.linop, hb}, rho, kfac);
+            else
+                do_rhs<KIND, MO, MC, Smem>(in, sm, EpiStageF{K[st + 1], K[7]}, rho, kfac);
             rhs_evals++;
             __syncthreads();
             if (sm.flag != LH_OK) status = sm.flag;
@@ -1986,8 +2011,8 @@ static int ensure_solver_buffers(lh_ctx* c, int save_n, bool want_saves) {
         HIPCK(hipMalloc(&c->Yin, sizeof(double2) * NS));
         HIPCK(hipMalloc(&c->Y, sizeof(double2) * NS));
         HIPCK(hipMalloc(&c->YN, sizeof(double2) * NS));
-        // the seven stage vectors and the stage's exp(L·(t − t2)) (slot 7) in
-        // one block, each starting on a 256-byte boundary
+        // the seven stage vectors in one block, each starting on a 256-byte
+        // boundary, and slot 7 for the envelope stage factor exp(L·(t − t2))
         const size_t NSp = (NS + 15) & ~(size_t)15;
         HIPCK(hipMalloc(&c->K[0], sizeof(double2) * NSp * 8));
         for (int j = 1; j < 7; j++) c->K[j] = c->K[0] + j * NSp;
