@@ -94,3 +94,23 @@ def test_fno_arg_checks(capi):
     h = ctypes.c_void_p()
     cfg = capi.LhFnoCfg(n_x=100, cin=4, channels=64, modes=32, layers=4, hidden=128, max_batch=4)
     assert L.lh_fno_create(ctypes.byref(h), ctypes.byref(cfg)) == capi.LH_E_ARG  # n_x not a multiple of 64
+
+
+def test_envelope_beyond_largest_class_rejected(capi, pkg):
+    """The soliton self-frequency-shift case of test/test_gnlse.jl:33-56
+    (trange 13 ps over 450-8000 nm) needs an Nt = 16384 envelope grid, above
+    the envelope kernels' largest class (Nt = 8192); lh_create refuses it with
+    LH_E_ARG before touching a device."""
+    import math
+    from luna_jl_amd import interface
+    gamma, beta2, tau0, fr = 0.1, -1e-26, 10e-15, 0.18
+    P0 = abs(beta2) / ((1 - fr) * gamma * tau0 ** 2)
+    flength = math.pi / 2 * tau0 ** 2 / abs(beta2) * 90
+    Ew, grid, linop, tr, _, _ = interface.prop_gnlse_args(
+        gamma, flength, [0.0, 0.0, beta2], lambda0=835e-9, tau_fwhm=1.763 * tau0, power=P0,
+        pulseshape="sech", lambda_lims=(450e-9, 8000e-9), trange=13e-12, raman=True, shock=False,
+        fr=fr, shotnoise=False, ramanmodel="sdo", tau1=12.2e-15, tau2=32e-15)
+    assert len(grid.t) == 16384
+    holder = tr.descriptor(linop)
+    h = ctypes.c_void_p()
+    assert capi.lib().lh_create(ctypes.byref(h), holder.ref) == capi.LH_E_ARG
