@@ -279,6 +279,35 @@ __device__ __forceinline__ void c2r_prepack(double2* buf, const TwLds& tw) {
     }
 }
 
+// c2r pre-pack of one bin when X is zero above M/2 (a field spectrum of
+// M/2 + 1 bins zero-padded to M): the pair (k, M − k) then depends on X[k]
+// alone, so the owner of bin k writes Z[k] and Z[M − k] itself, by the same
+// operations as c2r_prepack with X[M − k] = 0.
+template <int M>
+__device__ __forceinline__ void c2r_pack_bin(double2* buf, const TwLds& tw, int k, double2 Xk) {
+    constexpr int TS = LH_TWN / (2 * M);
+    const double2 Xm = make_double2(0.0, 0.0);
+    if (k == 0) {
+        Xk.y = 0.0;
+        double2 A = cadd(Xk, cconj(Xm));
+        double2 B = csub(Xk, cconj(Xm));
+        buf[pad(0)] = cadd(A, cmul_pi(B));
+    } else if (k == M / 2) {
+        double2 A = cadd(Xk, cconj(Xk));
+        double2 B = cmul(csub(Xk, cconj(Xk)), cconj(tw_get(tw, k * TS)));
+        buf[pad(k)] = cadd(A, cmul_pi(B));
+    } else {
+        const int m = M - k;
+        double2 wk = cconj(tw_get(tw, k * TS)), wm = cconj(tw_get(tw, m * TS));
+        double2 Ak = cadd(Xk, cconj(Xm));
+        double2 Bk = cmul(csub(Xk, cconj(Xm)), wk);
+        double2 Am = cadd(Xm, cconj(Xk));
+        double2 Bm = cmul(csub(Xm, cconj(Xk)), wm);
+        buf[pad(k)] = cadd(Ak, cmul_pi(Bk));
+        buf[pad(m)] = cadd(Am, cmul_pi(Bm));
+    }
+}
+
 // r2c post-step for bin k (0 <= k <= M): Z = FFT_M(z) in LDS →
 //   X[k] = E + W_{2M}^k O,  E = ½(Z[k] + conj(Z[M-k])),  O = -½ i (Z[k] - conj(Z[M-k])).
 template <int M>

@@ -535,12 +535,15 @@ struct EpiStageF {  // out[k] = v·F[k], F = exp(L·hb) stored by the envelope s
 };
 
 // Field-grid mode-averaged transform on LDS data.  On entry buf[0..N) holds
-// sc_t·Eω (zero to MO).  Hands the output bins k < N to epi.
-template <int MO, class Smem, class Epi>
+// sc_t·Eω (zero to MO), or (PACKED) its c2r pre-pack already.  Hands the
+// output bins k < N to epi.
+template <int MO, class Smem, class Epi, bool PACKED>
 __device__ __forceinline__ void rhs_field(const DInst& in, Smem& sm, Epi epi, double rho, double kfac) {
     __syncthreads();
-    c2r_prepack<MO>(sm.buf, sm.tw);
-    __syncthreads();
+    if constexpr (!PACKED) {
+        c2r_prepack<MO>(sm.buf, sm.tw);
+        __syncthreads();
+    }
     fft_lds<MO, true>(sm.buf, sm.tw);
     pointwise_field<MO, Smem>(in, sm, rho, kfac);
     constexpr int N = Bins<MO>::N, NPT = Bins<MO>::NPT;
@@ -586,7 +589,7 @@ struct KSet {
 // propagated by h0 (evaluate!, src/RK45.jl:209-223).  Loops run over the
 // k-vectors outside and this thread's bins inside, so each vector's loads
 // for all bins are in flight together.
-template <int MO, class Smem, class KS>
+template <int MO, class Smem, class KS, bool PACK>
 __device__ __forceinline__ void stage_prologue_cols(const double2* __restrict__ linop, double sc_t, Smem& sm,
                                                     const double2* Y, const KS& K, int stage, double dt,
                                                     double h0, double hf);
@@ -594,11 +597,11 @@ __device__ __forceinline__ void stage_prologue_cols(const double2* __restrict__ 
 template <int MO, class Smem>
 __device__ __forceinline__ void stage_prologue(const DInst& in, Smem& sm, const double2* Y, const KSet& K,
                                                int stage, double dt, double h0, double hf) {
-    stage_prologue_cols<MO, Smem, KSet>(in.linop, in.sc_t, sm, Y, K, stage, dt, h0, hf);
+    stage_prologue_cols<MO, Smem, KSet, true>(in.linop, in.sc_t, sm, Y, K, stage, dt, h0, hf);
 }
 
 // Same on explicit per-column pointers (one column of a radial state).
-template <int MO, class Smem, class KS>
+template <int MO, class Smem, class KS, bool PACK>
 __device__ __forceinline__ void stage_prologue_cols(const double2* __restrict__ linop, double sc_t, Smem& sm,
                                                     const double2* Y, const KS& K, int stage, double dt,
                                                     double h0, double hf) {
@@ -657,10 +660,14 @@ __device__ __forceinline__ void stage_prologue_cols(const double2* __restrict__ 
         int k = tid + i * LH_T;
         if (k < N) {
             double2 yp = (hf != 0.0) ? cmul(yb[i], cexp_lh(L[i], hf)) : yb[i];
-            sm.buf[pad(k)] = cscal(sc_t, yp);
+            if constexpr (PACK)  // the c2r pre-pack of the RHS, bin by bin (N − 1 = MO/2)
+                c2r_pack_bin<MO>(sm.buf, sm.tw, k, cscal(sc_t, yp));
+            else
+                sm.buf[pad(k)] = cscal(sc_t, yp);
         }
     }
-    for (int k = N + tid; k <= MO; k += LH_T) sm.buf[pad(k)] = make_double2(0.0, 0.0);
+    if constexpr (!PACK)
+        for (int k = N + tid; k <= MO; k += LH_T) sm.buf[pad(k)] = make_double2(0.0, 0.0);
 }
 
 // Statistics of the spectrum held in LDS (defined with the statistics below).
@@ -1336,10 +1343,12 @@ __device__ __forceinline__ void do_prologue(const DInst& in, Smem& sm, const dou
         stage_prologue_env<MO, MC, Smem>(in, sm, Y, K, stage, dt, h0, hf);
 }
 
-template <int KIND, int MO, int MC, class Smem, class Epi>
+// PACKED: the field-grid buffer already holds the c2r pre-pack (the stage
+// prologue writes it); the single-RHS entry fills plain bins instead.
+template <int KIND, int MO, int MC, class Smem, class Epi, bool PACKED = true>
 __device__ __forceinline__ void do_rhs(const DInst& in, Smem& sm, Epi epi, double rho, double kfac) {
     if constexpr (KIND == 0)
-        rhs_field<MO, Smem>(in, sm, epi, rho, kfac);
+        rhs_field<MO, Smem, Epi, PACKED>(in, sm, epi, rho, kfac);
     else
         rhs_env<MO, MC, Smem>(in, sm, epi);
 }
@@ -1598,7 +1607,7 @@ __device__ __forceinline__ void rhs_once(const DInst& in, Smem& sm) {
         for (int k = MC / 2 + threadIdx.x; k < MO - MC / 2; k += LH_T) sm.buf[pad(k)] = make_double2(0, 0);
     }
     double2* out = in.rhs_out;
-    do_rhs<KIND, MO, MC, Smem>(in, sm, EpiStore{out}, rho, kfac);
+    do_rhs<KIND, MO, MC, Smem, EpiStore, false>(in, sm, EpiStore{out}, rho, kfac);
 }
 
 template <int KIND, int MO, int MC>

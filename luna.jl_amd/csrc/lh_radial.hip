@@ -80,7 +80,7 @@ __global__ __launch_bounds__(LH_T) void rad_stage_a(const RadArgs* __restrict__ 
         const double t = ctl.t, dt = ctl.dt;
         const double t2 = t + c_nodes[stage] * dt;
         const double h0 = (stage == 0) ? ctl.h0 : 0.0;
-        stage_prologue_cols<MO, SmemT<MO>, RadK>(a.linop + off, a.sc_t, sm, a.Y[ctl.ycur] + off, Kc, stage, dt, h0,
+        stage_prologue_cols<MO, SmemT<MO>, RadK, false>(a.linop + off, a.sc_t, sm, a.Y[ctl.ycur] + off, Kc, stage, dt, h0,
                                                  t2 - t);
     }
     __syncthreads();
