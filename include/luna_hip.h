@@ -236,19 +236,26 @@ enum {
     LH_STAT_ELECTRONDENSITY = 8,   /* ρ·(1 - exp(-∫rate dt)), :219-241; NaN without plasma */
     LH_STAT_PEAK_IONISATION_RATE = 9, /* max rate, same function; NaN without plasma */
     LH_STAT_DENSITY = 10,          /* ρ (density(dfun), :325-329)               */
-    LH_NSTAT = 11
+    LH_STAT_WINDOW0 = 11,          /* energy in window w = 0..3 (Stats.default's */
+    LH_STAT_WINDOW1 = 12,          /* `windows`: energy_λ/energy_window,         */
+    LH_STAT_WINDOW2 = 13,          /* :52-79): Σ_k window_w[w][k]·|Eω_k|²;       */
+    LH_STAT_WINDOW3 = 14,          /* NaN where no window is set                 */
+    LH_NSTAT = 15
 };
+#define LH_STAT_MAX_WINDOWS 4
 
 /* What the statistics need beyond the transform's own constants. */
 typedef struct lh_stats_desc {
     int32_t capacity;         /* records kept per run: the first `capacity`
                                  accepted steps                                   */
-    int32_t reserved;         /* 0                                               */
+    int32_t n_windows;        /* energy windows, 0..LH_STAT_MAX_WINDOWS          */
     const double *energy_w;   /* [n_omega] energy = Σ_k energy_w[k]·|Eω_k|²: the
                                  quadrature of Fields.energyfuncs' energy_ω
                                  (src/Fields.jl:510-534) with its prefactor      */
     const double *t;          /* [n_t] grid.t, the coarse time axis (fwhm_t)     */
     double aeff;              /* effective area Aeff (constant in z)             */
+    const double *window_w;   /* [n_windows][n_omega]: energy_w·window² of each
+                                 window (energyfun_ω(Eω .* window)); NULL if none */
 } lh_stats_desc;
 
 /* Switch per-step statistics on for later solves of ctx (sd = NULL: off).

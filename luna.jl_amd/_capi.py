@@ -15,8 +15,9 @@ LH_TRANS_MODEAVG, LH_TRANS_RADIAL = 0, 1
 LH_KERR_NONE, LH_KERR_FIELD, LH_KERR_ENV = 0, 1, 2
 # per-step statistics record: field order of LH_STAT_* (keys as Stats.jl names them)
 STAT_KEYS = ("z", "dz", "ω0", "energy", "peakpower", "peakintensity", "fwhm_t_min", "fwhm_t_max",
-             "electrondensity", "peak_ionisation_rate", "density")
+             "electrondensity", "peak_ionisation_rate", "density", "window0", "window1", "window2", "window3")
 LH_NSTAT = len(STAT_KEYS)
+LH_STAT_MAX_WINDOWS = 4
 
 _dp = C.POINTER(C.c_double)
 _u8p = C.POINTER(C.c_uint8)
@@ -76,8 +77,8 @@ class LhPptDesc(C.Structure):
 
 class LhStatsDesc(C.Structure):
     """``lh_stats_desc`` of include/luna_hip.h."""
-    _fields_ = [("capacity", C.c_int32), ("reserved", C.c_int32), ("energy_w", C.POINTER(C.c_double)),
-                ("t", C.POINTER(C.c_double)), ("aeff", C.c_double)]
+    _fields_ = [("capacity", C.c_int32), ("n_windows", C.c_int32), ("energy_w", C.POINTER(C.c_double)),
+                ("t", C.POINTER(C.c_double)), ("aeff", C.c_double), ("window_w", C.POINTER(C.c_double))]
 
 
 class LhFnoCfg(C.Structure):

@@ -356,6 +356,38 @@ __device__ __forceinline__ double block_exscan(double v, double* wsum, double* t
     return r;
 }
 
+// Sum of four values over the workgroup (fixed order → deterministic);
+// scratch: 4·LH_NW doubles.  Ends synchronised.
+__device__ __forceinline__ void block_sum4(double& a, double& b, double& c, double& d, double* scratch) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        a += __shfl_xor(a, off, 64);
+        b += __shfl_xor(b, off, 64);
+        c += __shfl_xor(c, off, 64);
+        d += __shfl_xor(d, off, 64);
+    }
+    if (lane == 0) {
+        scratch[4 * wid + 0] = a;
+        scratch[4 * wid + 1] = b;
+        scratch[4 * wid + 2] = c;
+        scratch[4 * wid + 3] = d;
+    }
+    __syncthreads();
+    double sa = 0, sb = 0, sc = 0, sd = 0;
+    for (int w = 0; w < LH_NW; w++) {
+        sa += scratch[4 * w + 0];
+        sb += scratch[4 * w + 1];
+        sc += scratch[4 * w + 2];
+        sd += scratch[4 * w + 3];
+    }
+    __syncthreads();
+    a = sa;
+    b = sb;
+    c = sc;
+    d = sd;
+}
+
 // Sum of three values over the workgroup (fixed tree → deterministic).
 __device__ __forceinline__ void block_sum3(double& a, double& b, double& c, double* scratch) {
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;

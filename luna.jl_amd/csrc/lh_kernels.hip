@@ -76,6 +76,8 @@ struct DInst {
     const double* st_t;      // [Nt] grid.t
     double st_aeff;          // Aeff
     double st_ediv;          // sqrt(ε0·c·Aeff/2)
+    const double* st_win;    // [st_nwin][Nω] energy_w·window² of each energy window
+    int st_nwin;
     double* steprec;         // [steprec_cap][LH_NSTAT]
     int steprec_cap;
     // z-dependent fill (pressure gradient); zg_np = 0: constant fill
@@ -671,9 +673,20 @@ __device__ __forceinline__ void stage_prologue_cols(const double2* __restrict__ 
 }
 
 // Statistics of the spectrum held in LDS (defined with the statistics below).
+// Partial energies Σ_k window_w[w][k]·|Eω_k|² of a thread's bins for the
+// energy windows (energy_window: energyfun_ω(Eω .* window), src/Stats.jl:69-79).
+struct WinSums {
+    double s[LH_STAT_MAX_WINDOWS];
+};
+__device__ __forceinline__ void win_add(const DInst& in, WinSums& w, int k, double a) {
+#pragma unroll
+    for (int q = 0; q < LH_STAT_MAX_WINDOWS; q++)
+        if (q < in.st_nwin) w.s[q] += in.st_win[(size_t)q * in.n_omega + k] * a;
+}
+
 template <int KIND, int NT, class Smem>
-__device__ __forceinline__ void stats_time(const DInst& in, Smem& sm, double s0, double s1, double se, double z,
-                                           double dz, double rho, double* rec);
+__device__ __forceinline__ void stats_time(const DInst& in, Smem& sm, double s0, double s1, double se,
+                                           const WinSums& ws, double z, double dz, double rho, double* rec);
 
 // Accepted-step window filter (stepfun, src/Luna.jl:366-370) on the coarse
 // grid: YN ← rfft(twin · irfft(ωwin · YN·exp(L h))).  With STATS and a
@@ -737,6 +750,7 @@ __device__ __forceinline__ void stepfun_filter_cols(const DInst& in, const doubl
             }
             __syncthreads();  // every bin read before the buffer is reused
             double s0 = 0.0, s1 = 0.0, se = 0.0;
+            WinSums ws = {{0.0, 0.0, 0.0, 0.0}};
 #pragma unroll
             for (int i = 0; i < NPT; i++) {
                 int k = threadIdx.x + i * LH_T;
@@ -744,10 +758,11 @@ __device__ __forceinline__ void stepfun_filter_cols(const DInst& in, const doubl
                 s0 += a;
                 s1 += wv[i] * a;
                 se += ev[i] * a;
+                if (k < N) win_add(in, ws, k, a);
                 if (k < N - 1) sm.buf[pad(k)] = make_double2(2.0 * yv[i].x, 2.0 * yv[i].y);
             }
             for (int k = N - 1 + threadIdx.x; k < 2 * MC; k += LH_T) sm.buf[pad(k)] = make_double2(0.0, 0.0);
-            stats_time<0, 2 * MC, Smem>(in, sm, s0, s1, se, z, dz, rho, rec);
+            stats_time<0, 2 * MC, Smem>(in, sm, s0, s1, se, ws, z, dz, rho, rec);
             return;
         }
     }
@@ -985,6 +1000,7 @@ __device__ __forceinline__ void stepfun_env(const DInst& in, Smem& sm, double2* 
     __syncthreads();
     fft_lds<NT, false>(sm.buf, sm.tw);
     double s0 = 0.0, s1 = 0.0, se = 0.0;
+    WinSums ws = {{0.0, 0.0, 0.0, 0.0}};
 #pragma unroll
     for (int i = 0; i < NPT; i++) {
         int k = threadIdx.x + i * LH_T;
@@ -996,11 +1012,12 @@ __device__ __forceinline__ void stepfun_env(const DInst& in, Smem& sm, double2* 
                 s0 += a;
                 s1 += in.st_omega[k] * a;
                 se += in.st_ew[k] * a;
+                win_add(in, ws, k, a);
             }
         }
     }
     if (rec)
-        stats_time<1, NT, Smem>(in, sm, s0, s1, se, z, dz, rho, rec);
+        stats_time<1, NT, Smem>(in, sm, s0, s1, se, ws, z, dz, rho, rec);
     else
         __syncthreads();
 }
@@ -1158,6 +1175,7 @@ __device__ __forceinline__ void step_stats(const DInst& in, Smem& sm, const doub
     constexpr int NT = MO;  // analytic-signal samples = length of grid.t
     const int tid = fresh_tid();
     double s0 = 0.0, s1 = 0.0, se = 0.0;
+    WinSums ws = {{0.0, 0.0, 0.0, 0.0}};
     for (int k = tid; k < NT; k += LH_T) {
         double2 v = (k < N) ? Y[k] : make_double2(0.0, 0.0);
         if (k < N) {
@@ -1165,23 +1183,41 @@ __device__ __forceinline__ void step_stats(const DInst& in, Smem& sm, const doub
             s0 += a;
             s1 += in.st_omega[k] * a;
             se += in.st_ew[k] * a;
+            win_add(in, ws, k, a);
         }
         if constexpr (KIND == 0)  // copyto_fft!: bins 0..Nω-2 doubled, the +fs/2 bin dropped
             sm.buf[pad(k)] = (k < N - 1) ? make_double2(2.0 * v.x, 2.0 * v.y) : make_double2(0.0, 0.0);
         else
             sm.buf[pad(k)] = v;
     }
-    stats_time<KIND, NT, Smem>(in, sm, s0, s1, se, z, dz, rho, rec);
+    stats_time<KIND, NT, Smem>(in, sm, s0, s1, se, ws, z, dz, rho, rec);
 }
 
 // The analytic spectrum is in LDS (not yet synchronised) and each thread holds
 // its partial sums Σ|E|², Σω|E|², Σw|E|²: the record of the state.
 template <int KIND, int NT, class Smem>
-__device__ __forceinline__ void stats_time(const DInst& in, Smem& sm, double s0, double s1, double se, double z,
-                                           double dz, double rho, double* rec) {
+__device__ __forceinline__ void stats_time(const DInst& in, Smem& sm, double s0, double s1, double se,
+                                           const WinSums& ws, double z, double dz, double rho, double* rec) {
     const int tid = fresh_tid();
     const int lane = tid & 63, wid = tid >> 6;
     block_sum3(s0, s1, se, sm.wsum);  // ends synchronised: the spectrum is complete
+    {
+        // energy windows: reduced and written here, so the sums do not stay live
+        const double nan = __longlong_as_double(0x7ff8000000000000LL);
+        double e[LH_STAT_MAX_WINDOWS] = {nan, nan, nan, nan};
+        if (in.st_nwin > 0) {
+            double w0 = ws.s[0], w1 = ws.s[1], w2 = ws.s[2], w3 = ws.s[3];
+            block_sum4(w0, w1, w2, w3, sm.wsum);
+            e[0] = w0;
+            e[1] = (in.st_nwin > 1) ? w1 : nan;
+            e[2] = (in.st_nwin > 2) ? w2 : nan;
+            e[3] = (in.st_nwin > 3) ? w3 : nan;
+        }
+        if (tid == 0) {
+#pragma unroll
+            for (int q = 0; q < LH_STAT_MAX_WINDOWS; q++) rec[LH_STAT_WINDOW0 + q] = e[q];
+        }
+    }
     fft_lds<NT, true>(sm.buf, sm.tw);
     // pass 1: P = |Et|² (kept in .x), rate (kept in .y); argmax, max/sum of
     // rate.  Sample j = tid + m·LH_T; rates in batches of QC (spline-table
@@ -1757,6 +1793,8 @@ struct lh_ctx {
     double* omega;       // [Nω] ω (shared)
     double* st_ew;       // [Nω] energy weights (shared)
     double* st_t;        // [Nt] grid.t (shared)
+    double* st_win;      // [st_nwin][Nω] window weights (shared)
+    int st_nwin;
     double st_aeff, st_ediv;
     int st_on, st_cap;
     double* steprec;     // [st_cap][LH_NSTAT]
@@ -1985,6 +2023,8 @@ static DInst base_inst(const lh_ctx* c) {
     in.st_t = c->st_t;
     in.st_aeff = c->st_aeff;
     in.st_ediv = c->st_ediv;
+    in.st_win = c->st_win;
+    in.st_nwin = c->st_nwin;
     in.steprec = c->st_on ? c->steprec : nullptr;
     in.steprec_cap = c->st_on ? c->st_cap : 0;
     in.sc_f = scale_to_freq(d);
@@ -2244,7 +2284,8 @@ int lh_destroy(lh_ctx* c) {
     hipSetDevice(c->device);
     if (c->stream) hipStreamSynchronize(c->stream);
     for (const void* p : {(const void*)c->omega_win, (const void*)c->t_win, (const void*)c->to_win, (const void*)c->ppt,
-                          (const void*)c->omega, (const void*)c->st_ew, (const void*)c->st_t, (const void*)c->zg_Z,
+                          (const void*)c->omega, (const void*)c->st_ew, (const void*)c->st_t, (const void*)c->st_win,
+                          (const void*)c->zg_Z,
                           (const void*)c->zg_P, (const void*)c->zfd, (const void*)c->zgam, (const void*)c->zd,
                           (const void*)c->znwg, (const void*)c->zpre, (const void*)c->zsidx})
         shared_release(p);
@@ -2556,10 +2597,15 @@ int lh_set_step_stats(lh_ctx* c, const lh_stats_desc* sd) {
     if (c->d.transform != LH_TRANS_MODEAVG) return LH_E_ARG;
     HIPCK(hipSetDevice(c->device));
     if (sd->capacity < 0 || !sd->energy_w || !sd->t || !(sd->aeff > 0) || !c->omega) return LH_E_ARG;
+    if (sd->n_windows < 0 || sd->n_windows > LH_STAT_MAX_WINDOWS || (sd->n_windows > 0 && !sd->window_w))
+        return LH_E_ARG;
     double* ew = nullptr;
     double* tg = nullptr;
+    double* wn = nullptr;
     int st = shared_upload(c->device, &ew, sd->energy_w, (size_t)c->d.n_omega);
     if (!st) st = shared_upload(c->device, &tg, sd->t, (size_t)c->d.n_t);
+    if (!st && sd->n_windows > 0)
+        st = shared_upload(c->device, &wn, sd->window_w, (size_t)sd->n_windows * (size_t)c->d.n_omega);
     if (!st && !c->st_rec1 && hipMalloc(&c->st_rec1, sizeof(double) * LH_NSTAT) != hipSuccess) st = LH_E_NOMEM;
     if (!st && sd->capacity > c->st_cap) {
         if (c->steprec) (void)hipFree(c->steprec);
@@ -2570,12 +2616,16 @@ int lh_set_step_stats(lh_ctx* c, const lh_stats_desc* sd) {
     if (st) {
         shared_release(ew);
         shared_release(tg);
+        shared_release(wn);
         return st;
     }
     shared_release(c->st_ew);
     shared_release(c->st_t);
+    shared_release(c->st_win);
     c->st_ew = ew;
     c->st_t = tg;
+    c->st_win = wn;
+    c->st_nwin = sd->n_windows;
     c->st_cap = sd->capacity;
     c->st_aeff = sd->aeff;
     c->st_ediv = sqrt(EPS0 * CLIGHT * sd->aeff / 2);  // sqrt(ε_0*c*aeff(z)/2), src/Stats.jl:236

@@ -14,11 +14,16 @@ of ``_capi.LH_NSTAT`` values per accepted step:
   grid with ``PlasmaCumtrapz`` only), density (``:325-329``), z and dz
   (``zdz!``, ``:409-412``).
 
+``windows`` (``energy_λ``/``energy_window``, ``:52-79``): up to four
+wavelength windows, each a Planck-taper window over ω whose energy
+energyfun_ω(Eω .* window) is recorded on the device as Σ_k energy_w[k]·win_k²·
+|Eω_k|², keyed ``energy_<λmin>nm_<λmax>nm`` as the reference labels them.
+
 ``pressure`` (``:335-339``, PhysData.pressure of the density) is filled on the
 host from each record's density with the gas model, and ``zdw`` (``zdw_linop``,
 ``:571-582``: ``Modes.zdw``, a root search on the mode's dispersion) from the
 mode: once for a constant fill, at each record's z for a gradient fill.  The
-``windows``/``userfuns``/``onaxis`` options are not provided.
+``userfuns``/``onaxis`` options are not provided.
 """
 import math
 
@@ -27,6 +32,35 @@ import numpy as np
 from . import _capi
 from . import physdata as pd
 from .nonlinear import PlasmaCumtrapz
+
+
+def planck_taper(x, left0, left1, right1, right0):
+    """``Maths.planck_taper(x, left0, left1, right1, right0)``
+    (``src/Maths.jl:422-449``): 0 below left0, rising to 1 between left0 and
+    left1, 1 up to right1 (both ends included), falling to 0 at right0."""
+    xc = np.asarray(x, dtype=float) - (right0 + left0) / 2
+    X = right0 - left0
+    el = abs(left1 - left0) / X
+    er = abs(right0 - right1) / X
+    x1, x2, x3, x4 = -X / 2, -X / 2 * (1 - 2 * el), X / 2 * (1 - 2 * er), X / 2
+    out = np.zeros_like(xc)
+    with np.errstate(divide="ignore", over="ignore", invalid="ignore"):
+        m = (x1 < xc) & (xc < x2)
+        out[m] = 1 / (1 + np.exp((x2 - x1) / (xc[m] - x1) + (x2 - x1) / (xc[m] - x2)))
+        out[(x2 <= xc) & (xc <= x3)] = 1.0
+        m = (x3 < xc) & (xc < x4)
+        out[m] = 1 / (1 + np.exp((x3 - x4) / (xc[m] - x3) + (x3 - x4) / (xc[m] - x4)))
+    return out
+
+
+def energy_window(grid, lambda_lims, winwidth=0.0):
+    """(label, window over grid.omega) of ``energy_λ(grid, energyfun_ω, λlims;
+    winwidth)``, ``src/Stats.jl:52-61``."""
+    lam = np.asarray(lambda_lims, dtype=float)
+    wmin, wmax = sorted(pd.wlfreq(lam))
+    win = planck_taper(grid.omega, wmin - winwidth, wmin, wmax, wmax + winwidth)
+    label = "%.2fnm_%.2fnm" % (1e9 * lam.min(), 1e9 * lam.max())
+    return label, win
 
 
 def energy_weights(grid):
@@ -57,7 +91,7 @@ class StatsFun:
     and the description the propagation kernel uses to record the same values
     at every accepted step (``desc()``)."""
 
-    def __init__(self, grid, transform, capacity=16384, gas=None, zdw=None):
+    def __init__(self, grid, transform, capacity=16384, gas=None, zdw=None, windows=None):
         if getattr(transform, "kind", None) != _capi.LH_TRANS_MODEAVG:
             raise NotImplementedError("per-step statistics are provided for mode-averaged transforms")
         self.grid = grid
@@ -69,6 +103,15 @@ class StatsFun:
         self.plasma = grid.kind == "real" and any(isinstance(r, PlasmaCumtrapz) for r in transform.responses)
         self.gas = gas
         self.zdw = zdw  # None, a number, or a function of z
+        # energy windows: λlims pairs, or (λlims, winwidth)
+        self.windows = []
+        for w in (windows or ()):
+            lims, ww = (w[0], w[1]) if np.ndim(w[0]) == 1 else (w, 0.0)
+            self.windows.append(energy_window(grid, lims, ww))
+        if len(self.windows) > _capi.LH_STAT_MAX_WINDOWS:
+            raise ValueError(f"at most {_capi.LH_STAT_MAX_WINDOWS} energy windows")
+        self.window_w = (np.ascontiguousarray([self.energy_w * win ** 2 for _, win in self.windows])
+                         if self.windows else None)
 
     def desc(self):
         d = _capi.LhStatsDesc()
@@ -76,11 +119,13 @@ class StatsFun:
         d.energy_w = _capi._ptr(self.energy_w)
         d.t = _capi._ptr(self.t)
         d.aeff = self.aeff
+        d.n_windows = len(self.windows)
+        d.window_w = _capi._ptr(self.window_w) if self.windows else None
         self._keep = d  # the arrays stay owned by self
         return d
 
     def keys(self):
-        keys = list(_capi.STAT_KEYS)
+        keys = [k for k in _capi.STAT_KEYS if not k.startswith("window")]
         if not self.plasma:
             keys = [k for k in keys if k not in ("electrondensity", "peak_ionisation_rate")]
         return keys
@@ -90,6 +135,8 @@ class StatsFun:
         from the recorded density when a gas is given)."""
         records = np.asarray(records, dtype=float).reshape(-1, _capi.LH_NSTAT)
         out = {k: records[:, _capi.STAT_KEYS.index(k)].copy() for k in self.keys()}
+        for q, (label, _) in enumerate(self.windows):
+            out["energy_" + label] = records[:, _capi.STAT_KEYS.index(f"window{q}")].copy()
         if self.gas is not None:
             out["pressure"] = np.array([pd.pressure(self.gas, r) for r in out["density"]])
         if callable(self.zdw):
@@ -109,8 +156,8 @@ def default(grid, Ew, mode, linop, transform, *, windows=None, gas=None, onaxis=
             capacity=16384):
     """``Stats.default(grid, Eω, mode, linop, transform; gas, windows, onaxis,
     userfuns)``, ``src/Stats.jl:492-532``, for a mode-averaged transform."""
-    if windows or onaxis or userfuns:
-        raise NotImplementedError("windows, onaxis and userfuns statistics are not provided")
+    if onaxis or userfuns:
+        raise NotImplementedError("onaxis and userfuns statistics are not provided")
     zdw = None
     if mode is not None and hasattr(mode, "beta"):
         from .modes import MarcatiliMode, zdw as mode_zdw
@@ -121,4 +168,4 @@ def default(grid, Ew, mode, linop, transform, *, windows=None, gas=None, onaxis=
                                                    kind=mode.kind, T=g.T, model=mode.model, loss=mode.loss))
         else:
             zdw = mode_zdw(mode)
-    return StatsFun(grid, transform, capacity=capacity, gas=gas, zdw=zdw)
+    return StatsFun(grid, transform, capacity=capacity, gas=gas, zdw=zdw, windows=windows)

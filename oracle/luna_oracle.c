@@ -791,10 +791,28 @@ static double fwhm_linear(int n, const double *x, const double *y, int minmax_ma
 
 /* collect_stats(Eω, z, dz) with the Stats.default functions of a mode-averaged
  * propagation (src/Stats.jl:454-532): analytic!, then ω0, energy, peakpower,
- * fwhm_t, density, peakintensity(aeff), electrondensity (plasma), zdz!.
+ * fwhm_t, density, peakintensity(aeff), electrondensity (plasma), zdz!, and
+ * energy_window for each of nwin windows (win: [nwin][n_omega] window values;
+ * energyfun_ω(Eω .* window), src/Stats.jl:52-79).
  * t: grid.t [n_t]; aeff: Aeff; rec[LH_NSTAT]. */
-static int step_stats(const lh_desc *d, const double *t, double aeff, const cplx *Ew, double z, double dz,
-                      double *rec) {
+static double energy_of(const lh_desc *d, const double *y) {
+    /* Fields.energyfuncs energy_ω of |Eω|² samples y, src/Fields.jl:510-534 */
+    const int N = d->n_omega;
+    const double *w = d->omega;
+    if (d->grid_kind == LH_GRID_REAL) {
+        double prefac = 2 * ORC_PI / (w[N - 1] * w[N - 1]);
+        return prefac * simpson_even(N, w[1] - w[0], y);
+    }
+    double dw = w[1] - w[0];
+    double Dw = (double)N * dw;
+    double prefac = 2 * ORC_PI * dw / (Dw * Dw);
+    double s = 0.0;
+    for (int i = 0; i < N; i++) s += y[i];
+    return prefac * s;
+}
+
+static int step_stats(const lh_desc *d, const double *t, double aeff, int nwin, const double *win, const cplx *Ew,
+                      double z, double dz, double *rec) {
     const int N = d->n_omega, nt = d->n_t;
     const double *w = d->omega;
     double *y = (double *)malloc(sizeof(double) * (size_t)(N > nt ? N : nt));
@@ -810,17 +828,18 @@ static int step_stats(const lh_desc *d, const double *t, double aeff, const cplx
         sy += y[i];
     }
     rec[LH_STAT_OMEGA0] = sxy / sy;
-    /* energy: Fields.energyfuncs energy_ω, src/Fields.jl:510-534 */
-    if (d->grid_kind == LH_GRID_REAL) {
-        double prefac = 2 * ORC_PI / (w[N - 1] * w[N - 1]);
-        rec[LH_STAT_ENERGY] = prefac * simpson_even(N, w[1] - w[0], y);
-    } else {
-        double dw = w[1] - w[0];
-        double Dw = (double)N * dw;
-        double prefac = 2 * ORC_PI * dw / (Dw * Dw);
-        double s = 0.0;
-        for (int i = 0; i < N; i++) s += y[i];
-        rec[LH_STAT_ENERGY] = prefac * s;
+    rec[LH_STAT_ENERGY] = energy_of(d, y);
+    /* energy_window: energyfun_ω(Eω .* window) */
+    for (int q = 0; q < LH_STAT_MAX_WINDOWS; q++) {
+        rec[LH_STAT_WINDOW0 + q] = NAN;
+        if (q >= nwin) continue;
+        double *yw = (double *)malloc(sizeof(double) * (size_t)N);
+        for (int i = 0; i < N; i++) {
+            cplx e = Ew[i] * win[(size_t)q * N + i];
+            yw[i] = creal(e) * creal(e) + cimag(e) * cimag(e);
+        }
+        rec[LH_STAT_WINDOW0 + q] = energy_of(d, yw);
+        free(yw);
     }
     /* analytic field: plan_analytic, src/Stats.jl:421-448 */
     if (d->grid_kind == LH_GRID_REAL) {
@@ -954,8 +973,8 @@ static void grad_constants(const lh_desc *d, const lh_gradient_desc *g, double z
 
 /* Luna.run + solve_precon + stepfun filter + GridCondition saves. */
 static int solve_impl(const lh_desc *d, double *Ew_inout, const lh_rk_opts *o, double *saves, double *zsaves,
-                      lh_stats *stats, const double *tgrid, double aeff, double *rec, int cap, int *nrec,
-                      const lh_gradient_desc *g) {
+                      lh_stats *stats, const double *tgrid, double aeff, int nwin, const double *win, double *rec,
+                      int cap, int *nrec, const lh_gradient_desc *g) {
     if (nrec) *nrec = 0;
     rhs_user u;
     memset(&u, 0, sizeof(u));
@@ -1012,7 +1031,8 @@ static int solve_impl(const lh_desc *d, double *Ew_inout, const lh_rk_opts *o, d
             if (rec && stats->accepted <= cap) {
                 lh_desc dz = *d; /* density(z) at tn */
                 if (g) grad_constants(d, g, s.tn, NULL, NULL, &dz.density);
-                st = step_stats(&dz, tgrid, aeff, s.yn, s.tn, s.dtn, rec + (size_t)(stats->accepted - 1) * LH_NSTAT);
+                st = step_stats(&dz, tgrid, aeff, nwin, win, s.yn, s.tn, s.dtn,
+                                rec + (size_t)(stats->accepted - 1) * LH_NSTAT);
                 *nrec = (int)stats->accepted;
             }
             /* MemoryOutput + GridCondition: src/Output.jl:65-81, 497-502 */
@@ -1042,13 +1062,13 @@ static int solve_impl(const lh_desc *d, double *Ew_inout, const lh_rk_opts *o, d
 
 int orc_solve(const lh_desc *d, double *Ew_inout, const lh_rk_opts *o, double *saves, double *zsaves,
               lh_stats *stats) {
-    return solve_impl(d, Ew_inout, o, saves, zsaves, stats, NULL, 0.0, NULL, 0, NULL, NULL);
+    return solve_impl(d, Ew_inout, o, saves, zsaves, stats, NULL, 0.0, 0, NULL, NULL, 0, NULL, NULL);
 }
 
 /* Propagation with a gradient fill (make_linop + make_prop!(linop!)). */
 int orc_solve_grad(const lh_desc *d, const lh_gradient_desc *g, double *Ew_inout, const lh_rk_opts *o,
                    double *saves, double *zsaves, lh_stats *stats) {
-    return solve_impl(d, Ew_inout, o, saves, zsaves, stats, NULL, 0.0, NULL, 0, NULL, g);
+    return solve_impl(d, Ew_inout, o, saves, zsaves, stats, NULL, 0.0, 0, NULL, NULL, 0, NULL, g);
 }
 
 /* One RHS t(nl, Eω, z) with a gradient fill. */
@@ -1074,14 +1094,16 @@ int orc_grad_constants(const lh_desc *d, const lh_gradient_desc *g, double z, do
 /* Same with per-step statistics: rec[cap][LH_NSTAT] receives the records of
  * the first cap accepted steps, *nrec their count. */
 int orc_solve_stats(const lh_desc *d, double *Ew_inout, const lh_rk_opts *o, double *saves, double *zsaves,
-                    lh_stats *stats, const double *t, double aeff, double *rec, int cap, int *nrec) {
-    return solve_impl(d, Ew_inout, o, saves, zsaves, stats, t, aeff, rec, cap, nrec, NULL);
+                    lh_stats *stats, const double *t, double aeff, int nwin, const double *win, double *rec, int cap,
+                    int *nrec) {
+    return solve_impl(d, Ew_inout, o, saves, zsaves, stats, t, aeff, nwin, win, rec, cap, nrec, NULL);
 }
 
 /* One record of a given state (the stats closure called directly). */
-int orc_step_stats(const lh_desc *d, const double *t, double aeff, const double *Ew, double z, double dz,
-                   double *rec) {
-    return step_stats(d, t, aeff, (const cplx *)Ew, z, dz, rec);
+int orc_step_stats(const lh_desc *d, const double *t, double aeff, int nwin, const double *win, const double *Ew,
+                   double z, double dz, double *rec) {
+    if (nwin < 0 || nwin > LH_STAT_MAX_WINDOWS || (nwin > 0 && !win)) return LH_E_ARG;
+    return step_stats(d, t, aeff, nwin, win, (const cplx *)Ew, z, dz, rec);
 }
 
 /* Maths.fwhm(x, y; method=:linear, minmax), src/Maths.jl:128-131. */

@@ -37,9 +37,9 @@ def lib(capi):
     L.orc_responses_real.argtypes = [D, dp, dp]
     L.orc_rhs.argtypes = [D, dp, dp, C.c_double]
     L.orc_solve.argtypes = [D, dp, C.POINTER(capi.LhRkOpts), dp, dp, C.POINTER(capi.LhStats)]
-    L.orc_step_stats.argtypes = [D, dp, C.c_double, dp, C.c_double, C.c_double, dp]
+    L.orc_step_stats.argtypes = [D, dp, C.c_double, C.c_int, dp, dp, C.c_double, C.c_double, dp]
     L.orc_solve_stats.argtypes = [D, dp, C.POINTER(capi.LhRkOpts), dp, dp, C.POINTER(capi.LhStats), dp,
-                                  C.c_double, dp, C.c_int, C.POINTER(C.c_int)]
+                                  C.c_double, C.c_int, dp, dp, C.c_int, C.POINTER(C.c_int)]
     L.orc_fwhm.argtypes = [C.c_int, dp, dp, C.c_int, dp]
     G = C.POINTER(capi.LhGradientDesc)
     L.orc_rhs_grad.argtypes = [D, G, dp, dp, C.c_double]
@@ -155,15 +155,26 @@ class Oracle:
         self.L.orc_fwhm(len(x), _p(x), _p(y), int(minmax == "max"), C.byref(out))
         return out.value
 
-    def step_stats(self, holder, t, aeff, Ew, z=0.0, dz=0.0):
-        """One statistics record (``capi.LH_NSTAT`` values) of the state Ew."""
+    @staticmethod
+    def _windows(windows):
+        """Window values [nwin][Nω] (or None) → (nwin, contiguous array or None)."""
+        if windows is None or len(windows) == 0:
+            return 0, None
+        w = np.ascontiguousarray(np.atleast_2d(np.asarray(windows, dtype=float)))
+        return w.shape[0], w
+
+    def step_stats(self, holder, t, aeff, Ew, z=0.0, dz=0.0, windows=None):
+        """One statistics record (``capi.LH_NSTAT`` values) of the state Ew;
+        ``windows``: window values [nwin][Nω] of the energy windows."""
         Ew = np.ascontiguousarray(Ew, dtype=np.complex128)
         t = np.ascontiguousarray(t, dtype=float)
         rec = np.zeros(self.capi.LH_NSTAT)
-        st = self.L.orc_step_stats(holder.ref, _p(t), float(aeff), _p(Ew), z, dz, _p(rec))
+        nw, w = self._windows(windows)
+        st = self.L.orc_step_stats(holder.ref, _p(t), float(aeff), nw, _p(w) if nw else None, _p(Ew), z, dz,
+                                   _p(rec))
         return st, rec
 
-    def solve_stats(self, holder, Ew0, opts, t, aeff, cap=4096, want_saves=False):
+    def solve_stats(self, holder, Ew0, opts, t, aeff, cap=4096, want_saves=False, windows=None):
         """``solve`` that also returns the per-step statistics records [n][LH_NSTAT]."""
         d = holder.desc
         n = d.n_omega * d.n_r
@@ -174,8 +185,10 @@ class Oracle:
         t = np.ascontiguousarray(t, dtype=float)
         rec = np.zeros((cap, self.capi.LH_NSTAT))
         nrec = C.c_int(0)
+        nw, w = self._windows(windows)
         st = self.L.orc_solve_stats(holder.ref, _p(Ew), C.byref(opts), _p(saves) if want_saves else None,
-                                    _p(zs), C.byref(stats), _p(t), float(aeff), _p(rec), cap, C.byref(nrec))
+                                    _p(zs), C.byref(stats), _p(t), float(aeff), nw, _p(w) if nw else None, _p(rec),
+                                    cap, C.byref(nrec))
         return st, Ew, stats, rec[:nrec.value]
 
     def solve_precon(self, f, linop, y0, t0, dt0, tmax, rtol=1e-6, atol=1e-10, outputN=201):

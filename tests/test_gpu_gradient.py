@@ -81,7 +81,7 @@ def test_batch_with_gradient_and_stats(capi):
     for i, ctx in enumerate(ctxs):
         assert st[i].status == 0
         assert np.array_equal(outs[i], single[i][0])
-        assert np.array_equal(ctx.step_stats(), single[i][1])
+        assert np.array_equal(ctx.step_stats(), single[i][1], equal_nan=True)
     rec = single[0][1]
     g = insts[0][3].gradient.gradient
     for r in rec:

@@ -11,7 +11,11 @@ the PPT rate, a steep function of |E| (about its 40th power here), carries that
 to ~1e-7 in peak_ionisation_rate and electrondensity, checked at 1e-6.
 z and dz agree to 1e-5 only: the step-size controller divides by an error
 estimate that is itself a near-cancelling sum (Σ k_j (b5_j - b4_j)), so the
-RHS's 1e-13 differences reach the proposed step at the 1e-7 level.
+RHS's 1e-13 differences reach the proposed step at the 1e-7 level.  Energy
+windows over a spectral region the pulse only reaches nonlinearly (the blue
+side, 4e-6 of the energy after 5 cm) magnify the trajectories' difference the
+same way (1.9e-7 measured) and are checked at 1e-6 in propagations; a window
+over the main spectrum is held to 1e-7.
 """
 import math
 
@@ -21,13 +25,15 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 
-def assert_rec_close(got, ref, rtol, keys, rtol_z=None, rtol_ion=None):
+def assert_rec_close(got, ref, rtol, keys, rtol_z=None, rtol_ion=None, rtol_win=None):
     for k, g, r in zip(keys, got, ref):
         tol = rtol
         if rtol_z is not None and k in ("z", "dz"):
             tol = rtol_z
         if rtol_ion is not None and k in ("electrondensity", "peak_ionisation_rate"):
             tol = rtol_ion
+        if rtol_win is not None and k.startswith("window"):
+            tol = rtol_win
         if math.isnan(r):
             assert math.isnan(g), (k, g, r)
         else:
@@ -149,12 +155,12 @@ def test_capacity_and_batch(orc, capi, pkg):
     st = rb.solve(opts, want_saves=False)
     for i, ctx in enumerate(ctxs):
         assert st[i].status == 0
-        assert np.array_equal(ctx.step_stats(), single[i])
+        assert np.array_equal(ctx.step_stats(), single[i], equal_nan=True)
     # a smaller capacity keeps the first records only
     ctxs[0].set_step_stats(stats.StatsFun(insts[0].grid, insts[0].transform, capacity=3))
     ctxs[0].solve(insts[0].Ew0, opts)
     r3 = ctxs[0].step_stats()
-    assert len(r3) == 3 and np.array_equal(r3, single[0][:3])
+    assert len(r3) == 3 and np.array_equal(r3, single[0][:3], equal_nan=True)
 
 
 def test_interface_output_stats(pkg, capi):
@@ -172,3 +178,57 @@ def test_interface_output_stats(pkg, capi):
     assert s["z"][-1] >= 0.02 and np.all(np.diff(s["z"]) > 0)
     assert s["pressure"][0] == pytest.approx(1.5, rel=1e-12)
     assert np.allclose(s["peakintensity"], s["peakpower"] / transform.aeff, rtol=1e-14)
+
+
+def windowed_ctx(inst, windows):
+    from luna_jl_amd import runtime, stats
+    ctx = runtime.Context(inst.holder)
+    sf = stats.StatsFun(inst.grid, inst.transform, capacity=4096, windows=windows)
+    ctx.set_step_stats(sf)
+    return ctx, sf, [win for _, win in sf.windows]
+
+
+def test_stats_eval_windows(small, orc, capi, pkg):
+    """Energy windows (energy_λ/energy_window, src/Stats.jl:52-79): a
+    rectangle, a Planck-tapered window and one over the whole grid, field and
+    envelope grids, one record against the oracle (1e-12)."""
+    print("case: stats_eval with energy windows", flush=True)
+    from luna_jl_amd import configs
+    wins = [(700e-9, 900e-9), ((600e-9, 1000e-9), 5e13), (100e-9, 10e-6)]
+    ctx, sf, vals = windowed_ctx(small, wins)
+    for scale in (1.0, 1.7):
+        rec = ctx.stats_eval(small.Ew0 * scale, 0.1, 1e-3)
+        st, ref = orc.step_stats(small.holder, small.grid.t, small.transform.aeff, small.Ew0 * scale, 0.1, 1e-3,
+                                 windows=vals)
+        assert st == 0
+        assert_rec_close(rec, ref, 1e-12, capi.STAT_KEYS)
+        assert rec[capi.STAT_KEYS.index("window2")] == pytest.approx(rec[capi.STAT_KEYS.index("energy")], rel=1e-13)
+        assert math.isnan(rec[capi.STAT_KEYS.index("window3")])
+    cfg1 = configs.gnlse_instance(shotnoise=False)
+    ctx, sf, vals = windowed_ctx(cfg1, [(800e-9, 900e-9), ((1000e-9, 1300e-9), 2e13)])
+    rec = ctx.stats_eval(cfg1.Ew0, 0.0, 1e-4)
+    st, ref = orc.step_stats(cfg1.holder, cfg1.grid.t, cfg1.transform.aeff, cfg1.Ew0, 0.0, 1e-4, windows=vals)
+    assert st == 0
+    assert_rec_close(rec, ref, 1e-12, capi.STAT_KEYS)
+
+
+def test_solve_records_windows(orc, capi, pkg):
+    print("case: solve with stats and energy windows", flush=True)
+    from luna_jl_amd import configs
+    inst = configs.capillary_instance(energy=180e-6, pressure=2.5, trange=0.5e-12, flength=0.05)
+    ctx, sf, vals = windowed_ctx(inst, [(700e-9, 900e-9), ((250e-9, 600e-9), 3e13)])
+    opts = capi.rk_opts(0.05, save_n=6)
+    Ew, sv, zs, stats = ctx.solve(inst.Ew0, opts)
+    rec = ctx.step_stats()
+    st, Ew_r, stats_r, rec_r = orc.solve_stats(inst.holder, inst.Ew0, opts, inst.grid.t, inst.transform.aeff,
+                                               windows=vals)
+    assert st == 0 and stats.status == 0 and stats.accepted == len(rec) == len(rec_r)
+    w0 = capi.STAT_KEYS.index("window0")
+    for i in range(len(rec)):
+        assert_rec_close(rec[i], rec_r[i], 1e-7, capi.STAT_KEYS, rtol_z=1e-5, rtol_ion=1e-6, rtol_win=1e-6)
+        assert rec[i][w0] == pytest.approx(rec_r[i][w0], rel=1e-7)  # the main spectrum
+    d = sf.to_dict(rec)
+    assert len(d["energy_700.00nm_900.00nm"]) == len(rec)
+    # the blue window gains energy as the pulse broadens
+    blue = d["energy_250.00nm_600.00nm"]
+    assert blue[-1] > blue[0]

@@ -133,3 +133,40 @@ def test_host_energy_weights_match_oracle(small, orc, capi):
     st, rec = orc.step_stats(small.holder, small.grid.t, small.transform.aeff, small.Ew0)
     e = np.sum(w * np.abs(small.Ew0) ** 2)
     assert e == pytest.approx(rec[capi.STAT_KEYS.index("energy")], rel=1e-13)
+
+
+def test_energy_windows(small, orc, capi):
+    """energy_window (src/Stats.jl:52-79): energyfun_ω(Eω .* window).
+    (a) a window over the whole grid gives the energy itself, bit for bit;
+    (b) two rectangular windows split between two bins add up to the energy;
+    (c) the 10 fs Gaussian input has |Eω|² ∝ g(ω) = exp(-(ω-ω0)²/2σ²) with
+        σ = 4 ln2/(τ·2√(2 ln2)): the energy fraction in a window is
+        Σ g·win² / Σ g — a rectangle at ω0 ± σ, and a Planck-tapered window
+        (which fails if the window multiplied |Eω|² instead of Eω)."""
+    from luna_jl_amd import physdata as pd
+    from luna_jl_amd import stats
+    g = small.grid
+    w = np.asarray(g.omega)
+    t, aeff = g.t, small.transform.aeff
+    ones = np.ones_like(w)
+    st, rec = orc.step_stats(small.holder, t, aeff, small.Ew0, windows=[ones])
+    r = dict(zip(capi.STAT_KEYS, rec))
+    assert st == 0 and r["window0"] == r["energy"]
+    assert all(math.isnan(r[f"window{q}"]) for q in (1, 2, 3))
+    kb = int(np.argmin(abs(w - pd.wlfreq(800e-9))))
+    lo = (np.arange(len(w)) <= kb).astype(float)
+    st, rec = orc.step_stats(small.holder, t, aeff, small.Ew0, windows=[lo, 1.0 - lo])
+    r = dict(zip(capi.STAT_KEYS, rec))
+    assert r["window0"] + r["window1"] == pytest.approx(r["energy"], rel=1e-13)
+    tau = 10e-15
+    w0 = pd.wlfreq(800e-9)
+    sig = 4 * math.log(2) / tau / (2 * math.sqrt(2 * math.log(2)))
+    gk = np.exp(-((w - w0) ** 2) / (2 * sig ** 2))
+    _, rect = stats.energy_window(g, (pd.wlfreq(w0 + sig), pd.wlfreq(w0 - sig)))
+    _, tap = stats.energy_window(g, (pd.wlfreq(w0 + sig), pd.wlfreq(w0 - sig)), winwidth=sig)
+    st, rec = orc.step_stats(small.holder, t, aeff, small.Ew0, windows=[rect, tap])
+    r = dict(zip(capi.STAT_KEYS, rec))
+    assert 0.6 < r["window0"] / r["energy"] < 0.75  # ≈ erf(1/√2) on the bins
+    assert r["window0"] / r["energy"] == pytest.approx(np.sum(gk * rect ** 2) / np.sum(gk), rel=1e-6)
+    assert r["window1"] / r["energy"] == pytest.approx(np.sum(gk * tap ** 2) / np.sum(gk), rel=1e-6)
+    assert abs(np.sum(gk * tap) / np.sum(gk) - r["window1"] / r["energy"]) > 1e-2

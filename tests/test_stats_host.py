@@ -35,9 +35,43 @@ def test_no_plasma_no_ionisation_keys(pkg, capi):
 def test_unsupported_options_raise(small):
     from luna_jl_amd import stats
     with pytest.raises(NotImplementedError):
-        stats.default(small.grid, small.Ew0, None, small.linop, small.transform, windows=[(200e-9, 300e-9)])
-    with pytest.raises(NotImplementedError):
         stats.default(small.grid, small.Ew0, None, small.linop, small.transform, onaxis=True)
+    with pytest.raises(ValueError):  # the record holds four windows
+        stats.default(small.grid, small.Ew0, None, small.linop, small.transform,
+                      windows=[(200e-9 + 1e-8 * i, 300e-9) for i in range(5)])
+
+
+def test_window_keys_and_columns(small, capi):
+    """energy_λ labels (src/Stats.jl:52-61): "energy_%.2fnm_%.2fnm" of the
+    sorted limits; (λlims, winwidth) pairs accepted; the columns map in order."""
+    from luna_jl_amd import stats
+    sf = stats.default(small.grid, small.Ew0, None, small.linop, small.transform,
+                       windows=[(300e-9, 200e-9), ((700e-9, 900e-9), 2e13)])
+    rec = np.zeros((2, capi.LH_NSTAT))
+    rec[:, capi.STAT_KEYS.index("window0")] = [1.0, 2.0]
+    rec[:, capi.STAT_KEYS.index("window1")] = [3.0, 4.0]
+    d = sf.to_dict(rec)
+    assert list(d["energy_200.00nm_300.00nm"]) == [1.0, 2.0]
+    assert list(d["energy_700.00nm_900.00nm"]) == [3.0, 4.0]
+    assert not any(k.startswith("window") for k in d)
+    desc = sf.desc()
+    assert desc.n_windows == 2 and sf.window_w.shape == (2, len(small.grid.omega))
+
+
+def test_planck_taper_shape():
+    """Maths.planck_taper (src/Maths.jl:422-449): 1 between left1 and right1
+    (ends included), 0 outside [left0, right0], 1/2 at the middle of a taper
+    (z = 0 there), and the zero-width case is a rectangle with both ends."""
+    from luna_jl_amd import stats
+    x = np.linspace(0.0, 10.0, 1001)
+    w = stats.planck_taper(x, 2.0, 3.0, 7.0, 9.0)
+    assert np.all(w[(x >= 3.0) & (x <= 7.0)] == 1.0)
+    assert np.all(w[(x <= 2.0) | (x >= 9.0)] == 0.0)
+    assert w[np.argmin(abs(x - 2.5))] == pytest.approx(0.5, abs=1e-12)
+    assert w[np.argmin(abs(x - 8.0))] == pytest.approx(0.5, abs=1e-12)
+    assert np.all(np.diff(w[(x > 2.0) & (x < 3.0)]) >= 0)  # saturates to 0 and 1 at the ends
+    r = stats.planck_taper(x, 3.0, 3.0, 7.0, 7.0)
+    assert np.array_equal(r, ((x >= 3.0) & (x <= 7.0)).astype(float))
 
 
 def test_pressure_inverts_density():
