@@ -79,6 +79,18 @@ class MarcatiliMode:
     def Aeff(self, z=0.0):
         return self.a ** 2 * self.aeff_intg
 
+    def aeff_onaxis(self):
+        """P/I(r = 0) of the mode: the area that turns power into the on-axis
+        intensity of Modes.to_space at (0, 0) with the y component
+        (src/Modes.jl:458-493, Exy = field/√N, src/Modes.jl:73): for HE1m,
+        E_y(0) = J0(0)/√N with N = π/2 a² J1(u)² √(ε0/μ0) (src/Capillary.jl:
+        270-287), so I(0) = cε0/2·P/N and the area is 2N/(cε0) = π a² J1(u)².
+        Other modes vanish on the axis."""
+        if self.kind != "HE" or self.n != 1:
+            raise NotImplementedError("on-axis statistics need an HE1m mode (the others vanish on the axis)")
+        N = math.pi / 2 * self.a ** 2 * special.jv(1, self.unm) ** 2 * math.sqrt(pd.eps_0 / pd.mu_0)
+        return 2 * N / (pd.c * pd.eps_0)
+
     def neff(self, omega):
         """``neff(m::MarcatiliMode, ω)``, ``Capillary.jl:136-186``."""
         omega = np.asarray(omega, dtype=float)

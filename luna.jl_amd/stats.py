@@ -19,11 +19,16 @@ wavelength windows, each a Planck-taper window over ω whose energy
 energyfun_ω(Eω .* window) is recorded on the device as Σ_k energy_w[k]·win_k²·
 |Eω_k|², keyed ``energy_<λmin>nm_<λmax>nm`` as the reference labels them.
 
+``onaxis`` (``peakintensity(grid, (mode,))`` and ``electrondensity(..., (mode,))``,
+``:159-174, 243-262``): the field at r = 0 of a single HE1m mode is the
+mode-averaged field scaled by a constant, so both statistics are the
+mode-averaged ones with Aeff replaced by the mode's P/I(0) (``aeff_onaxis``).
+
 ``pressure`` (``:335-339``, PhysData.pressure of the density) is filled on the
 host from each record's density with the gas model, and ``zdw`` (``zdw_linop``,
 ``:571-582``: ``Modes.zdw``, a root search on the mode's dispersion) from the
 mode: once for a constant fill, at each record's z for a gradient fill.  The
-``userfuns``/``onaxis`` options are not provided.
+``userfuns`` option is not provided.
 """
 import math
 
@@ -91,7 +96,7 @@ class StatsFun:
     and the description the propagation kernel uses to record the same values
     at every accepted step (``desc()``)."""
 
-    def __init__(self, grid, transform, capacity=16384, gas=None, zdw=None, windows=None):
+    def __init__(self, grid, transform, capacity=16384, gas=None, zdw=None, windows=None, aeff=None):
         if getattr(transform, "kind", None) != _capi.LH_TRANS_MODEAVG:
             raise NotImplementedError("per-step statistics are provided for mode-averaged transforms")
         self.grid = grid
@@ -99,7 +104,7 @@ class StatsFun:
         self.capacity = int(capacity)
         self.energy_w = np.ascontiguousarray(energy_weights(grid))
         self.t = np.ascontiguousarray(grid.t, dtype=float)
-        self.aeff = float(transform.aeff)
+        self.aeff = float(transform.aeff if aeff is None else aeff)  # on-axis statistics: P/I(0)
         self.plasma = grid.kind == "real" and any(isinstance(r, PlasmaCumtrapz) for r in transform.responses)
         self.gas = gas
         self.zdw = zdw  # None, a number, or a function of z
@@ -156,8 +161,15 @@ def default(grid, Ew, mode, linop, transform, *, windows=None, gas=None, onaxis=
             capacity=16384):
     """``Stats.default(grid, Eω, mode, linop, transform; gas, windows, onaxis,
     userfuns)``, ``src/Stats.jl:492-532``, for a mode-averaged transform."""
-    if onaxis or userfuns:
-        raise NotImplementedError("onaxis and userfuns statistics are not provided")
+    if userfuns:
+        raise NotImplementedError("userfuns statistics are not provided")
+    aeff = None
+    if onaxis:
+        if mode is None or not hasattr(mode, "aeff_onaxis"):
+            raise NotImplementedError("on-axis statistics need a capillary (Marcatili) mode")
+        if getattr(transform, "gradient", None) is not None:
+            raise NotImplementedError("on-axis statistics with a gradient fill are not provided")
+        aeff = mode.aeff_onaxis()
     zdw = None
     if mode is not None and hasattr(mode, "beta"):
         from .modes import MarcatiliMode, zdw as mode_zdw
@@ -168,4 +180,4 @@ def default(grid, Ew, mode, linop, transform, *, windows=None, gas=None, onaxis=
                                                    kind=mode.kind, T=g.T, model=mode.model, loss=mode.loss))
         else:
             zdw = mode_zdw(mode)
-    return StatsFun(grid, transform, capacity=capacity, gas=gas, zdw=zdw, windows=windows)
+    return StatsFun(grid, transform, capacity=capacity, gas=gas, zdw=zdw, windows=windows, aeff=aeff)

@@ -232,3 +232,21 @@ def test_solve_records_windows(orc, capi, pkg):
     # the blue window gains energy as the pulse broadens
     blue = d["energy_250.00nm_600.00nm"]
     assert blue[-1] > blue[0]
+
+
+def test_stats_eval_onaxis(small, orc, capi, pkg):
+    """On-axis statistics (Stats.default(...; onaxis=true)): peak intensity and
+    electron density with the mode's P/I(0) in place of Aeff, one record
+    against the oracle with the same area (1e-12)."""
+    print("case: stats_eval on axis", flush=True)
+    from luna_jl_amd import modes, runtime, stats
+    mode = modes.MarcatiliMode(125e-6, "He", 2.0)
+    sf = stats.default(small.grid, small.Ew0, mode, small.linop, small.transform, onaxis=True)
+    ctx = runtime.Context(small.holder)
+    ctx.set_step_stats(sf)
+    rec = ctx.stats_eval(small.Ew0 * 1.5, 0.0, 1e-3)
+    st, ref = orc.step_stats(small.holder, small.grid.t, mode.aeff_onaxis(), small.Ew0 * 1.5, 0.0, 1e-3)
+    assert st == 0
+    assert_rec_close(rec, ref, 1e-12, capi.STAT_KEYS)
+    k = capi.STAT_KEYS.index("peakintensity")
+    assert rec[k] == pytest.approx(rec[capi.STAT_KEYS.index("peakpower")] / mode.aeff_onaxis(), rel=1e-14)
