@@ -1,7 +1,7 @@
 // lh_device.h -- device building blocks for the propagation kernels (gfx950).
 //
-// Complex arithmetic on double2, LDS-resident Stockham FFTs, block-wide scans
-// and reductions.  One workgroup = LH_T threads (8 wave64s) works on one
+// Complex arithmetic on double2, LDS-resident Stockham FFTs, block-wide
+// reductions.  One workgroup = LH_T threads (8 wave64s) works on one
 // transform; the whole oversampled time series of one RHS lives in LDS.
 // 512 threads leave 256 registers per lane for the scan/spline phase.
 #pragma once
@@ -322,40 +322,7 @@ __device__ __forceinline__ double2 r2c_bin(const double2* buf, const TwLds& tw, 
     return cadd(E, cmul(w, O));
 }
 
-// ------------------------------------------------------------ block scans
-// Exclusive prefix sum of one value per thread over the workgroup; also
-// returns the workgroup total.  wsum: LDS scratch of 2*LH_NW+1 doubles.
-__device__ __forceinline__ double block_exscan(double v, double* wsum, double* total = nullptr) {
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    double x = v;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        double y = __shfl_up(x, off, 64);
-        if (lane >= off) x += y;
-    }
-    double ex = __shfl_up(x, 1, 64);
-    if (lane == 0) ex = 0.0;
-    if (lane == 63) wsum[wid] = x;
-    __syncthreads();
-    if (wid == 0) {
-        double w = (lane < LH_NW) ? wsum[lane] : 0.0;
-#pragma unroll
-        for (int off = 1; off < LH_NW; off <<= 1) {
-            double y = __shfl_up(w, off, 64);
-            if (lane >= off) w += y;
-        }
-        double wex = __shfl_up(w, 1, 64);
-        if (lane == 0) wex = 0.0;
-        if (lane < LH_NW) wsum[LH_NW + lane] = wex;
-        if (lane == LH_NW - 1) wsum[2 * LH_NW] = w;
-    }
-    __syncthreads();
-    double r = wsum[LH_NW + wid] + ex;
-    if (total) *total = wsum[2 * LH_NW];
-    __syncthreads();
-    return r;
-}
-
+// ------------------------------------------------------- block reductions
 // Sum of four values over the workgroup (fixed order → deterministic);
 // scratch: 4·LH_NW doubles.  Ends synchronised.
 __device__ __forceinline__ void block_sum4(double& a, double& b, double& c, double& d, double* scratch) {

@@ -130,7 +130,7 @@ template <int MO>
 struct SmemT {
     double2 buf[padded_len(MO)];
     TwLds tw;
-    double wsum[5 * LH_NW];      // block_exscan, block_sum3, statistics reductions
+    double wsum[5 * LH_NW];      // block_sum3/block_sum4 and the statistics reductions
     double scan[2][3 * LH_NW];   // one-barrier scans and min/max, used in alternation
     double plist[PList<MO>::CAP];
     double zs[2];  // gradient fill: ρ(z), β1(z) from thread 0
