@@ -280,6 +280,14 @@ int lh_get_step_stats(lh_ctx *ctx, double *rec, int capacity, int *n);
  * nclass must be >= 6. */
 int lh_last_launch_times(double *ms, int *n_inst, int nclass);
 
+/* Page-locked host memory for the states and saves a sweep copies back (no
+ * counterpart in the reference, whose arrays live in host memory already):
+ * device-to-host copies into it run at DMA speed and touch no unmapped pages.
+ * lh_host_alloc: *ptr = bytes of pinned memory on the current device's host
+ * (LH_E_NOMEM if refused); lh_host_free releases it (NULL is a no-op). */
+int lh_host_alloc(size_t bytes, void **ptr);
+int lh_host_free(void *ptr);
+
 #ifdef __cplusplus
 }
 #endif

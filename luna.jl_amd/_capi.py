@@ -174,6 +174,8 @@ def lib():
     L.lh_batch_solve_resident.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.POINTER(LhRkOpts), C.c_int,
                                           C.POINTER(LhStats)]
     L.lh_last_launch_times.argtypes = [_dp, C.POINTER(C.c_int), C.c_int]
+    L.lh_host_alloc.argtypes = [C.c_size_t, C.POINTER(C.c_void_p)]
+    L.lh_host_free.argtypes = [C.c_void_p]
     L.lh_set_step_stats.argtypes = [C.c_void_p, C.POINTER(LhStatsDesc)]
     L.lh_set_gradient.argtypes = [C.c_void_p, C.POINTER(LhGradientDesc)]
     L.lh_ppt_table.argtypes = [C.POINTER(LhPptDesc), C.c_int, _dp, _dp, _dp]
@@ -196,6 +198,7 @@ def lib():
 EXPORTED = ["lh_strerror", "lh_device_count", "lh_create", "lh_destroy", "lh_rhs",
             "lh_rhs_device", "lh_ctx_status", "lh_solve", "lh_batch_solve", "lh_set_state",
             "lh_batch_solve_resident", "lh_get_state", "lh_get_saves", "lh_last_launch_times",
+            "lh_host_alloc", "lh_host_free",
             "lh_set_step_stats", "lh_stats_eval", "lh_get_step_stats", "lh_set_gradient", "lh_ppt_table",
             "lh_fno_create", "lh_fno_destroy", "lh_fno_param_count", "lh_fno_set_params",
             "lh_fno_get_params", "lh_fno_forward", "lh_fno_loss_grad", "lh_fno_set_data", "lh_fno_train"]

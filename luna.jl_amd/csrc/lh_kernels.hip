@@ -2733,6 +2733,23 @@ int lh_last_launch_times(double* ms, int* n_inst, int nclass) {
     return LH_OK;
 }
 
+int lh_host_alloc(size_t bytes, void** ptr) {
+    if (!ptr || bytes == 0) return LH_E_ARG;
+    *ptr = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return LH_E_NODEVICE;
+    if (hipHostMalloc(ptr, bytes, hipHostMallocDefault) != hipSuccess) {
+        *ptr = nullptr;
+        return LH_E_NOMEM;
+    }
+    return LH_OK;
+}
+
+int lh_host_free(void* ptr) {
+    if (!ptr) return LH_OK;
+    return hipHostFree(ptr) == hipSuccess ? LH_OK : LH_E_HIP;
+}
+
 int lh_batch_solve(lh_ctx* const* ctxs, int n, double* const* Ew, const lh_rk_opts* o, double* const* saves,
                    double* const* zsaves, lh_stats* stats) {
     if (!ctxs || n <= 0 || !Ew || !o || !stats) return LH_E_ARG;

@@ -86,20 +86,71 @@ class Context:
         return rec[:n.value].copy()
 
 
-def batch_solve(contexts, Ews, opts, want_saves=True):
-    """Run every instance in one persistent launch (``lh_batch_solve``)."""
+class HostBuffers:
+    """Page-locked host arrays (``lh_host_alloc``) for the final states, saves
+    and save positions of a sweep: allocated once and handed to every
+    ``batch_solve(..., out=...)`` of the sweep, so the copies back run at DMA
+    speed into memory that is already mapped.  The arrays are views of the
+    pinned block and are invalid after ``close()``."""
+
+    def __init__(self, contexts, save_n):
+        n = len(contexts)
+        ns = [c.n_state for c in contexts]
+        nz = max(save_n, 1)
+        nbytes = sum(16 * m + 16 * save_n * m + 8 * nz for m in ns)
+        ptr = C.c_void_p()
+        _capi.check(_capi.lib().lh_host_alloc(nbytes, C.byref(ptr)), "lh_host_alloc")
+        self._ptr = ptr
+        raw = np.frombuffer((C.c_char * nbytes).from_address(ptr.value), dtype=np.uint8)
+        self.states, self.saves, self.zs = [], [], []
+        off = 0
+        for m in ns:
+            self.states.append(raw[off:off + 16 * m].view(np.complex128))
+            off += 16 * m
+            self.saves.append(raw[off:off + 16 * save_n * m].view(np.complex128).reshape(save_n, m))
+            off += 16 * save_n * m
+            self.zs.append(raw[off:off + 8 * nz].view(np.float64))
+            off += 8 * nz
+        self.save_n = save_n
+        self.n = n
+
+    def close(self):
+        if self._ptr is not None and self._ptr.value:
+            self.states = self.saves = self.zs = None
+            _capi.lib().lh_host_free(self._ptr)
+        self._ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def batch_solve(contexts, Ews, opts, want_saves=True, out=None):
+    """Run every instance in one persistent launch (``lh_batch_solve``).
+    ``out``: HostBuffers for this sweep's contexts (states, saves and save
+    positions are written there) or None (fresh arrays)."""
     n = len(contexts)
     handles = (C.c_void_p * n)(*[c.handle for c in contexts])
-    ew_arrs = [np.ascontiguousarray(np.asarray(e, dtype=np.complex128).reshape(-1, order="F")).copy()
-               for e in Ews]
+    if out is not None:
+        if out.n != n or out.save_n != opts.save_n:
+            raise ValueError("HostBuffers made for another sweep shape")
+        ew_arrs = out.states
+        for a, e in zip(ew_arrs, Ews):
+            a[:] = np.asarray(e, dtype=np.complex128).reshape(-1, order="F")
+    else:
+        ew_arrs = [np.ascontiguousarray(np.asarray(e, dtype=np.complex128).reshape(-1, order="F")).copy()
+                   for e in Ews]
     ew_ptrs = (C.POINTER(C.c_double) * n)(*[_p(a) for a in ew_arrs])
     if want_saves:
-        sv = [np.zeros((opts.save_n, c.n_state), dtype=np.complex128) for c in contexts]
+        sv = out.saves if out is not None else [np.zeros((opts.save_n, c.n_state), dtype=np.complex128)
+                                                for c in contexts]
         sv_ptrs = (C.POINTER(C.c_double) * n)(*[_p(a) for a in sv])
     else:
         sv = [None] * n
         sv_ptrs = None
-    zs = [np.zeros(max(opts.save_n, 1)) for _ in range(n)]
+    zs = out.zs if out is not None else [np.zeros(max(opts.save_n, 1)) for _ in range(n)]
     zs_ptrs = (C.POINTER(C.c_double) * n)(*[_p(a) for a in zs])
     stats = (_capi.LhStats * n)()
     st = _capi.lib().lh_batch_solve(handles, n, ew_ptrs, C.byref(opts), sv_ptrs, zs_ptrs, stats)

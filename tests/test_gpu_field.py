@@ -120,3 +120,26 @@ def test_resident_batch_matches_host_batch(pkg, capi):
             assert np.array_equal(rb.state(i), outs[i])
             sv, zs = rb.saves(i)
             assert np.array_equal(sv, svs[i])
+
+
+def test_batch_into_pinned_host_buffers(pkg, capi):
+    """batch_solve(..., out=HostBuffers): the same states, saves and save
+    positions as with fresh host arrays, twice in a row into the same
+    page-locked buffers."""
+    print("case: batch into pinned host buffers", flush=True)
+    from luna_jl_amd import configs, runtime
+    insts = [configs.capillary_instance(energy=e, pressure=p, trange=t, flength=0.05)
+             for e, p, t in [(120e-6, 1.2, 0.5e-12), (90e-6, 2.0, 1e-12), (150e-6, 0.8, 0.5e-12)]]
+    ctxs = [runtime.Context(c.holder) for c in insts]
+    opts = capi.rk_opts(0.05, save_n=5)
+    outs, svs, zss, st_ref = runtime.batch_solve(ctxs, [c.Ew0 for c in insts], opts)
+    hb = runtime.HostBuffers(ctxs, opts.save_n)
+    try:
+        for rep in range(2):
+            o2, s2, z2, st2 = runtime.batch_solve(ctxs, [c.Ew0 for c in insts], opts, out=hb)
+            for i in range(len(insts)):
+                assert st2[i].status == 0 and st2[i].steps == st_ref[i].steps
+                assert np.array_equal(o2[i], outs[i]) and np.array_equal(s2[i], svs[i])
+                assert np.array_equal(z2[i], zss[i])
+    finally:
+        hb.close()
