@@ -37,10 +37,11 @@ def parse():
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--stats", default="on", choices=["on", "off"],
                    help="record Stats.default's per-step statistics (as prop_capillary does)")
-    p.add_argument("--workload", default="scan", choices=["scan", "radial", "surrogate", "copy", "ppt"],
+    p.add_argument("--workload", default="scan", choices=["scan", "radial", "surrogate", "copy", "ppt", "gnlse"],
                    help="scan: batched capillary sweep (the reported metric); radial: the "
                         "Nr=256 x Nt=4096 radial propagation; surrogate: FNO training on solver "
-                        "spectra and screening of the sweep; the last two print their own line")
+                        "spectra and screening of the sweep; gnlse: a 256-seed shot-noise ensemble of "
+                        "the GNLSE supercontinuum; ppt: PPT tables; copy: counter calibration")
     p.add_argument("--n-train", type=int, default=4096)
     p.add_argument("--n-cal", type=int, default=512)
     p.add_argument("--train-steps", type=int, default=8000)
@@ -221,6 +222,53 @@ def surrogate_bench(args):
     print(json.dumps(out))
 
 
+def gnlse_ensemble(args):
+    """Shot-noise ensemble of the GNLSE supercontinuum (Dudley et al., RMP 78,
+    1135 (2006), Fig. 3: Nt = 8192 envelope grid, Kerr_env + RamanPolarEnv,
+    15 cm): --instances propagations that differ only in the shot-noise seed,
+    as coherence studies run them, in one resident batch with 201 saves each
+    kept in HBM; and one propagation by itself for comparison."""
+    import __graft_entry__ as ge
+    ge.load_package()
+    from luna_jl_amd import _capi, configs, fields, runtime
+    n = min(args.instances, 256)
+    base = configs.gnlse_instance(shotnoise=False)
+    Ews = [base.Ew0 + fields.ShotNoise(seed=i)(base.grid) for i in range(n)]
+    ctxs = [runtime.Context(base.holder) for _ in range(n)]
+    opts = _capi.rk_opts(base.zmax, save_n=201)
+    single = runtime.ResidentBatch(ctxs[:1], Ews[:1])
+    single.solve(opts)
+    t = time.perf_counter()
+    st1 = single.solve(opts)[0]
+    one = time.perf_counter() - t
+    log(f"bench gnlse: one propagation {one * 1e3:.1f} ms, {st1.steps} steps")
+    batch = runtime.ResidentBatch(ctxs, Ews)
+    for w in range(args.warmup):
+        batch.solve(opts)
+        log(f"bench gnlse: warmup {w + 1}/{args.warmup} done")
+    times = []
+    for s in range(args.steps):
+        t = time.perf_counter()
+        st = batch.solve(opts)
+        times.append(time.perf_counter() - t)
+        log(f"bench gnlse: step {s + 1}/{args.steps}: {times[-1] * 1e3:.1f} ms")
+    dt = float(np.mean(times))
+    ok = int(sum(x.status == 0 for x in st))
+    steps = np.array([x.steps for x in st], dtype=float)
+    out = {
+        "metric": "GNLSE shot-noise ensemble propagations/hour (Nt=8192 envelope, Kerr+Raman, 15 cm)",
+        "value": ok * 3600.0 / dt, "unit": "propagations/hour", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": dt * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded shot noise)",
+        "config": {"workload": f"gnlse_ensemble_{n}", "instances": n, "n_t": 8192, "flength_m": base.zmax,
+                   "saves": 201},
+        "instances_ok": ok, "single_propagation_ms": one * 1e3, "batched_over_single": one * n / dt,
+        "rhs_evals_per_s": float(sum(x.rhs_evals for x in st)) / dt,
+        "steps_per_instance": {"min": float(steps.min()), "mean": float(steps.mean()), "max": float(steps.max())},
+    }
+    print(json.dumps(out))
+
+
 def ppt_tables(args):
     """PPT rate tables (2^16 field values, makePPTcache) built by the GPU kernel
     and by the host loop, for He (φ_0 only) and Ar (m = ±1 terms)."""
@@ -288,6 +336,8 @@ def main():
         return copy_calibration(args)
     if args.workload == "ppt":
         return ppt_tables(args)
+    if args.workload == "gnlse":
+        return gnlse_ensemble(args)
     import __graft_entry__ as ge
     ge.load_package()
     from luna_jl_amd import _capi, configs, runtime
