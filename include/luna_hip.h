@@ -179,7 +179,10 @@ typedef struct lh_gradient_desc {
     const double *dens_rho;   /* [dens_n] density at dens_p                      */
     const double *dens_d;     /* [dens_n] CSpline derivatives                    */
     int32_t loss;             /* 1: complex neff (MarcatiliMode loss = true)     */
-    int32_t reserved;         /* 0                                               */
+    int32_t taper;            /* 1: linear taper a(z) = a0 + (a1 - a0)·clamp(z/L,
+                                 0, 1) (MarcatiliMode with radius a(z),
+                                 src/Capillary.jl:60-81, 202-214, 265-296);
+                                 0: constant radius                               */
     const double *gamma;      /* [n_omega] sellmeier_gas γ(λ) per particle       */
     const double *nwg;        /* complex [n_omega] neff_wg(mode, ω), z-independent */
     const double *pre;        /* complex [n_omega] norm_mode_average's pre(ω)    */
@@ -188,6 +191,15 @@ typedef struct lh_gradient_desc {
     const double *fd;         /* [9][4] β1 = dβ/dω at ω0 by the 8th-order central
                                  stencil: per point ω_j = ω0 + (j-4)h, γ(ω_j),
                                  neff_wg(ω_j) (re, im)                            */
+    /* taper only (otherwise ignored): neff_wg(ω, a) = (U/a)²(1 - iV/a)² (:full)
+     * or (U/a)²/2 + i(U/a)²V/a (:reduced, the second term with loss), with
+     * U = u_nm/k and V = v_n/k, k = ω/c (src/Capillary.jl:188-214)            */
+    double taper_a0, taper_a1; /* radius at z <= 0 and z >= taper_L (m)         */
+    double taper_L;           /* taper length (m)                                */
+    double sqrt_aeff_intg;    /* √(Aeff/a²): Aeff(z) = a(z)²·aeff_intg           */
+    const double *wgU;        /* [n_omega] U                                      */
+    const double *wgV;        /* complex [n_omega] V                              */
+    const double *fdt;        /* [9][3] U, Re V, Im V at the stencil points       */
 } lh_gradient_desc;
 
 /* Attach a gradient fill to a mode-averaged field-grid context (g = NULL:

@@ -62,8 +62,10 @@ class LhGradientDesc(C.Structure):
     """``lh_gradient_desc`` of include/luna_hip.h."""
     _fields_ = [("n_points", C.c_int32), ("two_point", C.c_int32), ("Z", _dp), ("P", _dp),
                 ("dens_n", C.c_int32), ("model", C.c_int32), ("dens_p", _dp), ("dens_rho", _dp),
-                ("dens_d", _dp), ("loss", C.c_int32), ("reserved", C.c_int32), ("gamma", _dp),
-                ("nwg", _dp), ("pre", _dp), ("sqrt_aeff", C.c_double), ("fd_h", C.c_double), ("fd", _dp)]
+                ("dens_d", _dp), ("loss", C.c_int32), ("taper", C.c_int32), ("gamma", _dp),
+                ("nwg", _dp), ("pre", _dp), ("sqrt_aeff", C.c_double), ("fd_h", C.c_double), ("fd", _dp),
+                ("taper_a0", C.c_double), ("taper_a1", C.c_double), ("taper_L", C.c_double),
+                ("sqrt_aeff_intg", C.c_double), ("wgU", _dp), ("wgV", _dp), ("fdt", _dp)]
 
 
 class LhPptDesc(C.Structure):

@@ -57,10 +57,11 @@ class Gradient:
         return float(cspline_eval(self.dens_p, self.dens_rho, self.dens_d, self.pressure(z)))
 
 
-def gradient_spec(grid, mode, gradient, ref_lambda, shock=True):
-    """The arrays of ``lh_gradient_desc`` for a Marcatili mode of constant radius
-    (``mode`` supplies the radius, cladding, model and loss switch; its own fill
-    pressure is not used)."""
+def gradient_spec(grid, mode, gradient, ref_lambda, shock=True, taper=None):
+    """The arrays of ``lh_gradient_desc`` for a Marcatili mode (``mode`` supplies
+    the radius, cladding, model and loss switch; its own fill pressure is not
+    used).  ``taper = (a0, a1, L)``: the radius runs linearly from a0 at z = 0
+    to a1 at z = L (constant beyond), the mode's radius is ignored."""
     if grid.kind != "real":
         raise NotImplementedError("gradient fills are provided on the field grid")
     w = grid.omega
@@ -82,13 +83,31 @@ def gradient_spec(grid, mode, gradient, ref_lambda, shock=True):
     fd[:, 1] = gamma_fun(pd.wlfreq(wj) * 1e6)
     nw = mode.neff_wg(wj)
     fd[:, 2], fd[:, 3] = nw.real, nw.imag
-    return GradientSpec(gradient, mode, gamma, nwg, pre, math.sqrt(mode.Aeff()), h, fd)
+    tp = None
+    if taper is not None:
+        # neff_wg(ω, a) from U = u/k and V = v_n/k (src/Capillary.jl:188-214)
+        U = np.zeros(len(w))
+        V = np.zeros(len(w), dtype=np.complex128)
+        k = w[s] / pd.c
+        U[s] = mode.unm / k
+        V[s] = mode.vn(w[s]) / k
+        kj = wj / pd.c
+        fdt = np.stack([mode.unm / kj, (mode.vn(wj) / kj).real, (mode.vn(wj) / kj).imag], axis=1)
+        tp = dict(a0=float(taper[0]), a1=float(taper[1]), L=float(taper[2]),
+                  sqrt_intg=math.sqrt(mode.aeff_intg), U=U, V=V, fdt=fdt)
+    return GradientSpec(gradient, mode, gamma, nwg, pre, math.sqrt(mode.Aeff()), h, fd, taper=tp)
+
+
+def taper_radius(taper, z):
+    """a(z) of the linear taper (a0, a1, L): a0 + (a1 - a0)·clamp(z/L, 0, 1)."""
+    a0, a1, L = taper
+    return a0 + (a1 - a0) * min(max(z / L, 0.0), 1.0)
 
 
 class GradientSpec:
     """Owns the arrays an ``LhGradientDesc`` points into."""
 
-    def __init__(self, gradient, mode, gamma, nwg, pre, sqrt_aeff, fd_h, fd):
+    def __init__(self, gradient, mode, gamma, nwg, pre, sqrt_aeff, fd_h, fd, taper=None):
         self.gradient = gradient
         self.model = 0 if mode.model == "full" else 1
         self.loss = 1 if mode.loss else 0
@@ -98,14 +117,33 @@ class GradientSpec:
         self.sqrt_aeff = float(sqrt_aeff)
         self.fd_h = float(fd_h)
         self.fd = np.ascontiguousarray(fd, dtype=float)
+        self.taper = taper
+        if taper is not None:
+            self.wgU = np.ascontiguousarray(taper["U"], dtype=float)
+            self.wgV = np.ascontiguousarray(taper["V"], dtype=np.complex128)
+            self.fdt = np.ascontiguousarray(taper["fdt"], dtype=float)
         g = gradient
         self.desc = _capi.LhGradientDesc(
             n_points=len(g.Z), two_point=int(g.two_point), Z=_capi._ptr(g.Z), P=_capi._ptr(g.P),
             dens_n=len(g.dens_p), model=self.model, dens_p=_capi._ptr(g.dens_p),
-            dens_rho=_capi._ptr(g.dens_rho), dens_d=_capi._ptr(g.dens_d), loss=self.loss, reserved=0,
+            dens_rho=_capi._ptr(g.dens_rho), dens_d=_capi._ptr(g.dens_d), loss=self.loss,
+            taper=1 if taper is not None else 0,
             gamma=_capi._ptr(self.gamma), nwg=_capi._ptr(self.nwg.view(np.float64)),
             pre=_capi._ptr(self.pre.view(np.float64)), sqrt_aeff=self.sqrt_aeff, fd_h=self.fd_h,
             fd=_capi._ptr(self.fd))
+        if taper is not None:
+            d = self.desc
+            d.taper_a0, d.taper_a1, d.taper_L = taper["a0"], taper["a1"], taper["L"]
+            d.sqrt_aeff_intg = taper["sqrt_intg"]
+            d.wgU = _capi._ptr(self.wgU)
+            d.wgV = _capi._ptr(self.wgV.view(np.float64))
+            d.fdt = _capi._ptr(self.fdt)
+
+    def radius(self, z):
+        """a(z): the taper's radius, or None for a constant radius."""
+        if self.taper is None:
+            return None
+        return taper_radius((self.taper["a0"], self.taper["a1"], self.taper["L"]), z)
 
     @property
     def ref(self):

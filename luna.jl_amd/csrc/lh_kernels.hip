@@ -93,6 +93,11 @@ struct DInst {
     const double2* zpre;     // [Nω] norm prefactor
     const unsigned char* zsidx;  // [Nω]
     const double* zfd;       // [9][4] β1 stencil (ω, γ, nwg)
+    int ztaper;              // 1: linear taper a(z); neff_wg from zU, zV
+    double za0, za1, zaL, zsqrt_intg;
+    const double* zU;        // [Nω] u_nm/k
+    const double2* zV;       // [Nω] v_n/k
+    const double* zfdt;      // [9][3] U, V at the β1 stencil points
     double2* zL;             // [Nω] linop at the last refresh (= linop)
     double2* zOF;            // [Nω] output factor at the last refresh (= outfac)
 };
@@ -133,7 +138,7 @@ struct SmemT {
     double wsum[5 * LH_NW];      // block_sum3/block_sum4 and the statistics reductions
     double scan[2][3 * LH_NW];   // one-barrier scans and min/max, used in alternation
     double plist[PList<MO>::CAP];
-    double zs[2];  // gradient fill: ρ(z), β1(z) from thread 0
+    double zs[3];  // gradient fill: ρ(z), β1(z), taper radius a(z) from thread 0
     int flag;
 };
 
@@ -1102,6 +1107,30 @@ __device__ __forceinline__ double2 zg_neff(const DInst& in, double eco, double2 
 // bin the operator -i(ω/c·conj_clamp(neff) - ω β1) (src/LinearOps.jl:337-358,
 // 254) and the output factor sc_f·pre/β·√Aeff·ωwin (β = ω/c·Re neff),
 // written to in.zL / in.zOF.  Returns ρ(z) and the Kerr factor ρ ε0 γ3.
+// Radius of the linear taper at z: a0 + (a1 - a0)·clamp(z/L, 0, 1).
+__device__ __forceinline__ double zg_radius(const DInst& in, double z) {
+    double f = z / in.zaL;
+    f = f < 0.0 ? 0.0 : (f > 1.0 ? 1.0 : f);
+    return in.za0 + (in.za1 - in.za0) * f;
+}
+
+// neff_wg(mode, ω; z) at radius a from U = u_nm/k and V = v_n/k
+// (src/Capillary.jl:188-214): (U/a)²(1 - iV/a)² (:full); (U/a)²/2, plus
+// i(U/a)²V/a with loss (:reduced).
+__device__ __forceinline__ double2 zg_nwg_taper(const DInst& in, double U, double2 V, double a) {
+    const double q = U / a, q2 = q * q;
+    if (in.zg_model == 0) {
+        const double tr = 1.0 + V.y / a, ti = -(V.x / a);  // 1 - iV/a
+        return make_double2(q2 * (tr * tr - ti * ti), q2 * (2.0 * tr * ti));
+    }
+    double2 r = make_double2(q2 / 2, 0.0);
+    if (in.zg_loss) {
+        r.x += -(q2 * (V.y / a));
+        r.y += q2 * (V.x / a);
+    }
+    return r;
+}
+
 template <class Smem>
 __device__ __forceinline__ void zg_refresh(const DInst& in, Smem& sm, double z, double& rho, double& kfac) {
     const int tid = fresh_tid();
@@ -1110,27 +1139,37 @@ __device__ __forceinline__ void zg_refresh(const DInst& in, Smem& sm, double z, 
         const double r = cspline_uniform(in.zd, in.zd_n, in.zd[0].x, in.zd[in.zd_n - 1].x, p);
         // β1 = Modes.dispersion(mode, 1, ω0, z): β(ω) = ω/c·Re neff(mode, ω; z)
         // with neff's own clamp (src/Capillary.jl:162-186)
+        const double a = in.ztaper ? zg_radius(in, z) : 0.0;
         double acc = 0.0;
         for (int j = 0; j < 9; j++) {
             const double* f = in.zfd + 4 * j;
             const double sc = sqrt(1 + f[1] * r);
-            double nr = zg_neff(in, sc * sc, make_double2(f[2], f[3])).x;
+            double2 nwj = make_double2(f[2], f[3]);
+            if (in.ztaper) {
+                const double* t = in.zfdt + 3 * j;
+                nwj = zg_nwg_taper(in, t[0], make_double2(t[1], t[2]), a);
+            }
+            double nr = zg_neff(in, sc * sc, nwj).x;
             if (in.zg_model == 0 && nr < 1e-3) nr = 1e-3;
             acc += c_fd8[j] * (f[0] / LH_CLIGHT * nr);
         }
         sm.zs[0] = r;
         sm.zs[1] = acc / in.zfd_h;
+        sm.zs[2] = a;
     }
     __syncthreads();
     rho = sm.zs[0];
     const double b1 = sm.zs[1];
+    const double a = sm.zs[2];
+    const double sqrt_aeff = in.ztaper ? a * in.zsqrt_intg : in.zsqrt_aeff;  // Aeff(z) = a(z)²·aeff_intg
     kfac = rho * LH_EPS0 * in.kerr_gamma3;
     for (int k = tid; k < in.n_omega; k += LH_T) {
         double2 L = make_double2(0.0, 0.0), of = make_double2(in.sc_f, 0.0);
         if (in.zsidx[k]) {
             const double w = in.st_omega[k];
             const double sc = sqrt(1 + in.zgam[k] * rho);  // coren(ω; z)^2
-            const double2 n = zg_neff(in, sc * sc, in.znwg[k]);
+            const double2 nw = in.ztaper ? zg_nwg_taper(in, in.zU[k], in.zV[k], a) : in.znwg[k];
+            const double2 n = zg_neff(in, sc * sc, nw);
             const double ncr = (n.x < 1e-3) ? 1e-3 : n.x;  // conj_clamp
             const double lim = 3000 * LH_CLIGHT / w;
             const double nci = -((n.y < 0.0) ? 0.0 : (n.y > lim ? lim : n.y));
@@ -1139,7 +1178,7 @@ __device__ __forceinline__ void zg_refresh(const DInst& in, Smem& sm, double z, 
             L = make_double2(zi, -zr);
             const double beta = wc * n.x;
             const double2 pre = in.zpre[k];
-            const double nfx = pre.x / beta * in.zsqrt_aeff, nfy = pre.y / beta * in.zsqrt_aeff;
+            const double nfx = pre.x / beta * sqrt_aeff, nfy = pre.y / beta * sqrt_aeff;
             of = make_double2((in.sc_f * nfx) * in.omega_win[k], (in.sc_f * nfy) * in.omega_win[k]);
         }
         in.zL[k] = L;
@@ -1201,6 +1240,14 @@ __device__ __forceinline__ void stats_time(const DInst& in, Smem& sm, double s0,
     const int tid = fresh_tid();
     const int lane = tid & 63, wid = tid >> 6;
     block_sum3(s0, s1, se, sm.wsum);  // ends synchronised: the spectrum is complete
+    // Aeff at this record's z: a taper's a(z)²·aeff_intg from the last refresh
+    // (at z: the step's final stage, or lh_stats_eval's own), else the constant
+    double st_aeff = in.st_aeff, st_ediv = in.st_ediv;
+    if (KIND == 0 && in.zg_np > 0 && in.ztaper) {
+        const double sa = sm.zs[2] * in.zsqrt_intg;
+        st_aeff = sa * sa;
+        st_ediv = sqrt(LH_EPS0 * LH_CLIGHT * st_aeff / 2);
+    }
     {
         // energy windows: reduced and written here, so the sums do not stay live
         const double nan = __longlong_as_double(0x7ff8000000000000LL);
@@ -1236,7 +1283,7 @@ __device__ __forceinline__ void stats_time(const DInst& in, Smem& sm, double s0,
             const double2 e = sm.buf[pad(tid + (c + q) * LH_T)];
             const double re = e.x * inv, im = e.y * inv;
             p[q] = re * re + im * im;
-            aE[q] = fabs(re / in.st_ediv);
+            aE[q] = fabs(re / st_ediv);
             r[q] = 0.0;
         }
         bool need = false;  // any sample at or above the table start (or NaN)
@@ -1360,7 +1407,7 @@ __device__ __forceinline__ void stats_time(const DInst& in, Smem& sm, double s0,
         rec[LH_STAT_OMEGA0] = s1 / s0;
         rec[LH_STAT_ENERGY] = se;
         rec[LH_STAT_PEAKPOWER] = pmax;
-        rec[LH_STAT_PEAKINTENSITY] = pmax / in.st_aeff;
+        rec[LH_STAT_PEAKINTENSITY] = pmax / st_aeff;
         rec[LH_STAT_FWHM_T_MIN] = fmin_;
         rec[LH_STAT_FWHM_T_MAX] = fmax_;
         rec[LH_STAT_ELECTRONDENSITY] = ed;
@@ -1398,7 +1445,9 @@ __device__ __forceinline__ void do_stepfun(const DInst& in, Smem& sm, double2* Y
         stepfun_env<MC, Smem>(in, sm, YN, h, rec, z, dz, rho);
 }
 
-template <int KIND, int MO, int MC, class Smem>
+// ZDEP: compiled with the z-dependent fill (the launch picks it only for a
+// class holding a gradient or taper; the rest run without its registers).
+template <int KIND, int MO, int MC, class Smem, bool ZDEP>
 __device__ __forceinline__ void solve_instance(const DInst& in, Smem& sm) {
     constexpr int N = Sizes<KIND, MO, MC>::N, NPT = Sizes<KIND, MO, MC>::NPT;
     KSet K{in.K[0], in.kstride, 0, 6};
@@ -1415,7 +1464,7 @@ __device__ __forceinline__ void solve_instance(const DInst& in, Smem& sm) {
     // density and Kerr factor of the current z (constant unless a gradient fill
     // re-evaluates them, with the operator, at every new z)
     double rho = in.density, kfac = in.kerr_fac;
-    const bool zdep = (KIND == 0) && in.zg_np > 0;
+    const bool zdep = ZDEP && (KIND == 0) && in.zg_np > 0;
     if (zdep) zg_refresh(in, sm, in.z0, rho, kfac);
 
     // PreconStepper init: k1 = fbar(y0, t, t)
@@ -1617,12 +1666,12 @@ __device__ __forceinline__ void solve_instance(const DInst& in, Smem& sm) {
 }
 
 // One persistent kernel per grid size: one workgroup per instance.
-template <int KIND, int MO, int MC>
+template <int KIND, int MO, int MC, bool ZDEP>
 __global__ __launch_bounds__(LH_T) void lh_solve_kernel(const DInst* insts, const double2* __restrict__ twg) {
     __shared__ SmemT<MO> sm;
     tw_load(sm.tw, twg);
     __syncthreads();
-    solve_instance<KIND, MO, MC, SmemT<MO>>(insts[blockIdx.x], sm);
+    solve_instance<KIND, MO, MC, SmemT<MO>, ZDEP>(insts[blockIdx.x], sm);
 }
 
 template <int KIND, int MO, int MC, class Smem>
@@ -1696,13 +1745,22 @@ static int size_class_of(int grid_kind, int n_to, int n_t) {
     return -1;
 }
 
-static void launch_solve(int cls, int n, const DInst* dinst, const double2* tw, hipStream_t s) {
+template <int KIND, int MO, int MC>
+static void launch_solve_k(bool zdep, int n, const DInst* dinst, const double2* tw, hipStream_t s) {
+    if (KIND == 0 && zdep)
+        hipLaunchKernelGGL((lh_solve_kernel<KIND, MO, MC, true>), dim3(n), dim3(LH_T), 0, s, dinst, tw);
+    else
+        hipLaunchKernelGGL((lh_solve_kernel<KIND, MO, MC, false>), dim3(n), dim3(LH_T), 0, s, dinst, tw);
+}
+
+// zdep: some instance of the class has a z-dependent fill (field grid only).
+static void launch_solve(int cls, int n, const DInst* dinst, const double2* tw, hipStream_t s, bool zdep) {
     switch (cls) {
-        case LH_CLASS_16384: hipLaunchKernelGGL((lh_solve_kernel<0, 8192, 4096>), dim3(n), dim3(LH_T), 0, s, dinst, tw); break;
-        case LH_CLASS_8192: hipLaunchKernelGGL((lh_solve_kernel<0, 4096, 2048>), dim3(n), dim3(LH_T), 0, s, dinst, tw); break;
-        case LH_CLASS_4096: hipLaunchKernelGGL((lh_solve_kernel<0, 2048, 1024>), dim3(n), dim3(LH_T), 0, s, dinst, tw); break;
-        case LH_CLASS_ENV8192: hipLaunchKernelGGL((lh_solve_kernel<1, 8192, 8192>), dim3(n), dim3(LH_T), 0, s, dinst, tw); break;
-        default: hipLaunchKernelGGL((lh_solve_kernel<1, 4096, 4096>), dim3(n), dim3(LH_T), 0, s, dinst, tw); break;
+        case LH_CLASS_16384: launch_solve_k<0, 8192, 4096>(zdep, n, dinst, tw, s); break;
+        case LH_CLASS_8192: launch_solve_k<0, 4096, 2048>(zdep, n, dinst, tw, s); break;
+        case LH_CLASS_4096: launch_solve_k<0, 2048, 1024>(zdep, n, dinst, tw, s); break;
+        case LH_CLASS_ENV8192: launch_solve_k<1, 8192, 8192>(false, n, dinst, tw, s); break;
+        default: launch_solve_k<1, 4096, 4096>(false, n, dinst, tw, s); break;
     }
 }
 
@@ -1808,6 +1866,10 @@ struct lh_ctx {
     double2 *znwg, *zpre;              // shared
     unsigned char* zsidx;              // shared
     double2 *zL, *zOF;                 // per context
+    int zg_taper;                      // linear taper
+    double zg_a0, zg_a1, zg_aL, zg_sqrt_intg;
+    double *zU, *zfdt;                 // shared
+    double2* zV;                       // shared
 };
 
 static std::mutex g_mu;
@@ -2045,6 +2107,14 @@ static DInst base_inst(const lh_ctx* c) {
         in.zpre = c->zpre;
         in.zsidx = c->zsidx;
         in.zfd = c->zfd;
+        in.ztaper = c->zg_taper;
+        in.za0 = c->zg_a0;
+        in.za1 = c->zg_a1;
+        in.zaL = c->zg_aL;
+        in.zsqrt_intg = c->zg_sqrt_intg;
+        in.zU = c->zU;
+        in.zV = c->zV;
+        in.zfdt = c->zfdt;
         in.zL = c->zL;
         in.zOF = c->zOF;
         in.linop = c->zL;  // refreshed in place at every new z
@@ -2551,7 +2621,10 @@ int lh_batch_solve_resident(lh_ctx* const* ctxs, int n, const lh_rk_opts* o, int
                     st = rad_solve(ctxs[i], o, want_saves != 0, cs[c], &rad_stats[i]);
             if (st) break;
         } else {
-            launch_solve(c, count[c], dinst + first[c], tw, cs[c]);
+            bool zdep = false;
+            for (int i = 0; i < n; i++)
+                if (ctx_class(ctxs[i]) == c && ctxs[i]->zg_on) zdep = true;
+            launch_solve(c, count[c], dinst + first[c], tw, cs[c], zdep);
         }
         if (hipGetLastError() != hipSuccess || hipEventRecord(t1[c], cs[c]) != hipSuccess ||
             hipEventRecord(done[c], cs[c]) != hipSuccess || hipStreamWaitEvent(s0, done[c], 0) != hipSuccess)
@@ -2636,11 +2709,15 @@ int lh_set_step_stats(lh_ctx* c, const lh_stats_desc* sd) {
 
 static void zg_release(lh_ctx* c) {
     for (const void* p : {(const void*)c->zg_Z, (const void*)c->zg_P, (const void*)c->zfd, (const void*)c->zgam,
-                          (const void*)c->zd, (const void*)c->znwg, (const void*)c->zpre})
+                          (const void*)c->zd, (const void*)c->znwg, (const void*)c->zpre, (const void*)c->zU,
+                          (const void*)c->zV, (const void*)c->zfdt})
         shared_release(p);
     c->zg_Z = c->zg_P = c->zfd = c->zgam = nullptr;
     c->zd = nullptr;
     c->znwg = c->zpre = nullptr;
+    c->zU = c->zfdt = nullptr;
+    c->zV = nullptr;
+    c->zg_taper = 0;
 }
 
 int lh_set_gradient(lh_ctx* c, const lh_gradient_desc* g) {
@@ -2655,7 +2732,10 @@ int lh_set_gradient(lh_ctx* c, const lh_gradient_desc* g) {
     if (d.transform != LH_TRANS_MODEAVG || d.grid_kind != LH_GRID_REAL || !c->omega || !c->zsidx) return LH_E_ARG;
     if (g->n_points < 2 || !g->Z || !g->P || g->dens_n < 4 || !g->dens_p || !g->dens_rho || !g->dens_d ||
         !g->gamma || !g->nwg || !g->pre || !g->fd || !(g->fd_h > 0) || !(g->sqrt_aeff > 0) ||
-        (g->model != 0 && g->model != 1) || (g->two_point && g->n_points != 2))
+        (g->model != 0 && g->model != 1) || (g->two_point && g->n_points != 2) || (g->taper != 0 && g->taper != 1))
+        return LH_E_ARG;
+    if (g->taper && (!(g->taper_a0 > 0) || !(g->taper_a1 > 0) || !(g->taper_L > 0) || !(g->sqrt_aeff_intg > 0) ||
+                     !g->wgU || !g->wgV || !g->fdt))
         return LH_E_ARG;
     for (int i = 0; i + 1 < g->n_points; i++)
         if (!(g->Z[i] < g->Z[i + 1])) return LH_E_ARG;
@@ -2672,6 +2752,9 @@ int lh_set_gradient(lh_ctx* c, const lh_gradient_desc* g) {
     if (!st) st = shared_upload(c->device, &c->znwg, g->nwg, N);
     if (!st) st = shared_upload(c->device, &c->zpre, g->pre, N);
     if (!st) st = shared_upload(c->device, &c->zfd, g->fd, (size_t)36);
+    if (!st && g->taper) st = shared_upload(c->device, &c->zU, g->wgU, N);
+    if (!st && g->taper) st = shared_upload(c->device, &c->zV, g->wgV, N);
+    if (!st && g->taper) st = shared_upload(c->device, &c->zfdt, g->fdt, (size_t)27);
     if (!st && !c->zL && hipMalloc(&c->zL, sizeof(double2) * N) != hipSuccess) st = LH_E_NOMEM;
     if (!st && !c->zOF && hipMalloc(&c->zOF, sizeof(double2) * N) != hipSuccess) st = LH_E_NOMEM;
     if (st) {
@@ -2686,6 +2769,11 @@ int lh_set_gradient(lh_ctx* c, const lh_gradient_desc* g) {
     c->zd_n = g->dens_n;
     c->zsqrt_aeff = g->sqrt_aeff;
     c->zfd_h = g->fd_h;
+    c->zg_taper = g->taper;
+    c->zg_a0 = g->taper_a0;
+    c->zg_a1 = g->taper_a1;
+    c->zg_aL = g->taper_L;
+    c->zg_sqrt_intg = g->sqrt_aeff_intg;
     c->zg_on = 1;
     return LH_OK;
 }

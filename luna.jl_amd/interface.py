@@ -96,6 +96,15 @@ def prop_capillary_args(radius, flength, gas, pressure, *, lambda0, lambda_lims,
     plasma = True if plasma is None else plasma
     grid = RealGrid(flength, lambda0, lambda_lims, trange)
     grad = None
+    taper = None
+    if isinstance(radius, (tuple, list)):
+        # a linear taper from radius[0] at z = 0 to radius[1] at the fibre's end
+        # (the reference takes any a(z), src/Capillary.jl:60-81); it runs
+        # through the z-dependent fill, so a constant pressure becomes a flat one
+        taper = (float(radius[0]), float(radius[1]), float(flength))
+        radius = taper[0]
+        if not isinstance(pressure, (tuple, list)):
+            pressure = (float(pressure), float(pressure))
     if isinstance(pressure, (tuple, list)):
         if np.ndim(pressure[0]) == 0:
             grad = Gradient.linear(gas, flength, float(pressure[0]), float(pressure[1]), T=temperature)
@@ -122,7 +131,7 @@ def prop_capillary_args(radius, flength, gas, pressure, *, lambda0, lambda_lims,
     linop, beta, beta1 = make_const_linop(grid, mode, grid.ref_lambda)
     aeff = mode.Aeff()
     norm = norm_mode_average(grid, beta, aeff)
-    spec = gradient_spec(grid, mode, grad, grid.ref_lambda) if grad is not None else None
+    spec = gradient_spec(grid, mode, grad, grid.ref_lambda, taper=taper) if grad is not None else None
     transform = TransModeAvg(grid, resp, rho, norm, aeff, gradient=spec)
     Ew = sum(f(grid) for f in inputs)
     stats = Stats.default(grid, Ew, mode, linop, transform, gas=gas)

@@ -175,9 +175,10 @@ def default(grid, Ew, mode, linop, transform, *, windows=None, gas=None, onaxis=
         from .modes import MarcatiliMode, zdw as mode_zdw
         spec = getattr(transform, "gradient", None)
         if spec is not None and isinstance(mode, MarcatiliMode):
-            g = spec.gradient  # the mode at the fill pressure of each z
-            zdw = lambda z: mode_zdw(MarcatiliMode(mode.a, g.gas, g.pressure(z), n=mode.n, m=mode.m,
-                                                   kind=mode.kind, T=g.T, model=mode.model, loss=mode.loss))
+            g = spec.gradient  # the mode at the fill pressure (and taper radius) of each z
+            zdw = lambda z: mode_zdw(MarcatiliMode(spec.radius(z) or mode.a, g.gas, g.pressure(z), n=mode.n,
+                                                   m=mode.m, kind=mode.kind, T=g.T, model=mode.model,
+                                                   loss=mode.loss))
         else:
             zdw = mode_zdw(mode)
     return StatsFun(grid, transform, capacity=capacity, gas=gas, zdw=zdw, windows=windows, aeff=aeff)

@@ -929,17 +929,48 @@ static const double FD8_[9] = {1.0 / 280, -4.0 / 105, 1.0 / 5, -4.0 / 5, 0.0, 4.
  *   β1(z) = dβ/dω at ω0 with β = ω/c*real(neff(mode, ω; z)) (8th-order stencil);
  * and norm_mode_average's factor pre/β(ω, z)*sqrt(aeff) (src/NonlinearRHS.jl:375-387).
  * L, nf, rho may be NULL. */
+/* radius of the linear taper at z: a0 + (a1 - a0)·clamp(z/L, 0, 1) */
+static double taper_a(const lh_gradient_desc *g, double z) {
+    double f = z / g->taper_L;
+    f = f < 0 ? 0 : (f > 1 ? 1 : f);
+    return g->taper_a0 + (g->taper_a1 - g->taper_a0) * f;
+}
+
+/* neff_wg(m, ω; z) at radius a from U = u_nm/k and V = v_n/k
+ * (src/Capillary.jl:188-214): (u/(ka))²(1 - i v_n/(ka))² for :full,
+ * c²u²/(2ω²a²) [+ i c³u² v_n/(a³ω³) with loss] for :reduced. */
+static cplx taper_nwg(const lh_gradient_desc *g, double U, cplx V, double a) {
+    double q = U / a;
+    if (g->model == 0) {
+        cplx t = 1 - I * V / a;
+        return (q * q) * (t * t);
+    }
+    cplx r = q * q / 2;
+    if (g->loss) r += I * (q * q) * (V / a);
+    return r;
+}
+
 static void grad_constants(const lh_desc *d, const lh_gradient_desc *g, double z, cplx *L, cplx *nf, double *rho) {
     double r = cspline_eval(g->dens_n, g->dens_p, g->dens_rho, g->dens_d, grad_pressure(g, z));
     if (rho) *rho = r;
     int N = d->n_omega;
-    const cplx *nwg = (const cplx *)g->nwg, *pre = (const cplx *)g->pre;
+    const cplx *nwg0 = (const cplx *)g->nwg, *pre = (const cplx *)g->pre;
+    const double a = g->taper ? taper_a(g, z) : 0.0;
+    const double sqrt_aeff = g->taper ? a * g->sqrt_aeff_intg : g->sqrt_aeff;
+    cplx *nwg = (cplx *)malloc(sizeof(cplx) * (size_t)N);
+    for (int i = 0; i < N; i++)
+        nwg[i] = g->taper ? taper_nwg(g, g->wgU[i], ((const cplx *)g->wgV)[i], a) : nwg0[i];
     if (L) {
         double acc = 0.0;
         for (int j = 0; j < 9; j++) {
             const double *f = g->fd + 4 * j;
             double co = sqrt(1 + f[1] * r);
-            double nr = creal(grad_neff(g, co * co, f[2], f[3]));
+            cplx nwj = f[2] + I * f[3];
+            if (g->taper) {
+                const double *t = g->fdt + 3 * j;
+                nwj = taper_nwg(g, t[0], t[1] + I * t[2], a);
+            }
+            double nr = creal(grad_neff(g, co * co, creal(nwj), cimag(nwj)));
             if (g->model == 0 && nr < 1e-3) nr = 1e-3; /* neff(mode, ω; z)'s clamp */
             acc += FD8_[j] * (f[0] / ORC_C_LIGHT * nr);
         }
@@ -966,9 +997,10 @@ static void grad_constants(const lh_desc *d, const lh_gradient_desc *g, double z
             double w = d->omega[i];
             double co = sqrt(1 + g->gamma[i] * r);
             double beta = w / ORC_C_LIGHT * creal(grad_neff(g, co * co, creal(nwg[i]), cimag(nwg[i])));
-            nf[i] = (creal(pre[i]) / beta * g->sqrt_aeff) + I * (cimag(pre[i]) / beta * g->sqrt_aeff);
+            nf[i] = (creal(pre[i]) / beta * sqrt_aeff) + I * (cimag(pre[i]) / beta * sqrt_aeff);
         }
     }
+    free(nwg);
 }
 
 /* Luna.run + solve_precon + stepfun filter + GridCondition saves. */
@@ -1030,8 +1062,13 @@ static int solve_impl(const lh_desc *d, double *Ew_inout, const lh_rk_opts *o, d
             /* MemoryOutput: statistics of the filtered state, src/Output.jl:65-67 */
             if (rec && stats->accepted <= cap) {
                 lh_desc dz = *d; /* density(z) at tn */
+                double aeff_z = aeff; /* aeff(z) of a taper */
                 if (g) grad_constants(d, g, s.tn, NULL, NULL, &dz.density);
-                st = step_stats(&dz, tgrid, aeff, nwin, win, s.yn, s.tn, s.dtn,
+                if (g && g->taper) {
+                    double sa = taper_a(g, s.tn) * g->sqrt_aeff_intg;
+                    aeff_z = sa * sa;
+                }
+                st = step_stats(&dz, tgrid, aeff_z, nwin, win, s.yn, s.tn, s.dtn,
                                 rec + (size_t)(stats->accepted - 1) * LH_NSTAT);
                 *nrec = (int)stats->accepted;
             }
@@ -1097,6 +1134,14 @@ int orc_solve_stats(const lh_desc *d, double *Ew_inout, const lh_rk_opts *o, dou
                     lh_stats *stats, const double *t, double aeff, int nwin, const double *win, double *rec, int cap,
                     int *nrec) {
     return solve_impl(d, Ew_inout, o, saves, zsaves, stats, t, aeff, nwin, win, rec, cap, nrec, NULL);
+}
+
+/* Same with a gradient fill (and taper): ρ(z), the operator, the
+ * normalisation and the statistics' Aeff at each record's z. */
+int orc_solve_grad_stats(const lh_desc *d, const lh_gradient_desc *g, double *Ew_inout, const lh_rk_opts *o,
+                         double *saves, double *zsaves, lh_stats *stats, const double *t, double aeff, int nwin,
+                         const double *win, double *rec, int cap, int *nrec) {
+    return solve_impl(d, Ew_inout, o, saves, zsaves, stats, t, aeff, nwin, win, rec, cap, nrec, g);
 }
 
 /* One record of a given state (the stats closure called directly). */

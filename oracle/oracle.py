@@ -41,6 +41,9 @@ def lib(capi):
     L.orc_solve_stats.argtypes = [D, dp, C.POINTER(capi.LhRkOpts), dp, dp, C.POINTER(capi.LhStats), dp,
                                   C.c_double, C.c_int, dp, dp, C.c_int, C.POINTER(C.c_int)]
     L.orc_fwhm.argtypes = [C.c_int, dp, dp, C.c_int, dp]
+    L.orc_solve_grad_stats.argtypes = [D, C.POINTER(capi.LhGradientDesc), dp, C.POINTER(capi.LhRkOpts), dp, dp,
+                                       C.POINTER(capi.LhStats), dp, C.c_double, C.c_int, dp, dp, C.c_int,
+                                       C.POINTER(C.c_int)]
     G = C.POINTER(capi.LhGradientDesc)
     L.orc_rhs_grad.argtypes = [D, G, dp, dp, C.c_double]
     L.orc_solve_grad.argtypes = [D, G, dp, C.POINTER(capi.LhRkOpts), dp, dp, C.POINTER(capi.LhStats)]
@@ -174,7 +177,7 @@ class Oracle:
                                    _p(rec))
         return st, rec
 
-    def solve_stats(self, holder, Ew0, opts, t, aeff, cap=4096, want_saves=False, windows=None):
+    def solve_stats(self, holder, Ew0, opts, t, aeff, cap=4096, want_saves=False, windows=None, spec=None):
         """``solve`` that also returns the per-step statistics records [n][LH_NSTAT]."""
         d = holder.desc
         n = d.n_omega * d.n_r
@@ -186,9 +189,14 @@ class Oracle:
         rec = np.zeros((cap, self.capi.LH_NSTAT))
         nrec = C.c_int(0)
         nw, w = self._windows(windows)
-        st = self.L.orc_solve_stats(holder.ref, _p(Ew), C.byref(opts), _p(saves) if want_saves else None,
-                                    _p(zs), C.byref(stats), _p(t), float(aeff), nw, _p(w) if nw else None, _p(rec),
-                                    cap, C.byref(nrec))
+        if spec is not None:  # a gradient fill (GradientSpec)
+            st = self.L.orc_solve_grad_stats(holder.ref, spec.ref, _p(Ew), C.byref(opts),
+                                             _p(saves) if want_saves else None, _p(zs), C.byref(stats), _p(t),
+                                             float(aeff), nw, _p(w) if nw else None, _p(rec), cap, C.byref(nrec))
+        else:
+            st = self.L.orc_solve_stats(holder.ref, _p(Ew), C.byref(opts), _p(saves) if want_saves else None,
+                                        _p(zs), C.byref(stats), _p(t), float(aeff), nw, _p(w) if nw else None,
+                                        _p(rec), cap, C.byref(nrec))
         return st, Ew, stats, rec[:nrec.value]
 
     def solve_precon(self, f, linop, y0, t0, dt0, tmax, rtol=1e-6, atol=1e-10, outputN=201):

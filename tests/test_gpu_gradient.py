@@ -86,3 +86,42 @@ def test_batch_with_gradient_and_stats(capi):
     g = insts[0][3].gradient.gradient
     for r in rec:
         assert r[capi.STAT_KEYS.index("density")] == pytest.approx(g.density(r[0]), rel=1e-12)
+
+
+def setup_taper(radius, pressure, flength, trange=0.5e-12, energy=150e-6):
+    from luna_jl_amd import interface
+    Ew, grid, linop, tr, _, out = interface.prop_capillary_args(radius, flength, "He", pressure, trange=trange,
+                                                                energy=energy, **KW)
+    return Ew, grid, linop, tr, tr.descriptor(linop)
+
+
+def test_taper_rhs_solve_and_stats(orc, capi):
+    """A linear taper a(z) with a pressure gradient: one RHS at several z
+    (1e-12), a 5 cm propagation (same steps, 1e-9) and its per-step records
+    (Aeff(z) in the peak intensity) against the oracle."""
+    print("case: taper 125 -> 90 um with a gradient", flush=True)
+    from luna_jl_amd import capillary, runtime, stats
+    L = 0.05
+    Ew, grid, linop, tr, h = setup_taper((125e-6, 90e-6), (0.5, 2.5), L)
+    ctx = runtime.Context(h)
+    for z in (0.0, 0.013, 0.04, 0.05, 0.08):
+        nl = ctx.rhs(Ew * 2.0, z)
+        st, ref = orc.rhs_grad(h, tr.gradient, Ew * 2.0, z)
+        assert st == 0
+        assert np.linalg.norm(nl - ref) / np.linalg.norm(ref) <= 1e-12, z
+    sf = stats.StatsFun(grid, tr, capacity=2048)
+    ctx.set_step_stats(sf)
+    opts = capi.rk_opts(L, save_n=6)
+    Eg, sv, zs, st_g = ctx.solve(Ew, opts)
+    rec = ctx.step_stats()
+    st, Er, st_r, rec_r = orc.solve_stats(h, Ew, opts, grid.t, tr.aeff, spec=tr.gradient)
+    assert st == 0 and st_g.status == 0
+    assert (st_g.steps, st_g.accepted) == (st_r.steps, st_r.accepted) and len(rec) == len(rec_r)
+    assert spec_err(Eg, Er, grid.sidx) <= 1e-9
+    k = {n: i for i, n in enumerate(capi.STAT_KEYS)}
+    intg = tr.gradient.taper["sqrt_intg"] ** 2
+    for r, q in zip(rec, rec_r):
+        for key in ("energy", "peakpower", "peakintensity", "density"):
+            assert r[k[key]] == pytest.approx(q[k[key]], rel=1e-7), key
+        a = capillary.taper_radius((125e-6, 90e-6, L), r[k["z"]])
+        assert r[k["peakintensity"]] == pytest.approx(r[k["peakpower"]] / (a * a * intg), rel=1e-12)

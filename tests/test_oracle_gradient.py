@@ -106,3 +106,63 @@ def test_linear_propagation_uses_later_time(orc, capi, pkg):
     err = np.linalg.norm((Eend - later)[big]) / np.linalg.norm(later[big])
     assert err < 1e-9
     assert np.linalg.norm((Eend - earlier)[big]) / np.linalg.norm(earlier[big]) > 1e-3
+
+
+def setup_taper(radius, pressure, flength, **kw):
+    from luna_jl_amd import interface
+    args = dict(KW, **kw)
+    Ew, grid, linop, tr, _, out = interface.prop_capillary_args(radius, flength, "He", pressure, **args)
+    return Ew, grid, linop, tr, tr.descriptor(linop)
+
+
+@pytest.mark.parametrize("model", ["full"])
+def test_taper_constants_match_mode_at_radius(orc, pkg, model):
+    """A linear taper a(z) (MarcatiliMode with radius a(z), src/Capillary.jl:
+    60-81, 188-214, 265-296) with a pressure gradient: the operator and the
+    normalisation at z equal the constant-fill ones of the host mode model at
+    radius a(z) and pressure p(z).  (Full model only: for the reduced model the
+    reference's z-dependent operator, neff_β_grid → neff(m, εco, nwg),
+    src/Capillary.jl:216-234, 345-359, subtracts the loss term of neff_wg that
+    neff(m, ω) adds, so conj_clamp drops the loss there; the oracle and the
+    device follow the z-dependent path.)"""
+    from luna_jl_amd import capillary, linops, modes, nonlinearrhs
+    L = 1.0
+    Ew, grid, linop, tr, h = setup_taper((150e-6, 75e-6), (0.5, 2.0), L, model=model)
+    s = grid.sidx
+    for z in (0.0, 0.3, 0.77, 1.0, 1.2):
+        Lz, nf, rho = orc.grad_constants(h, tr.gradient, z)
+        a = capillary.taper_radius((150e-6, 75e-6, L), z)
+        m = modes.MarcatiliMode(a, "He", tr.gradient.gradient.pressure(z), model=model)
+        Lc, beta, b1 = linops.make_const_linop(grid, m, 800e-9)
+        nfc = nonlinearrhs.norm_mode_average(grid, beta, m.Aeff())
+        assert np.linalg.norm(Lz[s] - Lc[s]) / np.linalg.norm(Lc[s]) < 1e-7, z
+        assert np.max(np.abs(Lz[s] - Lc[s])) < 1e-5, z
+        assert np.max(np.abs(nf[s] - nfc[s]) / np.abs(nfc[s])) < 1e-12, z
+
+
+def test_flat_taper_is_constant_radius(orc, capi, pkg):
+    Ewt, grid, lt, trt, ht = setup_taper((125e-6, 125e-6), 1.3, 0.05)
+    Ewg, _, lg, trg, hg = setup((1.3, 1.3), flength=0.05)
+    opts = capi.rk_opts(0.05, save_n=3)
+    st, Et, sv, zs, s1 = orc.solve_grad(ht, trt.gradient, Ewt, opts)
+    st2, Eg, sv2, zs2, s2 = orc.solve_grad(hg, trg.gradient, Ewg, opts)
+    assert st == st2 == 0 and s1.steps == s2.steps
+    s = grid.sidx
+    pa, pb = np.abs(Et[s]) ** 2, np.abs(Eg[s]) ** 2
+    assert np.linalg.norm(pa - pb) / np.linalg.norm(pb) < 1e-10
+
+
+def test_taper_statistics_use_aeff_at_z(orc, capi, pkg):
+    """Stats.default's peakintensity = peakpower/aeff(z) (src/Stats.jl:150-154)
+    with the taper's Aeff(z) = a(z)²·aeff_intg at each record's z."""
+    from luna_jl_amd import capillary, modes
+    L = 0.05
+    Ew, grid, linop, tr, h = setup_taper((125e-6, 100e-6), 1.0, L)
+    opts = capi.rk_opts(L, save_n=2)
+    st, E, stats, rec = orc.solve_stats(h, Ew, opts, grid.t, tr.aeff, spec=tr.gradient)
+    assert st == 0 and len(rec) == stats.accepted
+    intg = modes.MarcatiliMode(125e-6, "He", 1.0).aeff_intg
+    k = {n: i for i, n in enumerate(capi.STAT_KEYS)}
+    for r in rec:
+        a = capillary.taper_radius((125e-6, 100e-6, L), r[k["z"]])
+        assert r[k["peakintensity"]] == pytest.approx(r[k["peakpower"]] / (a * a * intg), rel=1e-13)
