@@ -454,7 +454,7 @@ def main():
         "batched_over_plain": value / plain_pph,
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": 78.6, "unit": "TFLOP/s",
                      "frac": achieved / 78.6, "traffic": measured_traffic(),
-                     "kernel": "lh_solve_kernel<8192,4096> (Nto=16384 class)", "kernel_ms": kernel_ms,
+                     "kernel": "lh_solve_kernel<0,8192,4096,false> (Nto=16384 class)", "kernel_ms": kernel_ms,
                      "algorithmic_bytes": sum(st.rhs_evals * 2 * c.params["n_omega"] * 16 for st, c in big)},
         "cpu_baseline": cpu,
     }
