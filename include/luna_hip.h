@@ -180,7 +180,9 @@ typedef struct lh_gradient_desc {
     const double *dens_d;     /* [dens_n] CSpline derivatives                    */
     int32_t loss;             /* 1: complex neff (MarcatiliMode loss = true)     */
     int32_t taper;            /* 1: linear taper a(z) = a0 + (a1 - a0)·clamp(z/L,
-                                 0, 1) (MarcatiliMode with radius a(z),
+                                 0, 1); 2: piecewise-linear a(z) through the
+                                 nodes (taper_Z, taper_A), constant outside
+                                 (MarcatiliMode with radius a(z),
                                  src/Capillary.jl:60-81, 202-214, 265-296);
                                  0: constant radius                               */
     const double *gamma;      /* [n_omega] sellmeier_gas γ(λ) per particle       */
@@ -200,6 +202,10 @@ typedef struct lh_gradient_desc {
     const double *wgU;        /* [n_omega] U                                      */
     const double *wgV;        /* complex [n_omega] V                              */
     const double *fdt;        /* [9][3] U, Re V, Im V at the stencil points       */
+    int32_t taper_n;          /* taper = 2: nodes (>= 2)                          */
+    int32_t taper_reserved;   /* 0                                               */
+    const double *taper_Z;    /* [taper_n] positions (m), increasing              */
+    const double *taper_A;    /* [taper_n] radii (m)                              */
 } lh_gradient_desc;
 
 /* Attach a gradient fill to a mode-averaged field-grid context (g = NULL:

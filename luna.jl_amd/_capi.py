@@ -65,7 +65,8 @@ class LhGradientDesc(C.Structure):
                 ("dens_d", _dp), ("loss", C.c_int32), ("taper", C.c_int32), ("gamma", _dp),
                 ("nwg", _dp), ("pre", _dp), ("sqrt_aeff", C.c_double), ("fd_h", C.c_double), ("fd", _dp),
                 ("taper_a0", C.c_double), ("taper_a1", C.c_double), ("taper_L", C.c_double),
-                ("sqrt_aeff_intg", C.c_double), ("wgU", _dp), ("wgV", _dp), ("fdt", _dp)]
+                ("sqrt_aeff_intg", C.c_double), ("wgU", _dp), ("wgV", _dp), ("fdt", _dp),
+                ("taper_n", C.c_int32), ("taper_reserved", C.c_int32), ("taper_Z", _dp), ("taper_A", _dp)]
 
 
 class LhPptDesc(C.Structure):

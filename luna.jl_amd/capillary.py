@@ -61,7 +61,9 @@ def gradient_spec(grid, mode, gradient, ref_lambda, shock=True, taper=None):
     """The arrays of ``lh_gradient_desc`` for a Marcatili mode (``mode`` supplies
     the radius, cladding, model and loss switch; its own fill pressure is not
     used).  ``taper = (a0, a1, L)``: the radius runs linearly from a0 at z = 0
-    to a1 at z = L (constant beyond), the mode's radius is ignored."""
+    to a1 at z = L (constant beyond); ``taper = (Z, A)`` (two sequences): the
+    piecewise-linear radius through those nodes, constant outside.  With a
+    taper the mode's radius is ignored."""
     if grid.kind != "real":
         raise NotImplementedError("gradient fills are provided on the field grid")
     w = grid.omega
@@ -93,13 +95,26 @@ def gradient_spec(grid, mode, gradient, ref_lambda, shock=True, taper=None):
         V[s] = mode.vn(w[s]) / k
         kj = wj / pd.c
         fdt = np.stack([mode.unm / kj, (mode.vn(wj) / kj).real, (mode.vn(wj) / kj).imag], axis=1)
-        tp = dict(a0=float(taper[0]), a1=float(taper[1]), L=float(taper[2]),
-                  sqrt_intg=math.sqrt(mode.aeff_intg), U=U, V=V, fdt=fdt)
+        tp = dict(sqrt_intg=math.sqrt(mode.aeff_intg), U=U, V=V, fdt=fdt)
+        if len(taper) == 2:
+            tp.update(kind=2, Z=np.ascontiguousarray(taper[0], dtype=float),
+                      A=np.ascontiguousarray(taper[1], dtype=float))
+        else:
+            tp.update(kind=1, a0=float(taper[0]), a1=float(taper[1]), L=float(taper[2]))
     return GradientSpec(gradient, mode, gamma, nwg, pre, math.sqrt(mode.Aeff()), h, fd, taper=tp)
 
 
 def taper_radius(taper, z):
-    """a(z) of the linear taper (a0, a1, L): a0 + (a1 - a0)·clamp(z/L, 0, 1)."""
+    """a(z) of a linear taper (a0, a1, L): a0 + (a1 - a0)·clamp(z/L, 0, 1), or
+    of a piecewise-linear one (Z, A), constant outside the nodes."""
+    if len(taper) == 2:
+        Z, A = np.asarray(taper[0], dtype=float), np.asarray(taper[1], dtype=float)
+        if z <= Z[0]:
+            return float(A[0])
+        if z >= Z[-1]:
+            return float(A[-1])
+        i = int(np.searchsorted(Z, z, side="left")) - 1  # Z[i] < z <= Z[i+1]
+        return float(A[i] + (A[i + 1] - A[i]) * ((z - Z[i]) / (Z[i + 1] - Z[i])))
     a0, a1, L = taper
     return a0 + (a1 - a0) * min(max(z / L, 0.0), 1.0)
 
@@ -127,13 +142,18 @@ class GradientSpec:
             n_points=len(g.Z), two_point=int(g.two_point), Z=_capi._ptr(g.Z), P=_capi._ptr(g.P),
             dens_n=len(g.dens_p), model=self.model, dens_p=_capi._ptr(g.dens_p),
             dens_rho=_capi._ptr(g.dens_rho), dens_d=_capi._ptr(g.dens_d), loss=self.loss,
-            taper=1 if taper is not None else 0,
+            taper=taper["kind"] if taper is not None else 0,
             gamma=_capi._ptr(self.gamma), nwg=_capi._ptr(self.nwg.view(np.float64)),
             pre=_capi._ptr(self.pre.view(np.float64)), sqrt_aeff=self.sqrt_aeff, fd_h=self.fd_h,
             fd=_capi._ptr(self.fd))
         if taper is not None:
             d = self.desc
-            d.taper_a0, d.taper_a1, d.taper_L = taper["a0"], taper["a1"], taper["L"]
+            if taper["kind"] == 1:
+                d.taper_a0, d.taper_a1, d.taper_L = taper["a0"], taper["a1"], taper["L"]
+            else:
+                self.taper_Z, self.taper_A = taper["Z"], taper["A"]
+                d.taper_n = len(self.taper_Z)
+                d.taper_Z, d.taper_A = _capi._ptr(self.taper_Z), _capi._ptr(self.taper_A)
             d.sqrt_aeff_intg = taper["sqrt_intg"]
             d.wgU = _capi._ptr(self.wgU)
             d.wgV = _capi._ptr(self.wgV.view(np.float64))
@@ -143,6 +163,8 @@ class GradientSpec:
         """a(z): the taper's radius, or None for a constant radius."""
         if self.taper is None:
             return None
+        if self.taper["kind"] == 2:
+            return taper_radius((self.taper["Z"], self.taper["A"]), z)
         return taper_radius((self.taper["a0"], self.taper["a1"], self.taper["L"]), z)
 
     @property

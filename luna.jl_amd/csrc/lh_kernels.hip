@@ -93,8 +93,11 @@ struct DInst {
     const double2* zpre;     // [Nω] norm prefactor
     const unsigned char* zsidx;  // [Nω]
     const double* zfd;       // [9][4] β1 stencil (ω, γ, nwg)
-    int ztaper;              // 1: linear taper a(z); neff_wg from zU, zV
+    int ztaper;              // 1: linear taper a(z), 2: piecewise linear; neff_wg from zU, zV
     double za0, za1, zaL, zsqrt_intg;
+    int zt_n;                // taper = 2: nodes
+    const double* zt_Z;      // [zt_n] positions
+    const double* zt_A;      // [zt_n] radii
     const double* zU;        // [Nω] u_nm/k
     const double2* zV;       // [Nω] v_n/k
     const double* zfdt;      // [9][3] U, V at the β1 stencil points
@@ -1107,8 +1110,19 @@ __device__ __forceinline__ double2 zg_neff(const DInst& in, double eco, double2 
 // bin the operator -i(ω/c·conj_clamp(neff) - ω β1) (src/LinearOps.jl:337-358,
 // 254) and the output factor sc_f·pre/β·√Aeff·ωwin (β = ω/c·Re neff),
 // written to in.zL / in.zOF.  Returns ρ(z) and the Kerr factor ρ ε0 γ3.
-// Radius of the linear taper at z: a0 + (a1 - a0)·clamp(z/L, 0, 1).
+// Radius of the taper at z: linear a0 + (a1 - a0)·clamp(z/L, 0, 1), or the
+// piecewise-linear profile through (zt_Z, zt_A), constant outside.
 __device__ __forceinline__ double zg_radius(const DInst& in, double z) {
+    if (in.ztaper == 2) {
+        const double *Z = in.zt_Z, *A = in.zt_A;
+        const int n = in.zt_n;
+        if (z <= Z[0]) return A[0];
+        if (z >= Z[n - 1]) return A[n - 1];
+        int i = 0;
+        for (int j = 0; j < n - 1; j++)
+            if (Z[j] < z) i = j;  // the segment [Z[i], Z[i+1]] holding z
+        return A[i] + (A[i + 1] - A[i]) * ((z - Z[i]) / (Z[i + 1] - Z[i]));
+    }
     double f = z / in.zaL;
     f = f < 0.0 ? 0.0 : (f > 1.0 ? 1.0 : f);
     return in.za0 + (in.za1 - in.za0) * f;
@@ -1866,8 +1880,10 @@ struct lh_ctx {
     double2 *znwg, *zpre;              // shared
     unsigned char* zsidx;              // shared
     double2 *zL, *zOF;                 // per context
-    int zg_taper;                      // linear taper
+    int zg_taper;                      // 1 linear, 2 piecewise-linear taper
     double zg_a0, zg_a1, zg_aL, zg_sqrt_intg;
+    int zt_n;
+    double *zt_Z, *zt_A;               // shared
     double *zU, *zfdt;                 // shared
     double2* zV;                       // shared
 };
@@ -2112,6 +2128,9 @@ static DInst base_inst(const lh_ctx* c) {
         in.za1 = c->zg_a1;
         in.zaL = c->zg_aL;
         in.zsqrt_intg = c->zg_sqrt_intg;
+        in.zt_n = c->zt_n;
+        in.zt_Z = c->zt_Z;
+        in.zt_A = c->zt_A;
         in.zU = c->zU;
         in.zV = c->zV;
         in.zfdt = c->zfdt;
@@ -2710,8 +2729,10 @@ int lh_set_step_stats(lh_ctx* c, const lh_stats_desc* sd) {
 static void zg_release(lh_ctx* c) {
     for (const void* p : {(const void*)c->zg_Z, (const void*)c->zg_P, (const void*)c->zfd, (const void*)c->zgam,
                           (const void*)c->zd, (const void*)c->znwg, (const void*)c->zpre, (const void*)c->zU,
-                          (const void*)c->zV, (const void*)c->zfdt})
+                          (const void*)c->zV, (const void*)c->zfdt, (const void*)c->zt_Z, (const void*)c->zt_A})
         shared_release(p);
+    c->zt_Z = c->zt_A = nullptr;
+    c->zt_n = 0;
     c->zg_Z = c->zg_P = c->zfd = c->zgam = nullptr;
     c->zd = nullptr;
     c->znwg = c->zpre = nullptr;
@@ -2732,11 +2753,15 @@ int lh_set_gradient(lh_ctx* c, const lh_gradient_desc* g) {
     if (d.transform != LH_TRANS_MODEAVG || d.grid_kind != LH_GRID_REAL || !c->omega || !c->zsidx) return LH_E_ARG;
     if (g->n_points < 2 || !g->Z || !g->P || g->dens_n < 4 || !g->dens_p || !g->dens_rho || !g->dens_d ||
         !g->gamma || !g->nwg || !g->pre || !g->fd || !(g->fd_h > 0) || !(g->sqrt_aeff > 0) ||
-        (g->model != 0 && g->model != 1) || (g->two_point && g->n_points != 2) || (g->taper != 0 && g->taper != 1))
+        (g->model != 0 && g->model != 1) || (g->two_point && g->n_points != 2) || g->taper < 0 || g->taper > 2)
         return LH_E_ARG;
-    if (g->taper && (!(g->taper_a0 > 0) || !(g->taper_a1 > 0) || !(g->taper_L > 0) || !(g->sqrt_aeff_intg > 0) ||
-                     !g->wgU || !g->wgV || !g->fdt))
-        return LH_E_ARG;
+    if (g->taper && (!(g->sqrt_aeff_intg > 0) || !g->wgU || !g->wgV || !g->fdt)) return LH_E_ARG;
+    if (g->taper == 1 && (!(g->taper_a0 > 0) || !(g->taper_a1 > 0) || !(g->taper_L > 0))) return LH_E_ARG;
+    if (g->taper == 2) {
+        if (g->taper_n < 2 || !g->taper_Z || !g->taper_A) return LH_E_ARG;
+        for (int i = 0; i < g->taper_n; i++)
+            if (!(g->taper_A[i] > 0) || (i > 0 && !(g->taper_Z[i - 1] < g->taper_Z[i]))) return LH_E_ARG;
+    }
     for (int i = 0; i + 1 < g->n_points; i++)
         if (!(g->Z[i] < g->Z[i + 1])) return LH_E_ARG;
     if (g->two_point && !(g->Z[0] == 0.0)) return LH_E_ARG;
@@ -2755,6 +2780,8 @@ int lh_set_gradient(lh_ctx* c, const lh_gradient_desc* g) {
     if (!st && g->taper) st = shared_upload(c->device, &c->zU, g->wgU, N);
     if (!st && g->taper) st = shared_upload(c->device, &c->zV, g->wgV, N);
     if (!st && g->taper) st = shared_upload(c->device, &c->zfdt, g->fdt, (size_t)27);
+    if (!st && g->taper == 2) st = shared_upload(c->device, &c->zt_Z, g->taper_Z, (size_t)g->taper_n);
+    if (!st && g->taper == 2) st = shared_upload(c->device, &c->zt_A, g->taper_A, (size_t)g->taper_n);
     if (!st && !c->zL && hipMalloc(&c->zL, sizeof(double2) * N) != hipSuccess) st = LH_E_NOMEM;
     if (!st && !c->zOF && hipMalloc(&c->zOF, sizeof(double2) * N) != hipSuccess) st = LH_E_NOMEM;
     if (st) {
@@ -2774,6 +2801,7 @@ int lh_set_gradient(lh_ctx* c, const lh_gradient_desc* g) {
     c->zg_a1 = g->taper_a1;
     c->zg_aL = g->taper_L;
     c->zg_sqrt_intg = g->sqrt_aeff_intg;
+    c->zt_n = (g->taper == 2) ? g->taper_n : 0;
     c->zg_on = 1;
     return LH_OK;
 }

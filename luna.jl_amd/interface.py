@@ -14,6 +14,7 @@ from . import _capi
 from . import stats as Stats
 from . import physdata as pd
 from .capillary import Gradient, gradient_spec
+from .capillary import taper_radius as capillary_taper_radius
 from .fields import GaussField, SechField, ShotNoise
 from .grid import EnvGrid, RealGrid
 from .ionisation import ppt_spline_cached
@@ -97,12 +98,21 @@ def prop_capillary_args(radius, flength, gas, pressure, *, lambda0, lambda_lims,
     grid = RealGrid(flength, lambda0, lambda_lims, trange)
     grad = None
     taper = None
+    if callable(radius):
+        # a(z) as the reference takes it (src/Capillary.jl:60-81): sampled on
+        # 1025 nodes over the fibre, piecewise linear between them
+        Zt = np.linspace(0.0, flength, 1025)
+        radius = (Zt, np.array([float(radius(z)) for z in Zt]))
     if isinstance(radius, (tuple, list)):
-        # a linear taper from radius[0] at z = 0 to radius[1] at the fibre's end
-        # (the reference takes any a(z), src/Capillary.jl:60-81); it runs
-        # through the z-dependent fill, so a constant pressure becomes a flat one
-        taper = (float(radius[0]), float(radius[1]), float(flength))
-        radius = taper[0]
+        # (a0, a1): linear from a0 at z = 0 to a1 at the fibre's end; (Z, A):
+        # piecewise linear through the nodes.  Either runs through the
+        # z-dependent fill, so a constant pressure becomes a flat one.
+        if np.ndim(radius[0]) == 0:
+            taper = (float(radius[0]), float(radius[1]), float(flength))
+            radius = taper[0]
+        else:
+            taper = (np.asarray(radius[0], dtype=float), np.asarray(radius[1], dtype=float))
+            radius = float(capillary_taper_radius(taper, 0.0))
         if not isinstance(pressure, (tuple, list)):
             pressure = (float(pressure), float(pressure))
     if isinstance(pressure, (tuple, list)):

@@ -929,8 +929,19 @@ static const double FD8_[9] = {1.0 / 280, -4.0 / 105, 1.0 / 5, -4.0 / 5, 0.0, 4.
  *   β1(z) = dβ/dω at ω0 with β = ω/c*real(neff(mode, ω; z)) (8th-order stencil);
  * and norm_mode_average's factor pre/β(ω, z)*sqrt(aeff) (src/NonlinearRHS.jl:375-387).
  * L, nf, rho may be NULL. */
-/* radius of the linear taper at z: a0 + (a1 - a0)·clamp(z/L, 0, 1) */
+/* radius of the taper at z: linear a0 + (a1 - a0)·clamp(z/L, 0, 1), or the
+ * piecewise-linear profile through (taper_Z, taper_A), constant outside */
 static double taper_a(const lh_gradient_desc *g, double z) {
+    if (g->taper == 2) {
+        const double *Z = g->taper_Z, *A = g->taper_A;
+        int n = g->taper_n;
+        if (z <= Z[0]) return A[0];
+        if (z >= Z[n - 1]) return A[n - 1];
+        int i = 0;
+        for (int j = 0; j < n - 1; j++)
+            if (Z[j] < z) i = j; /* the segment [Z[i], Z[i+1]] holding z */
+        return A[i] + (A[i + 1] - A[i]) * ((z - Z[i]) / (Z[i + 1] - Z[i]));
+    }
     double f = z / g->taper_L;
     f = f < 0 ? 0 : (f > 1 ? 1 : f);
     return g->taper_a0 + (g->taper_a1 - g->taper_a0) * f;

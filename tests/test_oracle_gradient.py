@@ -166,3 +166,28 @@ def test_taper_statistics_use_aeff_at_z(orc, capi, pkg):
     for r in rec:
         a = capillary.taper_radius((125e-6, 100e-6, L), r[k["z"]])
         assert r[k["peakintensity"]] == pytest.approx(r[k["peakpower"]] / (a * a * intg), rel=1e-13)
+
+
+def test_piecewise_taper_constants_match_mode_at_radius(orc, pkg):
+    """A piecewise-linear radius through nodes (Z, A) — and a callable a(z)
+    sampled on the fibre — with a constant fill: at z inside each segment, at
+    the nodes and beyond them, the operator and normalisation equal the host
+    mode model's at that radius."""
+    from luna_jl_amd import capillary, linops, modes, nonlinearrhs
+    L = 0.4
+    Z, A = [0.0, 0.1, 0.25, 0.4], [150e-6, 100e-6, 120e-6, 80e-6]
+    for radius in ((Z, A), lambda z: 150e-6 - 150e-6 * z):
+        Ew, grid, linop, tr, h = setup_taper(radius, 1.2, L)
+        s = grid.sidx
+        for z in (0.0, 0.05, 0.1, 0.2, 0.33, 0.4, 0.5):
+            Lz, nf, rho = orc.grad_constants(h, tr.gradient, z)
+            a = tr.gradient.radius(z)
+            if callable(radius):
+                assert a == pytest.approx(radius(min(z, L)), rel=1e-12)
+            else:
+                assert a == pytest.approx(np.interp(z, Z, A), rel=1e-14)
+            m = modes.MarcatiliMode(a, "He", 1.2)
+            Lc, beta, b1 = linops.make_const_linop(grid, m, 800e-9)
+            nfc = nonlinearrhs.norm_mode_average(grid, beta, m.Aeff())
+            assert np.linalg.norm(Lz[s] - Lc[s]) / np.linalg.norm(Lc[s]) < 1e-7, z
+            assert np.max(np.abs(nf[s] - nfc[s]) / np.abs(nfc[s])) < 1e-12, z
