@@ -114,3 +114,13 @@ def test_envelope_beyond_largest_class_rejected(capi, pkg):
     holder = tr.descriptor(linop)
     h = ctypes.c_void_p()
     assert capi.lib().lh_create(ctypes.byref(h), holder.ref) == capi.LH_E_ARG
+
+
+def test_host_alloc_argument_checks(capi):
+    """lh_host_alloc refuses a zero size or a missing result pointer before
+    touching a device; lh_host_free(NULL) is a no-op."""
+    L = capi.lib()
+    p = ctypes.c_void_p()
+    assert L.lh_host_alloc(0, ctypes.byref(p)) == capi.LH_E_ARG and not p.value
+    assert L.lh_host_alloc(64, None) == capi.LH_E_ARG
+    assert L.lh_host_free(None) == capi.LH_OK

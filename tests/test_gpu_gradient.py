@@ -148,3 +148,37 @@ def test_piecewise_taper_rhs_and_solve(orc, capi):
     st, Er, sv_r, zs_r, st_r = orc.solve_grad(h, tr.gradient, Ew, opts)
     assert st == 0 and st_g.status == 0 and st_g.steps == st_r.steps
     assert spec_err(Eg, Er, grid.sidx) <= 1e-7
+
+
+def test_invalid_taper_and_window_descriptors(orc, capi):
+    """Descriptors the device path cannot honour are refused with LH_E_ARG and
+    leave the context usable: a taper kind out of range, nodes that do not
+    increase, a non-positive radius; more than four energy windows."""
+    print("case: invalid taper and window descriptors", flush=True)
+    import ctypes
+    from luna_jl_amd import runtime, stats
+    Ew, grid, linop, tr, h = setup_taper(([0.0, 0.02, 0.05], [125e-6, 100e-6, 110e-6]), 1.0, 0.05)
+    ctx = runtime.Context(h)
+    L = capi.lib()
+    d = tr.gradient.desc
+    zgood, agood = tr.gradient.taper_Z, tr.gradient.taper_A  # (ctypes pointer fields read as views)
+    d.taper = 3
+    assert L.lh_set_gradient(ctx.handle, ctypes.byref(d)) == capi.LH_E_ARG
+    d.taper = 2
+    d.taper_Z = None
+    assert L.lh_set_gradient(ctx.handle, ctypes.byref(d)) == capi.LH_E_ARG
+    zbad = np.array([0.0, 0.03, 0.02])
+    d.taper_Z = capi._ptr(zbad)
+    assert L.lh_set_gradient(ctx.handle, ctypes.byref(d)) == capi.LH_E_ARG
+    abad = np.array([125e-6, 0.0, 110e-6])
+    d.taper_Z, d.taper_A = capi._ptr(zgood), capi._ptr(abad)
+    assert L.lh_set_gradient(ctx.handle, ctypes.byref(d)) == capi.LH_E_ARG
+    d.taper_A = capi._ptr(agood)
+    assert L.lh_set_gradient(ctx.handle, ctypes.byref(d)) == capi.LH_OK
+    sf = stats.StatsFun(grid, tr, capacity=16)
+    sd = sf.desc()
+    sd.n_windows = 5
+    assert L.lh_set_step_stats(ctx.handle, ctypes.byref(sd)) == capi.LH_E_ARG
+    nl = ctx.rhs(Ew, 0.01)  # still usable
+    st, ref = orc.rhs_grad(h, tr.gradient, Ew, 0.01)
+    assert st == 0 and np.linalg.norm(nl - ref) / np.linalg.norm(ref) <= 1e-12
