@@ -17,6 +17,7 @@
 #ifndef LUNA_HIP_H
 #define LUNA_HIP_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus

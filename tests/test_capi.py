@@ -124,3 +124,20 @@ def test_host_alloc_argument_checks(capi):
     assert L.lh_host_alloc(0, ctypes.byref(p)) == capi.LH_E_ARG and not p.value
     assert L.lh_host_alloc(64, None) == capi.LH_E_ARG
     assert L.lh_host_free(None) == capi.LH_OK
+
+
+@pytest.mark.parametrize("header", HEADERS)
+@pytest.mark.parametrize("lang", ["c99", "c11", "c++17"])
+def test_header_compiles_on_its_own(header, lang):
+    """Each public header compiles as the only include of a C or C++ file."""
+    import subprocess
+    import tempfile
+    cpp = lang.startswith("c++")
+    with tempfile.TemporaryDirectory() as td:
+        src = os.path.join(td, "h.cpp" if cpp else "h.c")
+        open(src, "w").write(f'#include "{header}"\nint main(void) {{ return 0; }}\n')
+        cc = "g++" if cpp else "gcc"
+        res = subprocess.run([cc, f"-std={lang}", "-Wall", "-Wextra", "-Werror", "-pedantic",
+                              "-I", os.path.join(ROOT, "include"), "-o", os.path.join(td, "h"), src],
+                             capture_output=True, text=True)
+        assert res.returncode == 0, res.stderr

@@ -55,28 +55,46 @@ def _const_rate_table(r0, emin=1.0, emax=1e12, n=64):
     return x, np.full(n, math.log(r0)), np.zeros(n)
 
 
-def test_plasma_constant_rate_closed_form(orc, capi):
-    # Constant field E0 and constant rate r0: F = r0 t exactly (trapezoid),
-    # f = 1 - e^{-r0 t}, J = ∫ f e²/m E0 + Ip r0 (1-f)/E0, P = ∫ J.
-    r0, E0, ip, dt = 2e13, 5e9, 3e-18, 1e-16
-    n = 4096
+def _plasma_closed_form(t, r0, E0, ip):
+    """P(t) of PlasmaCumtrapz for a constant field E0 and a constant rate r0
+    (preionfrac 0): F = r0 t, f = 1 - e^{-r0 t}, J = ∫ f e²/m E0 dt + Ip r0 (1-f)/E0,
+    P = ∫ J dt, integrated exactly.  Returns (phase part, loss part)."""
+    phase = E_RATIO * E0 * (t ** 2 / 2 - t / r0 + (1 - np.exp(-r0 * t)) / r0 ** 2)
+    loss = ip / E0 * (1 - np.exp(-r0 * t))
+    return phase, loss
+
+
+def _plasma_case(orc, capi, r0, E0, ip, dt, n=4096):
     ppt = _const_rate_table(r0)
     h = _plain_desc(capi, n_to=n, kerr=False, plasma=True, ppt=ppt, rho=1.0, ionpot=ip)
     h.desc.dt_o = dt
-    E = np.full(n, E0)
-    st, P = orc.responses_real(h, E)
+    st, P = orc.responses_real(h, np.full(n, E0))
     assert st == 0
-    t = np.arange(n) * dt
-    f = 1 - np.exp(-r0 * t)
-    # exact integrals of the continuous expressions; trapezoid error is O(dt² r0²)
-    J = E_RATIO * E0 * (t - (1 - np.exp(-r0 * t)) / r0) + ip * r0 * np.exp(-r0 * t) / E0
-    Pex = E_RATIO * E0 * (t ** 2 / 2 - t / r0 + (1 - np.exp(-r0 * t)) / r0 ** 2) \
-        + ip / E0 * (1 - np.exp(-r0 * t)) - ip * r0 / E0 * 0.0
-    # the loss term enters J at every sample (J[i] += ...), P integrates it from t=0
-    Pex = Pex + 0.0
-    rel = np.max(np.abs(P[1:] - Pex[1:])) / np.max(np.abs(Pex))
-    assert rel < 1e-4, rel
-    assert np.all(np.diff(f) >= 0)
+    return np.arange(n) * dt, P
+
+
+def test_plasma_constant_rate_phase_term(orc, capi):
+    # no ionisation loss: P is the free-electron current integrated twice.
+    # The trapezoid error of the two outer integrals is O((r0 dt)²) ≈ 4e-6.
+    t, P = _plasma_case(orc, capi, r0=2e13, E0=5e9, ip=0.0, dt=1e-16)
+    phase, loss = _plasma_closed_form(t, 2e13, 5e9, 0.0)
+    rel = np.max(np.abs(P[1:] - phase[1:])) / np.max(np.abs(phase))
+    assert rel < 2e-5, rel
+
+
+def test_plasma_constant_rate_loss_term(orc, capi):
+    # A weak field (1e7 V/m) and a large Ip: the ionisation-loss current
+    # Ip·rate·(1-f)/E (src/Nonlinear.jl:128-131) dominates P by ~600x, so
+    # dropping it, doubling it or dividing by E² instead of E misses by far
+    # more than the tolerance ((r0 dt)² = 1e-6 trapezoid error).
+    r0, E0, ip, dt = 1e14, 1e7, 3.9e-18, 1e-17
+    t, P = _plasma_case(orc, capi, r0=r0, E0=E0, ip=ip, dt=dt)
+    phase, loss = _plasma_closed_form(t, r0, E0, ip)
+    assert loss[-1] > 300 * abs(phase[-1])
+    rel = np.max(np.abs(P[1:] - (phase + loss)[1:])) / np.max(np.abs(phase + loss))
+    assert rel < 1e-5, rel
+    for wrong in (phase, phase + 2 * loss, phase + loss / E0):
+        assert np.max(np.abs(P[1:] - wrong[1:])) / np.max(np.abs(phase + loss)) > 0.1
 
 
 def test_plasma_zero_field(orc, capi):
