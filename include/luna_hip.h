@@ -36,7 +36,15 @@ enum {
     LH_E_REPEAT_LIMIT = 4, /* more than repeat_limit consecutive rejections  */
     LH_E_NONFINITE = 5,    /* non-finite state                               */
     LH_E_NOMEM = 6,        /* device allocation failed                       */
-    LH_E_NODEVICE = 7      /* no HIP device available                        */
+    LH_E_NODEVICE = 7,     /* no HIP device available                        */
+    LH_E_MAXSTEPS = 8,     /* step-attempt cap (lh_rk_opts.max_steps) reached:
+                              no counterpart in the reference, whose loop is
+                              unbounded (src/RK45.jl:55-89)                  */
+    LH_E_STOPPED = 9       /* stopped at lh_rk_opts.zstop before zmax: the
+                              analogue of a run interrupted after an accepted
+                              step (the cached-continuation test,
+                              test/test_output.jl:203-276); resume from the
+                              returned state, z, dz and saved count          */
 };
 
 enum { LH_GRID_REAL = 0, LH_GRID_ENV = 1 };           /* Grid.RealGrid / Grid.EnvGrid */
@@ -95,6 +103,19 @@ typedef struct lh_rk_opts {
     double init_dz;           /* 1e-4                                            */
     int32_t repeat_limit;     /* 10                                              */
     int32_t save_n;           /* 201 (MemoryOutput GridCondition points)         */
+    /* The save grid is range(save_z0, zmax, save_n): GridCondition(tmin, tmax,
+     * saveN) takes the output's tmin, not the solver's z0 (src/Output.jl:490-496). */
+    double save_z0;           /* 0                                               */
+    int32_t save_first;       /* save points already written: a run resumed from a
+                                 cache continues GridCondition at this index, as
+                                 HDF5Output's `saved` (src/Output.jl:170-176,
+                                 473-487; Luna.run src/Luna.jl:374-379) (0)      */
+    int32_t reserved;         /* 0                                               */
+    double zstop;             /* > 0: end with LH_E_STOPPED after the first
+                                 accepted step with tn > zstop (its window filter
+                                 and saves done); 0: run to zmax                 */
+    int64_t max_steps;        /* cap on step attempts, LH_E_MAXSTEPS when reached;
+                                 0: 500000                                       */
 } lh_rk_opts;
 
 /* Per-run counters (src/RK45.jl:38-82). */

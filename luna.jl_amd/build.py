@@ -1,4 +1,8 @@
-"""Builds libluna_hip.so in-tree with hipcc for gfx950."""
+"""Builds libluna_hip.so in-tree with hipcc for gfx950.
+
+Each source is compiled to an object in its own hipcc process (they run
+concurrently), then the objects are linked into one shared library.
+"""
 import os
 import subprocess
 import sys
@@ -10,8 +14,9 @@ OUT = os.path.join(OUT_DIR, "libluna_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 FLAGS = [
-    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
-    # packed FP32 instructions stay off: every kernel here is FP64 anyway
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
+    # packed FP32 instructions stay off: the solver kernels are FP64, and the
+    # surrogate's FP32 elementwise code is not worth the risk
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-value", "-Wno-unused-command-line-argument",
 ]
@@ -19,15 +24,32 @@ FLAGS = [
 
 def build(verbose=False):
     os.makedirs(OUT_DIR, exist_ok=True)
-    tmp = OUT + f".tmp{os.getpid()}"
-    cmd = [HIPCC] + FLAGS + ["-o", tmp] + SRCS
-    res = subprocess.run(cmd, capture_output=True, text=True)
-    if res.returncode != 0:
-        sys.stderr.write(res.stdout + res.stderr)
-        raise RuntimeError("hipcc failed for " + " ".join(SRCS))
-    if verbose:
-        sys.stderr.write(res.stderr)
-    os.replace(tmp, OUT)
+    tag = f".tmp{os.getpid()}"
+    objs = [os.path.join(OUT_DIR, os.path.basename(s).replace(".hip", ".o") + tag) for s in SRCS]
+    procs = [subprocess.Popen([HIPCC] + FLAGS + ["-c", "-o", o, s], stdout=subprocess.PIPE,
+                              stderr=subprocess.PIPE, text=True) for s, o in zip(SRCS, objs)]
+    failed = []
+    for p, s in zip(procs, SRCS):
+        out, err = p.communicate()
+        if p.returncode != 0:
+            sys.stderr.write(out + err)
+            failed.append(s)
+        elif verbose:
+            sys.stderr.write(err)
+    try:
+        if failed:
+            raise RuntimeError("hipcc failed for " + " ".join(failed))
+        tmp = OUT + tag
+        res = subprocess.run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs,
+                             capture_output=True, text=True)
+        if res.returncode != 0:
+            sys.stderr.write(res.stdout + res.stderr)
+            raise RuntimeError("linking libluna_hip.so failed")
+        os.replace(tmp, OUT)
+    finally:
+        for o in objs:
+            if os.path.exists(o):
+                os.remove(o)
     return OUT
 
 

@@ -34,8 +34,7 @@
 #include "../../include/luna_hip.h"
 #include "lh_device.h"
 
-#define LH_E_MAXSTEPS_INTERNAL LH_E_REPEAT_LIMIT
-static const long long LH_MAX_STEPS = 500000;  // hard cap on step attempts per instance
+static const long long LH_MAX_STEPS = 500000;  // default cap on step attempts per instance
 
 // --------------------------------------------------------------------------
 // Device-side instance description
@@ -67,6 +66,9 @@ struct DInst {
     lh_stats* stats;
     double z0, zmax, rtol, atol, safety, max_dz, min_dz, init_dz;
     int repeat_limit, save_n;
+    int save_first;       // saves already written (a resumed run continues at this index)
+    double zstop;         // > 0: stop after the first accepted step with tn > zstop
+    long long max_steps;  // cap on step attempts
     // single-RHS I/O
     const double2* rhs_in;
     double2* rhs_out;
@@ -1473,7 +1475,7 @@ __device__ __forceinline__ void solve_instance(const DInst& in, Smem& sm) {
     __syncthreads();
 
     long long steps = 0, accepted = 0, rejected = 0, rhs_evals = 0;
-    int status = LH_OK, saved = 0, repeated = 0;
+    int status = LH_OK, saved = in.save_first, repeated = 0;
     double t = in.z0, tn = in.z0, dt = in.init_dz, dtn = in.init_dz, errlast = 0.0;
     // density and Kerr factor of the current z (constant unless a gradient fill
     // re-evaluates them, with the operator, at every new z)
@@ -1491,8 +1493,8 @@ __device__ __forceinline__ void solve_instance(const DInst& in, Smem& sm) {
     if (sm.flag != LH_OK) status = sm.flag;
 
     while (status == LH_OK && tn <= in.zmax) {
-        if (steps >= LH_MAX_STEPS) {
-            status = LH_E_MAXSTEPS_INTERNAL;
+        if (steps >= in.max_steps) {
+            status = LH_E_MAXSTEPS;
             break;
         }
         // evaluate!
@@ -1653,6 +1655,7 @@ __device__ __forceinline__ void solve_instance(const DInst& in, Smem& sm) {
             YN = tmp2;
             accepted++;
             repeated = 0;
+            if (in.zstop > 0.0 && tn > in.zstop && tn <= in.zmax) status = LH_E_STOPPED;
         } else {
             rejected++;
             repeated++;
@@ -2172,18 +2175,30 @@ static int ensure_solver_buffers(lh_ctx* c, int save_n, bool want_saves) {
     return LH_OK;
 }
 
-// range(z0, stop=zmax, length=n) as Julia evaluates it (correctly rounded lerp).
-static void save_grid(const lh_rk_opts* o, std::vector<double>& g) {
-    int n = o->save_n;
-    g.assign(n > 0 ? n : 1, o->z0);
-    long double a = o->z0, b = o->zmax;
+// range(save_z0, stop=zmax, length=n) as Julia evaluates it (correctly rounded
+// lerp): GridCondition(tmin, tmax, saveN), src/Output.jl:490-496.
+static void save_grid_into(const lh_rk_opts* o, double* g) {
+    const int n = o->save_n;
+    const long double a = o->save_z0, b = o->zmax;
     for (int i = 0; i < n; i++)
-        g[i] = (n == 1) ? o->z0 : (double)((a * (long double)(n - 1 - i) + b * (long double)i) / (long double)(n - 1));
+        g[i] = (n == 1) ? o->save_z0
+                        : (double)((a * (long double)(n - 1 - i) + b * (long double)i) / (long double)(n - 1));
+    if (n <= 0) g[0] = o->save_z0;
+}
+
+static void save_grid(const lh_rk_opts* o, std::vector<double>& g) {
+    g.assign(o->save_n > 0 ? o->save_n : 1, o->save_z0);
+    save_grid_into(o, g.data());
+}
+
+static bool opts_ok(const lh_rk_opts* o) {
+    return o->save_n >= 0 && o->repeat_limit >= 0 && o->rtol > 0 && o->zmax >= o->z0 && o->save_first >= 0 &&
+           o->save_first <= o->save_n && o->max_steps >= 0 && o->zstop >= 0;
 }
 
 static int prepare_instance(lh_ctx* c, const lh_rk_opts* o, bool want_saves, DInst* in, lh_stats* dstats_slot,
                             hipStream_t s) {
-    if (o->save_n < 0 || o->repeat_limit < 0 || !(o->rtol > 0) || !(o->zmax >= o->z0)) return LH_E_ARG;
+    if (!opts_ok(o)) return LH_E_ARG;
     if (!c->have_state) return LH_E_ARG;
     int st = ensure_solver_buffers(c, o->save_n, want_saves);
     if (st) return st;
@@ -2210,6 +2225,9 @@ static int prepare_instance(lh_ctx* c, const lh_rk_opts* o, bool want_saves, DIn
     in->init_dz = o->init_dz;
     in->repeat_limit = o->repeat_limit;
     in->save_n = o->save_n;
+    in->save_first = o->save_first;
+    in->zstop = o->zstop;
+    in->max_steps = o->max_steps > 0 ? o->max_steps : LH_MAX_STEPS;
     c->have_result = 0;
     c->result_save_n = want_saves ? o->save_n : 0;
     return LH_OK;
@@ -2240,6 +2258,7 @@ static void rad_ctl_init(RadCtl* h, const lh_rk_opts* o) {
     if (o) {
         h->t = h->tn = o->z0;
         h->dt = h->dtn = o->init_dz;
+        h->saved = o->save_first;
     }
     for (int j = 0; j < 8; j++) h->perm[j] = j;
     h->status = LH_OK;
@@ -2290,6 +2309,8 @@ static int rad_solve(lh_ctx* c, const lh_rk_opts* o, bool want_saves, hipStream_
     a.min_dz = o->min_dz;
     a.repeat_limit = o->repeat_limit;
     a.save_n = o->save_n;
+    a.zstop = o->zstop;
+    a.max_steps = o->max_steps > 0 ? o->max_steps : LH_MAX_STEPS;
     RadCtl h;
     rad_ctl_init(&h, o);
     h.rhs_evals = 1;
@@ -2356,6 +2377,8 @@ const char* lh_strerror(int status) {
         case LH_E_NONFINITE: return "non-finite state";
         case LH_E_NOMEM: return "device allocation failed";
         case LH_E_NODEVICE: return "no HIP device";
+        case LH_E_MAXSTEPS: return "reached the cap on step attempts";
+        case LH_E_STOPPED: return "stopped before the end of the propagation (zstop)";
         default: return "unknown status";
     }
 }

@@ -55,6 +55,8 @@ struct RadArgs {
     RadCtl* ctl;
     double zmax, rtol, atol, safety, max_dz, min_dz;
     int repeat_limit, save_n;
+    double zstop;         // > 0: stop after the first accepted step with tn > zstop
+    long long max_steps;  // cap on step attempts
     // single RHS mode: input/output columns
     const double2* rhs_in;
     double2* rhs_out;
@@ -381,8 +383,11 @@ __global__ void rad_step_end(const RadArgs* __restrict__ ap) {
     s.dt = s.dtn;
     if (!(s.tn <= a.zmax)) {
         s.done = 1;
-    } else if (s.steps >= LH_MAX_STEPS) {
-        s.status = LH_E_MAXSTEPS_INTERNAL;
+    } else if (s.ok && a.zstop > 0.0 && s.tn > a.zstop) {
+        s.status = LH_E_STOPPED;
+        s.done = 1;
+    } else if (s.steps >= a.max_steps) {
+        s.status = LH_E_MAXSTEPS;
         s.done = 1;
     }
     *ctl = s;

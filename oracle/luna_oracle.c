@@ -1036,14 +1036,21 @@ static int solve_impl(const lh_desc *d, double *Ew_inout, const lh_rk_opts *o, d
     cplx *Etc = (cplx *)malloc(sizeof(cplx) * (size_t)nt);
     cplx *work = (cplx *)malloc(sizeof(cplx) * (size_t)(nt / 2 + 2));
     double *grid = (double *)malloc(sizeof(double) * (size_t)(o->save_n > 0 ? o->save_n : 1));
+    /* GridCondition(tmin, tmax, saveN): the output's own tmin (src/Output.jl:490-496) */
     for (int i = 0; i < o->save_n; i++) {
-        long double a = (long double)o->z0, b = (long double)o->zmax;
-        grid[i] = (o->save_n == 1) ? o->z0
+        long double a = (long double)o->save_z0, b = (long double)o->zmax;
+        grid[i] = (o->save_n == 1) ? o->save_z0
                                    : (double)((a * (long double)(o->save_n - 1 - i) + b * (long double)i) /
                                               (long double)(o->save_n - 1));
     }
-    int saved = 0, repeated = 0;
+    /* a resumed run continues the saves at the cache's count (src/Output.jl:170-176) */
+    int saved = o->save_first, repeated = 0;
+    const long long max_steps = o->max_steps > 0 ? o->max_steps : 500000;
     while (st == LH_OK && s.tn <= o->zmax) {
+        if (stats->steps >= max_steps) {
+            st = LH_E_MAXSTEPS;
+            break;
+        }
         st = step(&s);
         if (st) break;
         stats->steps++;
@@ -1090,6 +1097,9 @@ static int solve_impl(const lh_desc *d, double *Ew_inout, const lh_rk_opts *o, d
                 saved++;
             }
             repeated = 0;
+            /* an interruption after this accepted step (the erroring stepfun of
+               test/test_output.jl:226-231) */
+            if (o->zstop > 0.0 && s.tn > o->zstop && s.tn <= o->zmax) st = LH_E_STOPPED;
         } else {
             stats->rejected++;
             repeated++;
