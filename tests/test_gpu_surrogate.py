@@ -49,4 +49,4 @@ def test_surrogate_screening_gain(pkg):
     # the screen bounds the error statistically (calibrated on held-out runs):
     # at least 98 % of the accepted points must be within the bound
     assert np.mean(true_err[acc] <= BOUND) >= 0.98
-    assert gain >= 3.0
+    assert gain >= 5.0  # the target: surrogate screening cuts full-solver calls by at least 5x
