@@ -160,6 +160,41 @@ int lh_ctx_status(lh_ctx *ctx);
 int lh_solve(lh_ctx *ctx, double *Ew_inout, const lh_rk_opts *opts,
              double *saves, double *zsaves, lh_stats *stats);
 
+/* Per-step output while the device solve runs: the output callable of
+ * Luna.run's stepfun, output(y, z, dz, interpolant) (src/Luna.jl:366-372,
+ * src/Output.jl:65-81, 262-302), for outputs other than a fixed save buffer
+ * (HDF5Output, user callbacks).  The kernel publishes every accepted step that
+ * writes a GridCondition save; the host hands each publication to cb on the
+ * calling thread while the kernel goes on.  One call carries the saves written
+ * since the previous call, the state, z and dz of the latest saving step
+ * (HDF5Output's cache: t, dt, y, saved) and the statistics records of the
+ * steps since the previous call.  Pointers are valid during the call only. */
+typedef struct lh_step_out {
+    int32_t first_save;       /* index of the first save in this call           */
+    int32_t n_saves;          /* saves in this call (>= 1)                      */
+    const double *saves;      /* complex [n_saves][n_omega*n_r]                 */
+    const double *zsaves;     /* [n_saves] their positions                      */
+    double z;                 /* tn of the latest saving step                   */
+    double dz;                /* dtn after it                                   */
+    const double *y;          /* complex [n_omega*n_r]: the filtered state at z */
+    int64_t accepted;         /* accepted steps up to that step                 */
+    int32_t first_record;     /* index of the first statistics record          */
+    int32_t n_records;        /* records in this call                           */
+    const double *records;    /* [n_records][LH_NSTAT]                          */
+} lh_step_out;
+
+/* Return nonzero to stop the run (as an error thrown by the reference's
+ * output or stepfun ends the propagation): nothing more is handed over, the
+ * kernel ends after its next saving step with LH_E_STOPPED. */
+typedef int (*lh_output_cb)(void *user, const lh_step_out *out);
+
+/* lh_solve with the saves and records streamed to cb instead of filled into
+ * buffers; same trajectory, bit for bit.  Mode-averaged contexts with a
+ * constant fill (LH_E_ARG for radial contexts and z-dependent fills, whose
+ * outputs are filled from lh_solve's buffers after the run). */
+int lh_solve_stream(lh_ctx *ctx, double *Ew_inout, const lh_rk_opts *opts, lh_output_cb cb, void *user,
+                    lh_stats *stats);
+
 /* Batched solve: n independent instances (each its own grid, constants, step
  * size control and status) in one persistent launch per grid class, one
  * workgroup per instance.  Radial contexts (whose one instance fills the GPU)
@@ -170,6 +205,12 @@ int lh_solve(lh_ctx *ctx, double *Ew_inout, const lh_rk_opts *opts,
 int lh_batch_solve(lh_ctx *const *ctxs, int n, double *const *Ew_inout,
                    const lh_rk_opts *opts, double *const *saves, double *const *zsaves,
                    lh_stats *stats);
+
+/* lh_batch_solve with options per instance, opts[i] for ctxs[i] (its own z
+ * range, tolerances, save grid, stop and cap): the batch of Scans.runscan's
+ * points when they differ in length or output (src/Scans.jl:275-289). */
+int lh_batch_solve_opts(lh_ctx *const *ctxs, int n, double *const *Ew_inout, const lh_rk_opts *opts,
+                        double *const *saves, double *const *zsaves, lh_stats *stats);
 
 /* Device-resident batch path (no counterpart in the reference; it splits
  * lh_batch_solve so a sweep can be rerun on inputs already in HBM):

@@ -61,6 +61,15 @@ class LhStats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class LhStepOut(C.Structure):
+    _fields_ = [("first_save", C.c_int32), ("n_saves", C.c_int32), ("saves", _dp), ("zsaves", _dp),
+                ("z", C.c_double), ("dz", C.c_double), ("y", _dp), ("accepted", C.c_int64),
+                ("first_record", C.c_int32), ("n_records", C.c_int32), ("records", _dp)]
+
+
+LhOutputCb = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(LhStepOut))
+
+
 class LhGradientDesc(C.Structure):
     """``lh_gradient_desc`` of include/luna_hip.h."""
     _fields_ = [("n_points", C.c_int32), ("two_point", C.c_int32), ("Z", _dp), ("P", _dp),
@@ -176,8 +185,12 @@ def lib():
     L.lh_rhs_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_void_p]
     L.lh_ctx_status.argtypes = [C.c_void_p]
     L.lh_solve.argtypes = [C.c_void_p, _dp, C.POINTER(LhRkOpts), _dp, _dp, C.POINTER(LhStats)]
+    L.lh_solve_stream.argtypes = [C.c_void_p, _dp, C.POINTER(LhRkOpts), LhOutputCb, C.c_void_p,
+                                  C.POINTER(LhStats)]
     L.lh_batch_solve.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.POINTER(_dp), C.POINTER(LhRkOpts),
                                  C.POINTER(_dp), C.POINTER(_dp), C.POINTER(LhStats)]
+    L.lh_batch_solve_opts.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.POINTER(_dp), C.POINTER(LhRkOpts),
+                                      C.POINTER(_dp), C.POINTER(_dp), C.POINTER(LhStats)]
     L.lh_set_state.argtypes = [C.c_void_p, _dp]
     L.lh_get_state.argtypes = [C.c_void_p, _dp]
     L.lh_get_saves.argtypes = [C.c_void_p, _dp, _dp]
@@ -206,7 +219,7 @@ def lib():
 
 
 EXPORTED = ["lh_strerror", "lh_device_count", "lh_create", "lh_destroy", "lh_rhs",
-            "lh_rhs_device", "lh_ctx_status", "lh_solve", "lh_batch_solve", "lh_set_state",
+            "lh_rhs_device", "lh_ctx_status", "lh_solve", "lh_solve_stream", "lh_batch_solve", "lh_batch_solve_opts", "lh_set_state",
             "lh_batch_solve_resident", "lh_get_state", "lh_get_saves", "lh_last_launch_times",
             "lh_host_alloc", "lh_host_free",
             "lh_set_step_stats", "lh_stats_eval", "lh_get_step_stats", "lh_set_gradient", "lh_ppt_table",

@@ -22,6 +22,31 @@ FLAGS = [
 ]
 
 
+H5_SRC = os.path.join(HERE, "csrc", "lh_h5.c")
+H5_OUT = os.path.join(OUT_DIR, "libluna_h5.so")
+HDF5_ROOT = os.environ.get("LUNA_HDF5_ROOT", "/opt/conda")
+
+
+def build_h5(verbose=False):
+    """libluna_h5.so, the HDF5 file layer of HDF5Output (host code, gcc),
+    built only where the HDF5 C library and its header are found.  Returns its
+    path, or None when HDF5 is absent (HDF5Output then raises on use)."""
+    inc, lib = os.path.join(HDF5_ROOT, "include"), os.path.join(HDF5_ROOT, "lib")
+    if not (os.path.isfile(os.path.join(inc, "hdf5.h")) and os.path.isfile(os.path.join(lib, "libhdf5.so"))):
+        if verbose:
+            sys.stderr.write(f"HDF5 not found under {HDF5_ROOT}: HDF5Output not built\n")
+        return None
+    os.makedirs(OUT_DIR, exist_ok=True)
+    tmp = H5_OUT + f".tmp{os.getpid()}"
+    res = subprocess.run(["gcc", "-O2", "-std=gnu99", "-fPIC", "-shared", "-Wall", "-I" + inc, H5_SRC,
+                          "-L" + lib, "-lhdf5", "-Wl,-rpath," + lib, "-o", tmp], capture_output=True, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout + res.stderr)
+        raise RuntimeError("building libluna_h5.so failed")
+    os.replace(tmp, H5_OUT)
+    return H5_OUT
+
+
 def build(verbose=False):
     os.makedirs(OUT_DIR, exist_ok=True)
     tag = f".tmp{os.getpid()}"
@@ -46,6 +71,7 @@ def build(verbose=False):
             sys.stderr.write(res.stdout + res.stderr)
             raise RuntimeError("linking libluna_hip.so failed")
         os.replace(tmp, OUT)
+        build_h5(verbose)
     finally:
         for o in objs:
             if os.path.exists(o):

@@ -29,6 +29,7 @@
 #include <string.h>
 
 #include <mutex>
+#include <time.h>
 #include <vector>
 
 #include "../../include/luna_hip.h"
@@ -105,6 +106,14 @@ struct DInst {
     const double* zfdt;      // [9][3] U, V at the β1 stencil points
     double2* zL;             // [Nω] linop at the last refresh (= linop)
     double2* zOF;            // [Nω] output factor at the last refresh (= outfac)
+    // streaming output (lh_solve_stream; pub null: off): device-mapped,
+    // page-locked host memory.  A step that writes saves publishes itself:
+    // its filtered state and (z, dz, saved, accepted) go to slot k & 1 of
+    // publication k, then pub is set to k + 1 with a system-scope release.
+    unsigned long long* pub;
+    double* pub_info;        // [2][4]
+    double2* pub_y;          // [2][n_state]
+    const int* pub_stop;     // set by the host: end with LH_E_STOPPED at the next publication
 };
 
 // Dormand–Prince tableau, src/dopri.jl:1-28 (rational literals evaluated in double).
@@ -1463,7 +1472,7 @@ __device__ __forceinline__ void do_stepfun(const DInst& in, Smem& sm, double2* Y
 
 // ZDEP: compiled with the z-dependent fill (the launch picks it only for a
 // class holding a gradient or taper; the rest run without its registers).
-template <int KIND, int MO, int MC, class Smem, bool ZDEP>
+template <int KIND, int MO, int MC, class Smem, bool ZDEP, bool STREAM>
 __device__ __forceinline__ void solve_instance(const DInst& in, Smem& sm) {
     constexpr int N = Sizes<KIND, MO, MC>::N, NPT = Sizes<KIND, MO, MC>::NPT;
     KSet K{in.K[0], in.kstride, 0, 6};
@@ -1474,7 +1483,7 @@ __device__ __forceinline__ void solve_instance(const DInst& in, Smem& sm) {
     if (threadIdx.x == 0) sm.flag = LH_OK;
     __syncthreads();
 
-    long long steps = 0, accepted = 0, rejected = 0, rhs_evals = 0;
+    long long steps = 0, accepted = 0, rejected = 0, rhs_evals = 0, npub = 0;
     int status = LH_OK, saved = in.save_first, repeated = 0;
     double t = in.z0, tn = in.z0, dt = in.init_dz, dtn = in.init_dz, errlast = 0.0;
     // density and Kerr factor of the current z (constant unless a gradient fill
@@ -1598,6 +1607,7 @@ __device__ __forceinline__ void solve_instance(const DInst& in, Smem& sm) {
             do_stepfun<KIND, MO, MC, Smem>(in, sm, YN, tn - t, rec, tn, dtn, rho);
             if (rec && sm.flag != LH_OK) status = sm.flag;
             // GridCondition saves via dense output; ks[1] and ks[7] both hold k7
+            const int saved_before = saved;
             while (saved < in.save_n && in.zgrid[saved] <= tn) {
                 double ti = in.zgrid[saved];
                 if (zdep && in.saves && ti != t && ti != tn) zg_refresh(in, sm, ti, rho, kfac);  // prop!(out, t, ti)
@@ -1650,6 +1660,31 @@ __device__ __forceinline__ void solve_instance(const DInst& in, Smem& sm) {
                 }
                 saved++;
             }
+            if (STREAM && saved > saved_before) {
+                // the output call of this step (src/Output.jl:65-81, 262-302):
+                // its saves are in place; hand over the state, then publish
+                const int slot = (int)(npub & 1);
+                double2* py = in.pub_y + (size_t)slot * N;
+                for (int k = threadIdx.x; k < N; k += LH_T) py[k] = YN[k];
+                if (threadIdx.x == 0) {
+                    double* info = in.pub_info + 4 * slot;
+                    info[0] = tn;
+                    info[1] = dtn;
+                    info[2] = (double)saved;
+                    info[3] = (double)(accepted + 1);
+                }
+                __threadfence_system();
+                __syncthreads();
+                if (threadIdx.x == 0) {
+                    __hip_atomic_store(in.pub, (unsigned long long)(npub + 1), __ATOMIC_RELEASE,
+                                       __HIP_MEMORY_SCOPE_SYSTEM);
+                    if (__hip_atomic_load(in.pub_stop, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM))
+                        sm.flag = LH_E_STOPPED;
+                }
+                __syncthreads();
+                if (sm.flag == LH_E_STOPPED && status == LH_OK && tn <= in.zmax) status = LH_E_STOPPED;
+                npub++;
+            }
             double2* tmp2 = Y;
             Y = YN;
             YN = tmp2;
@@ -1682,13 +1717,15 @@ __device__ __forceinline__ void solve_instance(const DInst& in, Smem& sm) {
     __syncthreads();
 }
 
-// One persistent kernel per grid size: one workgroup per instance.
-template <int KIND, int MO, int MC, bool ZDEP>
+// One persistent kernel per grid size: one workgroup per instance.  STREAM
+// builds in the per-step publication of lh_solve_stream (a separate kernel,
+// so the others keep their register allocation).
+template <int KIND, int MO, int MC, bool ZDEP, bool STREAM = false>
 __global__ __launch_bounds__(LH_T) void lh_solve_kernel(const DInst* insts, const double2* __restrict__ twg) {
     __shared__ SmemT<MO> sm;
     tw_load(sm.tw, twg);
     __syncthreads();
-    solve_instance<KIND, MO, MC, SmemT<MO>, ZDEP>(insts[blockIdx.x], sm);
+    solve_instance<KIND, MO, MC, SmemT<MO>, ZDEP, STREAM>(insts[blockIdx.x], sm);
 }
 
 template <int KIND, int MO, int MC, class Smem>
@@ -1764,10 +1801,23 @@ static int size_class_of(int grid_kind, int n_to, int n_t) {
 
 template <int KIND, int MO, int MC>
 static void launch_solve_k(bool zdep, int n, const DInst* dinst, const double2* tw, hipStream_t s) {
-    if (KIND == 0 && zdep)
-        hipLaunchKernelGGL((lh_solve_kernel<KIND, MO, MC, true>), dim3(n), dim3(LH_T), 0, s, dinst, tw);
-    else
-        hipLaunchKernelGGL((lh_solve_kernel<KIND, MO, MC, false>), dim3(n), dim3(LH_T), 0, s, dinst, tw);
+    if constexpr (KIND == 0) {
+        if (zdep) {
+            hipLaunchKernelGGL((lh_solve_kernel<KIND, MO, MC, true>), dim3(n), dim3(LH_T), 0, s, dinst, tw);
+            return;
+        }
+    }
+    hipLaunchKernelGGL((lh_solve_kernel<KIND, MO, MC, false>), dim3(n), dim3(LH_T), 0, s, dinst, tw);
+}
+
+static void launch_solve_stream(int cls, const DInst* dinst, const double2* tw, hipStream_t s) {
+    switch (cls) {
+        case LH_CLASS_16384: hipLaunchKernelGGL((lh_solve_kernel<0, 8192, 4096, false, true>), dim3(1), dim3(LH_T), 0, s, dinst, tw); break;
+        case LH_CLASS_8192: hipLaunchKernelGGL((lh_solve_kernel<0, 4096, 2048, false, true>), dim3(1), dim3(LH_T), 0, s, dinst, tw); break;
+        case LH_CLASS_4096: hipLaunchKernelGGL((lh_solve_kernel<0, 2048, 1024, false, true>), dim3(1), dim3(LH_T), 0, s, dinst, tw); break;
+        case LH_CLASS_ENV8192: hipLaunchKernelGGL((lh_solve_kernel<1, 8192, 8192, false, true>), dim3(1), dim3(LH_T), 0, s, dinst, tw); break;
+        default: hipLaunchKernelGGL((lh_solve_kernel<1, 4096, 4096, false, true>), dim3(1), dim3(LH_T), 0, s, dinst, tw); break;
+    }
 }
 
 // zdep: some instance of the class has a z-dependent fill (field grid only).
@@ -1875,6 +1925,7 @@ struct lh_ctx {
     double* steprec;     // [st_cap][LH_NSTAT]
     double* st_rec1;     // [LH_NSTAT], lh_stats_eval
     int result_nrec;     // records kept by the last solve
+    int have_records;    // the last solve (lh_batch_solve* or lh_solve_stream) ran
     // gradient fill
     int zg_on, zg_np, zg_two, zg_model, zg_loss, zd_n;
     double zsqrt_aeff, zfd_h;
@@ -2177,13 +2228,16 @@ static int ensure_solver_buffers(lh_ctx* c, int save_n, bool want_saves) {
 
 // range(save_z0, stop=zmax, length=n) as Julia evaluates it (correctly rounded
 // lerp): GridCondition(tmin, tmax, saveN), src/Output.jl:490-496.
-static void save_grid_into(const lh_rk_opts* o, double* g) {
+static double save_grid_at(const lh_rk_opts* o, int i) {
     const int n = o->save_n;
     const long double a = o->save_z0, b = o->zmax;
-    for (int i = 0; i < n; i++)
-        g[i] = (n == 1) ? o->save_z0
-                        : (double)((a * (long double)(n - 1 - i) + b * (long double)i) / (long double)(n - 1));
-    if (n <= 0) g[0] = o->save_z0;
+    if (n <= 1) return o->save_z0;
+    return (double)((a * (long double)(n - 1 - i) + b * (long double)i) / (long double)(n - 1));
+}
+
+static void save_grid_into(const lh_rk_opts* o, double* g) {
+    const int n = o->save_n > 0 ? o->save_n : 1;
+    for (int i = 0; i < n; i++) g[i] = save_grid_at(o, i);
 }
 
 static void save_grid(const lh_rk_opts* o, std::vector<double>& g) {
@@ -2194,6 +2248,22 @@ static void save_grid(const lh_rk_opts* o, std::vector<double>& g) {
 static bool opts_ok(const lh_rk_opts* o) {
     return o->save_n >= 0 && o->repeat_limit >= 0 && o->rtol > 0 && o->zmax >= o->z0 && o->save_first >= 0 &&
            o->save_first <= o->save_n && o->max_steps >= 0 && o->zstop >= 0;
+}
+
+static void inst_options(DInst* in, const lh_rk_opts* o) {
+    in->z0 = o->z0;
+    in->zmax = o->zmax;
+    in->rtol = o->rtol;
+    in->atol = o->atol;
+    in->safety = o->safety;
+    in->max_dz = o->max_dz;
+    in->min_dz = o->min_dz;
+    in->init_dz = o->init_dz;
+    in->repeat_limit = o->repeat_limit;
+    in->save_n = o->save_n;
+    in->save_first = o->save_first;
+    in->zstop = o->zstop;
+    in->max_steps = o->max_steps > 0 ? o->max_steps : LH_MAX_STEPS;
 }
 
 static int prepare_instance(lh_ctx* c, const lh_rk_opts* o, bool want_saves, DInst* in, lh_stats* dstats_slot,
@@ -2215,20 +2285,9 @@ static int prepare_instance(lh_ctx* c, const lh_rk_opts* o, bool want_saves, DIn
     in->saves = want_saves ? c->saves : nullptr;
     in->zgrid = c->zgrid;
     in->stats = dstats_slot;
-    in->z0 = o->z0;
-    in->zmax = o->zmax;
-    in->rtol = o->rtol;
-    in->atol = o->atol;
-    in->safety = o->safety;
-    in->max_dz = o->max_dz;
-    in->min_dz = o->min_dz;
-    in->init_dz = o->init_dz;
-    in->repeat_limit = o->repeat_limit;
-    in->save_n = o->save_n;
-    in->save_first = o->save_first;
-    in->zstop = o->zstop;
-    in->max_steps = o->max_steps > 0 ? o->max_steps : LH_MAX_STEPS;
+    inst_options(in, o);
     c->have_result = 0;
+    c->have_records = 0;
     c->result_save_n = want_saves ? o->save_n : 0;
     return LH_OK;
 }
@@ -2578,6 +2637,130 @@ int lh_solve(lh_ctx* c, double* Ew, const lh_rk_opts* o, double* saves, double* 
     return sp->status;
 }
 
+int lh_solve_stream(lh_ctx* c, double* Ew, const lh_rk_opts* o, lh_output_cb cb, void* user, lh_stats* stats) {
+    if (!c || !Ew || !o || !cb || !stats || !opts_ok(o)) return LH_E_ARG;
+    const int cls = ctx_class(c);
+    if (cls < 0 || cls == LH_CLASS_RADIAL || c->zg_on) return LH_E_ARG;
+    int st = lh_set_state(c, Ew);
+    if (st) return st;
+    const double2* tw;
+    st = twiddles(c->device, &tw);
+    if (st) return st;
+    // one page-locked, device-mapped, coherent block: publication counter and
+    // stop flag, slot info, two state slots, every save and its position, the
+    // statistics records
+    const size_t NS = (size_t)c->n_state;
+    const size_t nsave = (size_t)(o->save_n > 0 ? o->save_n : 0);
+    const size_t ncap = (size_t)(c->st_on ? c->st_cap : 0);
+    const size_t off_info = 256, off_y = 512, off_sv = off_y + 2 * NS * sizeof(double2),
+                 off_rec = off_sv + nsave * NS * sizeof(double2), total = off_rec + ncap * LH_NSTAT * sizeof(double);
+    unsigned char* hb = nullptr;
+    HIPCK(hipHostMalloc((void**)&hb, total, hipHostMallocMapped | hipHostMallocCoherent));
+    unsigned char* db = nullptr;
+    if (hipHostGetDevicePointer((void**)&db, hb, 0) != hipSuccess) {
+        (void)hipHostFree(hb);
+        return LH_E_HIP;
+    }
+    memset(hb, 0, off_y);
+    unsigned long long* pub = (unsigned long long*)hb;
+    int* stop = (int*)(hb + 64);
+    const double* info = (const double*)(hb + off_info);
+    const double2* ys = (const double2*)(hb + off_y);
+    const double2* sv = (const double2*)(hb + off_sv);
+    const double* recs = (const double*)(hb + off_rec);
+    DInst in;
+    st = prepare_instance(c, o, false, &in, c->dstats, c->stream);
+    std::vector<double> zg;
+    save_grid(o, zg);
+    if (!st) {
+        in.saves = nsave ? (double2*)(db + off_sv) : nullptr;
+        if (ncap) in.steprec = (double*)(db + off_rec);
+        in.pub = (unsigned long long*)db;
+        in.pub_stop = (const int*)(db + 64);
+        in.pub_info = (double*)(db + off_info);
+        in.pub_y = (double2*)(db + off_y);
+        if (hipMemcpyAsync(c->dinst, &in, sizeof(DInst), hipMemcpyHostToDevice, c->stream) != hipSuccess) st = LH_E_HIP;
+    }
+    if (!st) {
+        launch_solve_stream(cls, c->dinst, tw, c->stream);
+        if (hipGetLastError() != hipSuccess) st = LH_E_HIP;
+    }
+    // Drain the publications while the kernel runs.  A publication's state
+    // slot is rewritten two publications later, so a copy of it counts only
+    // if the counter has not moved on meanwhile; saves and records have slots
+    // of their own.  After the callback asks to stop, nothing more is handed
+    // over (the run ends at its next publication).
+    std::vector<double2> ycopy(NS);
+    unsigned long long done = 0;
+    int saves_done = o->save_first;
+    long long recs_done = 0;
+    bool stop_req = false;
+    while (!st) {
+        const hipError_t q = hipStreamQuery(c->stream);
+        if (q != hipSuccess && q != hipErrorNotReady) {
+            st = LH_E_HIP;
+            break;
+        }
+        unsigned long long p = __atomic_load_n(pub, __ATOMIC_ACQUIRE);
+        if (p > done && !stop_req) {
+            double z = 0, dz = 0, fsaved = 0, facc = 0;
+            for (;;) {
+                const int slot = (int)((p - 1) & 1);
+                z = info[4 * slot];
+                dz = info[4 * slot + 1];
+                fsaved = info[4 * slot + 2];
+                facc = info[4 * slot + 3];
+                memcpy(ycopy.data(), ys + (size_t)slot * NS, NS * sizeof(double2));
+                const unsigned long long p2 = __atomic_load_n(pub, __ATOMIC_ACQUIRE);
+                if (p2 == p) break;
+                p = p2;
+            }
+            const int nsaved = (int)fsaved;
+            const long long nacc = (long long)facc;
+            const long long nrec = ncap ? (nacc < (long long)ncap ? nacc : (long long)ncap) : 0;
+            lh_step_out out;
+            out.first_save = saves_done;
+            out.n_saves = nsaved - saves_done;
+            out.saves = (const double*)(sv + (size_t)saves_done * NS);
+            out.zsaves = zg.data() + saves_done;
+            out.z = z;
+            out.dz = dz;
+            out.y = (const double*)ycopy.data();
+            out.accepted = nacc;
+            out.first_record = (int32_t)recs_done;
+            out.n_records = (int32_t)(nrec - recs_done);
+            out.records = recs + (size_t)recs_done * LH_NSTAT;
+            if (cb(user, &out) != 0) {
+                stop_req = true;
+                __atomic_store_n(stop, 1, __ATOMIC_RELEASE);
+            }
+            done = p;
+            saves_done = nsaved;
+            recs_done = nrec;
+        }
+        if (q == hipSuccess) break;  // p was read after the kernel had ended
+        struct timespec ts = {0, 20000};
+        nanosleep(&ts, nullptr);
+    }
+    if (!st) {
+        if (hipMemcpy(stats, c->dstats, sizeof(lh_stats), hipMemcpyDeviceToHost) != hipSuccess) st = LH_E_HIP;
+    } else {
+        (void)hipStreamSynchronize(c->stream);  // never free a block the kernel may still write
+    }
+    if (!st && hipMemcpy(Ew, c->Y, sizeof(double2) * NS, hipMemcpyDeviceToHost) != hipSuccess) st = LH_E_HIP;
+    if (!st && ncap && c->steprec &&
+        hipMemcpy(c->steprec, recs, sizeof(double) * LH_NSTAT * ncap, hipMemcpyHostToDevice) != hipSuccess)
+        st = LH_E_HIP;
+    (void)hipHostFree(hb);
+    if (st) return st;
+    c->last_status = stats->status;
+    c->have_result = 1;
+    c->result_save_n = 0;  // the saves went to the callback
+    c->result_nrec = ncap ? (int)(stats->accepted < (long long)ncap ? stats->accepted : (long long)ncap) : 0;
+    c->have_records = 1;
+    return stats->status;
+}
+
 int lh_set_state(lh_ctx* c, const double* Ew) {
     if (!c || !Ew) return LH_E_ARG;
     HIPCK(hipSetDevice(c->device));
@@ -2605,8 +2788,11 @@ int lh_get_saves(lh_ctx* c, double* saves, double* zsaves) {
     return LH_OK;
 }
 
-int lh_batch_solve_resident(lh_ctx* const* ctxs, int n, const lh_rk_opts* o, int want_saves, lh_stats* stats) {
-    if (!ctxs || n <= 0 || !o || !stats) return LH_E_ARG;
+// The resident batch with options per instance: op[i] for instance i.
+static int batch_resident(lh_ctx* const* ctxs, int n, const lh_rk_opts* const* op, int want_saves, lh_stats* stats) {
+    if (!ctxs || n <= 0 || !op || !stats) return LH_E_ARG;
+    for (int i = 0; i < n; i++)
+        if (!op[i]) return LH_E_ARG;
     const int dev = ctxs[0]->device;
     for (int i = 0; i < n; i++)
         if (!ctxs[i] || ctxs[i]->device != dev || ctx_class(ctxs[i]) < 0) return LH_E_ARG;
@@ -2636,7 +2822,8 @@ int lh_batch_solve_resident(lh_ctx* const* ctxs, int n, const lh_rk_opts* o, int
     HIPCK(hipMalloc(&dinst, sizeof(DInst) * (size_t)n));
     HIPCK(hipMalloc(&dstats, sizeof(lh_stats) * (size_t)n));
     std::vector<DInst> h(n);
-    for (int i = 0; i < n && !st; i++) st = prepare_instance(ctxs[i], o, want_saves != 0, &h[pos[i]], dstats + pos[i], s0);
+    for (int i = 0; i < n && !st; i++)
+        st = prepare_instance(ctxs[i], op[i], want_saves != 0, &h[pos[i]], dstats + pos[i], s0);
     if (!st && hipMemcpyAsync(dinst, h.data(), sizeof(DInst) * (size_t)n, hipMemcpyHostToDevice, s0) != hipSuccess)
         st = LH_E_HIP;
     std::vector<lh_stats> rad_stats(n);
@@ -2660,7 +2847,7 @@ int lh_batch_solve_resident(lh_ctx* const* ctxs, int n, const lh_rk_opts* o, int
             // context by context (each fills the GPU); field classes run meanwhile
             for (int i = 0; i < n && !st; i++)
                 if (ctx_class(ctxs[i]) == LH_CLASS_RADIAL)
-                    st = rad_solve(ctxs[i], o, want_saves != 0, cs[c], &rad_stats[i]);
+                    st = rad_solve(ctxs[i], op[i], want_saves != 0, cs[c], &rad_stats[i]);
             if (st) break;
         } else {
             bool zdep = false;
@@ -2699,8 +2886,38 @@ int lh_batch_solve_resident(lh_ctx* const* ctxs, int n, const lh_rk_opts* o, int
         ctxs[i]->last_status = stats[i].status;
         ctxs[i]->have_result = 1;
         ctxs[i]->result_nrec = ctxs[i]->st_on ? (int)(stats[i].accepted < ctxs[i]->st_cap ? stats[i].accepted : ctxs[i]->st_cap) : 0;
+        ctxs[i]->have_records = 1;
     }
     return LH_OK;
+}
+
+int lh_batch_solve_resident(lh_ctx* const* ctxs, int n, const lh_rk_opts* o, int want_saves, lh_stats* stats) {
+    if (!o || n <= 0) return LH_E_ARG;
+    std::vector<const lh_rk_opts*> op((size_t)n, o);
+    return batch_resident(ctxs, n, op.data(), want_saves, stats);
+}
+
+int lh_batch_solve_opts(lh_ctx* const* ctxs, int n, double* const* Ew, const lh_rk_opts* opts, double* const* saves,
+                        double* const* zsaves, lh_stats* stats) {
+    if (!ctxs || n <= 0 || !Ew || !opts || !stats) return LH_E_ARG;
+    for (int i = 0; i < n; i++) {
+        if (!ctxs[i] || !Ew[i]) return LH_E_ARG;
+        int st = lh_set_state(ctxs[i], Ew[i]);
+        if (st) return st;
+    }
+    int want = 0;
+    for (int i = 0; i < n; i++)
+        if (saves && saves[i]) want = 1;
+    std::vector<const lh_rk_opts*> op((size_t)n);
+    for (int i = 0; i < n; i++) op[i] = opts + i;
+    int st = batch_resident(ctxs, n, op.data(), want, stats);
+    if (st) return st;
+    for (int i = 0; i < n && !st; i++) {
+        st = lh_get_state(ctxs[i], Ew[i]);
+        if (!st && ((saves && saves[i]) || (zsaves && zsaves[i])))
+            st = lh_get_saves(ctxs[i], saves ? saves[i] : nullptr, zsaves ? zsaves[i] : nullptr);
+    }
+    return st;
 }
 
 int lh_set_step_stats(lh_ctx* c, const lh_stats_desc* sd) {
@@ -2855,7 +3072,7 @@ int lh_stats_eval(lh_ctx* c, const double* Ew, double z, double dz, double* rec)
 }
 
 int lh_get_step_stats(lh_ctx* c, double* rec, int capacity, int* n) {
-    if (!c || !n || capacity < 0 || (!rec && capacity > 0) || !c->have_result) return LH_E_ARG;
+    if (!c || !n || capacity < 0 || (!rec && capacity > 0) || !c->have_records) return LH_E_ARG;
     HIPCK(hipSetDevice(c->device));
     int k = c->result_nrec < capacity ? c->result_nrec : capacity;
     if (k > 0) HIPCK(hipMemcpy(rec, c->steprec, sizeof(double) * LH_NSTAT * (size_t)k, hipMemcpyDeviceToHost));

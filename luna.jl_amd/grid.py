@@ -67,6 +67,12 @@ class RealGrid:
     def dt_o(self):
         return self.to[1] - self.to[0]
 
+    def to_dict(self):
+        """``Grid.to_dict`` (``Grid.jl:238-244``): the struct's fields by name."""
+        return {"zmax": self.zmax, "referenceλ": self.ref_lambda, "t": self.t, "ω": self.omega,
+                "to": self.to, "ωo": self.omega_o, "sidx": self.sidx, "ωwin": self.omega_win,
+                "twin": self.t_win, "towin": self.to_win}
+
 
 class EnvGrid:
     """``Grid.EnvGrid(zmax, referenceλ, λ_lims, trange; δt=1, thg=false)``, ``Grid.jl:115-190``."""
@@ -127,3 +133,9 @@ class EnvGrid:
     n_t = RealGrid.n_t
     n_to = RealGrid.n_to
     dt_o = RealGrid.dt_o
+
+    def to_dict(self):
+        """``Grid.to_dict`` (``Grid.jl:238-244``): the struct's fields by name."""
+        return {"zmax": self.zmax, "referenceλ": self.ref_lambda, "ω0": self.omega0, "t": self.t,
+                "ω": self.omega, "to": self.to, "ωo": self.omega_o, "sidx": self.sidx,
+                "ωwin": self.omega_win, "twin": self.t_win, "towin": self.to_win}

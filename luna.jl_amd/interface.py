@@ -5,6 +5,7 @@ builders ``:453-616, 773-835``; ``prop_gnlse_args`` ``:921-979``) and the driver
 ``Luna.run`` (``src/Luna.jl:358-391``).  The propagation itself (RK45 loop,
 window filter, saves) runs on the GPU via ``lh_solve``.
 """
+import logging
 import math
 import warnings
 
@@ -23,50 +24,66 @@ from .modes import MarcatiliMode, SimpleMode
 from .nonlinear import (CombinedRamanResponse, Kerr_env, Kerr_field, PlasmaCumtrapz, RamanPolarEnv,
                         RamanRespNormedSingleDampedOscillator)
 from .nonlinearrhs import TransModeAvg, TransRadial, norm_mode_average, norm_mode_average_gnlse
+from .output import HDF5Output, MemoryOutput  # noqa: F401 (re-exported: Output.MemoryOutput)
+
+log = logging.getLogger("luna_jl_amd")
 
 
-class MemoryOutput:
-    """``Output.MemoryOutput(0, zmax, saveN, statsfun)`` with ``GridCondition``
-    saves (``Output.jl:15-141``); ``statsfun`` from ``stats.default`` is
-    recorded on the device at every accepted step."""
-
-    def __init__(self, tmin, tmax, saveN, statsfun=None):
-        self.tmin, self.tmax, self.saveN = tmin, tmax, saveN
-        self.statsfun = statsfun
-        self.data = {"stats": {}, "meta": {}}
-
-    def __getitem__(self, k):
-        return self.data[k]
-
-    def __contains__(self, k):
-        return k in self.data
+def simtype(grid, transform, linop):
+    """``simtype`` (``src/Luna.jl:327-336``): the ``simulation_type`` group."""
+    variable = callable(linop) or getattr(transform, "gradient", None) is not None
+    return {"field": "field-resolved" if grid.kind == "real" else "envelope",
+            "transform": str(transform), "linop": "variable" if variable else "constant"}
 
 
 def run(Ew, grid, linop, transform, FT=None, output=None, min_dz=0.0, max_dz=None, init_dz=1e-4,
         z0=0.0, rtol=1e-6, atol=1e-10, safety=0.9, status_period=1):
-    """``Luna.run``: propagate to ``grid.zmax`` on the GPU; fills ``output``."""
-    saveN = output.saveN if output is not None else 201
+    """``Luna.run`` (``src/Luna.jl:358-391``): propagate to ``grid.zmax`` on the
+    GPU.  A cached propagation in the output (``HDF5Output``) is resumed from
+    its cache; the grid and simulation type are written to the output; the
+    device solve fills a ``MemoryOutput`` from its save buffer and streams
+    every saving step to any other output (``lh_solve_stream``)."""
+    if output is None:
+        output = MemoryOutput(0, grid.zmax, 201)
+    save_first = 0
+    Ewc, zc, dzc = output.check_cache(Ew, z0, init_dz)
+    if zc > z0:
+        log.info("Found cached propagation. Resuming...")
+        Ew, z0, init_dz = Ewc, zc, dzc
+        save_first = output.saved
+    output(grid.to_dict(), group="grid")
+    output(simtype(grid, transform, linop), group="simulation_type")
     opts = _capi.rk_opts(grid.zmax, z0=z0, rtol=rtol, atol=atol, safety=safety,
                          max_dz=grid.zmax / 2 if max_dz is None else max_dz, min_dz=min_dz,
-                         init_dz=init_dz, save_n=saveN)
+                         init_dz=init_dz, save_n=output.saveN, save_z0=output.tmin, save_first=save_first)
     ctx = transform.context(linop)
-    statsfun = getattr(output, "statsfun", None)
+    statsfun = output.statsfun
     ctx.set_step_stats(statsfun)
-    Ew_out, saves, zs, stats = ctx.solve(Ew, opts)
-    if statsfun is not None:
-        rec = ctx.step_stats()
-        if stats.accepted > len(rec):
-            warnings.warn(f"statistics kept for the first {len(rec)} of {stats.accepted} steps "
-                          "(raise the capacity of stats.default)", RuntimeWarning)
-        output.data["stats"] = statsfun.to_dict(rec)
-    if output is not None:
-        n = grid.n_omega
+    shape = (grid.n_omega,) + ((transform.n_r,) if transform.n_r > 1 else ())
+    if (isinstance(output, MemoryOutput) or transform.kind == _capi.LH_TRANS_RADIAL
+            or getattr(transform, "gradient", None) is not None):
+        # the whole run in the device save buffer, handed over at the end
+        # (radial runs and z-dependent fills are not streamed)
+        Ew_out, saves, zs, stats = ctx.solve(Ew, opts)
+        _capi.check(stats.status, "Luna.run")
+        rec = ctx.step_stats() if statsfun is not None else np.zeros((0, _capi.LH_NSTAT))
+        n = output.saveN - save_first
         # saves[s] holds the state column-major (ω fastest): → (Nω[, Nr], saveN)
-        output.data["Eω"] = saves.reshape(saveN, -1, n).transpose(2, 1, 0).squeeze(1)
-        output.data["z"] = zs[:saveN]
-        output.data["grid"] = {k: getattr(grid, k) for k in ("t", "omega", "to", "omega_o", "sidx",
-                                                              "omega_win", "t_win", "to_win")}
-        output.data["meta"]["solver"] = stats.as_dict()
+        sv = saves[save_first:].reshape(n, -1, grid.n_omega).transpose(2, 1, 0).reshape(shape + (n,), order="F")
+        output.on_step(save_first, sv, zs[save_first:output.saveN], stats.z, stats.dz,
+                       Ew_out.reshape(shape, order="F"), rec)
+    else:
+        def on_step(first, saves, zsv, z, dz, y, records):
+            saves = saves.reshape(shape + (saves.shape[-1],), order="F")
+            return output.on_step(first, saves, zsv, z, dz, y.reshape(shape, order="F"), records)
+
+        Ew_out, stats = ctx.solve_stream(Ew, opts, on_step)
+        if stats.status != _capi.LH_E_STOPPED:
+            _capi.check(stats.status, "Luna.run")
+    if statsfun is not None and stats.accepted > statsfun.capacity:
+        warnings.warn(f"statistics kept for the first {statsfun.capacity} of {stats.accepted} steps "
+                      "(raise the capacity of stats.default)", RuntimeWarning)
+    output("solver", stats.as_dict(), meta=True, force=True)
     return Ew_out, stats
 
 

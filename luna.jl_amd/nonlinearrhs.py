@@ -139,6 +139,16 @@ class _TransformBase:
         out = self.context().rhs(Ew, z)
         nl[...] = out.reshape(nl.shape, order="F")
 
+    def __str__(self):
+        """``show(io, t::TransModeAvg)`` / ``TransRadial`` (``NonlinearRHS.jl:336-342,
+        420-428``), the ``transform`` entry of the output's ``simulation_type``."""
+        g = self.grid
+        parts = [type(self).__name__, f"grid type: {type(g).__name__}", f"time grid size: {g.n_t} / {g.n_to}"]
+        if self.kind == _capi.LH_TRANS_RADIAL:
+            parts += [f"radial points: {self.q.N}", f"aperture: {self.q.R}"]
+        parts.append("responses: " + "\n    ".join(type(r).__name__ for r in self.responses))
+        return "\n  ".join(parts)
+
 
 class TransModeAvg(_TransformBase):
     """``TransModeAvg(grid, FT, resp, densityfun, norm!, aeff)`` (FT implied)."""

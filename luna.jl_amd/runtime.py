@@ -55,6 +55,43 @@ class Context:
         return Ew, saves, zs, stats
 
 
+    def solve_stream(self, Ew0, opts, on_step):
+        """``lh_solve_stream``: the device solve with every accepted step that
+        saves handed to ``on_step(first_save, saves, zs, z, dz, y, records)``
+        while the kernel runs (saves: (n_state, n) column per save; y: the
+        filtered state at z; records: [n][LH_NSTAT] statistics of the steps
+        since the previous call).  ``on_step`` returning True stops the run.
+        An exception raised by ``on_step`` stops it too and is re-raised."""
+        Ew = np.ascontiguousarray(np.asarray(Ew0, dtype=np.complex128).reshape(-1, order="F")).copy()
+        ns = self.n_state
+        failure = []
+
+        def cb(user, out):
+            try:
+                o = out.contents
+                sv = np.ctypeslib.as_array(o.saves, shape=(o.n_saves * ns * 2,)).view(np.complex128)
+                saves = sv.reshape(o.n_saves, ns).T.copy()
+                zs = np.ctypeslib.as_array(o.zsaves, shape=(o.n_saves,)).copy()
+                y = np.ctypeslib.as_array(o.y, shape=(2 * ns,)).view(np.complex128).copy()
+                if o.n_records > 0:
+                    rec = np.ctypeslib.as_array(o.records, shape=(o.n_records * _capi.LH_NSTAT,))
+                    rec = rec.reshape(o.n_records, _capi.LH_NSTAT).copy()
+                else:
+                    rec = np.zeros((0, _capi.LH_NSTAT))
+                return 1 if on_step(o.first_save, saves, zs, o.z, o.dz, y, rec) else 0
+            except BaseException as e:  # noqa: BLE001 -- handed back to the caller below
+                failure.append(e)
+                return 1
+
+        stats = _capi.LhStats()
+        st = _capi.lib().lh_solve_stream(self.handle, _p(Ew), C.byref(opts), _capi.LhOutputCb(cb), None,
+                                         C.byref(stats))
+        if failure:
+            raise failure[0]
+        if st not in (_capi.LH_OK, _capi.LH_E_STOPPED):
+            _capi.check(st, "lh_solve_stream")
+        return Ew, stats
+
     # -- per-step statistics (Stats.default, src/Stats.jl:476-532) ------------
     def set_step_stats(self, statsfun):
         """Record ``statsfun``'s values at every accepted step of later solves
@@ -155,6 +192,25 @@ def batch_solve(contexts, Ews, opts, want_saves=True, out=None):
     stats = (_capi.LhStats * n)()
     st = _capi.lib().lh_batch_solve(handles, n, ew_ptrs, C.byref(opts), sv_ptrs, zs_ptrs, stats)
     _capi.check(st, "lh_batch_solve")
+    return ew_arrs, sv, zs, list(stats)
+
+
+def batch_solve_opts(contexts, Ews, opts_list, want_saves=True):
+    """``lh_batch_solve_opts``: one batch, each instance with its own options
+    (save grid, z range, tolerances)."""
+    n = len(contexts)
+    handles = (C.c_void_p * n)(*[c.handle for c in contexts])
+    ew_arrs = [np.ascontiguousarray(np.asarray(e, dtype=np.complex128).reshape(-1, order="F")).copy() for e in Ews]
+    ew_ptrs = (C.POINTER(C.c_double) * n)(*[_p(a) for a in ew_arrs])
+    sv = [np.zeros((o.save_n, c.n_state), dtype=np.complex128) for c, o in zip(contexts, opts_list)] \
+        if want_saves else [None] * n
+    sv_ptrs = (C.POINTER(C.c_double) * n)(*[_p(a) for a in sv]) if want_saves else None
+    zs = [np.zeros(max(o.save_n, 1)) for o in opts_list]
+    zs_ptrs = (C.POINTER(C.c_double) * n)(*[_p(a) for a in zs])
+    opts = (_capi.LhRkOpts * n)(*opts_list)
+    stats = (_capi.LhStats * n)()
+    _capi.check(_capi.lib().lh_batch_solve_opts(handles, n, ew_ptrs, opts, sv_ptrs, zs_ptrs, stats),
+                "lh_batch_solve_opts")
     return ew_arrs, sv, zs, list(stats)
 
 

@@ -8,12 +8,14 @@ import pytest
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-HEADERS = ("luna_hip.h", "luna_fno.h")
+HIP_HEADERS = ("luna_hip.h", "luna_fno.h")   # libluna_hip.so
+H5_HEADERS = ("luna_h5.h",)                  # libluna_h5.so (built where HDF5 is installed)
+HEADERS = HIP_HEADERS + H5_HEADERS
 
 
-def declared_symbols():
+def declared_symbols(headers=HIP_HEADERS):
     syms = set()
-    for h in HEADERS:
+    for h in headers:
         text = open(os.path.join(ROOT, "include", h)).read()
         text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
         syms |= set(re.findall(r"\b(lh_[a-z_0-9]+)\s*\(", text))
@@ -36,6 +38,15 @@ def test_library_exports_all_declared(capi):
     for s in declared_symbols():
         assert hasattr(L, s), s
     assert set(capi.EXPORTED) == set(declared_symbols())
+
+
+def test_h5_library_exports_all_declared():
+    from luna_jl_amd import output
+    if not os.path.isfile(output.H5_LIB_PATH):
+        pytest.skip("libluna_h5.so not built (no HDF5 library)")
+    L = output.h5lib()
+    for s in declared_symbols(H5_HEADERS):
+        assert hasattr(L, s), s
 
 
 def test_strerror_and_arg_checks(capi):

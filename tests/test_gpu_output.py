@@ -1,0 +1,168 @@
+"""Per-step output of the device solve (lh_solve_stream) and HDF5Output.
+
+* Streaming changes nothing: the saves, the final state and the statistics
+  records handed out step by step equal lh_solve's bit for bit.
+* "HDF5 vs Memory" and "Continuing" of test/test_output.jl:122-276 on the same
+  configuration (13 μm Ar capillary at 5 bar, no loss, Kerr only, 1 μJ 30 fs
+  at 800 nm, 5 cm, 51 saves): the file written while the kernel runs equals the
+  MemoryOutput run; a run whose output raises after the first step beyond 3 cm
+  leaves a cache from which a second run resumes, and the joined file agrees
+  with the uninterrupted run as that test requires.
+* A batched scan (runscan_batched) with ScanHDF5Output files, one point of
+  which fails mid-run: the failure is reported and skipped, the other points'
+  files equal their single runs.
+"""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _continuation_case(pkg, statsfun=True):
+    from luna_jl_amd import interface, stats
+    Ew0, grid, linop, tr, _, _ = interface.prop_capillary_args(
+        13e-6, 5e-2, "Ar", 5.0, lambda0=800e-9, lambda_lims=(160e-9, 3000e-9), trange=1e-12,
+        tau_fwhm=30e-15, energy=1e-6, shotnoise=False, plasma=False, loss=False)
+    sf = stats.StatsFun(grid, tr, capacity=4096) if statsfun else None
+    return Ew0, grid, linop, tr, sf
+
+
+def test_stream_equals_solve(pkg, capi):
+    print("case: lh_solve_stream vs lh_solve", flush=True)
+    from luna_jl_amd import configs, runtime, stats
+    inst = configs.capillary_instance(energy=150e-6, pressure=2.0, trange=0.5e-12, flength=0.2)
+    ctx = runtime.Context(inst.holder)
+    ctx.set_step_stats(stats.StatsFun(inst.grid, inst.transform, capacity=4096))
+    opts = capi.rk_opts(inst.zmax, save_n=41)
+    Ew_a, sv_a, zs_a, st_a = ctx.solve(inst.Ew0, opts)
+    rec_a = ctx.step_stats()
+    got = {"saves": np.zeros_like(sv_a), "zs": np.zeros(41), "rec": [], "calls": 0, "next": 0}
+
+    def on_step(first, saves, zs, z, dz, y, rec):
+        assert first == got["next"] and saves.shape[1] >= 1
+        got["saves"][first:first + saves.shape[1]] = saves.T
+        got["zs"][first:first + saves.shape[1]] = zs
+        got["next"] = first + saves.shape[1]
+        got["rec"].append(rec)
+        got["calls"] += 1
+        # the state handed over is the filtered state of a step at z
+        assert z >= zs[-1] and dz > 0 and y.shape == (ctx.n_state,)
+        return False
+
+    Ew_b, st_b = ctx.solve_stream(inst.Ew0, opts, on_step)
+    assert st_a.status == st_b.status == 0
+    assert (st_a.steps, st_a.accepted, st_a.saved) == (st_b.steps, st_b.accepted, st_b.saved)
+    assert np.array_equal(Ew_a, Ew_b)
+    assert got["next"] == 41 and 2 <= got["calls"] <= 41
+    assert np.array_equal(got["saves"], sv_a) and np.array_equal(got["zs"], zs_a[:41])
+    rec_b = np.concatenate(got["rec"])
+    assert len(rec_b) <= len(rec_a)
+    assert np.array_equal(rec_b, rec_a[:len(rec_b)], equal_nan=True)
+    assert np.array_equal(ctx.step_stats(), rec_a, equal_nan=True)
+
+
+def test_hdf5_vs_memory(pkg, tmp_path):
+    print("case: HDF5Output vs MemoryOutput, 5 cm Ar", flush=True)
+    from luna_jl_amd import interface, output
+    Ew0, grid, linop, tr, sf = _continuation_case(pkg)
+    fpath = str(tmp_path / "run.h5")
+    hdf5 = output.HDF5Output(fpath, 0, grid.zmax, 51, sf)
+    mem = output.MemoryOutput(0, grid.zmax, 51, sf)
+    for o in (hdf5, mem):
+        o({"λ0": 800e-9})
+        o("τ", 30e-15)
+    interface.run(Ew0, grid, linop, tr, None, hdf5)
+    interface.run(Ew0, grid, linop, tr, None, mem)
+    with output.H5File(fpath) as f:
+        assert f.read("λ0") == mem.data["λ0"]
+        assert np.array_equal(f.read("Eω"), mem.data["Eω"])
+        assert np.array_equal(f.read("stats/ω0"), mem.data["stats"]["ω0"])
+        assert np.array_equal(f.read("z"), mem.data["z"])
+        g = f.read("grid")
+        ref = grid.to_dict()
+        assert set(g) == set(ref)
+        for k, v in ref.items():
+            assert np.array_equal(np.asarray(g[k]).astype(np.asarray(v).dtype), v), k
+        assert f.read("simulation_type/field") == "field-resolved"
+        assert f.read("simulation_type/transform") == str(tr)
+        assert f.read("simulation_type/linop") == "constant"
+    ro = output.HDF5Output.open(fpath)
+    assert np.array_equal(ro["Eω", slice(None), 0], mem["Eω"][:, 0])
+    assert np.array_equal(ro["Eω", 1, slice(None)], mem["Eω"][1, :])
+    assert np.array_equal(ro["Eω", slice(None), [0, 1, 49]], mem["Eω"][:, [0, 1, 49]])
+
+
+def test_continuing(pkg, tmp_path):
+    print("case: HDF5Output interrupted at 3 cm and continued", flush=True)
+    from luna_jl_amd import interface, output
+    Ew0, grid, linop, tr, sf = _continuation_case(pkg)
+    fpath = str(tmp_path / "cont.h5")
+
+    class Failing(output.HDF5Output):
+        def on_step(self, *a):
+            super().on_step(*a)
+            if a[3] > 3e-2:
+                raise RuntimeError("Oh no!")
+
+    with pytest.raises(RuntimeError, match="Oh no"):
+        interface.run(Ew0, grid, linop, tr, None, Failing(fpath, 0, grid.zmax, 51, sf))
+    first = output.HDF5Output.open(fpath)
+    tc = first["meta", "cache", "t"]
+    saved_c = first["meta", "cache", "saved"]
+    assert 3e-2 < tc < 5e-2 and 0 < saved_c < 51
+    # run again: the cache is found and the propagation resumes from it
+    o = output.HDF5Output(fpath, 0, grid.zmax, 51, sf)
+    assert o.saved == saved_c
+    _, st = interface.run(Ew0, grid, linop, tr, None, o)
+    assert st.status == 0 and st.saved == 51
+    mem = output.MemoryOutput(0, grid.zmax, 51, sf)
+    interface.run(Ew0, grid, linop, tr, None, mem)
+    band = np.flatnonzero(grid.omega_win == 1)
+    idx1, idx2 = np.argmax(grid.omega_win != 1), band[-1]
+    Ewm = mem["Eω"][idx1:idx2 + 1, :]
+    Ewh = o["Eω"][idx1:idx2 + 1, :]
+    Iw, Iwm = np.abs(Ewh) ** 2, np.abs(Ewm) ** 2
+    assert np.linalg.norm(Iw - Iwm) / np.linalg.norm(Iw) < 1e-7
+    assert np.allclose(Ewm, Ewh, atol=1e-4 * np.max(np.abs(Ewm)), rtol=0)
+    assert np.allclose(o["stats"]["ω0"], mem.data["stats"]["ω0"], rtol=1e-6)
+    assert np.allclose(o["stats"]["energy"], mem.data["stats"]["energy"], rtol=1e-12)
+    assert np.array_equal(o["z"], mem.data["z"])
+    assert o["simulation_type"]["field"] == "field-resolved"
+    assert o["simulation_type"]["transform"] == str(tr)
+
+
+def test_batched_scan_with_files(pkg, capi, tmp_path):
+    print("case: runscan_batched, 2 x 2 points, one failing mid-run", flush=True)
+    import warnings
+    from luna_jl_amd import interface, output, scans
+
+    scan = scans.Scan("capscan", energy=np.array([80e-6, 150e-6]), pressure=np.array([0.7, 2.5]))
+    fdir = str(tmp_path / "scanout")
+
+    def point(energy, pressure):
+        return interface.prop_capillary_args(
+            125e-6, 0.1, "He", pressure, lambda0=800e-9, lambda_lims=(150e-9, 4e-6), trange=0.5e-12,
+            tau_fwhm=10e-15, energy=energy, shotnoise=False, ppt_N=2 ** 12)[:4]
+
+    def make_point(scanidx, energy, pressure):
+        Ew, grid, linop, tr = point(energy, pressure)
+        out = output.ScanHDF5Output(scan, scanidx, 0, grid.zmax, 11, fdir=fdir)
+        # point 3 cannot meet its tolerance: every attempt is rejected and the
+        # instance ends with the repeat limit while the others run on
+        return Ew, grid, linop, tr, None, out, ({"rtol": 1e-200} if scanidx == 3 else {})
+
+    with warnings.catch_warnings(record=True) as w:
+        warnings.simplefilter("always")
+        done = scans.runscan_batched(scan, make_point)
+    assert sorted(done) == [1, 2, 4]
+    assert any("Error at scanidx 3" in str(x.message) and "repetition" in str(x.message) for x in w)
+    for idx in (1, 2, 4):
+        e, p = scan.values(idx)
+        Ew, grid, linop, tr = point(e, p)
+        mem = output.MemoryOutput(0, grid.zmax, 11)
+        interface.run(Ew, grid, linop, tr, None, mem)
+        f = output.HDF5Output.open(os.path.join(fdir, scans.makefilename(scan, idx)))
+        assert np.array_equal(f["Eω"], mem["Eω"]) and np.array_equal(f["z"], mem["z"])
+        assert f["meta", "scanidx"] == idx and f["meta", "scanvars", "energy"] == e
