@@ -40,11 +40,13 @@ enum {
     LH_E_MAXSTEPS = 8,     /* step-attempt cap (lh_rk_opts.max_steps) reached:
                               no counterpart in the reference, whose loop is
                               unbounded (src/RK45.jl:55-89)                  */
-    LH_E_STOPPED = 9       /* stopped at lh_rk_opts.zstop before zmax: the
+    LH_E_STOPPED = 9,      /* stopped at lh_rk_opts.zstop before zmax: the
                               analogue of a run interrupted after an accepted
                               step (the cached-continuation test,
                               test/test_output.jl:203-276); resume from the
                               returned state, z, dz and saved count          */
+    LH_E_TOO_LARGE = 10    /* the problem does not fit the fixed working
+                              memory of lh_solve_noalloc                     */
 };
 
 enum { LH_GRID_REAL = 0, LH_GRID_ENV = 1 };           /* Grid.RealGrid / Grid.EnvGrid */
@@ -368,6 +370,30 @@ int lh_last_launch_times(double *ms, int *n_inst, int nclass);
  * (LH_E_NOMEM if refused); lh_host_free releases it (NULL is a no-op). */
 int lh_host_alloc(size_t bytes, void **ptr);
 int lh_host_free(void *ptr);
+
+/* Allocation-free solve.  No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  The same device solve as
+ * lh_solve on one mode-averaged context (field or envelope grid), with the
+ * solver's working memory -- the instance record, the run counters, the save
+ * grid, the second state buffer, the seven stage vectors and the envelope
+ * Raman stash -- in one fixed buffer of LH_NOALLOC_SCRATCH_BYTES per GPU,
+ * which the library holds until the process ends.  The call allocates nothing
+ * (device or host): set the context up beforehand (lh_create, and
+ * lh_set_step_stats / lh_set_gradient if used).
+ *   Ew_dev    device pointer, complex [n_omega]: initial field in, final state out
+ *   saves_dev device pointer, complex [save_n][n_omega], or NULL (no saves)
+ *   zsaves    host array [save_n] receiving the save positions, or NULL
+ *   stream    hipStream_t to run on (NULL: the context's stream); the call
+ *             returns once that stream has finished the solve
+ * Returns the run's status as lh_solve does; LH_E_TOO_LARGE when the working
+ * memory for n_omega and save_n exceeds LH_NOALLOC_SCRATCH_BYTES (every
+ * supported grid fits with up to 30000 saves), LH_E_ARG for radial contexts.
+ * Every call on one GPU uses that GPU's buffer: calls that overlap in time on
+ * one GPU race and must not be issued. */
+#define LH_NOALLOC_SCRATCH_BYTES (1536u * 1024u)
+int lh_solve_noalloc(lh_ctx *ctx, void *Ew_dev, const lh_rk_opts *opts, void *saves_dev, double *zsaves,
+                     lh_stats *stats, void *stream);
 
 #ifdef __cplusplus
 }

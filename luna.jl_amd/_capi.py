@@ -10,7 +10,8 @@ import os
 import numpy as np
 
 (LH_OK, LH_E_ARG, LH_E_HIP, LH_E_PPT_RANGE, LH_E_REPEAT_LIMIT, LH_E_NONFINITE, LH_E_NOMEM, LH_E_NODEVICE,
- LH_E_MAXSTEPS, LH_E_STOPPED) = range(10)
+ LH_E_MAXSTEPS, LH_E_STOPPED, LH_E_TOO_LARGE) = range(11)
+LH_NOALLOC_SCRATCH_BYTES = 1536 * 1024
 LH_GRID_REAL, LH_GRID_ENV = 0, 1
 LH_TRANS_MODEAVG, LH_TRANS_RADIAL = 0, 1
 LH_KERR_NONE, LH_KERR_FIELD, LH_KERR_ENV = 0, 1, 2
@@ -199,6 +200,8 @@ def lib():
     L.lh_last_launch_times.argtypes = [_dp, C.POINTER(C.c_int), C.c_int]
     L.lh_host_alloc.argtypes = [C.c_size_t, C.POINTER(C.c_void_p)]
     L.lh_host_free.argtypes = [C.c_void_p]
+    L.lh_solve_noalloc.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(LhRkOpts), C.c_void_p, _dp,
+                                   C.POINTER(LhStats), C.c_void_p]
     L.lh_set_step_stats.argtypes = [C.c_void_p, C.POINTER(LhStatsDesc)]
     L.lh_set_gradient.argtypes = [C.c_void_p, C.POINTER(LhGradientDesc)]
     L.lh_ppt_table.argtypes = [C.POINTER(LhPptDesc), C.c_int, _dp, _dp, _dp]
@@ -221,7 +224,7 @@ def lib():
 EXPORTED = ["lh_strerror", "lh_device_count", "lh_create", "lh_destroy", "lh_rhs",
             "lh_rhs_device", "lh_ctx_status", "lh_solve", "lh_solve_stream", "lh_batch_solve", "lh_batch_solve_opts", "lh_set_state",
             "lh_batch_solve_resident", "lh_get_state", "lh_get_saves", "lh_last_launch_times",
-            "lh_host_alloc", "lh_host_free",
+            "lh_host_alloc", "lh_host_free", "lh_solve_noalloc",
             "lh_set_step_stats", "lh_stats_eval", "lh_get_step_stats", "lh_set_gradient", "lh_ppt_table",
             "lh_fno_create", "lh_fno_destroy", "lh_fno_param_count", "lh_fno_set_params",
             "lh_fno_get_params", "lh_fno_forward", "lh_fno_loss_grad", "lh_fno_set_data", "lh_fno_train"]

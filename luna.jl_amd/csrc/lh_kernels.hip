@@ -1925,7 +1925,7 @@ struct lh_ctx {
     double* steprec;     // [st_cap][LH_NSTAT]
     double* st_rec1;     // [LH_NSTAT], lh_stats_eval
     int result_nrec;     // records kept by the last solve
-    int have_records;    // the last solve (lh_batch_solve* or lh_solve_stream) ran
+    int have_records;    // the last solve (lh_batch_solve* or lh_solve_noalloc) ran
     // gradient fill
     int zg_on, zg_np, zg_two, zg_model, zg_loss, zd_n;
     double zsqrt_aeff, zfd_h;
@@ -1944,6 +1944,13 @@ struct lh_ctx {
 
 static std::mutex g_mu;
 static double2* g_tw[64] = {nullptr};
+
+// Working memory of lh_solve_noalloc: one fixed array per GPU, held until the
+// process ends.  Its name is a host-side stand-in, not a device address; host
+// code uses only the per-device address that hipGetSymbolAddress returns
+// (g_scratch, set once per device by lh_create with that device current).
+static __device__ __align__(256) unsigned char scratch[LH_NOALLOC_SCRATCH_BYTES];
+static unsigned char* g_scratch[64] = {nullptr};
 // HIP-event durations of the last resident batch, per size class
 static float g_last_ms[8] = {0};
 static int g_last_n[8] = {0};
@@ -1989,6 +1996,19 @@ static void sincos_frac(long m, long N, double* c, double* s) {
     *s = (double)S;
 }
 
+// Device address of `scratch` on dev (the current device), looked up once.
+static int scratch_init(int dev) {
+    std::lock_guard<std::mutex> g(g_mu);
+    if (dev < 0 || dev >= 64) return LH_E_ARG;
+    if (!g_scratch[dev]) {
+        void* p = nullptr;
+        HIPCK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return LH_E_HIP;
+        g_scratch[dev] = (unsigned char*)p;
+    }
+    return LH_OK;
+}
+
 // Global twiddle table tw[m] = exp(-2πi m/LH_TWN), one per device.
 static int twiddles(int dev, const double2** out) {
     std::lock_guard<std::mutex> g(g_mu);
@@ -2013,6 +2033,17 @@ static int twiddles(int dev, const double2** out) {
 // grid) are uploaded once per device and reference-counted: identical content
 // maps to one device buffer, which then stays resident in the L2/MALL caches
 // for every instance of a batch instead of one private copy each.
+// The twiddle table and scratch address of dev as lh_create left them; null
+// if no context was ever created there (nothing is allocated here).
+static const double2* twiddles_existing(int dev) {
+    std::lock_guard<std::mutex> g(g_mu);
+    return (dev >= 0 && dev < 64) ? g_tw[dev] : nullptr;
+}
+static unsigned char* scratch_existing(int dev) {
+    std::lock_guard<std::mutex> g(g_mu);
+    return (dev >= 0 && dev < 64) ? g_scratch[dev] : nullptr;
+}
+
 struct SharedBuf {
     int device;
     uint64_t hash;
@@ -2438,6 +2469,7 @@ const char* lh_strerror(int status) {
         case LH_E_NODEVICE: return "no HIP device";
         case LH_E_MAXSTEPS: return "reached the cap on step attempts";
         case LH_E_STOPPED: return "stopped before the end of the propagation (zstop)";
+        case LH_E_TOO_LARGE: return "problem exceeds the fixed working memory of lh_solve_noalloc";
         default: return "unknown status";
     }
 }
@@ -2489,6 +2521,7 @@ int lh_create(lh_ctx** out, const lh_desc* d) {
     c->n_state = d->n_omega * d->n_r;
     int st = LH_OK;
     if (hipGetDevice(&c->device) != hipSuccess) st = LH_E_HIP;
+    if (!st) st = scratch_init(c->device);
     if (!st && hipStreamCreate(&c->stream) != hipSuccess) st = LH_E_HIP;
     const size_t N = (size_t)d->n_omega, NS = (size_t)c->n_state;
     if (!st) st = shared_upload(c->device, &c->omega_win, d->omega_win, N);
@@ -3104,6 +3137,81 @@ int lh_host_alloc(size_t bytes, void** ptr) {
 int lh_host_free(void* ptr) {
     if (!ptr) return LH_OK;
     return hipHostFree(ptr) == hipSuccess ? LH_OK : LH_E_HIP;
+}
+
+// Offsets of the working memory of one lh_solve_noalloc call inside `scratch`
+// (each part 256-byte aligned): instance record, run counters, save grid, the
+// second state buffer YN, the stage-vector block (8 slots as in
+// ensure_solver_buffers) and the envelope Raman stash.  The first state buffer
+// is the caller's Ew_dev.
+struct NoallocLayout {
+    size_t inst, stats, zgrid, yn, k, escr, total, kstride;
+};
+static NoallocLayout noalloc_layout(size_t ns, int save_n, size_t n_escr) {
+    auto up = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    NoallocLayout L;
+    L.kstride = (ns + 15) & ~(size_t)15;
+    L.inst = 0;
+    L.stats = up(sizeof(DInst));
+    L.zgrid = L.stats + up(sizeof(lh_stats));
+    L.yn = L.zgrid + up(sizeof(double) * (size_t)(save_n > 0 ? save_n : 1));
+    L.k = L.yn + up(sizeof(double2) * ns);
+    L.escr = L.k + sizeof(double2) * L.kstride * 8;
+    L.total = L.escr + up(sizeof(double2) * n_escr);
+    return L;
+}
+
+int lh_solve_noalloc(lh_ctx* c, void* Ew_dev, const lh_rk_opts* o, void* saves_dev, double* zsaves, lh_stats* stats,
+                     void* stream) {
+    if (!c || !Ew_dev || !o || !stats || !opts_ok(o)) return LH_E_ARG;
+    const int cls = ctx_class(c);
+    if (c->d.transform != LH_TRANS_MODEAVG || cls < 0 || cls == LH_CLASS_RADIAL) return LH_E_ARG;
+    const size_t NS = (size_t)c->n_state;
+    const NoallocLayout L = noalloc_layout(NS, o->save_n, c->escr ? (size_t)c->d.n_to : 0);
+    if (L.total > LH_NOALLOC_SCRATCH_BYTES) return LH_E_TOO_LARGE;
+    HIPCK(hipSetDevice(c->device));
+    unsigned char* base = scratch_existing(c->device);
+    const double2* tw = twiddles_existing(c->device);
+    if (!base || !tw) return LH_E_HIP;  // set by lh_create on this device
+    hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+    DInst in = base_inst(c);
+    in.Yin = (const double2*)Ew_dev;  // read once into Y, the same buffer
+    in.Y = (double2*)Ew_dev;          // receives the final state
+    in.YN = (double2*)(base + L.yn);
+    for (int j = 0; j < 7; j++) in.K[j] = (double2*)(base + L.k) + (size_t)j * L.kstride;
+    in.kstride = (long long)L.kstride;
+    if (c->escr) in.escr = (double2*)(base + L.escr);
+    in.saves = (double2*)saves_dev;
+    in.zgrid = (const double*)(base + L.zgrid);
+    in.stats = (lh_stats*)(base + L.stats);
+    inst_options(&in, o);
+    // Nothing in the working memory needs clearing: the kernel writes every
+    // state and stage vector before reading it, and the counters at the end.
+    // Host staging lives on this stack frame; each copy is waited for before
+    // the frame's buffer is reused or left.
+    HIPCK(hipMemcpyAsync(base + L.inst, &in, sizeof(DInst), hipMemcpyHostToDevice, s));
+    const int zn = o->save_n > 0 ? o->save_n : 1;
+    double chunk[256];
+    for (int i0 = 0; i0 < zn; i0 += 256) {
+        const int m = (zn - i0 < 256) ? zn - i0 : 256;
+        for (int i = 0; i < m; i++) chunk[i] = save_grid_at(o, i0 + i);
+        HIPCK(hipMemcpyAsync(base + L.zgrid + sizeof(double) * (size_t)i0, chunk, sizeof(double) * (size_t)m,
+                             hipMemcpyHostToDevice, s));
+        HIPCK(hipStreamSynchronize(s));
+    }
+    launch_solve(cls, 1, (const DInst*)(base + L.inst), tw, s, c->zg_on != 0);
+    HIPCK(hipGetLastError());
+    lh_stats hs;
+    HIPCK(hipMemcpyAsync(&hs, base + L.stats, sizeof(lh_stats), hipMemcpyDeviceToHost, s));
+    HIPCK(hipStreamSynchronize(s));
+    if (zsaves && o->save_n > 0) save_grid_into(o, zsaves);
+    *stats = hs;
+    c->last_status = hs.status;
+    // the state and saves went to the caller's buffers, not the context's
+    c->have_result = 0;
+    c->result_nrec = c->st_on ? (int)(hs.accepted < c->st_cap ? hs.accepted : c->st_cap) : 0;
+    c->have_records = 1;
+    return hs.status;
 }
 
 int lh_batch_solve(lh_ctx* const* ctxs, int n, double* const* Ew, const lh_rk_opts* o, double* const* saves,
