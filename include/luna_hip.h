@@ -359,8 +359,8 @@ int lh_get_step_stats(lh_ctx *ctx, double *rec, int capacity, int *n);
 /* Measurement hook (no counterpart in the reference): HIP-event durations of
  * the persistent launches of the last batch, one per grid-size class
  * (field grid Nto = 16384, 8192, 4096; envelope grid Nto = 8192, 4096; all
- * radial contexts together), with the number of instances in each.
- * nclass must be >= 6. */
+ * radial contexts together; envelope grid Nto = 16384), with the number of
+ * instances in each.  nclass must be >= 7. */
 int lh_last_launch_times(double *ms, int *n_inst, int nclass);
 
 /* Page-locked host memory for the states and saves a sweep copies back (no
@@ -387,8 +387,9 @@ int lh_host_free(void *ptr);
  *   stream    hipStream_t to run on (NULL: the context's stream); the call
  *             returns once that stream has finished the solve
  * Returns the run's status as lh_solve does; LH_E_TOO_LARGE when the working
- * memory for n_omega and save_n exceeds LH_NOALLOC_SCRATCH_BYTES (every
- * supported grid fits with up to 30000 saves), LH_E_ARG for radial contexts.
+ * memory for n_omega and save_n exceeds LH_NOALLOC_SCRATCH_BYTES (the field
+ * grids and the envelope grids up to Nt = 8192 fit with up to 30000 saves;
+ * the Nt = 16384 envelope grid does not), LH_E_ARG for radial contexts.
  * Every call on one GPU uses that GPU's buffer: calls that overlap in time on
  * one GPU race and must not be issued. */
 #define LH_NOALLOC_SCRATCH_BYTES (1536u * 1024u)

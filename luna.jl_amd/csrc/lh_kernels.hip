@@ -361,7 +361,7 @@ __device__ __forceinline__ double field_at(const Smem& sm, int g, double e_scale
 // The series is processed in rounds of QR samples per thread (contiguous
 // chunks), with the three cumtrapz integrals carried from round to round;
 // each thread's samples are touched only by that thread.
-template <int MO, class Smem>
+template <int MO, class Smem, bool RAMAN = false>
 __device__ __forceinline__ void pointwise_field(const DInst& in, Smem& sm, double rho, double kfac) {
     constexpr int QT = 2 * MO / LH_T;          // samples per thread over all rounds
     constexpr int QR = (QT > 16) ? 16 : QT;    // samples per thread per round
@@ -369,6 +369,7 @@ __device__ __forceinline__ void pointwise_field(const DInst& in, Smem& sm, doubl
     const int tid = fresh_tid();
     const double es = in.e_scale;
     const bool kerr = (in.kerr == LH_KERR_FIELD), plasma = in.plasma;
+    const double2* praman = RAMAN ? in.escr + 3 * MO : nullptr;  // P(t) of raman_field, packed pairs
     const double* tw = in.to_win;
     const double dt = in.dt_o, pp1 = in.preion_p1, er = in.e_ratio, ip = in.ionpot;
     double cF = 0.0, cJ = 0.0, cP = 0.0;           // carried integrals
@@ -493,6 +494,11 @@ __device__ __forceinline__ void pointwise_field(const DInst& in, Smem& sm, doubl
                 p0 = p0 + rho * a[2 * s2];
                 p1 = p1 + rho * a[2 * s2 + 1];
             }
+            if constexpr (RAMAN) {  // RamanPolarField: Pout = ρ·E·P (src/Nonlinear.jl:323-325)
+                const double2 pr = praman[g >> 1];
+                p0 = p0 + (rho * E0) * pr.x;
+                p1 = p1 + (rho * E1) * pr.y;
+            }
             sm.buf[pad(g >> 1)] = make_double2(p0 * tw[g], p1 * tw[g + 1]);
         }
     }
@@ -558,7 +564,7 @@ struct EpiStageF {  // out[k] = v·F[k], F = exp(L·hb) stored by the envelope s
 // Field-grid mode-averaged transform on LDS data.  On entry buf[0..N) holds
 // sc_t·Eω (zero to MO), or (PACKED) its c2r pre-pack already.  Hands the
 // output bins k < N to epi.
-template <int MO, class Smem, class Epi, bool PACKED>
+template <int MO, class Smem, class Epi, bool PACKED, bool RAMAN>
 __device__ __forceinline__ void rhs_field(const DInst& in, Smem& sm, Epi epi, double rho, double kfac) {
     __syncthreads();
     if constexpr (!PACKED) {
@@ -566,7 +572,8 @@ __device__ __forceinline__ void rhs_field(const DInst& in, Smem& sm, Epi epi, do
         __syncthreads();
     }
     fft_lds<MO, true>(sm.buf, sm.tw);
-    pointwise_field<MO, Smem>(in, sm, rho, kfac);
+    if constexpr (RAMAN) raman_field<MO, Smem>(in, sm);
+    pointwise_field<MO, Smem, RAMAN>(in, sm, rho, kfac);
     constexpr int N = Bins<MO>::N, NPT = Bins<MO>::NPT;
     double2 of[NPT];  // loads in flight during the forward FFT
 #pragma unroll
@@ -1041,6 +1048,301 @@ __device__ __forceinline__ void stepfun_env(const DInst& in, Smem& sm, double2* 
         __syncthreads();
 }
 
+// ==========================================================================
+// Envelope grid beyond one workgroup's LDS (Nt = Nto = 16384, KIND 2).  Each
+// transform of NO = 2·M points runs as two M-point LDS transforms (even and
+// odd samples) joined by one radix-2 step in HBM; between transforms the
+// series live in the instance's HBM work area escr = [E(t) | W | T]
+// (NO + NO + M complex): E the scaled field, W the working series, T the
+// half-length stash.  Bins per thread exceed the register budget of the
+// LDS-resident classes, so bin loops run in chunks of 8.
+// ==========================================================================
+// exp(−2πi k/L) for the zero-padded Raman transforms: L up to LH_TWN from
+// the table; L = 2·LH_TWN gives even k from the table and odd k one more
+// factor exp(−2πi/L).
+template <int L>
+__device__ __forceinline__ double2 tw_frac(const TwLds& tw, int k) {
+    if constexpr (L <= LH_TWN) {
+        return tw_get(tw, k * (LH_TWN / L));
+    } else {
+        static_assert(L == 2 * LH_TWN, "tw_frac: L up to 2·LH_TWN");
+        double2 w = tw_get(tw, k >> 1);
+        if (k & 1) {
+            double sn, cs;
+            sincos(-6.283185307179586476925286766559 / (double)L, &sn, &cs);
+            w = cmul(w, make_double2(cs, sn));
+        }
+        return w;
+    }
+}
+
+// Unnormalised complex FFT of NO = 2·M points held in HBM (natural order, in
+// place in g); tmp: M points of HBM.  X[k] = E[k] + w^k O[k], X[k+M] =
+// E[k] − w^k O[k], with E, O the transforms of the even and odd samples and
+// w = exp(∓2πi/NO).  Starts and ends with a workgroup barrier.
+template <int M, bool INV, class Smem>
+__device__ __forceinline__ void fft_big(double2* g, double2* tmp, Smem& sm) {
+    const int tid = fresh_tid();
+    __syncthreads();
+    for (int j = tid; j < M; j += LH_T) sm.buf[pad(j)] = g[2 * j];
+    __syncthreads();
+    fft_lds<M, INV>(sm.buf, sm.tw);
+    for (int j = tid; j < M; j += LH_T) tmp[j] = sm.buf[pad(j)];
+    __syncthreads();
+    for (int j = tid; j < M; j += LH_T) sm.buf[pad(j)] = g[2 * j + 1];
+    __syncthreads();
+    fft_lds<M, INV>(sm.buf, sm.tw);
+    constexpr int TS = LH_TWN / (2 * M);
+    for (int k = tid; k < M; k += LH_T) {
+        double2 w = tw_get(sm.tw, k * TS);
+        if (INV) w.y = -w.y;
+        const double2 o = cmul(w, sm.buf[pad(k)]), e = tmp[k];
+        g[k] = cadd(e, o);
+        g[k + M] = csub(e, o);
+    }
+    __syncthreads();
+}
+
+// r2c post-step of a 2·NO-point real transform whose packed NO-point
+// transform Z is in HBM: X[k] = E + w^k O (as r2c_bin).
+template <int NO, class Smem>
+__device__ __forceinline__ double2 r2c_bin_big(const double2* Z, const Smem& sm, int k) {
+    double2 zk = Z[k == NO ? 0 : k];
+    double2 zc = cconj(Z[k == 0 ? 0 : NO - k]);
+    double2 E = cscal(0.5, cadd(zk, zc));
+    double2 d = csub(zk, zc);
+    double2 O = make_double2(0.5 * d.y, -(0.5 * d.x));
+    double2 w = (k == NO) ? make_double2(-1.0, 0.0) : tw_frac<2 * NO>(sm.tw, k);
+    return cadd(E, cmul(w, O));
+}
+
+// The convolution step of the zero-padded Raman transform (2·NO reals packed
+// into the NO-point transform Z in HBM): r2c post-step, ×hω·δt, and the c2r
+// pre-pack of the product, pairwise (k, NO − k) in place.
+template <int NO, class Smem>
+__device__ __forceinline__ void raman_spectrum_big(const DInst& in, const Smem& sm, double2* W) {
+    const int tid = fresh_tid();
+    for (int k = tid; k <= NO / 2; k += LH_T) {
+        const int m = NO - k;
+        if (k == 0) {
+            double2 Y0 = cmul(in.hwdt[0], r2c_bin_big<NO>(W, sm, 0));
+            double2 YM = cmul(in.hwdt[NO], r2c_bin_big<NO>(W, sm, NO));
+            Y0.y = 0.0;
+            YM.y = 0.0;
+            const double2 A = cadd(Y0, cconj(YM)), B = csub(Y0, cconj(YM));
+            W[0] = cadd(A, cmul_pi(B));
+        } else if (k == NO / 2) {
+            const double2 Yk = cmul(in.hwdt[k], r2c_bin_big<NO>(W, sm, k));
+            const double2 A = cadd(Yk, cconj(Yk));
+            const double2 B = cmul(csub(Yk, cconj(Yk)), cconj(tw_frac<2 * NO>(sm.tw, k)));
+            W[k] = cadd(A, cmul_pi(B));
+        } else {
+            const double2 Yk = cmul(in.hwdt[k], r2c_bin_big<NO>(W, sm, k));
+            const double2 Ym = cmul(in.hwdt[m], r2c_bin_big<NO>(W, sm, m));
+            const double2 wk = cconj(tw_frac<2 * NO>(sm.tw, k)), wm = cconj(tw_frac<2 * NO>(sm.tw, m));
+            const double2 Ak = cadd(Yk, cconj(Ym)), Bk = cmul(csub(Yk, cconj(Ym)), wk);
+            const double2 Am = cadd(Ym, cconj(Yk)), Bm = cmul(csub(Ym, cconj(Yk)), wm);
+            W[k] = cadd(Ak, cmul_pi(Bk));
+            W[m] = cadd(Am, cmul_pi(Bm));
+        }
+    }
+}
+
+// RamanPolarField with third-harmonic terms (src/Nonlinear.jl:183-247,
+// 284-333, thg = true) on the field grid, for a series of Nto = 2·MO reals
+// held packed in LDS (z[j] = x[2j] + i x[2j+1], before the e_scale factor).
+// E² over the zero-padded 2·Nto reals is one 2·MO-point complex transform,
+// run through HBM as two MO-point LDS transforms (fft_big); ×hω·δt; back;
+// P(t) = irfft/(2·Nto) lands in escr[3·MO, 4·MO) as packed pairs, which the
+// pointwise phase adds as ρ·E·P.  The LDS series is restored unchanged from
+// its HBM copy escr[0, MO); escr[MO, 3·MO) is the transform's series.
+template <int MO, class Smem>
+__device__ __forceinline__ void raman_field(const DInst& in, Smem& sm) {
+    constexpr int NR = 2 * MO;
+    double2* S = in.escr;
+    double2* W = in.escr + MO;
+    double2* T = in.escr + 3 * MO;
+    const int tid = fresh_tid();
+    const double es = in.e_scale;
+    for (int j = tid; j < MO; j += LH_T) {
+        const double2 z = sm.buf[pad(j)];
+        const double e0 = z.x * es, e1 = z.y * es;
+        S[j] = z;
+        W[j] = make_double2(e0 * e0, e1 * e1);  // E² (thg: E.^2), reals packed in pairs
+        W[j + MO] = make_double2(0.0, 0.0);     // zero padding to 2·Nto reals
+    }
+    fft_big<MO, false>(W, T, sm);
+    raman_spectrum_big<NR>(in, sm, W);
+    fft_big<MO, true>(W, T, sm);
+    const double inv2n = in.inv_2n;
+    for (int j = tid; j < MO; j += LH_T) {
+        const double2 p = W[j];
+        T[j] = make_double2(p.x * inv2n, p.y * inv2n);
+        sm.buf[pad(j)] = S[j];
+    }
+    __syncthreads();
+}
+
+// Stage prologue on the big envelope grid: W = sc_t·(y + Δz Σ B k)·exp(L hf),
+// exp(−L hf) into slot 7 (as stage_prologue_env, chunked).
+template <int NN, class Smem>
+__device__ __forceinline__ void stage_prologue_big(const DInst& in, Smem& sm, const double2* Y, const KSet& K,
+                                                   int stage, double dt, double h0, double hf) {
+    constexpr int NPT = NN / LH_T, CH = 8;
+    static_assert(NN % (LH_T * CH) == 0, "big-grid chunks");
+    double2* W = in.escr + NN;
+    const int tid = fresh_tid();
+    const double sc_t = in.sc_t;
+    double2* K7 = K[7];
+#pragma unroll 1
+    for (int c = 0; c < NPT; c += CH) {
+        double2 yb[CH], L[CH];
+#pragma unroll
+        for (int i = 0; i < CH; i++) {
+            const int k = tid + (c + i) * LH_T;
+            yb[i] = Y[k];
+            L[i] = in.linop[k];
+        }
+        if (stage == 0 && h0 != 0.0) {
+            double2* K0 = K[0];
+            double2 kv[CH];
+#pragma unroll
+            for (int i = 0; i < CH; i++) kv[i] = K0[tid + (c + i) * LH_T];
+#pragma unroll
+            for (int i = 0; i < CH; i++) K0[tid + (c + i) * LH_T] = cmul(kv[i], cexp_lh(L[i], h0));
+        }
+#pragma unroll 1
+        for (int j = 0; j <= stage; j++) {
+            const double cj = dt * c_B[stage][j];
+            if (cj == 0.0) continue;
+            const double2* Kj = K[j];
+#pragma unroll
+            for (int i = 0; i < CH; i++) yb[i] = cadd(yb[i], cscal(cj, Kj[tid + (c + i) * LH_T]));
+        }
+#pragma unroll
+        for (int i = 0; i < CH; i++) {
+            const int k = tid + (c + i) * LH_T;
+            double2 yp = yb[i], inv = make_double2(1.0, 0.0);
+            if (hf != 0.0) yp = cmul(yb[i], cexp_lh_inv(L[i], hf, inv));
+            W[k] = cscal(sc_t, yp);
+            K7[k] = inv;
+        }
+    }
+}
+
+// TransModeAvg on the big envelope grid (the operations of rhs_env with the
+// transforms in HBM): W holds sc_t·Eω on entry.
+template <int NO, class Smem, class Epi>
+__device__ __forceinline__ void rhs_big(const DInst& in, Smem& sm, Epi epi) {
+    constexpr int M = NO / 2;
+    double2* E = in.escr;
+    double2* W = in.escr + NO;
+    double2* T = in.escr + 2 * NO;
+    const int tid = fresh_tid();
+    fft_big<M, true>(W, T, sm);
+    const double es = in.e_scale, k34 = in.kerr34, rho = in.density;
+    const bool kerr = (in.kerr == LH_KERR_ENV);
+    const double* twin = in.to_win;
+    if (in.raman) {
+        for (int i = tid; i < NO; i += LH_T) {
+            const double2 z = W[i];
+            E[i] = make_double2(z.x * es, z.y * es);
+        }
+        __syncthreads();
+        for (int j = tid; j < NO; j += LH_T) {
+            if (j < M) {
+                const double2 a = E[2 * j], b = E[2 * j + 1];
+                W[j] = make_double2(0.5 * (a.x * a.x + a.y * a.y), 0.5 * (b.x * b.x + b.y * b.y));
+            } else {
+                W[j] = make_double2(0.0, 0.0);  // zero padding to 2·NO reals
+            }
+        }
+        fft_big<M, false>(W, T, sm);
+        raman_spectrum_big<NO>(in, sm, W);
+        fft_big<M, true>(W, T, sm);  // P(t) on the 2·NO grid, packed in pairs: the first NO values in W[0..M)
+        for (int j = tid; j < M; j += LH_T) T[j] = W[j];
+        __syncthreads();
+        const double inv2n = in.inv_2n;
+        for (int i = tid; i < NO; i += LH_T) {
+            const double2 pp = T[i >> 1];
+            const double P = ((i & 1) ? pp.y : pp.x) * inv2n;
+            const double2 e = E[i];
+            double2 p = make_double2(0.0, 0.0);
+            if (kerr) {
+                const double s = k34 * (e.x * e.x + e.y * e.y);
+                p = make_double2(s * e.x, s * e.y);
+            }
+            p = cadd(p, make_double2((rho * e.x) * P, (rho * e.y) * P));
+            W[i] = make_double2(p.x * twin[i], p.y * twin[i]);
+        }
+    } else {
+        for (int i = tid; i < NO; i += LH_T) {
+            const double2 z = W[i];
+            const double2 e = make_double2(z.x * es, z.y * es);
+            double2 p = make_double2(0.0, 0.0);
+            if (kerr) {
+                const double s = k34 * (e.x * e.x + e.y * e.y);
+                p = make_double2(s * e.x, s * e.y);
+            }
+            W[i] = make_double2(p.x * twin[i], p.y * twin[i]);
+        }
+    }
+    fft_big<M, false>(W, T, sm);
+    constexpr int NPT = NO / LH_T, CH = 8;
+#pragma unroll 1
+    for (int c = 0; c < NPT; c += CH) {
+        double2 res[CH];
+#pragma unroll
+        for (int i = 0; i < CH; i++) {
+            const int k = tid + (c + i) * LH_T;
+            res[i] = cmul(W[k], in.outfac[k]);
+        }
+        epi(res, NO, c);
+    }
+}
+
+// stepfun on the big envelope grid: YN ← fft(twin · ifft(ωwin · YN·exp(L h))).
+template <int NT, class Smem>
+__device__ __forceinline__ void stepfun_big(const DInst& in, Smem& sm, double2* YN, double h) {
+    double2* W = in.escr + NT;
+    double2* T = in.escr + 2 * NT;
+    const int tid = fresh_tid();
+    for (int k = tid; k < NT; k += LH_T) {
+        double2 v = YN[k];
+        if (h != 0.0) v = cmul(v, cexp_lh(in.linop[k], h));
+        W[k] = cscal(in.omega_win[k], v);
+    }
+    fft_big<NT / 2, true>(W, T, sm);
+    const double inv_nt = in.inv_nt;
+    for (int j = tid; j < NT; j += LH_T) {
+        const double2 z = W[j];
+        const double w = in.t_win[j];
+        W[j] = make_double2((inv_nt * z.x) * w, (inv_nt * z.y) * w);
+    }
+    fft_big<NT / 2, false>(W, T, sm);
+    for (int k = tid; k < NT; k += LH_T) YN[k] = W[k];
+    __syncthreads();
+}
+
+// hω·δt on the big grid: h zero-padded to 2·NO reals, packed into NO complex
+// points in W, transformed, post-stepped into hwdt[0..NO].  One workgroup.
+template <int NO>
+__global__ __launch_bounds__(LH_T) void lh_raman_hw_big_kernel(const double* __restrict__ h, double dt,
+                                                               const double2* __restrict__ twg, double2* work,
+                                                               double2* hwdt) {
+    __shared__ SmemT<NO / 2> sm;
+    tw_load(sm.tw, twg);
+    double2* W = work;
+    double2* T = work + NO;
+    for (int j = threadIdx.x; j < NO; j += LH_T)
+        W[j] = (j < NO / 2) ? make_double2(h[2 * j], h[2 * j + 1]) : make_double2(0.0, 0.0);
+    fft_big<NO / 2, false>(W, T, sm);
+    for (int k = threadIdx.x; k <= NO; k += LH_T) {
+        const double2 X = r2c_bin_big<NO>(W, sm, k);
+        hwdt[k] = make_double2(X.x * dt, X.y * dt);
+    }
+}
+
 // hω·δt for the Raman convolution: rfft of h(t) zero-padded to 2·Nto reals
 // (the reference's FT * h on the doubled grid, src/Nonlinear.jl:307-312).
 template <int NO>
@@ -1063,10 +1365,12 @@ __global__ __launch_bounds__(LH_T) void lh_raman_hw_kernel(const double* __restr
 // --------------------------------------------------------------------------
 // KIND 0: field grid, MO = Nto/2 complex points, MC = Nt/2.
 // KIND 1: envelope grid, MO = Nto complex points, MC = Nt = Nω.
+// KIND 2: envelope grid with the transforms staged through HBM, MO = MC = Nt.
 template <int KIND, int MO, int MC>
 struct Sizes {
     static constexpr int N = (KIND == 0) ? Bins<MO>::N : MC;
     static constexpr int NPT = (N + LH_T - 1) / LH_T;
+    static constexpr int LDS = (KIND == 2) ? MO / 2 : MO;  // LDS transform length (SmemT)
 };
 
 // ==========================================================================
@@ -1447,18 +1751,24 @@ __device__ __forceinline__ void do_prologue(const DInst& in, Smem& sm, const dou
                                             double dt, double h0, double hf) {
     if constexpr (KIND == 0)
         stage_prologue<MO, Smem>(in, sm, Y, K, stage, dt, h0, hf);
-    else
+    else if constexpr (KIND == 1)
         stage_prologue_env<MO, MC, Smem>(in, sm, Y, K, stage, dt, h0, hf);
+    else
+        stage_prologue_big<MC, Smem>(in, sm, Y, K, stage, dt, h0, hf);
 }
 
 // PACKED: the field-grid buffer already holds the c2r pre-pack (the stage
 // prologue writes it); the single-RHS entry fills plain bins instead.
-template <int KIND, int MO, int MC, class Smem, class Epi, bool PACKED = true>
+// RAMAN (field grid): compiled with RamanPolarField (the launch picks it for a
+// class holding a Raman instance; the rest run without its registers).
+template <int KIND, int MO, int MC, class Smem, class Epi, bool PACKED = true, bool RAMAN = false>
 __device__ __forceinline__ void do_rhs(const DInst& in, Smem& sm, Epi epi, double rho, double kfac) {
     if constexpr (KIND == 0)
-        rhs_field<MO, Smem, Epi, PACKED>(in, sm, epi, rho, kfac);
-    else
+        rhs_field<MO, Smem, Epi, PACKED, RAMAN>(in, sm, epi, rho, kfac);
+    else if constexpr (KIND == 1)
         rhs_env<MO, MC, Smem>(in, sm, epi);
+    else
+        rhs_big<MO, Smem>(in, sm, epi);
 }
 
 template <int KIND, int MO, int MC, class Smem>
@@ -1466,13 +1776,15 @@ __device__ __forceinline__ void do_stepfun(const DInst& in, Smem& sm, double2* Y
                                            double dz, double rho) {
     if constexpr (KIND == 0)
         stepfun_filter<MC, Smem>(in, sm, YN, h, rec, z, dz, rho);
-    else
+    else if constexpr (KIND == 1)
         stepfun_env<MC, Smem>(in, sm, YN, h, rec, z, dz, rho);
+    else
+        stepfun_big<MC, Smem>(in, sm, YN, h);  // no statistics on this grid
 }
 
 // ZDEP: compiled with the z-dependent fill (the launch picks it only for a
 // class holding a gradient or taper; the rest run without its registers).
-template <int KIND, int MO, int MC, class Smem, bool ZDEP, bool STREAM>
+template <int KIND, int MO, int MC, class Smem, bool ZDEP, bool STREAM, bool RAMAN>
 __device__ __forceinline__ void solve_instance(const DInst& in, Smem& sm) {
     constexpr int N = Sizes<KIND, MO, MC>::N, NPT = Sizes<KIND, MO, MC>::NPT;
     KSet K{in.K[0], in.kstride, 0, 6};
@@ -1495,7 +1807,7 @@ __device__ __forceinline__ void solve_instance(const DInst& in, Smem& sm) {
     // PreconStepper init: k1 = fbar(y0, t, t)
     do_prologue<KIND, MO, MC, Smem>(in, sm, Y, K, 0, 0.0, 0.0, 0.0);
     {
-        do_rhs<KIND, MO, MC, Smem>(in, sm, EpiStore{K[0]}, rho, kfac);
+        do_rhs<KIND, MO, MC, Smem, EpiStore, true, RAMAN>(in, sm, EpiStore{K[0]}, rho, kfac);
     }
     rhs_evals++;
     __syncthreads();
@@ -1528,9 +1840,10 @@ __device__ __forceinline__ void solve_instance(const DInst& in, Smem& sm) {
             if (zdep && st != 5) zg_refresh(in, sm, t2, rho, kfac);  // stages 5 and 6 share t + dt
             do_prologue<KIND, MO, MC, Smem>(in, sm, Y, K, st, dt, h0, hf);
             if constexpr (KIND == 0)
-                do_rhs<KIND, MO, MC, Smem>(in, sm, EpiStage{K[st + 1], in.linop, hb}, rho, kfac);
+                do_rhs<KIND, MO, MC, Smem, EpiStage, true, RAMAN>(in, sm, EpiStage{K[st + 1], in.linop, hb}, rho,
+                                                                  kfac);
             else
-                do_rhs<KIND, MO, MC, Smem>(in, sm, EpiStageF{K[st + 1], K[7]}, rho, kfac);
+                do_rhs<KIND, MO, MC, Smem, EpiStageF, true, RAMAN>(in, sm, EpiStageF{K[st + 1], K[7]}, rho, kfac);
             rhs_evals++;
             __syncthreads();
             if (sm.flag != LH_OK) status = sm.flag;
@@ -1539,7 +1852,39 @@ __device__ __forceinline__ void solve_instance(const DInst& in, Smem& sm) {
         steps++;
         // yn = y + Δz Σ b5 k ; yerr = Σ (Δz k) e ; weaknorm (src/RK45.jl:170-181, 343-354)
         double sy = 0.0, syn = 0.0, se = 0.0;
-        {
+        if constexpr (NPT > 16) {
+            // the big envelope grid: the same sums over chunks of 8 bins
+            constexpr int CH = 8;
+#pragma unroll 1
+            for (int c = 0; c < NPT; c += CH) {
+                double2 yv[CH], yn[CH], ye[CH];
+#pragma unroll
+                for (int i = 0; i < CH; i++) {
+                    yv[i] = Y[threadIdx.x + (c + i) * LH_T];
+                    yn[i] = yv[i];
+                    ye[i] = make_double2(0.0, 0.0);
+                }
+#pragma unroll 1
+                for (int j = 0; j < 7; j++) {
+                    if (j == 1) continue;
+                    const double cb = dt * c_b5[j], ce = c_b5[j] - c_b4[j];
+                    const double2* Kj = K[j];
+#pragma unroll
+                    for (int i = 0; i < CH; i++) {
+                        const double2 kv = Kj[threadIdx.x + (c + i) * LH_T];
+                        yn[i] = cadd(yn[i], cscal(cb, kv));
+                        ye[i] = cadd(ye[i], cscal(ce, cscal(dt, kv)));
+                    }
+                }
+#pragma unroll
+                for (int i = 0; i < CH; i++) {
+                    sy += yv[i].x * yv[i].x + yv[i].y * yv[i].y;
+                    syn += yn[i].x * yn[i].x + yn[i].y * yn[i].y;
+                    se += ye[i].x * ye[i].x + ye[i].y * ye[i].y;
+                    YN[threadIdx.x + (c + i) * LH_T] = yn[i];
+                }
+            }
+        } else {
             double2 yv[NPT], yn[NPT], ye[NPT];
 #pragma unroll
             for (int i = 0; i < NPT; i++) {
@@ -1720,12 +2065,13 @@ __device__ __forceinline__ void solve_instance(const DInst& in, Smem& sm) {
 // One persistent kernel per grid size: one workgroup per instance.  STREAM
 // builds in the per-step publication of lh_solve_stream (a separate kernel,
 // so the others keep their register allocation).
-template <int KIND, int MO, int MC, bool ZDEP, bool STREAM = false>
+template <int KIND, int MO, int MC, bool ZDEP, bool STREAM = false, bool RAMAN = false>
 __global__ __launch_bounds__(LH_T) void lh_solve_kernel(const DInst* insts, const double2* __restrict__ twg) {
-    __shared__ SmemT<MO> sm;
+    using Smem = SmemT<Sizes<KIND, MO, MC>::LDS>;
+    __shared__ Smem sm;
     tw_load(sm.tw, twg);
     __syncthreads();
-    solve_instance<KIND, MO, MC, SmemT<MO>, ZDEP, STREAM>(insts[blockIdx.x], sm);
+    solve_instance<KIND, MO, MC, Smem, ZDEP, STREAM, RAMAN>(insts[blockIdx.x], sm);
 }
 
 template <int KIND, int MO, int MC, class Smem>
@@ -1737,8 +2083,10 @@ __device__ __forceinline__ void rhs_once(const DInst& in, Smem& sm) {
         double2 v = cscal(in.sc_t, in.rhs_in[k]);
         if constexpr (KIND == 0)
             sm.buf[pad(k)] = v;
-        else
+        else if constexpr (KIND == 1)
             sm.buf[pad(env_slot<MO, MC>(k))] = v;
+        else
+            in.escr[MO + k] = v;  // the big grid's working series W
     }
     if constexpr (KIND == 0) {
         for (int k = N + threadIdx.x; k <= MO; k += LH_T) sm.buf[pad(k)] = make_double2(0, 0);
@@ -1746,17 +2094,21 @@ __device__ __forceinline__ void rhs_once(const DInst& in, Smem& sm) {
         for (int k = MC / 2 + threadIdx.x; k < MO - MC / 2; k += LH_T) sm.buf[pad(k)] = make_double2(0, 0);
     }
     double2* out = in.rhs_out;
-    do_rhs<KIND, MO, MC, Smem, EpiStore, false>(in, sm, EpiStore{out}, rho, kfac);
+    if (KIND == 0 && in.raman)
+        do_rhs<KIND, MO, MC, Smem, EpiStore, false, true>(in, sm, EpiStore{out}, rho, kfac);
+    else
+        do_rhs<KIND, MO, MC, Smem, EpiStore, false, false>(in, sm, EpiStore{out}, rho, kfac);
 }
 
 template <int KIND, int MO, int MC>
 __global__ __launch_bounds__(LH_T) void lh_rhs_kernel(const DInst* insts, const double2* __restrict__ twg,
                                                       int* status) {
-    __shared__ SmemT<MO> sm;
+    using Smem = SmemT<Sizes<KIND, MO, MC>::LDS>;
+    __shared__ Smem sm;
     tw_load(sm.tw, twg);
     if (threadIdx.x == 0) sm.flag = LH_OK;
     __syncthreads();
-    rhs_once<KIND, MO, MC, SmemT<MO>>(insts[blockIdx.x], sm);
+    rhs_once<KIND, MO, MC, Smem>(insts[blockIdx.x], sm);
     __syncthreads();
     if (threadIdx.x == 0) status[blockIdx.x] = sm.flag;
 }
@@ -1784,7 +2136,8 @@ enum {
     LH_CLASS_ENV8192 = 3,  // envelope grid, Nto = Nt = 8192
     LH_CLASS_ENV4096 = 4,  // envelope grid, Nto = Nt = 4096
     LH_CLASS_RADIAL = 5,   // radial transform (solved context by context)
-    LH_NCLASS = 6
+    LH_CLASS_ENV16384 = 6, // envelope grid, Nto = Nt = 16384, transforms staged through HBM
+    LH_NCLASS = 7
 };
 
 static int size_class_of(int grid_kind, int n_to, int n_t) {
@@ -1793,6 +2146,7 @@ static int size_class_of(int grid_kind, int n_to, int n_t) {
         if (n_to == 8192 && n_t == 4096) return LH_CLASS_8192;
         if (n_to == 4096 && n_t == 2048) return LH_CLASS_4096;
     } else if (grid_kind == LH_GRID_ENV) {
+        if (n_to == 16384 && n_t == 16384) return LH_CLASS_ENV16384;
         if (n_to == 8192 && n_t == 8192) return LH_CLASS_ENV8192;
         if (n_to == 4096 && n_t == 4096) return LH_CLASS_ENV4096;
     }
@@ -1800,34 +2154,51 @@ static int size_class_of(int grid_kind, int n_to, int n_t) {
 }
 
 template <int KIND, int MO, int MC>
-static void launch_solve_k(bool zdep, int n, const DInst* dinst, const double2* tw, hipStream_t s) {
+static void launch_solve_k(bool zdep, bool raman, int n, const DInst* dinst, const double2* tw, hipStream_t s) {
     if constexpr (KIND == 0) {
-        if (zdep) {
+        if (zdep) {  // a z-dependent fill never carries Raman (lh_set_gradient refuses it)
             hipLaunchKernelGGL((lh_solve_kernel<KIND, MO, MC, true>), dim3(n), dim3(LH_T), 0, s, dinst, tw);
+            return;
+        }
+        if (raman) {
+            hipLaunchKernelGGL((lh_solve_kernel<KIND, MO, MC, false, false, true>), dim3(n), dim3(LH_T), 0, s, dinst,
+                               tw);
             return;
         }
     }
     hipLaunchKernelGGL((lh_solve_kernel<KIND, MO, MC, false>), dim3(n), dim3(LH_T), 0, s, dinst, tw);
 }
 
-static void launch_solve_stream(int cls, const DInst* dinst, const double2* tw, hipStream_t s) {
+static void launch_solve_stream(int cls, bool raman, const DInst* dinst, const double2* tw, hipStream_t s) {
+    if (raman) {
+        switch (cls) {
+            case LH_CLASS_16384: hipLaunchKernelGGL((lh_solve_kernel<0, 8192, 4096, false, true, true>), dim3(1), dim3(LH_T), 0, s, dinst, tw); return;
+            case LH_CLASS_8192: hipLaunchKernelGGL((lh_solve_kernel<0, 4096, 2048, false, true, true>), dim3(1), dim3(LH_T), 0, s, dinst, tw); return;
+            case LH_CLASS_4096: hipLaunchKernelGGL((lh_solve_kernel<0, 2048, 1024, false, true, true>), dim3(1), dim3(LH_T), 0, s, dinst, tw); return;
+            default: break;  // the envelope classes handle Raman at run time
+        }
+    }
     switch (cls) {
         case LH_CLASS_16384: hipLaunchKernelGGL((lh_solve_kernel<0, 8192, 4096, false, true>), dim3(1), dim3(LH_T), 0, s, dinst, tw); break;
         case LH_CLASS_8192: hipLaunchKernelGGL((lh_solve_kernel<0, 4096, 2048, false, true>), dim3(1), dim3(LH_T), 0, s, dinst, tw); break;
         case LH_CLASS_4096: hipLaunchKernelGGL((lh_solve_kernel<0, 2048, 1024, false, true>), dim3(1), dim3(LH_T), 0, s, dinst, tw); break;
         case LH_CLASS_ENV8192: hipLaunchKernelGGL((lh_solve_kernel<1, 8192, 8192, false, true>), dim3(1), dim3(LH_T), 0, s, dinst, tw); break;
+        case LH_CLASS_ENV16384: hipLaunchKernelGGL((lh_solve_kernel<2, 16384, 16384, false, true>), dim3(1), dim3(LH_T), 0, s, dinst, tw); break;
         default: hipLaunchKernelGGL((lh_solve_kernel<1, 4096, 4096, false, true>), dim3(1), dim3(LH_T), 0, s, dinst, tw); break;
     }
 }
 
-// zdep: some instance of the class has a z-dependent fill (field grid only).
-static void launch_solve(int cls, int n, const DInst* dinst, const double2* tw, hipStream_t s, bool zdep) {
+// zdep: some instance of the class has a z-dependent fill; raman: some
+// instance carries a field-grid Raman response (field grid only, each).
+static void launch_solve(int cls, int n, const DInst* dinst, const double2* tw, hipStream_t s, bool zdep,
+                         bool raman = false) {
     switch (cls) {
-        case LH_CLASS_16384: launch_solve_k<0, 8192, 4096>(zdep, n, dinst, tw, s); break;
-        case LH_CLASS_8192: launch_solve_k<0, 4096, 2048>(zdep, n, dinst, tw, s); break;
-        case LH_CLASS_4096: launch_solve_k<0, 2048, 1024>(zdep, n, dinst, tw, s); break;
-        case LH_CLASS_ENV8192: launch_solve_k<1, 8192, 8192>(false, n, dinst, tw, s); break;
-        default: launch_solve_k<1, 4096, 4096>(false, n, dinst, tw, s); break;
+        case LH_CLASS_16384: launch_solve_k<0, 8192, 4096>(zdep, raman, n, dinst, tw, s); break;
+        case LH_CLASS_8192: launch_solve_k<0, 4096, 2048>(zdep, raman, n, dinst, tw, s); break;
+        case LH_CLASS_4096: launch_solve_k<0, 2048, 1024>(zdep, raman, n, dinst, tw, s); break;
+        case LH_CLASS_ENV8192: launch_solve_k<1, 8192, 8192>(false, false, n, dinst, tw, s); break;
+        case LH_CLASS_ENV16384: launch_solve_k<2, 16384, 16384>(false, false, n, dinst, tw, s); break;
+        default: launch_solve_k<1, 4096, 4096>(false, false, n, dinst, tw, s); break;
     }
 }
 
@@ -1837,6 +2208,7 @@ static void launch_rhs(int cls, const DInst* dinst, const double2* tw, int* stat
         case LH_CLASS_8192: hipLaunchKernelGGL((lh_rhs_kernel<0, 4096, 2048>), dim3(1), dim3(LH_T), 0, s, dinst, tw, status); break;
         case LH_CLASS_4096: hipLaunchKernelGGL((lh_rhs_kernel<0, 2048, 1024>), dim3(1), dim3(LH_T), 0, s, dinst, tw, status); break;
         case LH_CLASS_ENV8192: hipLaunchKernelGGL((lh_rhs_kernel<1, 8192, 8192>), dim3(1), dim3(LH_T), 0, s, dinst, tw, status); break;
+        case LH_CLASS_ENV16384: hipLaunchKernelGGL((lh_rhs_kernel<2, 16384, 16384>), dim3(1), dim3(LH_T), 0, s, dinst, tw, status); break;
         default: hipLaunchKernelGGL((lh_rhs_kernel<1, 4096, 4096>), dim3(1), dim3(LH_T), 0, s, dinst, tw, status); break;
     }
 }
@@ -1852,7 +2224,12 @@ static void launch_stats(int cls, const DInst* dinst, const double2* tw, double 
     }
 }
 
-static void launch_raman_hw(int n_to, const double* h, double dt, const double2* tw, double2* hwdt, hipStream_t s) {
+static void launch_raman_hw(int n_to, const double* h, double dt, const double2* tw, double2* hwdt, double2* work,
+                            hipStream_t s) {
+    if (n_to == 16384) {
+        hipLaunchKernelGGL((lh_raman_hw_big_kernel<16384>), dim3(1), dim3(LH_T), 0, s, h, dt, tw, work, hwdt);
+        return;
+    }
     if (n_to == 8192)
         hipLaunchKernelGGL((lh_raman_hw_kernel<8192>), dim3(1), dim3(LH_T), 0, s, h, dt, tw, hwdt);
     else
@@ -1883,7 +2260,8 @@ struct lh_ctx {
     double2* outfac;
     double4* ppt;
     double2* hwdt;  // envelope Raman: rfft(h, 2·Nto)·δt
-    double2* escr;  // envelope Raman: E(t) stash
+    double2* escr;  // envelope Raman: E(t) stash; big envelope grid: its HBM work area
+    size_t escr_n;  // points of escr
     // solver buffers (allocated on first use)
     double2* Yin;
     int have_state;
@@ -2122,7 +2500,7 @@ static bool supported(const lh_desc* d) {
     if (size_class_of(d->grid_kind, d->n_to, d->n_t) < 0) return false;
     if (d->grid_kind == LH_GRID_REAL) {
         if (d->n_omega != d->n_t / 2 + 1) return false;
-        if (d->raman || d->kerr == LH_KERR_ENV) return false;
+        if ((d->raman && !d->raman_h) || d->kerr == LH_KERR_ENV) return false;
     } else {
         if (d->n_omega != d->n_t) return false;
         if (d->plasma || d->kerr == LH_KERR_FIELD) return false;
@@ -2560,15 +2938,24 @@ int lh_create(lh_ctx** out, const lh_desc* d) {
         c->ppt_emax = d->ppt_x[d->ppt_n - 1];
         st = shared_upload(c->device, &c->ppt, p.data(), p.size());
     }
-    if (!st && d->grid_kind == LH_GRID_ENV && d->raman) {
+    const bool big_env = (ctx_class(c) == LH_CLASS_ENV16384);
+    const bool field_raman = (d->grid_kind == LH_GRID_REAL && d->transform == LH_TRANS_MODEAVG && d->raman);
+    if (!st && (big_env || d->raman)) {
+        // envelope Raman: E(t) stash of Nto points; the big envelope grid:
+        // [E(t) | W | T], Nto + Nto + Nto/2 points, with or without Raman;
+        // field-grid Raman: [series | W | P], Nto/2 + Nto + Nto/2 points
+        c->escr_n = big_env ? (size_t)d->n_to * 5 / 2 : field_raman ? (size_t)d->n_to * 2 : (size_t)d->n_to;
+        if (hipMalloc(&c->escr, sizeof(double2) * c->escr_n) != hipSuccess) st = LH_E_NOMEM;
+    }
+    if (!st && d->raman) {
         const double2* tw;
         double* dh = nullptr;
         st = twiddles(c->device, &tw);
         if (!st) st = upload(&dh, d->raman_h, (size_t)d->n_to);
         if (!st && hipMalloc(&c->hwdt, sizeof(double2) * (size_t)(d->n_to + 1)) != hipSuccess) st = LH_E_NOMEM;
-        if (!st && hipMalloc(&c->escr, sizeof(double2) * (size_t)d->n_to) != hipSuccess) st = LH_E_NOMEM;
         if (!st) {
-            launch_raman_hw(d->n_to, dh, d->dt_o, tw, c->hwdt, c->stream);
+            double2* work = big_env ? c->escr + d->n_to : field_raman ? c->escr + d->n_to / 2 : nullptr;
+            launch_raman_hw(d->n_to, dh, d->dt_o, tw, c->hwdt, work, c->stream);
             if (hipGetLastError() != hipSuccess || hipStreamSynchronize(c->stream) != hipSuccess) st = LH_E_HIP;
         }
         if (dh) (void)hipFree(dh);
@@ -2715,7 +3102,7 @@ int lh_solve_stream(lh_ctx* c, double* Ew, const lh_rk_opts* o, lh_output_cb cb,
         if (hipMemcpyAsync(c->dinst, &in, sizeof(DInst), hipMemcpyHostToDevice, c->stream) != hipSuccess) st = LH_E_HIP;
     }
     if (!st) {
-        launch_solve_stream(cls, c->dinst, tw, c->stream);
+        launch_solve_stream(cls, c->d.grid_kind == LH_GRID_REAL && c->d.raman, c->dinst, tw, c->stream);
         if (hipGetLastError() != hipSuccess) st = LH_E_HIP;
     }
     // Drain the publications while the kernel runs.  A publication's state
@@ -2883,10 +3270,19 @@ static int batch_resident(lh_ctx* const* ctxs, int n, const lh_rk_opts* const* o
                     st = rad_solve(ctxs[i], op[i], want_saves != 0, cs[c], &rad_stats[i]);
             if (st) break;
         } else {
-            bool zdep = false;
+            bool zdep = false, raman = false;
             for (int i = 0; i < n; i++)
-                if (ctx_class(ctxs[i]) == c && ctxs[i]->zg_on) zdep = true;
-            launch_solve(c, count[c], dinst + first[c], tw, cs[c], zdep);
+                if (ctx_class(ctxs[i]) == c) {
+                    if (ctxs[i]->zg_on) zdep = true;
+                    if (ctxs[i]->d.grid_kind == LH_GRID_REAL && ctxs[i]->d.raman) raman = true;
+                }
+            if (zdep && raman) {
+                // one class holding both z-dependent fills and Raman instances
+                // would need two kernels: refused
+                st = LH_E_ARG;
+                break;
+            }
+            launch_solve(c, count[c], dinst + first[c], tw, cs[c], zdep, raman);
         }
         if (hipGetLastError() != hipSuccess || hipEventRecord(t1[c], cs[c]) != hipSuccess ||
             hipEventRecord(done[c], cs[c]) != hipSuccess || hipStreamWaitEvent(s0, done[c], 0) != hipSuccess)
@@ -2959,7 +3355,7 @@ int lh_set_step_stats(lh_ctx* c, const lh_stats_desc* sd) {
         c->st_on = 0;
         return LH_OK;
     }
-    if (c->d.transform != LH_TRANS_MODEAVG) return LH_E_ARG;
+    if (c->d.transform != LH_TRANS_MODEAVG || ctx_class(c) == LH_CLASS_ENV16384) return LH_E_ARG;
     HIPCK(hipSetDevice(c->device));
     if (sd->capacity < 0 || !sd->energy_w || !sd->t || !(sd->aeff > 0) || !c->omega) return LH_E_ARG;
     if (sd->n_windows < 0 || sd->n_windows > LH_STAT_MAX_WINDOWS || (sd->n_windows > 0 && !sd->window_w))
@@ -3023,7 +3419,9 @@ int lh_set_gradient(lh_ctx* c, const lh_gradient_desc* g) {
         return LH_OK;
     }
     const lh_desc& d = c->d;
-    if (d.transform != LH_TRANS_MODEAVG || d.grid_kind != LH_GRID_REAL || !c->omega || !c->zsidx) return LH_E_ARG;
+    // (a Raman response depends on the density; with ρ(z) it would change along z)
+    if (d.transform != LH_TRANS_MODEAVG || d.grid_kind != LH_GRID_REAL || !c->omega || !c->zsidx || d.raman)
+        return LH_E_ARG;
     if (g->n_points < 2 || !g->Z || !g->P || g->dens_n < 4 || !g->dens_p || !g->dens_rho || !g->dens_d ||
         !g->gamma || !g->nwg || !g->pre || !g->fd || !(g->fd_h > 0) || !(g->sqrt_aeff > 0) ||
         (g->model != 0 && g->model != 1) || (g->two_point && g->n_points != 2) || g->taper < 0 || g->taper > 2)
@@ -3167,7 +3565,7 @@ int lh_solve_noalloc(lh_ctx* c, void* Ew_dev, const lh_rk_opts* o, void* saves_d
     const int cls = ctx_class(c);
     if (c->d.transform != LH_TRANS_MODEAVG || cls < 0 || cls == LH_CLASS_RADIAL) return LH_E_ARG;
     const size_t NS = (size_t)c->n_state;
-    const NoallocLayout L = noalloc_layout(NS, o->save_n, c->escr ? (size_t)c->d.n_to : 0);
+    const NoallocLayout L = noalloc_layout(NS, o->save_n, c->escr ? c->escr_n : 0);
     if (L.total > LH_NOALLOC_SCRATCH_BYTES) return LH_E_TOO_LARGE;
     HIPCK(hipSetDevice(c->device));
     unsigned char* base = scratch_existing(c->device);
@@ -3199,7 +3597,8 @@ int lh_solve_noalloc(lh_ctx* c, void* Ew_dev, const lh_rk_opts* o, void* saves_d
                              hipMemcpyHostToDevice, s));
         HIPCK(hipStreamSynchronize(s));
     }
-    launch_solve(cls, 1, (const DInst*)(base + L.inst), tw, s, c->zg_on != 0);
+    launch_solve(cls, 1, (const DInst*)(base + L.inst), tw, s, c->zg_on != 0,
+                 c->d.grid_kind == LH_GRID_REAL && c->d.raman);
     HIPCK(hipGetLastError());
     lh_stats hs;
     HIPCK(hipMemcpyAsync(&hs, base + L.stats, sizeof(lh_stats), hipMemcpyDeviceToHost, s));

@@ -22,7 +22,7 @@ from .ionisation import ppt_spline_cached
 from .linops import make_const_linop
 from .modes import MarcatiliMode, SimpleMode
 from .nonlinear import (CombinedRamanResponse, Kerr_env, Kerr_field, PlasmaCumtrapz, RamanPolarEnv,
-                        RamanRespNormedSingleDampedOscillator)
+                        RamanPolarField, RamanRespNormedSingleDampedOscillator)
 from .nonlinearrhs import TransModeAvg, TransRadial, norm_mode_average, norm_mode_average_gnlse
 from .output import HDF5Output, MemoryOutput  # noqa: F401 (re-exported: Output.MemoryOutput)
 
@@ -102,7 +102,7 @@ def prop_capillary_args(radius, flength, gas, pressure, *, lambda0, lambda_lims,
                         tau_fwhm=None, energy=None, power=None, pulseshape="gauss",
                         shotnoise=True, noise_seed=0, kerr=True, plasma=None, loss=True,
                         model="full", temperature=pd.roomtemp, preionfrac=0.0, saveN=201,
-                        ppt_N=2 ** 16, density=None):
+                        ppt_N=2 ** 16, density=None, raman=None, rotation=True, vibration=True):
     """Returns ``(Eω, grid, linop, transform, FT, output)`` like ``Interface.jl:360-401``.
 
     ``pressure``: a number (constant fill), ``(p0, p1)`` (a gradient from z = 0
@@ -110,7 +110,9 @@ def prop_capillary_args(radius, flength, gas, pressure, *, lambda0, lambda_lims,
     multi-point gradient), as ``makemode_s``/``makedensity`` accept
     (``Interface.jl:491-532``).  ``density`` overrides the number density of a
     constant fill (the reference obtains it from CoolProp, which is not
-    available here)."""
+    available here).  ``raman`` (default: on for the molecular gases, as
+    ``makeresponse``, ``Interface.jl:534-558``) adds ``RamanPolarField`` with
+    the gas's rotational and vibrational response."""
     plasma = True if plasma is None else plasma
     grid = RealGrid(flength, lambda0, lambda_lims, trange)
     grad = None
@@ -149,6 +151,12 @@ def prop_capillary_args(radius, flength, gas, pressure, *, lambda0, lambda_lims,
     if plasma:
         spl = ppt_spline_cached(gas, grid.ref_lambda, N=ppt_N)
         resp.append(PlasmaCumtrapz(grid.to, spl, pd.ionisation_potential(gas), preionfrac=preionfrac))
+    if raman is None:
+        raman = gas in ("N2", "H2", "D2", "N2O", "CH4", "SF6")
+    if raman:
+        from .raman import raman_response
+        resp.append(RamanPolarField(grid.to, raman_response(grid.to, gas, rotation=rotation, vibration=vibration,
+                                                            temp=temperature)))
     if pulseshape == "gauss":
         inputs = [GaussField(lambda0, tau_fwhm, energy=energy, power=power)]
     else:

@@ -6,6 +6,7 @@ the arithmetic runs in the HIP kernels:
   - ``Kerr_env(γ3)``              Nonlinear.jl:63-73  (P += ¾ ρ ε0 γ3 |E|² E)
   - ``PlasmaCumtrapz(t, E, rate, ionpot; preionfrac)``  Nonlinear.jl:88-177
   - ``RamanPolarEnv(t, response)``                       Nonlinear.jl:262-333
+  - ``RamanPolarField(t, response)``                     Nonlinear.jl:183-247, 284-333
 Raman response functions: ``RamanRespNormedSingleDampedOscillator``
 (``Raman.jl:53-65``) combined with the Planck window of
 ``CombinedRamanResponse`` (``Raman.jl:242-262``).
@@ -75,3 +76,15 @@ class RamanPolarEnv:
 
     def h(self, rho):
         return self.response(rho)
+
+
+class RamanPolarField(RamanPolarEnv):
+    """``RamanPolarField(t, r; thg=true)`` (``Nonlinear.jl:183-247``): the
+    field-resolved Raman polarisation ρ·E·(h ⊛ E²), the convolution over the
+    zero-padded 2·Nt grid.  Only the form with third-harmonic terms (E², the
+    default) is provided."""
+
+    def __init__(self, t, response, thg=True):
+        if not thg:
+            raise NotImplementedError("RamanPolarField(thg=false) (the Hilbert-transform form) is not provided")
+        super().__init__(t, response)

@@ -13,7 +13,7 @@ import numpy as np
 
 from . import _capi
 from . import physdata as pd
-from .nonlinear import Kerr_env, Kerr_field, PlasmaCumtrapz, RamanPolarEnv
+from .nonlinear import Kerr_env, Kerr_field, PlasmaCumtrapz, RamanPolarEnv, RamanPolarField
 
 NLSCALE = math.sqrt(pd.eps_0 * pd.c / 2)  # NonlinearRHS.jl:360
 
@@ -97,7 +97,9 @@ class _TransformBase:
                 spl = r.ratefunc
                 f.update(plasma=1, ppt_n=len(spl.x), ppt_x=spl.x, ppt_y=spl.y, ppt_d=spl.d,
                          ionpot=r.ionpot, preionfrac=r.preionfrac)
-            elif isinstance(r, RamanPolarEnv):
+            elif isinstance(r, RamanPolarEnv):  # and RamanPolarField, its field-grid form
+                if (self.grid.kind == "real") != isinstance(r, RamanPolarField):
+                    raise TypeError("RamanPolarField belongs to a field grid, RamanPolarEnv to an envelope grid")
                 f.update(raman=1, raman_h=np.ascontiguousarray(r.h(self.density), dtype=float))
             else:
                 raise TypeError(f"unsupported response {type(r).__name__}")

@@ -16,6 +16,7 @@
  *   - Kerr_field / Kerr_env      src/Nonlinear.jl:8-10, 24-34, 47-49, 63-73
  *   - PlasmaCumtrapz (scalar)    src/Nonlinear.jl:121-133, 164-177
  *   - RamanPolarEnv              src/Nonlinear.jl:201-333 (h(t) given as input)
+ *   - RamanPolarField (thg)      src/Nonlinear.jl:183-247, 284-333 (h(t) given as input)
  *   - TransModeAvg               src/NonlinearRHS.jl:362-399
  *   - TransRadial                src/NonlinearRHS.jl:466-474 (QDHT as a dense product)
  *   - RK45.solve / PreconStepper src/RK45.jl:27-89, 128-309, 343-406, dopri.jl:1-28
@@ -242,6 +243,8 @@ static void ws_free(rhs_ws *w) {
 /* ------------------------------------------------------------------------ */
 /* Responses on one real column (Et_to_Pt!, src/NonlinearRHS.jl:121-126)     */
 /* ------------------------------------------------------------------------ */
+static void raman_field(const lh_desc *d, rhs_ws *w, const double *E, double *Pt);
+
 static int responses_real(const lh_desc *d, rhs_ws *w, const double *E, double *Pt) {
     int n = d->n_to;
     double rho = d->density;
@@ -265,7 +268,32 @@ static int responses_real(const lh_desc *d, rhs_ws *w, const double *E, double *
         cumtrapz(n, w->J, dt, w->P);
         for (int i = 0; i < n; i++) Pt[i] = Pt[i] + rho * w->P[i];
     }
+    if (d->raman) raman_field(d, w, E, Pt);
     return LH_OK;
+}
+
+/* RamanPolarField call with thg = true, src/Nonlinear.jl:183-247, 284-333:
+ * E.^2 on the doubled grid, FT, ×hω·δt, inverse, Pout = ρ·E·P (R.r(ht, ρ) =
+ * raman_h).  The reference's rfft/irfft pair on real data is the c2c pair of
+ * the same length restricted to real input. */
+static void raman_field(const lh_desc *d, rhs_ws *w, const double *E, double *Pt) {
+    int n = d->n_to, n2 = 2 * d->n_to;
+    double rho = d->density;
+    for (int i = 0; i < n2; i++) w->E2[i] = 0.0;
+    for (int i = 0; i < n; i++) w->E2[i] = E[i] * E[i];
+    if (!w->hw_ready) {
+        for (int i = 0; i < n2; i++) w->hw[i] = (i < n) ? d->raman_h[i] : 0.0;
+        fft_c2c(n2, w->hw, -1);
+        w->hw_ready = 1;
+    }
+    fft_c2c(n2, w->E2, -1);
+    for (int i = 0; i < n2; i++) w->tmp[i] = cmul(w->hw[i], w->E2[i]) * d->dt_o;
+    fft_c2c(n2, w->tmp, +1);
+    double s = 1.0 / (double)n2;
+    for (int i = 0; i < n; i++) {
+        double P = creal(w->tmp[i]) * s;
+        Pt[i] = Pt[i] + (rho * E[i]) * P;
+    }
 }
 
 /* RamanPolarEnv call, src/Nonlinear.jl:282-333, with R.r(ht, ρ) = raman_h. */

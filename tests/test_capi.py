@@ -107,24 +107,31 @@ def test_fno_arg_checks(capi):
     assert L.lh_fno_create(ctypes.byref(h), ctypes.byref(cfg)) == capi.LH_E_ARG  # n_x not a multiple of 64
 
 
-def test_envelope_beyond_largest_class_rejected(capi, pkg):
-    """The soliton self-frequency-shift case of test/test_gnlse.jl:33-56
-    (trange 13 ps over 450-8000 nm) needs an Nt = 16384 envelope grid, above
-    the envelope kernels' largest class (Nt = 8192); lh_create refuses it with
-    LH_E_ARG before touching a device."""
-    import math
+def test_grids_beyond_largest_classes_rejected(capi, pkg):
+    """An envelope grid of Nt = 32768 (trange 26 ps over 450-8000 nm) and a
+    field grid of Nto = 32768 lie above the largest compiled classes
+    (envelope Nt = 16384, field Nto = 16384); lh_create refuses them with
+    LH_E_ARG before touching a device.  The Nt = 16384 envelope grid of the
+    soliton self-frequency-shift case (test/test_gnlse.jl:33-56) is a
+    supported class."""
     from luna_jl_amd import interface
     gamma, beta2, tau0, fr = 0.1, -1e-26, 10e-15, 0.18
     P0 = abs(beta2) / ((1 - fr) * gamma * tau0 ** 2)
-    flength = math.pi / 2 * tau0 ** 2 / abs(beta2) * 90
-    Ew, grid, linop, tr, _, _ = interface.prop_gnlse_args(
-        gamma, flength, [0.0, 0.0, beta2], lambda0=835e-9, tau_fwhm=1.763 * tau0, power=P0,
-        pulseshape="sech", lambda_lims=(450e-9, 8000e-9), trange=13e-12, raman=True, shock=False,
-        fr=fr, shotnoise=False, ramanmodel="sdo", tau1=12.2e-15, tau2=32e-15)
-    assert len(grid.t) == 16384
-    holder = tr.descriptor(linop)
+    sizes = {}
+    for trange in (13e-12, 26e-12):
+        Ew, grid, linop, tr, _, _ = interface.prop_gnlse_args(
+            gamma, 1.0, [0.0, 0.0, beta2], lambda0=835e-9, tau_fwhm=1.763 * tau0, power=P0,
+            pulseshape="sech", lambda_lims=(450e-9, 8000e-9), trange=trange, raman=True, shock=False,
+            fr=fr, shotnoise=False, ramanmodel="sdo", tau1=12.2e-15, tau2=32e-15)
+        sizes[trange] = (len(grid.t), tr.descriptor(linop))
+    assert sizes[13e-12][0] == 16384 and sizes[26e-12][0] == 32768
     h = ctypes.c_void_p()
-    assert capi.lib().lh_create(ctypes.byref(h), holder.ref) == capi.LH_E_ARG
+    assert capi.lib().lh_create(ctypes.byref(h), sizes[26e-12][1].ref) == capi.LH_E_ARG
+    Ew, grid, linop, tr, _, _ = interface.prop_capillary_args(
+        125e-6, 1.0, "He", 1.0, lambda0=800e-9, lambda_lims=(150e-9, 4e-6), trange=2e-12,
+        tau_fwhm=10e-15, energy=120e-6, shotnoise=False, ppt_N=2 ** 10)
+    assert grid.n_to == 32768
+    assert capi.lib().lh_create(ctypes.byref(h), tr.descriptor(linop).ref) == capi.LH_E_ARG
 
 
 def test_host_alloc_argument_checks(capi):

@@ -88,3 +88,75 @@ def test_env_solve_full_cfg1(cfg1, orc, capi):
     assert stats.status == 0 and stats.saved == 201
     assert stats.steps == stats_r.steps
     assert spec_err(sv[-1], sv_r[-1], cfg1.sidx) <= 1e-6
+
+
+# -- the Nt = 16384 envelope grid (transforms staged through HBM) -------------
+
+def _ssfs_args(pkg, flength=None):
+    """test/test_gnlse.jl:33-56: a fundamental soliton (τ0 = 10 fs, β2 = -1e-26
+    s²/m, γ = 0.1) with the SDO Raman response shifts its frequency over 90
+    soliton periods; its 13 ps grid has Nt = 16384."""
+    from luna_jl_amd import interface
+    gamma, beta2, N, tau0, fr = 0.1, -1e-26, 1.0, 10e-15, 0.18
+    P0 = N ** 2 * abs(beta2) / ((1 - fr) * gamma * tau0 ** 2)
+    L = math.pi / 2 * tau0 ** 2 / abs(beta2) * 90 if flength is None else flength
+    return interface.prop_gnlse_args(
+        gamma, L, [0.0, 0.0, beta2], lambda0=835e-9, tau_fwhm=1.763 * tau0, power=P0,
+        pulseshape="sech", lambda_lims=(450e-9, 8000e-9), trange=13e-12, raman=True, shock=False,
+        fr=fr, shotnoise=False, ramanmodel="sdo", tau1=12.2e-15, tau2=32e-15), L
+
+
+def test_env16384_rhs(pkg, orc):
+    print("case: envelope rhs Nt=16384 (Kerr+Raman, HBM-staged transforms)", flush=True)
+    from luna_jl_amd import configs, runtime
+    (Ew, grid, linop, tr, _, _), _L = _ssfs_args(pkg)
+    assert grid.n_t == 16384
+    h = tr.descriptor(linop)
+    ctx = runtime.Context(h)
+    rng = np.random.default_rng(5)
+    evolved = Ew * np.exp(1j * rng.uniform(0, 0.3, Ew.size))  # a chirped state
+    for E in (Ew, 40.0 * evolved):
+        nl = ctx.rhs(E)
+        st, ref = orc.rhs(h, E)
+        assert st == 0
+        assert rel_l2(nl, ref) <= 1e-12, rel_l2(nl, ref)
+    # the supercontinuum example's 12.5 ps grid (examples/simple_interface/gnlse_scg.jl:14)
+    g = configs.gnlse_instance(trange=12.5e-12, flength=0.15)
+    assert g.grid.n_t == 16384
+    ctx2 = runtime.Context(g.holder)
+    nl = ctx2.rhs(g.Ew0)
+    st, ref = orc.rhs(g.holder, g.Ew0)
+    assert st == 0 and rel_l2(nl, ref) <= 1e-12, rel_l2(nl, ref)
+
+
+def test_env16384_ssfs_golden(pkg, orc, capi):
+    """The reference's golden values, computed by the GPU solver: peak
+    frequency 1.3975031072069625e15 rad/s and peak time 6.079566076444579e-12 s
+    at z = flength (rtol 1e-14 there, i.e. the same grid bins)."""
+    print("case: soliton self-frequency shift, Nt=16384, 1.41 m", flush=True)
+    from luna_jl_amd import runtime
+    (Ew, grid, linop, tr, _, _), L = _ssfs_args(pkg)
+    h = tr.descriptor(linop)
+    ctx = runtime.Context(h)
+    opts = capi.rk_opts(L, save_n=2)
+    Eg, sv, zs, stats = ctx.solve(Ew, opts)
+    assert stats.status == 0 and zs[-1] == L
+    Eout = sv[-1]
+    assert grid.omega[np.argmax(np.abs(Eout) ** 2)] == 1.3975031072069625e15
+    assert grid.t[np.argmax(np.abs(np.fft.ifft(Eout)) ** 2)] == 6.079566076444579e-12
+    st, Er, sv_r, zs_r, stats_r = orc.solve(h, Ew, opts)
+    assert st == 0 and stats.steps == stats_r.steps
+    assert spec_err(sv[-1], sv_r[-1], grid.sidx) <= 1e-6
+
+
+def test_env16384_scg_short(pkg, orc, capi):
+    print("case: supercontinuum on the 12.5 ps grid (Nt=16384), 1 cm", flush=True)
+    from luna_jl_amd import configs, runtime
+    g = configs.gnlse_instance(trange=12.5e-12, flength=0.01)
+    ctx = runtime.Context(g.holder)
+    opts = capi.rk_opts(g.zmax, save_n=11)
+    Ew, sv, zs, stats = ctx.solve(g.Ew0, opts)
+    st, Ew_r, sv_r, zs_r, stats_r = orc.solve(g.holder, g.Ew0, opts)
+    assert stats.status == 0 and stats.steps == stats_r.steps
+    for k in range(11):
+        assert spec_err(sv[k], sv_r[k], g.sidx) <= 1e-8

@@ -15,10 +15,8 @@ reference's own golden values computed by the GPU solver.
   the per-step exponentials: 3e-9, the oracle itself reaching 3.8e-10).
 
 The reference's soliton self-frequency-shift golden values (test/test_gnlse.jl:
-33-56) need an Nt = 16384 envelope grid, beyond the envelope kernels' largest
-class (Nt = 8192: the Raman convolution's 2·Nt-point transform must fit one
-workgroup's LDS); the oracle holds them (test_oracle_gnlse.py) and lh_create
-rejects that grid (test_capi.py).
+33-56) need an Nt = 16384 envelope grid, whose transforms are staged through
+HBM; the GPU solver reproduces them in test_gpu_env.py.
 """
 
 import numpy as np
