@@ -371,6 +371,46 @@ int lh_last_launch_times(double *ms, int *n_inst, int nclass);
 int lh_host_alloc(size_t bytes, void **ptr);
 int lh_host_free(void *ptr);
 
+/* Plans of the radial RHS and their wisdom.  Counterpart of the reference's
+ * FFTW planning: Luna.setup plans its transforms with settings["fftw_flag"]
+ * (FFTW.PATIENT by default, src/Luna.jl:13-14, 239-258) between
+ * Utils.loadFFTwisdom and Utils.saveFFTwisdom (src/Utils.jl:84-108).  A plan
+ * is one variant of the radial RHS launch sequence: the radix plan of the
+ * per-column transforms (0: radix 8, radix 16 from 8192 points; 1: radix 16
+ * throughout) and the FP64 MFMA tiles per wave along t of the two Hankel
+ * GEMMs (1, 2 or 4).  Every plan computes the same RHS; the radix plan moves
+ * its last bits.  Radial contexts only (LH_E_ARG otherwise): the
+ * mode-averaged grids run one persistent kernel per size class with a fixed
+ * radix plan.
+ *   lh_get_plan / lh_set_plan  the context's plan (a set drops the captured
+ *                              step graph, re-captured on the next solve)
+ *   lh_autotune   times every plan on one RHS of the field Ew (host, complex
+ *                 [n_omega·n_r]; NULL: a zero field), reps timed launches
+ *                 each after one untimed; installs the fastest plan whose RHS
+ *                 agrees with the default plan's to 1e-12 (relative L2) in
+ *                 the context and in the wisdom; tried[0..min(cap,n)-1]
+ *                 receives every plan with its time (default plan first) and
+ *                 *n_tried their count (6)
+ *   lh_wisdom_export  the wisdom as text ("luna-hip-wisdom 1", then one line
+ *                 "radial n_to n_r fft_radix_plan gemm_tiles rhs_us" per
+ *                 shape); *len = bytes needed with the NUL; buf may be NULL
+ *   lh_wisdom_import  merges such text (LH_E_ARG and nothing merged if any
+ *                 line is malformed); contexts created afterwards for a shape
+ *                 in the wisdom take its plan
+ *   lh_wisdom_forget  empties the wisdom */
+typedef struct lh_plan {
+    int fft_radix_plan;
+    int gemm_tiles;
+    double rhs_us; /* measured microseconds per RHS (lh_autotune), else 0 */
+    int agrees;    /* 1 if the plan's RHS agreed with the default plan's */
+} lh_plan;
+int lh_get_plan(const lh_ctx *ctx, lh_plan *plan);
+int lh_set_plan(lh_ctx *ctx, const lh_plan *plan);
+int lh_autotune(lh_ctx *ctx, const double *Ew, int reps, lh_plan *tried, int cap, int *n_tried);
+int lh_wisdom_export(char *buf, size_t cap, size_t *len);
+int lh_wisdom_import(const char *text);
+int lh_wisdom_forget(void);
+
 /* Allocation-free solve.  No counterpart in the reference: it is for callers
  * that cannot allocate device memory (code under HIP graph capture, tight
  * serving loops, harnesses that forbid hipMalloc).  The same device solve as

@@ -103,6 +103,12 @@ class LhFnoCfg(C.Structure):
                 ("layers", C.c_int32), ("hidden", C.c_int32), ("max_batch", C.c_int32)]
 
 
+class LhPlan(C.Structure):
+    """``lh_plan`` of include/luna_hip.h: one variant of the radial RHS."""
+    _fields_ = [("fft_radix_plan", C.c_int32), ("gemm_tiles", C.c_int32), ("rhs_us", C.c_double),
+                ("agrees", C.c_int32)]
+
+
 class LhAdam(C.Structure):
     _fields_ = [("lr", C.c_float), ("beta1", C.c_float), ("beta2", C.c_float), ("eps", C.c_float),
                 ("weight_decay", C.c_float)]
@@ -202,6 +208,12 @@ def lib():
     L.lh_host_free.argtypes = [C.c_void_p]
     L.lh_solve_noalloc.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(LhRkOpts), C.c_void_p, _dp,
                                    C.POINTER(LhStats), C.c_void_p]
+    L.lh_get_plan.argtypes = [C.c_void_p, C.POINTER(LhPlan)]
+    L.lh_set_plan.argtypes = [C.c_void_p, C.POINTER(LhPlan)]
+    L.lh_autotune.argtypes = [C.c_void_p, _dp, C.c_int, C.POINTER(LhPlan), C.c_int, C.POINTER(C.c_int)]
+    L.lh_wisdom_export.argtypes = [C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]
+    L.lh_wisdom_import.argtypes = [C.c_char_p]
+    L.lh_wisdom_forget.argtypes = []
     L.lh_set_step_stats.argtypes = [C.c_void_p, C.POINTER(LhStatsDesc)]
     L.lh_set_gradient.argtypes = [C.c_void_p, C.POINTER(LhGradientDesc)]
     L.lh_ppt_table.argtypes = [C.POINTER(LhPptDesc), C.c_int, _dp, _dp, _dp]
@@ -225,6 +237,7 @@ EXPORTED = ["lh_strerror", "lh_device_count", "lh_create", "lh_destroy", "lh_rhs
             "lh_rhs_device", "lh_ctx_status", "lh_solve", "lh_solve_stream", "lh_batch_solve", "lh_batch_solve_opts", "lh_set_state",
             "lh_batch_solve_resident", "lh_get_state", "lh_get_saves", "lh_last_launch_times",
             "lh_host_alloc", "lh_host_free", "lh_solve_noalloc",
+            "lh_get_plan", "lh_set_plan", "lh_autotune", "lh_wisdom_export", "lh_wisdom_import", "lh_wisdom_forget",
             "lh_set_step_stats", "lh_stats_eval", "lh_get_step_stats", "lh_set_gradient", "lh_ppt_table",
             "lh_fno_create", "lh_fno_destroy", "lh_fno_param_count", "lh_fno_set_params",
             "lh_fno_get_params", "lh_fno_forward", "lh_fno_loss_grad", "lh_fno_set_data", "lh_fno_train"]

@@ -179,9 +179,11 @@ __device__ __forceinline__ double2 tw_get(const TwLds& t, int m) { return cmul(t
 // radix-16 pass of 8192 points).  The
 // twiddle w = exp(∓2πi k/(NS·R)) comes from the LDS table; its powers w^r by
 // repeated multiplication (error ≤ R ulp, far below the 1e-12 parity budget).
-template <int M, int R, int NS, bool INV>
+template <int M, int R, int NS, bool INV, int PLAN = 0>
 __device__ __forceinline__ void fft_pass(double2* buf, const TwLds& tw) {
-    constexpr int TA = (M / 8 < LH_T) ? M / 8 : LH_T;
+    // plan 0: min(M/8, LH_T) threads (one radix-8 butterfly each where they
+    // fit); plan 1: one thread per radix-R butterfly up to LH_T
+    constexpr int TA = (PLAN == 1) ? ((M / R < LH_T) ? M / R : LH_T) : ((M / 8 < LH_T) ? M / 8 : LH_T);
     constexpr int NB = M / R;
     constexpr int BPT = NB / TA;
     static_assert(BPT >= 1 && NB % TA == 0, "FFT pass split");
@@ -221,29 +223,34 @@ __device__ __forceinline__ void fft_pass(double2* buf, const TwLds& tw) {
     __syncthreads();
 }
 
-// Radix choice per pass: 8, except that transforms of 8192 points and more
-// take radix 16 after the first pass (8·16·16·4 for 8192: four LDS passes
-// instead of five, each still one butterfly per thread).
-template <int M, int NS>
+// Radix choice per pass.  Plan 0: 8, except that transforms of 8192 points
+// and more take radix 16 after the first pass (8·16·16·4 for 8192: four LDS
+// passes instead of five, each still one butterfly per thread).  Plan 1:
+// radix 16 wherever 16 points remain (16·16·16 for 4096: three passes with
+// half the threads active instead of four with all).  Which plan is faster
+// for a given kernel is measured, not assumed (lh_autotune).
+template <int M, int NS, int PLAN = 0>
 struct PassRadix {
-    static constexpr int R = (M >= 8192 && NS > 1 && M / NS >= 16) ? 16 : ((M / NS >= 8) ? 8 : (M / NS));
+    static constexpr int R =
+        (PLAN == 1) ? ((M / NS >= 16) ? 16 : (M / NS))
+                    : ((M >= 8192 && NS > 1 && M / NS >= 16) ? 16 : ((M / NS >= 8) ? 8 : (M / NS)));
 };
 
-template <int M, int NS, bool INV>
+template <int M, int NS, bool INV, int PLAN = 0>
 __device__ __forceinline__ void fft_passes(double2* buf, const TwLds& tw) {
     if constexpr (NS < M) {
-        constexpr int R = PassRadix<M, NS>::R;
-        fft_pass<M, R, NS, INV>(buf, tw);
-        fft_passes<M, NS * R, INV>(buf, tw);
+        constexpr int R = PassRadix<M, NS, PLAN>::R;
+        fft_pass<M, R, NS, INV, PLAN>(buf, tw);
+        fft_passes<M, NS * R, INV, PLAN>(buf, tw);
     }
 }
 
 // Unnormalised complex FFT of length M held in LDS (natural order in and out).
 // INV=false: exp(-2πi jk/M) (FFTW_FORWARD); INV=true: exp(+2πi jk/M).
 // Caller must __syncthreads() before (data complete); returns synchronised.
-template <int M, bool INV>
+template <int M, bool INV, int PLAN = 0>
 __device__ __forceinline__ void fft_lds(double2* buf, const TwLds& tw) {
-    fft_passes<M, 1, INV>(buf, tw);
+    fft_passes<M, 1, INV, PLAN>(buf, tw);
 }
 
 // Half-length real-transform packing.  For a real series x of length 2M held

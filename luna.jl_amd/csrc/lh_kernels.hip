@@ -29,6 +29,7 @@
 #include <string.h>
 
 #include <mutex>
+#include <string>
 #include <time.h>
 #include <vector>
 
@@ -2292,6 +2293,7 @@ struct lh_ctx {
     RadArgs* rargs;      // [0]: solver, [1]: single RHS
     RadCtl* rctl;        // [0]: solver, [1]: single RHS
     hipGraphExec_t rgraph;
+    RadPlan rplan;       // variant of the RHS launch sequence (rgraph is captured with it)
     // per-step statistics
     double* omega;       // [Nω] ω (shared)
     double* st_ew;       // [Nω] energy weights (shared)
@@ -2702,6 +2704,39 @@ static int prepare_instance(lh_ctx* c, const lh_rk_opts* o, bool want_saves, DIn
 }
 
 // --------------------------------------------------------------------------
+// Plan wisdom: measured radial plans keyed by (n_to, n_r)
+// --------------------------------------------------------------------------
+// The role of FFTW's wisdom in the reference (src/Utils.jl:84-108 imports it
+// before planning and exports it after): a context created for a shape with
+// wisdom takes the recorded plan without measuring; any other takes
+// RAD_DEFAULT_PLAN.  Process-wide, guarded by g_wisdom_mu.
+struct WisdomEntry {
+    int n_to, n_r;
+    RadPlan plan;
+    double us;
+};
+static std::mutex g_wisdom_mu;
+static std::vector<WisdomEntry> g_wisdom;
+
+static RadPlan wisdom_lookup(int n_to, int n_r) {
+    std::lock_guard<std::mutex> lk(g_wisdom_mu);
+    for (const WisdomEntry& w : g_wisdom)
+        if (w.n_to == n_to && w.n_r == n_r) return w.plan;
+    return RAD_DEFAULT_PLAN;
+}
+
+static void wisdom_record(int n_to, int n_r, RadPlan p, double us) {
+    std::lock_guard<std::mutex> lk(g_wisdom_mu);
+    for (WisdomEntry& w : g_wisdom)
+        if (w.n_to == n_to && w.n_r == n_r) {
+            w.plan = p;
+            w.us = us;
+            return;
+        }
+    g_wisdom.push_back({n_to, n_r, p, us});
+}
+
+// --------------------------------------------------------------------------
 // Radial transform: set-up, one RHS, solve
 // --------------------------------------------------------------------------
 static int rad_setup(lh_ctx* c, const lh_desc* d) {
@@ -2716,6 +2751,7 @@ static int rad_setup(lh_ctx* c, const lh_desc* d) {
     HIPCK(hipMalloc(&c->partial, sizeof(double) * 3 * nr));
     HIPCK(hipMalloc(&c->rargs, sizeof(RadArgs) * 2));
     HIPCK(hipMalloc(&c->rctl, sizeof(RadCtl) * 2));
+    c->rplan = wisdom_lookup(d->n_to, d->n_r);
     DInst pw = base_inst(c);
     pw.e_scale = 1.0;  // the 1/Nto of the inverse transform is folded into the k→r contraction
     return upload(&c->pw, &pw, 1);
@@ -2786,7 +2822,7 @@ static int rad_solve(lh_ctx* c, const lh_rk_opts* o, bool want_saves, hipStream_
     HIPCK(hipMemcpy(c->rargs, &a, sizeof(a), hipMemcpyHostToDevice));
     HIPCK(hipMemcpy(c->rctl, &h, sizeof(h), hipMemcpyHostToDevice));
     HIPCK(hipMemcpyAsync(c->Y, c->Yin, sizeof(double2) * (size_t)c->n_state, hipMemcpyDeviceToDevice, s));
-    rad_rhs_dispatch(n_to, n_r, c->rargs, tw, -1, s);  // PreconStepper init: k1 = fbar(y0)
+    rad_rhs_dispatch(c->rplan, n_to, n_r, c->rargs, tw, -1, s);  // PreconStepper init: k1 = fbar(y0)
     HIPCK(hipGetLastError());
     HIPCK(hipMemcpyAsync(&h, c->rctl, sizeof(h), hipMemcpyDeviceToHost, s));
     HIPCK(hipStreamSynchronize(s));
@@ -2797,7 +2833,7 @@ static int rad_solve(lh_ctx* c, const lh_rk_opts* o, bool want_saves, hipStream_
     if (!h.done && !c->rgraph) {
         hipGraph_t g = nullptr;
         HIPCK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-        rad_step_dispatch(n_to, n_r, c->rargs, tw, s);
+        rad_step_dispatch(c->rplan, n_to, n_r, c->rargs, tw, s);
         hipError_t e1 = hipGetLastError();
         hipError_t e2 = hipStreamEndCapture(s, &g);
         if (e1 != hipSuccess || e2 != hipSuccess || !g) {
@@ -3009,7 +3045,7 @@ int lh_rhs_device(lh_ctx* c, void* nl_dev, const void* Ew_dev, double z, void* s
         rad_ctl_init(&h, nullptr);
         HIPCK(hipMemcpyAsync(c->rargs + 1, &a, sizeof(a), hipMemcpyHostToDevice, s));
         HIPCK(hipMemcpyAsync(c->rctl + 1, &h, sizeof(h), hipMemcpyHostToDevice, s));
-        rad_rhs_dispatch(c->d.n_to, c->d.n_r, c->rargs + 1, tw, -1, s);
+        rad_rhs_dispatch(c->rplan, c->d.n_to, c->d.n_r, c->rargs + 1, tw, -1, s);
     } else {
         DInst in = base_inst(c);
         in.rhs_in = (const double2*)Ew_dev;
@@ -3535,6 +3571,176 @@ int lh_host_alloc(size_t bytes, void** ptr) {
 int lh_host_free(void* ptr) {
     if (!ptr) return LH_OK;
     return hipHostFree(ptr) == hipSuccess ? LH_OK : LH_E_HIP;
+}
+
+int lh_get_plan(const lh_ctx* c, lh_plan* p) {
+    if (!c || !p) return LH_E_ARG;
+    if (c->d.transform != LH_TRANS_RADIAL) return LH_E_ARG;
+    p->fft_radix_plan = c->rplan.fft;
+    p->gemm_tiles = c->rplan.mt;
+    p->rhs_us = 0.0;
+    p->agrees = 1;
+    return LH_OK;
+}
+
+int lh_set_plan(lh_ctx* c, const lh_plan* p) {
+    if (!c || !p || c->d.transform != LH_TRANS_RADIAL) return LH_E_ARG;
+    const RadPlan np = {p->fft_radix_plan, p->gemm_tiles};
+    if (!rad_plan_ok(np)) return LH_E_ARG;
+    if (np.fft != c->rplan.fft || np.mt != c->rplan.mt) {
+        HIPCK(hipSetDevice(c->device));
+        HIPCK(hipStreamSynchronize(c->stream));
+        if (c->rgraph) {  // captured with the old plan
+            (void)hipGraphExecDestroy(c->rgraph);
+            c->rgraph = nullptr;
+        }
+        c->rplan = np;
+    }
+    return LH_OK;
+}
+
+// Measures every plan on one RHS of ctx's shape (the field Ew, or zero when
+// NULL) and installs the fastest in ctx and in the wisdom.  Each plan runs
+// once untimed, then reps times between two events on the context's stream;
+// a plan whose RHS differs from the default plan's by more than 1e-12
+// (relative L2) is never chosen.
+int lh_autotune(lh_ctx* c, const double* Ew, int reps, lh_plan* tried, int cap, int* n_tried) {
+    if (!c || reps < 1 || (cap > 0 && !tried) || c->d.transform != LH_TRANS_RADIAL) return LH_E_ARG;
+    HIPCK(hipSetDevice(c->device));
+    const double2* tw;
+    int st = twiddles(c->device, &tw);
+    if (st) return st;
+    hipStream_t s = c->stream;
+    const size_t n = (size_t)c->n_state;
+    double2 *din = nullptr, *dout = nullptr;
+    if (hipMalloc(&din, sizeof(double2) * n) != hipSuccess) return LH_E_NOMEM;
+    if (hipMalloc(&dout, sizeof(double2) * n) != hipSuccess) {
+        (void)hipFree(din);
+        return LH_E_NOMEM;
+    }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    std::vector<double2> ref(n), out(n);
+    RadPlan plans[6];
+    int np = 0;
+    plans[np++] = RAD_DEFAULT_PLAN;
+    for (int f = 0; f < 2; f++)
+        for (int mt : {1, 2, 4})
+            if (!(f == RAD_DEFAULT_PLAN.fft && mt == RAD_DEFAULT_PLAN.mt)) plans[np++] = {f, mt};
+    RadPlan best = RAD_DEFAULT_PLAN;
+    double best_us = 0.0;
+    st = LH_OK;
+    do {
+        if (Ew) {
+            if (hipMemcpy(din, Ew, sizeof(double2) * n, hipMemcpyHostToDevice) != hipSuccess) { st = LH_E_HIP; break; }
+        } else if (hipMemset(din, 0, sizeof(double2) * n) != hipSuccess) { st = LH_E_HIP; break; }
+        if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) { st = LH_E_HIP; break; }
+        RadArgs a = rad_args(c, 1);
+        a.rhs_in = din;
+        a.rhs_out = dout;
+        RadCtl h;
+        rad_ctl_init(&h, nullptr);
+        if (hipStreamSynchronize(s) != hipSuccess ||
+            hipMemcpy(c->rargs + 1, &a, sizeof(a), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(c->rctl + 1, &h, sizeof(h), hipMemcpyHostToDevice) != hipSuccess) { st = LH_E_HIP; break; }
+        double refn = 0.0;
+        for (int i = 0; i < np && !st; i++) {
+            const RadPlan p = plans[i];
+            rad_rhs_dispatch(p, c->d.n_to, c->d.n_r, c->rargs + 1, tw, -1, s);
+            if (hipGetLastError() != hipSuccess || hipStreamSynchronize(s) != hipSuccess) { st = LH_E_HIP; break; }
+            if (hipMemcpy(i == 0 ? ref.data() : out.data(), dout, sizeof(double2) * n, hipMemcpyDeviceToHost) !=
+                hipSuccess) { st = LH_E_HIP; break; }
+            double dn = 0.0;
+            if (i == 0) {
+                for (size_t k = 0; k < n; k++) refn += ref[k].x * ref[k].x + ref[k].y * ref[k].y;
+            } else {
+                for (size_t k = 0; k < n; k++) {
+                    const double dx = out[k].x - ref[k].x, dy = out[k].y - ref[k].y;
+                    dn += dx * dx + dy * dy;
+                }
+            }
+            const bool agrees = !(dn > 1e-24 * refn);
+            (void)hipEventRecord(e0, s);
+            for (int r = 0; r < reps; r++) rad_rhs_dispatch(p, c->d.n_to, c->d.n_r, c->rargs + 1, tw, -1, s);
+            (void)hipEventRecord(e1, s);
+            float ms = 0.0f;
+            if (hipGetLastError() != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                hipEventElapsedTime(&ms, e0, e1) != hipSuccess) { st = LH_E_HIP; break; }
+            const double us = 1e3 * (double)ms / reps;
+            if (i < cap) {
+                tried[i].fft_radix_plan = p.fft;
+                tried[i].gemm_tiles = p.mt;
+                tried[i].rhs_us = us;
+                tried[i].agrees = agrees ? 1 : 0;
+            }
+            if (agrees && (i == 0 || us < best_us)) {
+                best = p;
+                best_us = us;
+            }
+        }
+    } while (0);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(din);
+    (void)hipFree(dout);
+    if (st) return st;
+    if (n_tried) *n_tried = np;
+    lh_plan bp = {best.fft, best.mt, best_us, 1};
+    st = lh_set_plan(c, &bp);
+    if (!st) wisdom_record(c->d.n_to, c->d.n_r, best, best_us);
+    return st;
+}
+
+// Wisdom as text: a header line, then one line per shape
+// "radial <n_to> <n_r> <fft_radix_plan> <gemm_tiles> <rhs_us>".
+int lh_wisdom_export(char* buf, size_t cap, size_t* len) {
+    std::string t = "luna-hip-wisdom 1\n";
+    {
+        std::lock_guard<std::mutex> lk(g_wisdom_mu);
+        char line[128];
+        for (const WisdomEntry& w : g_wisdom) {
+            snprintf(line, sizeof(line), "radial %d %d %d %d %.3f\n", w.n_to, w.n_r, w.plan.fft, w.plan.mt, w.us);
+            t += line;
+        }
+    }
+    if (len) *len = t.size() + 1;
+    if (!buf) return LH_OK;
+    if (cap < t.size() + 1) return LH_E_ARG;
+    memcpy(buf, t.c_str(), t.size() + 1);
+    return LH_OK;
+}
+
+int lh_wisdom_import(const char* text) {
+    if (!text) return LH_E_ARG;
+    std::vector<WisdomEntry> got;
+    const char* p = text;
+    int lineno = 0;
+    while (*p) {
+        const char* e = strchr(p, '\n');
+        std::string line = e ? std::string(p, e - p) : std::string(p);
+        p = e ? e + 1 : p + line.size();
+        if (lineno++ == 0) {
+            if (line != "luna-hip-wisdom 1") return LH_E_ARG;
+            continue;
+        }
+        if (line.empty()) continue;
+        WisdomEntry w;
+        char kind[16];
+        int used = 0;
+        if (sscanf(line.c_str(), "%15s %d %d %d %d %lf%n", kind, &w.n_to, &w.n_r, &w.plan.fft, &w.plan.mt, &w.us,
+                   &used) != 6 || strcmp(kind, "radial") != 0 || (size_t)used != line.size() || !rad_plan_ok(w.plan) ||
+            !rad_size_ok(w.n_to) || w.n_r < 64 || w.n_r % 64 != 0)
+            return LH_E_ARG;
+        got.push_back(w);
+    }
+    if (lineno == 0) return LH_E_ARG;
+    for (const WisdomEntry& w : got) wisdom_record(w.n_to, w.n_r, w.plan, w.us);
+    return LH_OK;
+}
+
+int lh_wisdom_forget(void) {
+    std::lock_guard<std::mutex> lk(g_wisdom_mu);
+    g_wisdom.clear();
+    return LH_OK;
 }
 
 // Offsets of the working memory of one lh_solve_noalloc call inside `scratch`

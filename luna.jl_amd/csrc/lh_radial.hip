@@ -63,7 +63,9 @@ struct RadArgs {
 };
 
 // ---------------------------------------------------------------- A: prologue + c2r
-template <int MO>
+// PLAN selects the radix plan of the column transforms (fft_lds), measured
+// per shape by lh_autotune.
+template <int MO, int PLAN>
 __global__ __launch_bounds__(LH_T) void rad_stage_a(const RadArgs* __restrict__ ap, const double2* __restrict__ twg,
                                                     int stage) {
     const RadArgs& a = *ap;
@@ -88,7 +90,7 @@ __global__ __launch_bounds__(LH_T) void rad_stage_a(const RadArgs* __restrict__ 
     __syncthreads();
     c2r_prepack<MO>(sm.buf, sm.tw);
     __syncthreads();
-    fft_lds<MO, true>(sm.buf, sm.tw);
+    fft_lds<MO, true, PLAN>(sm.buf, sm.tw);
     double* out = a.Ek + (size_t)a.n_to * c;
     for (int j = threadIdx.x; j < MO; j += LH_T) {
         double2 z = sm.buf[pad(j)];
@@ -184,7 +186,7 @@ __global__ __launch_bounds__(LH_T) void rad_pointwise(const RadArgs* __restrict_
 }
 
 // ---------------------------------------------------------------- E: r2c + epilogue
-template <int MO>
+template <int MO, int PLAN>
 __global__ __launch_bounds__(LH_T) void rad_stage_e(const RadArgs* __restrict__ ap, const double2* __restrict__ twg,
                                                     int stage) {
     const RadArgs& a = *ap;
@@ -197,7 +199,7 @@ __global__ __launch_bounds__(LH_T) void rad_stage_e(const RadArgs* __restrict__ 
     const double* col = a.Ek + (size_t)a.n_to * c;
     for (int j = threadIdx.x; j < MO; j += LH_T) sm.buf[pad(j)] = make_double2(col[2 * j], col[2 * j + 1]);
     __syncthreads();
-    fft_lds<MO, false>(sm.buf, sm.tw);
+    fft_lds<MO, false, PLAN>(sm.buf, sm.tw);
     const double2* of = a.outfac + off;
     if (stage < 0) {
         double2* out = (a.rhs_out ? a.rhs_out : a.K[ctl.perm[0]]) + off;
@@ -394,23 +396,31 @@ __global__ void rad_step_end(const RadArgs* __restrict__ ap) {
 }
 
 // Launch helpers for one RHS and one step (also used during graph capture).
-#ifndef RAD_MT
-#define RAD_MT 2  // MFMA tiles per wave along t (n_to is a multiple of 32·RAD_MT); 2 measured faster than 4
-#endif
+// A plan is one variant of the RHS launch sequence: the radix plan of the
+// column transforms (0: radix 8, radix 16 from 8192 points; 1: radix 16
+// throughout) and the MFMA tiles per wave along t of the two GEMMs (1, 2 or
+// 4; n_to is a multiple of 32·4).  Every plan computes the same transform;
+// the last bits of the result differ with the radix plan.
+struct RadPlan {
+    int fft;  // 0 or 1
+    int mt;   // 1, 2 or 4
+};
+static constexpr RadPlan RAD_DEFAULT_PLAN = {0, 2};
+static bool rad_plan_ok(const RadPlan& p) { return (p.fft == 0 || p.fft == 1) && (p.mt == 1 || p.mt == 2 || p.mt == 4); }
 
-template <int MO>
+template <int MO, int PLAN, int MT>
 static void rad_launch_rhs(const RadArgs* dargs, const double2* tw, int n_to, int n_r, int stage, hipStream_t s) {
-    dim3 g(n_to / (32 * RAD_MT), n_r / 64);
-    hipLaunchKernelGGL((rad_stage_a<MO>), dim3(n_r), dim3(LH_T), 0, s, dargs, tw, stage);
-    hipLaunchKernelGGL((rad_gemm<RAD_MT>), g, dim3(256), 0, s, dargs, 0);
+    dim3 g(n_to / (32 * MT), n_r / 64);
+    hipLaunchKernelGGL((rad_stage_a<MO, PLAN>), dim3(n_r), dim3(LH_T), 0, s, dargs, tw, stage);
+    hipLaunchKernelGGL((rad_gemm<MT>), g, dim3(256), 0, s, dargs, 0);
     hipLaunchKernelGGL((rad_pointwise<MO>), dim3(n_r), dim3(LH_T), 0, s, dargs);
-    hipLaunchKernelGGL((rad_gemm<RAD_MT>), g, dim3(256), 0, s, dargs, 1);
-    hipLaunchKernelGGL((rad_stage_e<MO>), dim3(n_r), dim3(LH_T), 0, s, dargs, tw, stage);
+    hipLaunchKernelGGL((rad_gemm<MT>), g, dim3(256), 0, s, dargs, 1);
+    hipLaunchKernelGGL((rad_stage_e<MO, PLAN>), dim3(n_r), dim3(LH_T), 0, s, dargs, tw, stage);
 }
 
-template <int MO, int MC>
+template <int MO, int MC, int PLAN, int MT>
 static void rad_launch_step(const RadArgs* dargs, const double2* tw, int n_to, int n_r, hipStream_t s) {
-    for (int st = 0; st < 6; st++) rad_launch_rhs<MO>(dargs, tw, n_to, n_r, st, s);
+    for (int st = 0; st < 6; st++) rad_launch_rhs<MO, PLAN, MT>(dargs, tw, n_to, n_r, st, s);
     hipLaunchKernelGGL(rad_combine, dim3(n_r), dim3(LH_T), 0, s, dargs);
     hipLaunchKernelGGL(rad_control, dim3(1), dim3(64), 0, s, dargs);
     hipLaunchKernelGGL((rad_stepfun<MC>), dim3(n_r), dim3(LH_T), 0, s, dargs, tw);
@@ -421,18 +431,49 @@ static void rad_launch_step(const RadArgs* dargs, const double2* tw, int n_to, i
 // Oversampled lengths with compiled radial kernels (Nt = Nto/2, Nω = Nt/2+1).
 static bool rad_size_ok(int n_to) { return n_to == 2048 || n_to == 4096 || n_to == 8192; }
 
-static void rad_rhs_dispatch(int n_to, int n_r, const RadArgs* a, const double2* tw, int stage, hipStream_t s) {
-    switch (n_to) {
-        case 8192: rad_launch_rhs<4096>(a, tw, n_to, n_r, stage, s); break;
-        case 4096: rad_launch_rhs<2048>(a, tw, n_to, n_r, stage, s); break;
-        default: rad_launch_rhs<1024>(a, tw, n_to, n_r, stage, s); break;
+// One RHS (stage < 0: of rhs_in or the current state) or one step, under plan p.
+template <int MO, int MC, int PLAN, int MT>
+static void rad_run(bool step, const RadArgs* a, const double2* tw, int n_to, int n_r, int stage, hipStream_t s) {
+    if (step)
+        rad_launch_step<MO, MC, PLAN, MT>(a, tw, n_to, n_r, s);
+    else
+        rad_launch_rhs<MO, PLAN, MT>(a, tw, n_to, n_r, stage, s);
+}
+
+template <int MO, int MC, int PLAN>
+static void rad_run_mt(const RadPlan& p, bool step, const RadArgs* a, const double2* tw, int n_to, int n_r, int stage,
+                       hipStream_t s) {
+    switch (p.mt) {
+        case 1: rad_run<MO, MC, PLAN, 1>(step, a, tw, n_to, n_r, stage, s); break;
+        case 4: rad_run<MO, MC, PLAN, 4>(step, a, tw, n_to, n_r, stage, s); break;
+        default: rad_run<MO, MC, PLAN, 2>(step, a, tw, n_to, n_r, stage, s); break;
     }
 }
 
-static void rad_step_dispatch(int n_to, int n_r, const RadArgs* a, const double2* tw, hipStream_t s) {
+template <int MO, int MC>
+static void rad_run_plan(const RadPlan& p, bool step, const RadArgs* a, const double2* tw, int n_to, int n_r,
+                         int stage, hipStream_t s) {
+    if (p.fft == 1)
+        rad_run_mt<MO, MC, 1>(p, step, a, tw, n_to, n_r, stage, s);
+    else
+        rad_run_mt<MO, MC, 0>(p, step, a, tw, n_to, n_r, stage, s);
+}
+
+static void rad_dispatch(const RadPlan& p, bool step, int n_to, int n_r, const RadArgs* a, const double2* tw,
+                         int stage, hipStream_t s) {
     switch (n_to) {
-        case 8192: rad_launch_step<4096, 2048>(a, tw, n_to, n_r, s); break;
-        case 4096: rad_launch_step<2048, 1024>(a, tw, n_to, n_r, s); break;
-        default: rad_launch_step<1024, 512>(a, tw, n_to, n_r, s); break;
+        case 8192: rad_run_plan<4096, 2048>(p, step, a, tw, n_to, n_r, stage, s); break;
+        case 4096: rad_run_plan<2048, 1024>(p, step, a, tw, n_to, n_r, stage, s); break;
+        default: rad_run_plan<1024, 512>(p, step, a, tw, n_to, n_r, stage, s); break;
     }
+}
+
+static void rad_rhs_dispatch(const RadPlan& p, int n_to, int n_r, const RadArgs* a, const double2* tw, int stage,
+                             hipStream_t s) {
+    rad_dispatch(p, false, n_to, n_r, a, tw, stage, s);
+}
+
+static void rad_step_dispatch(const RadPlan& p, int n_to, int n_r, const RadArgs* a, const double2* tw,
+                              hipStream_t s) {
+    rad_dispatch(p, true, n_to, n_r, a, tw, -1, s);
 }

@@ -19,6 +19,9 @@ class Context:
         self.n_omega, self.n_r = d.n_omega, d.n_r
         self.n_state = d.n_omega * d.n_r
         L = _capi.lib()
+        radial = d.transform == _capi.LH_TRANS_RADIAL
+        if radial and settings["fftw_flag"] in _REPS:
+            load_wisdom()  # before lh_create, which takes a recorded plan for its shape
         h = C.c_void_p()
         _capi.check(L.lh_create(C.byref(h), holder.ref), "lh_create")
         self.handle = h
@@ -26,6 +29,37 @@ class Context:
         if grad is not None:  # z-dependent fill (pressure gradient)
             _capi.check(L.lh_set_gradient(h, grad.ref), "lh_set_gradient")
         self.statsfun = None
+        if radial:
+            plan_radial(self)
+
+    # -- plans of the radial RHS (lh_get_plan / lh_set_plan / lh_autotune) ---
+    @property
+    def plan(self):
+        """(fft_radix_plan, gemm_tiles) of a radial context."""
+        p = _capi.LhPlan()
+        _capi.check(_capi.lib().lh_get_plan(self.handle, C.byref(p)), "lh_get_plan")
+        return p.fft_radix_plan, p.gemm_tiles
+
+    @plan.setter
+    def plan(self, value):
+        p = _capi.LhPlan()
+        p.fft_radix_plan, p.gemm_tiles = value
+        _capi.check(_capi.lib().lh_set_plan(self.handle, C.byref(p)), "lh_set_plan")
+
+    def autotune(self, Ew=None, reps=10):
+        """Times every plan on one RHS of ``Ew`` (zero field if None) and
+        installs the fastest; returns [(fft_radix_plan, gemm_tiles, µs per RHS,
+        agrees), ...] with the default plan first."""
+        ptr = None
+        if Ew is not None:
+            Ew = np.ascontiguousarray(np.asarray(Ew, dtype=np.complex128).reshape(-1, order="F"))
+            if Ew.size != self.n_state:
+                raise ValueError("Eω has the wrong size")
+            ptr = _p(Ew)
+        tried = (_capi.LhPlan * 8)()
+        n = C.c_int(0)
+        _capi.check(_capi.lib().lh_autotune(self.handle, ptr, int(reps), tried, 8, C.byref(n)), "lh_autotune")
+        return [(t.fft_radix_plan, t.gemm_tiles, t.rhs_us, bool(t.agrees)) for t in tried[:n.value]]
 
     def __del__(self):
         h = getattr(self, "handle", None)
@@ -121,6 +155,105 @@ class Context:
         n = C.c_int(0)
         _capi.check(_capi.lib().lh_get_step_stats(self.handle, _p(rec), cap, C.byref(n)), "lh_get_step_stats")
         return rec[:n.value].copy()
+
+
+# ---------------------------------------------------------------- planning
+# ``Luna.settings`` and ``set_fftw_mode`` (src/Luna.jl:13-36): the planning
+# mode of the transforms.  "estimate" takes the recorded wisdom or the
+# default plan without measuring; "measure", "patient" and "exhaustive" time
+# the radial plans (3, 10 and 30 launches each) for a shape with no wisdom.
+# The default here is "estimate" (the reference's is FFTW.PATIENT): a
+# measured choice can change with the machine's load, and with it the last
+# bits of a result.
+settings = {"fftw_flag": "estimate", "fftw_threads": 0}
+_REPS = {"measure": 3, "patient": 10, "exhaustive": 30}
+
+
+def set_fftw_mode(mode):
+    m = str(mode).lstrip(":").lower()
+    if m not in ("estimate", "measure", "patient", "exhaustive"):
+        raise ValueError(f"unknown planning mode {mode}")
+    settings["fftw_flag"] = m
+    return m
+
+
+def plan_radial(ctx):
+    """Plan a new radial context as ``Luna.setup`` plans its FFTs
+    (src/Luna.jl:239-258): a shape in the wisdom (loaded before the context
+    was created) keeps its plan; another is measured if the mode asks, and
+    the wisdom file rewritten."""
+    reps = _REPS.get(settings["fftw_flag"])
+    if reps is None:
+        return
+    known = f"\nradial {ctx.holder.desc.n_to} {ctx.n_r} " in wisdom_export()
+    if not known:
+        ctx.autotune(None, reps)
+        save_wisdom()
+
+
+def wisdom_export():
+    """The wisdom as text (``lh_wisdom_export``)."""
+    L = _capi.lib()
+    n = C.c_size_t(0)
+    _capi.check(L.lh_wisdom_export(None, 0, C.byref(n)), "lh_wisdom_export")
+    buf = C.create_string_buffer(n.value)
+    _capi.check(L.lh_wisdom_export(buf, n.value, C.byref(n)), "lh_wisdom_export")
+    return buf.value.decode()
+
+
+def wisdom_import(text):
+    _capi.check(_capi.lib().lh_wisdom_import(text.encode()), "lh_wisdom_import")
+
+
+def wisdom_forget():
+    _capi.check(_capi.lib().lh_wisdom_forget(), "lh_wisdom_forget")
+
+
+def _wisdom_path():
+    import os
+    d = os.environ.get("LUNA_CACHE_DIR") or os.path.join(os.path.expanduser("~"), ".cache", "luna_jl_amd")
+    return d, os.path.join(d, "plan_wisdom")
+
+
+def load_wisdom():
+    """``Utils.loadFFTwisdom`` (src/Utils.jl:84-97): merge the cached wisdom
+    file, under a file lock, if there is one."""
+    import fcntl
+    import os
+    d, path = _wisdom_path()
+    if not os.path.isfile(path):
+        return False
+    with open(path + ".lock", "a") as lk:
+        fcntl.flock(lk, fcntl.LOCK_SH)
+        try:
+            with open(path) as f:
+                text = f.read()
+        finally:
+            fcntl.flock(lk, fcntl.LOCK_UN)
+    try:
+        wisdom_import(text)
+    except _capi.LunaHipError:
+        return False  # a malformed file is ignored, as FFTW ignores unusable wisdom
+    return True
+
+
+def save_wisdom():
+    """``Utils.saveFFTwisdom`` (src/Utils.jl:99-108)."""
+    import fcntl
+    import os
+    d, path = _wisdom_path()
+    os.makedirs(d, exist_ok=True)
+    text = wisdom_export()
+    with open(path + ".lock", "a") as lk:
+        fcntl.flock(lk, fcntl.LOCK_EX)
+        try:
+            tmp = f"{path}.tmp{os.getpid()}"
+            with open(tmp, "w") as f:
+                f.write(text)
+            os.replace(tmp, path)
+        finally:
+            fcntl.flock(lk, fcntl.LOCK_UN)
+    return path
 
 
 class HostBuffers:

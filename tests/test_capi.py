@@ -78,6 +78,62 @@ def test_struct_sizes_match_header(capi):
                      ctypes.sizeof(capi.LhStatsDesc), capi.LhStatsDesc.aeff.offset, capi.LH_NSTAT]
 
 
+def test_plan_struct_matches_header(capi):
+    import subprocess
+    import tempfile
+    src = '#include <stdio.h>\n#include <stddef.h>\n#include "luna_hip.h"\nint main(){printf("%zu %zu %zu",' \
+          'sizeof(lh_plan), offsetof(lh_plan, rhs_us), offsetof(lh_plan, agrees));}'
+    with tempfile.TemporaryDirectory() as td:
+        c = os.path.join(td, "s.c")
+        open(c, "w").write(src)
+        exe = os.path.join(td, "s")
+        subprocess.check_call(["gcc", "-I", os.path.join(ROOT, "include"), "-o", exe, c])
+        out = [int(x) for x in subprocess.check_output([exe]).decode().split()]
+    assert out == [ctypes.sizeof(capi.LhPlan), capi.LhPlan.rhs_us.offset, capi.LhPlan.agrees.offset]
+
+
+def test_wisdom_text_round_trip(capi, pkg):
+    """The wisdom is host state: import, export and forget need no GPU."""
+    from luna_jl_amd import runtime
+    L = capi.lib()
+    runtime.wisdom_forget()
+    assert runtime.wisdom_export() == "luna-hip-wisdom 1\n"
+    good = "luna-hip-wisdom 1\nradial 8192 256 1 4 123.500\nradial 2048 64 0 1 7.250\n"
+    runtime.wisdom_import(good)
+    assert runtime.wisdom_export() == good
+    # a later import of the same shape replaces its entry
+    runtime.wisdom_import("luna-hip-wisdom 1\nradial 2048 64 1 2 6.000\n")
+    assert runtime.wisdom_export() == "luna-hip-wisdom 1\nradial 8192 256 1 4 123.500\nradial 2048 64 1 2 6.000\n"
+    # any malformed line refuses the whole text, merging nothing
+    for bad in ["", "luna-hip-wisdom 2\n", "luna-hip-wisdom 1\nradial 8192 256 2 4 1.0\n",
+                "luna-hip-wisdom 1\nradial 8192 256 0 3 1.0\n", "luna-hip-wisdom 1\nradial 1000 256 0 2 1.0\n",
+                "luna-hip-wisdom 1\nradial 8192 100 0 2 1.0\n", "luna-hip-wisdom 1\nmodeavg 8192 1 0 2 1.0\n",
+                "luna-hip-wisdom 1\nradial 4096 64 0 2 1.0\nradial 8192 256 0 2 1.0 extra\n"]:
+        assert L.lh_wisdom_import(bad.encode()) == capi.LH_E_ARG, bad
+    assert "radial 4096 64" not in runtime.wisdom_export()
+    assert L.lh_wisdom_import(None) == capi.LH_E_ARG
+    n = ctypes.c_size_t(0)
+    assert L.lh_wisdom_export(None, 0, ctypes.byref(n)) == capi.LH_OK and n.value == len(runtime.wisdom_export()) + 1
+    small = ctypes.create_string_buffer(4)
+    assert L.lh_wisdom_export(small, 4, ctypes.byref(n)) == capi.LH_E_ARG
+    runtime.wisdom_forget()
+    assert runtime.wisdom_export() == "luna-hip-wisdom 1\n"
+    assert L.lh_get_plan(None, None) == capi.LH_E_ARG and L.lh_set_plan(None, None) == capi.LH_E_ARG
+    assert L.lh_autotune(None, None, 1, None, 0, None) == capi.LH_E_ARG
+
+
+def test_planning_mode_mirror(pkg):
+    from luna_jl_amd import runtime
+    old = runtime.settings["fftw_flag"]
+    try:
+        assert runtime.set_fftw_mode(":patient") == "patient" and runtime.settings["fftw_flag"] == "patient"
+        assert runtime.set_fftw_mode("ESTIMATE") == "estimate"
+        with pytest.raises(ValueError):
+            runtime.set_fftw_mode(":wild")
+    finally:
+        runtime.settings["fftw_flag"] = old
+
+
 def test_stat_keys_follow_header_order(capi):
     import re
     text = open(os.path.join(ROOT, "include", "luna_hip.h")).read()
