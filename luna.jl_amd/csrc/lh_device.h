@@ -329,6 +329,84 @@ __device__ __forceinline__ double2 r2c_bin(const double2* buf, const TwLds& tw, 
     return cadd(E, cmul(w, O));
 }
 
+// ------------------------------------------------------- lane movement (DPP)
+// Data-parallel-primitive moves run on the vector ALU (a move with a lane
+// pattern, no LDS round trip as a lane shuffle takes).  A double moves as its
+// two 32-bit halves; a lane whose source lies outside its row (or whose row
+// ROW_MASK leaves out) receives `fill`'s bits, 0.0 by default.
+enum : int {
+    DPP_ROW_SHR1 = 0x111,  // lane l ← lane l−1 of the same row of 16
+    DPP_ROW_SHR2 = 0x112,
+    DPP_ROW_SHR4 = 0x114,
+    DPP_ROW_SHR8 = 0x118,
+    DPP_WAVE_SHR1 = 0x138,  // lane l ← lane l−1 of the wavefront
+    DPP_ROW_BCAST15 = 0x142,  // row r+1 ← lane 15 of row r
+    DPP_ROW_BCAST31 = 0x143,  // rows 2, 3 ← lane 31
+};
+
+template <int CTRL, int ROW_MASK = 0xf>
+__device__ __forceinline__ double dpp_f64(double x, double fill = 0.0) {
+    const long long b = __double_as_longlong(x), f = __double_as_longlong(fill);
+    const int lo = __builtin_amdgcn_update_dpp((int)f, (int)b, CTRL, ROW_MASK, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp((int)(f >> 32), (int)(b >> 32), CTRL, ROW_MASK, 0xf, false);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+template <int CTRL, int ROW_MASK = 0xf>
+__device__ __forceinline__ int dpp_i32(int x, int fill) {
+    return __builtin_amdgcn_update_dpp(fill, x, CTRL, ROW_MASK, 0xf, false);
+}
+
+// Value of x in lane `lane` (wave-uniform) for every lane.
+__device__ __forceinline__ double lane_value(double x, int lane) {
+    const long long b = __double_as_longlong(x);
+    const int l = __builtin_amdgcn_readfirstlane(lane);
+    const int lo = __builtin_amdgcn_readlane((int)b, l);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// Inclusive prefix sum over the 64 lanes: within rows of 16 by row shifts,
+// then across rows by the two row broadcasts.
+__device__ __forceinline__ double wave_incl_scan(double x) {
+    x += dpp_f64<DPP_ROW_SHR1>(x);
+    x += dpp_f64<DPP_ROW_SHR2>(x);
+    x += dpp_f64<DPP_ROW_SHR4>(x);
+    x += dpp_f64<DPP_ROW_SHR8>(x);
+    x += dpp_f64<DPP_ROW_BCAST15, 0xa>(x);
+    x += dpp_f64<DPP_ROW_BCAST31, 0xc>(x);
+    return x;
+}
+
+// Inclusive prefix sum over lanes 0..7 (row 0; the cross-wave step of the
+// workgroup scans, one value per wave).
+__device__ __forceinline__ double lanes8_incl_scan(double x) {
+    static_assert(LH_NW <= 8, "one value per wave in row 0");
+    x += dpp_f64<DPP_ROW_SHR1>(x);
+    x += dpp_f64<DPP_ROW_SHR2>(x);
+    x += dpp_f64<DPP_ROW_SHR4>(x);
+    return x;
+}
+
+// Minimum of lo and maximum of hi over the 64 lanes, in every lane.
+__device__ __forceinline__ void wave_minmax(int& lo, int& hi) {
+    constexpr int BIG = 0x7fffffff, SMALL = -0x7fffffff - 1;
+    lo = min(lo, dpp_i32<DPP_ROW_SHR1>(lo, BIG));
+    hi = max(hi, dpp_i32<DPP_ROW_SHR1>(hi, SMALL));
+    lo = min(lo, dpp_i32<DPP_ROW_SHR2>(lo, BIG));
+    hi = max(hi, dpp_i32<DPP_ROW_SHR2>(hi, SMALL));
+    lo = min(lo, dpp_i32<DPP_ROW_SHR4>(lo, BIG));
+    hi = max(hi, dpp_i32<DPP_ROW_SHR4>(hi, SMALL));
+    lo = min(lo, dpp_i32<DPP_ROW_SHR8>(lo, BIG));
+    hi = max(hi, dpp_i32<DPP_ROW_SHR8>(hi, SMALL));
+    lo = min(lo, dpp_i32<DPP_ROW_BCAST15, 0xa>(lo, BIG));
+    hi = max(hi, dpp_i32<DPP_ROW_BCAST15, 0xa>(hi, SMALL));
+    lo = min(lo, dpp_i32<DPP_ROW_BCAST31, 0xc>(lo, BIG));
+    hi = max(hi, dpp_i32<DPP_ROW_BCAST31, 0xc>(hi, SMALL));
+    lo = __builtin_amdgcn_readlane(lo, 63);
+    hi = __builtin_amdgcn_readlane(hi, 63);
+}
+
 // ------------------------------------------------------- block reductions
 // Sum of four values over the workgroup (fixed order → deterministic);
 // scratch: 4·LH_NW doubles.  Ends synchronised.

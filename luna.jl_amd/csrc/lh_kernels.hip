@@ -143,7 +143,7 @@ __constant__ double c_iC[4][7] = {
 // other buffers leave 11 KB (1408 samples) for it.
 template <int MO>
 struct PList {
-    static constexpr int CAP = (MO >= 8192) ? 1408 : 4096;
+    static constexpr int CAP = (MO >= 8192) ? 1344 : 4096;
 };
 
 template <int MO>
@@ -151,7 +151,7 @@ struct SmemT {
     double2 buf[padded_len(MO)];
     TwLds tw;
     double wsum[5 * LH_NW];      // block_sum3/block_sum4 and the statistics reductions
-    double scan[2][3 * LH_NW];   // one-barrier scans and min/max, used in alternation
+    double scan[2][7 * LH_NW];   // one-barrier scans and min/max, used in alternation
     double plist[PList<MO>::CAP];
     double zs[3];  // gradient fill: ρ(z), β1(z), taper radius a(z) from thread 0
     int flag;
@@ -243,19 +243,9 @@ __device__ __forceinline__ void ppt_rates(const DInst& in, const double* aE, dou
     }
 }
 
-// Inclusive prefix sum of x over the 64 lanes of a wavefront.
-__device__ __forceinline__ double wave_incl_scan(double x, int lane) {
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const double y = __shfl_up(x, off, 64);
-        if (lane >= off) x += y;
-    }
-    return x;
-}
-
 // cumtrapz over the workgroup for one round of QR samples per thread (global
 // index g0+q), with a single barrier.  The trapezoid joining a thread's first
-// sample to its predecessor's last comes from a lane shuffle inside a
+// sample to its predecessor's last comes from a lane move inside a
 // wavefront; for lane 0 of waves ≥ 1 it is added after the barrier from the
 // previous wave's last sample in LDS.  S: a scan slot of 3·LH_NW doubles (wave
 // totals, first and last samples of each wave); consecutive calls alternate
@@ -270,7 +260,7 @@ __device__ __forceinline__ double block_cumtrapz(double* y, double dx, int g0, d
     const int tid = fresh_tid();
     const int lane = tid & 63, wid = tid >> 6;
     const double first = y[0], lastv = y[QR - 1];
-    double prev = __shfl_up(lastv, 1, 64);
+    double prev = dpp_f64<DPP_WAVE_SHR1>(lastv);
     if (tid == 0) prev = prev0;
     const bool joined = (g0 != 0) && !(lane == 0 && wid > 0);
     double acc = 0.0;
@@ -283,9 +273,8 @@ __device__ __forceinline__ double block_cumtrapz(double* y, double dx, int g0, d
         acc += term;
         y[q] = acc;
     }
-    const double x = wave_incl_scan(acc, lane);
-    double ex = __shfl_up(x, 1, 64);
-    if (lane == 0) ex = 0.0;
+    const double x = wave_incl_scan(acc);
+    const double ex = dpp_f64<DPP_WAVE_SHR1>(x);  // exclusive (0 in lane 0)
     if (lane == 63) {
         S[wid] = x;
         S[2 * LH_NW + wid] = lastv;
@@ -298,16 +287,122 @@ __device__ __forceinline__ double block_cumtrapz(double* y, double dx, int g0, d
         Ww = S[lane];
         c = (lane > 0) ? (0.5 * (S[2 * LH_NW + lane - 1] + S[LH_NW + lane])) * dx + Ww : Ww;
     }
-#pragma unroll
-    for (int o = 1; o < LH_NW; o <<= 1) {
-        const double t = __shfl_up(c, o, 64);
-        if (lane >= o) c += t;
-    }
-    const double off = (carry + (__shfl(c, wid, 64) - __shfl(Ww, wid, 64))) + ex;
+    c = lanes8_incl_scan(c);
+    const double off = (carry + (lane_value(c, wid) - lane_value(Ww, wid))) + ex;
 #pragma unroll
     for (int q = 0; q < QR; q++) y[q] = off + y[q];
-    carry = carry + __shfl(c, LH_NW - 1, 64);
+    carry = carry + lane_value(c, LH_NW - 1);
     return S[3 * LH_NW - 1];
+}
+
+// Two chained cumtrapz over the workgroup with a single barrier: J = ∫a and
+// P = ∫(J + l) for one round of QR samples per thread (global index g0+q).
+// A run of samples acts on the integrals at the sample before it as
+// J ← J + SJ, P ← P + n·dx·J + SP (n: its trapezoids; SJ, SP: its integrals
+// from J = P = 0), and runs compose as
+//   (n1, SJ1, SP1)·(n2, SJ2, SP2) = (n1 + n2, SJ1 + SJ2, SP1 + SP2 + n2·dx·SJ1),
+// which the wave scan (DPP) and the cross-wave step apply in that order.  The
+// trapezoid joining lane 0 of wave w ≥ 1 to wave w−1's last sample is a run
+// of its own, formed after the barrier from the first and last samples in S
+// (7·LH_NW doubles: wave totals n, SJ, SP; first a, l; last a, l).
+// cJ, cP: J and P at the sample before the round; pa0, pl0: a and l there
+// (thread 0).  On return a[q] = P at the sample, cJ, cP, pa0, pl0 those of the
+// round's last sample.
+struct Run2 {
+    int n;
+    double sj, sp;
+};
+__device__ __forceinline__ Run2 run2_cat(const Run2& x, const Run2& y, double dx) {
+    return Run2{x.n + y.n, x.sj + y.sj, (x.sp + y.sp) + ((double)y.n * dx) * x.sj};
+}
+
+template <int CTRL, int ROW_MASK = 0xf>
+__device__ __forceinline__ Run2 dpp_run2(const Run2& x) {
+    return Run2{dpp_i32<CTRL, ROW_MASK>(x.n, 0), dpp_f64<CTRL, ROW_MASK>(x.sj), dpp_f64<CTRL, ROW_MASK>(x.sp)};
+}
+
+template <int QR>
+__device__ __forceinline__ void block_cumtrapz2(double* a, const double* l, double dx, int g0, double* S, double& cJ,
+                                                double& cP, double& pa0, double& pl0) {
+    const int tid = fresh_tid();
+    const int lane = tid & 63, wid = tid >> 6;
+    const double afirst = a[0], lfirst = l[0], alast = a[QR - 1], llast = l[QR - 1];
+    double aprev = dpp_f64<DPP_WAVE_SHR1>(alast), pprev = dpp_f64<DPP_WAVE_SHR1>(llast);
+    if (tid == 0) {
+        aprev = pa0;
+        pprev = pl0;
+    }
+    const bool joined = (g0 != 0) && !(lane == 0 && wid > 0);
+    // this thread's run; a[q] := its SP up to sample q
+    double sj = 0.0, sp = 0.0;
+#pragma unroll
+    for (int q = 0; q < QR; q++) {
+        const double cur = a[q];
+        if (q > 0 || joined) {
+            sj += (0.5 * (aprev + cur)) * dx;
+            const double pc = sj + l[q];
+            sp += (0.5 * (pprev + pc)) * dx;
+            pprev = pc;
+        } else {
+            pprev = l[q];
+        }
+        aprev = cur;
+        a[q] = sp;
+    }
+    const int nt = QR - (joined ? 0 : 1);
+    // inclusive scan of the runs over the wave (DPP), then exclusive by one lane
+    Run2 x{nt, sj, sp};
+    x = run2_cat(dpp_run2<DPP_ROW_SHR1>(x), x, dx);
+    x = run2_cat(dpp_run2<DPP_ROW_SHR2>(x), x, dx);
+    x = run2_cat(dpp_run2<DPP_ROW_SHR4>(x), x, dx);
+    x = run2_cat(dpp_run2<DPP_ROW_SHR8>(x), x, dx);
+    x = run2_cat(dpp_run2<DPP_ROW_BCAST15, 0xa>(x), x, dx);
+    x = run2_cat(dpp_run2<DPP_ROW_BCAST31, 0xc>(x), x, dx);
+    const Run2 ex = dpp_run2<DPP_WAVE_SHR1>(x);
+    if (lane == 63) {
+        S[wid] = (double)x.n;
+        S[LH_NW + wid] = x.sj;
+        S[2 * LH_NW + wid] = x.sp;
+        S[5 * LH_NW + wid] = alast;
+        S[6 * LH_NW + wid] = llast;
+    }
+    if (lane == 0) {
+        S[3 * LH_NW + wid] = afirst;
+        S[4 * LH_NW + wid] = lfirst;
+    }
+    __syncthreads();
+    // lane w < LH_NW: the join into wave w (w ≥ 1), then wave w's total
+    Run2 e{0, 0.0, 0.0}, jn{0, 0.0, 0.0};
+    if (lane < LH_NW) {
+        if (lane > 0) {
+            const double ta = (0.5 * (S[5 * LH_NW + lane - 1] + S[3 * LH_NW + lane])) * dx;
+            const double sp1 = (0.5 * (S[6 * LH_NW + lane - 1] + (ta + S[4 * LH_NW + lane]))) * dx;
+            jn = Run2{1, ta, sp1};
+        }
+        e = run2_cat(jn, Run2{(int)S[lane], S[LH_NW + lane], S[2 * LH_NW + lane]}, dx);
+    }
+    Run2 c = e;  // inclusive over lanes 0..LH_NW-1
+    c = run2_cat(dpp_run2<DPP_ROW_SHR1>(c), c, dx);
+    c = run2_cat(dpp_run2<DPP_ROW_SHR2>(c), c, dx);
+    c = run2_cat(dpp_run2<DPP_ROW_SHR4>(c), c, dx);
+    const Run2 win = run2_cat(dpp_run2<DPP_ROW_SHR1>(c), jn, dx);  // runs before wave w's first lane
+    // runs before this thread: the wave's, then the lanes' before it in the wave
+    const Run2 w_in{__builtin_amdgcn_readlane(win.n, __builtin_amdgcn_readfirstlane(wid)), lane_value(win.sj, wid),
+                    lane_value(win.sp, wid)};
+    const Run2 t_in = run2_cat(w_in, ex, dx);
+    const double J = cJ + t_in.sj;
+    const double P = (cP + ((double)t_in.n * dx) * cJ) + t_in.sp;
+#pragma unroll
+    for (int q = 0; q < QR; q++) {
+        const int nq = q + (joined ? 1 : 0);
+        a[q] = (P + ((double)nq * dx) * J) + a[q];
+    }
+    const Run2 all{__builtin_amdgcn_readlane(c.n, LH_NW - 1), lane_value(c.sj, LH_NW - 1),
+                   lane_value(c.sp, LH_NW - 1)};
+    cP = (cP + ((double)all.n * dx) * cJ) + all.sp;
+    cJ = cJ + all.sj;
+    pa0 = S[6 * LH_NW - 1];
+    pl0 = S[7 * LH_NW - 1];
 }
 
 // Exclusive prefix sum of one value per thread with a single barrier (scan
@@ -315,30 +410,20 @@ __device__ __forceinline__ double block_cumtrapz(double* y, double dx, int g0, d
 __device__ __forceinline__ double block_exscan1(double v, double* S, double* total) {
     const int tid = fresh_tid();
     const int lane = tid & 63, wid = tid >> 6;
-    const double x = wave_incl_scan(v, lane);
-    double ex = __shfl_up(x, 1, 64);
-    if (lane == 0) ex = 0.0;
+    const double x = wave_incl_scan(v);
+    const double ex = dpp_f64<DPP_WAVE_SHR1>(x);
     if (lane == 63) S[wid] = x;
     __syncthreads();
     const double Ww = (lane < LH_NW) ? S[lane] : 0.0;
-    double c = Ww;
-#pragma unroll
-    for (int o = 1; o < LH_NW; o <<= 1) {
-        const double t = __shfl_up(c, o, 64);
-        if (lane >= o) c += t;
-    }
-    *total = __shfl(c, LH_NW - 1, 64);
-    return (__shfl(c, wid, 64) - __shfl(Ww, wid, 64)) + ex;
+    const double c = lanes8_incl_scan(Ww);
+    *total = lane_value(c, LH_NW - 1);
+    return (lane_value(c, wid) - lane_value(Ww, wid)) + ex;
 }
 
 // Workgroup min of lo and max of hi with a single barrier (scan slot S).
 __device__ __forceinline__ void block_minmax(int& lo, int& hi, double* S) {
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        lo = min(lo, __shfl_xor(lo, off, 64));
-        hi = max(hi, __shfl_xor(hi, off, 64));
-    }
+    wave_minmax(lo, hi);
     if (lane == 0) {
         S[2 * wid] = (double)lo;
         S[2 * wid + 1] = (double)hi;
@@ -374,7 +459,7 @@ __device__ __forceinline__ void pointwise_field(const DInst& in, Smem& sm, doubl
     const double* tw = in.to_win;
     const double dt = in.dt_o, pp1 = in.preion_p1, er = in.e_ratio, ip = in.ionpot;
     double cF = 0.0, cJ = 0.0, cP = 0.0;           // carried integrals
-    double lastRate = 0.0, lastPhi = 0.0, lastJ = 0.0;
+    double lastRate = 0.0, lastPhi = 0.0, lastLoss = 0.0;
     double Fprev = 0.0, fprev = pp1 - 1.0;         // exp(-0) = 1
     int sl = 0;                                    // scan slot of the next workgroup scan
 #pragma unroll 1
@@ -477,11 +562,8 @@ __device__ __forceinline__ void pointwise_field(const DInst& in, Smem& sm, doubl
                 a[q] = (f * er) * E;                                                     // phase
                 jf[q] = (rate != 0.0 && E != 0.0) ? ((ip * rate) * (1 - f)) / E : 0.0;  // loss current
             }
-            lastPhi = block_cumtrapz<QR>(a, dt, g0, sm.scan[sl], cJ, lastPhi);  // a = J without loss term
-            sl ^= 1;
-#pragma unroll
-            for (int q = 0; q < QR; q++) a[q] = a[q] + jf[q];
-            lastJ = block_cumtrapz<QR>(a, dt, g0, sm.scan[sl], cP, lastJ);  // a = P
+            // a = P = ∫(J + loss), J = ∫(phase term)
+            block_cumtrapz2<QR>(a, jf, dt, g0, sm.scan[sl], cJ, cP, lastPhi, lastLoss);
             sl ^= 1;
         }
 #pragma unroll
@@ -509,6 +591,7 @@ __device__ __forceinline__ void pointwise_field(const DInst& in, Smem& sm, doubl
 // k = threadIdx.x + (i0 + i)·LH_T for a chunk of at most 9 of this thread's
 // bins (bins k ≥ n hold zero and are not written).
 struct EpiStore {  // out[k] = v
+    static constexpr bool FOLDS_OUTFAC = false;
     double2* out;
     template <int NP>
     __device__ __forceinline__ void operator()(const double2 (&v)[NP], int n, int i0) const {
@@ -519,36 +602,18 @@ struct EpiStore {  // out[k] = v
         }
     }
 };
-struct EpiStage {  // out[k] = v·exp(L_k·hb), the stage output of fbar
-    double2* out;
-    const double2* L;
-    double hb;
-    template <int NP>
-    __device__ __forceinline__ void operator()(const double2 (&v)[NP], int n, int i0) const {
-        static_assert(NP <= 9, "epilogue chunk");
-        // the chunk's operator loads are issued before its first store (out may
-        // alias L as far as the compiler knows, which would otherwise serialise them)
-        double2 Lk[NP];
-#pragma unroll
-        for (int i = 0; i < NP; i++) {
-            const int k = threadIdx.x + (i0 + i) * LH_T;
-            Lk[i] = (k < n) ? L[k] : make_double2(0.0, 0.0);
-        }
-#pragma unroll
-        for (int i = 0; i < NP; i++) {
-            const int k = threadIdx.x + (i0 + i) * LH_T;
-            if (k < n) out[k] = cmul(v[i], cexp_lh(Lk[i], hb));
-        }
-    }
-};
-
-struct EpiStageF {  // out[k] = v·F[k], F = exp(L·hb) stored by the envelope stage prologue
+// out[k] = v·F[k], F stored by the stage prologue: on the envelope grids
+// F = exp(L·(t − t2)) and v carries the output factor (FOLD false); on the
+// field grid F = outfac·exp(L·(t − t2)) and v is the bare r2c bin (FOLD true).
+template <bool FOLD>
+struct EpiStageFactor {
+    static constexpr bool FOLDS_OUTFAC = FOLD;
     double2* out;
     const double2* F;
     template <int NP>
     __device__ __forceinline__ void operator()(const double2 (&v)[NP], int n, int i0) const {
         static_assert(NP <= 9, "epilogue chunk");
-        double2 f[NP];  // loads before stores, as in EpiStage
+        double2 f[NP];  // the chunk's loads before its first store (out may alias F as far as the compiler knows)
 #pragma unroll
         for (int i = 0; i < NP; i++) {
             const int k = threadIdx.x + (i0 + i) * LH_T;
@@ -561,6 +626,8 @@ struct EpiStageF {  // out[k] = v·F[k], F = exp(L·hb) stored by the envelope s
         }
     }
 };
+using EpiStageF = EpiStageFactor<false>;
+using EpiStageFO = EpiStageFactor<true>;
 
 // Field-grid mode-averaged transform on LDS data.  On entry buf[0..N) holds
 // sc_t·Eω (zero to MO), or (PACKED) its c2r pre-pack already.  Hands the
@@ -576,20 +643,30 @@ __device__ __forceinline__ void rhs_field(const DInst& in, Smem& sm, Epi epi, do
     if constexpr (RAMAN) raman_field<MO, Smem>(in, sm);
     pointwise_field<MO, Smem, RAMAN>(in, sm, rho, kfac);
     constexpr int N = Bins<MO>::N, NPT = Bins<MO>::NPT;
-    double2 of[NPT];  // loads in flight during the forward FFT
-#pragma unroll
-    for (int i = 0; i < NPT; i++) {
-        int k = threadIdx.x + i * LH_T;
-        of[i] = (k < N) ? in.outfac[k] : make_double2(0.0, 0.0);
-    }
-    __syncthreads();
-    fft_lds<MO, false>(sm.buf, sm.tw);
     static_assert(NPT <= 9, "one epilogue chunk");
     double2 res[NPT];
+    if constexpr (Epi::FOLDS_OUTFAC) {
+        __syncthreads();
+        fft_lds<MO, false>(sm.buf, sm.tw);
 #pragma unroll
-    for (int i = 0; i < NPT; i++) {
-        int k = threadIdx.x + i * LH_T;
-        res[i] = (k < N) ? cmul(r2c_bin<MO>(sm.buf, sm.tw, k), of[i]) : make_double2(0.0, 0.0);
+        for (int i = 0; i < NPT; i++) {
+            int k = threadIdx.x + i * LH_T;
+            res[i] = (k < N) ? r2c_bin<MO>(sm.buf, sm.tw, k) : make_double2(0.0, 0.0);
+        }
+    } else {
+        double2 of[NPT];  // loads in flight during the forward FFT
+#pragma unroll
+        for (int i = 0; i < NPT; i++) {
+            int k = threadIdx.x + i * LH_T;
+            of[i] = (k < N) ? in.outfac[k] : make_double2(0.0, 0.0);
+        }
+        __syncthreads();
+        fft_lds<MO, false>(sm.buf, sm.tw);
+#pragma unroll
+        for (int i = 0; i < NPT; i++) {
+            int k = threadIdx.x + i * LH_T;
+            res[i] = (k < N) ? cmul(r2c_bin<MO>(sm.buf, sm.tw, k), of[i]) : make_double2(0.0, 0.0);
+        }
     }
     epi(res, N, 0);
 }
@@ -618,22 +695,26 @@ struct KSet {
 // propagated by h0 (evaluate!, src/RK45.jl:209-223).  Loops run over the
 // k-vectors outside and this thread's bins inside, so each vector's loads
 // for all bins are in flight together.
-template <int MO, class Smem, class KS, bool PACK>
+// FOLD (field grid): also stores the stage epilogue's factor
+// outfac·exp(L·(t − t2)) = outfac·exp(−L·hf) into slot 7 of K, from the same
+// exp and sincos as exp(L·hf) (the epilogue then needs neither a
+// transcendental pass nor the output factor held across the forward FFT).
+template <int MO, class Smem, class KS, bool PACK, bool FOLD = false>
 __device__ __forceinline__ void stage_prologue_cols(const double2* __restrict__ linop, double sc_t, Smem& sm,
                                                     const double2* Y, const KS& K, int stage, double dt,
-                                                    double h0, double hf);
+                                                    double h0, double hf, const double2* outfac = nullptr);
 
 template <int MO, class Smem>
 __device__ __forceinline__ void stage_prologue(const DInst& in, Smem& sm, const double2* Y, const KSet& K,
                                                int stage, double dt, double h0, double hf) {
-    stage_prologue_cols<MO, Smem, KSet, true>(in.linop, in.sc_t, sm, Y, K, stage, dt, h0, hf);
+    stage_prologue_cols<MO, Smem, KSet, true, true>(in.linop, in.sc_t, sm, Y, K, stage, dt, h0, hf, in.outfac);
 }
 
 // Same on explicit per-column pointers (one column of a radial state).
-template <int MO, class Smem, class KS, bool PACK>
+template <int MO, class Smem, class KS, bool PACK, bool FOLD>
 __device__ __forceinline__ void stage_prologue_cols(const double2* __restrict__ linop, double sc_t, Smem& sm,
                                                     const double2* Y, const KS& K, int stage, double dt,
-                                                    double h0, double hf) {
+                                                    double h0, double hf, const double2* outfac) {
     constexpr int N = Bins<MO>::N, NPT = Bins<MO>::NPT;
     const int tid = fresh_tid();
     double2 yb[NPT], L[NPT];
@@ -684,15 +765,30 @@ __device__ __forceinline__ void stage_prologue_cols(const double2* __restrict__ 
             if (c1 != 0.0) yb[i] = cadd(yb[i], cscal(c1, b[i]));
         }
     }
+    if constexpr (FOLD) {
+        double2* F = K[7];
 #pragma unroll
-    for (int i = 0; i < NPT; i++) {
-        int k = tid + i * LH_T;
-        if (k < N) {
-            double2 yp = (hf != 0.0) ? cmul(yb[i], cexp_lh(L[i], hf)) : yb[i];
-            if constexpr (PACK)  // the c2r pre-pack of the RHS, bin by bin (N − 1 = MO/2)
+        for (int i = 0; i < NPT; i++) {
+            int k = tid + i * LH_T;
+            if (k < N) {
+                const double2 of = outfac[k];
+                double2 yp = yb[i], inv = make_double2(1.0, 0.0);
+                if (hf != 0.0) yp = cmul(yb[i], cexp_lh_inv(L[i], hf, inv));
                 c2r_pack_bin<MO>(sm.buf, sm.tw, k, cscal(sc_t, yp));
-            else
-                sm.buf[pad(k)] = cscal(sc_t, yp);
+                F[k] = cmul(of, inv);
+            }
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < NPT; i++) {
+            int k = tid + i * LH_T;
+            if (k < N) {
+                double2 yp = (hf != 0.0) ? cmul(yb[i], cexp_lh(L[i], hf)) : yb[i];
+                if constexpr (PACK)  // the c2r pre-pack of the RHS, bin by bin (N − 1 = MO/2)
+                    c2r_pack_bin<MO>(sm.buf, sm.tw, k, cscal(sc_t, yp));
+                else
+                    sm.buf[pad(k)] = cscal(sc_t, yp);
+            }
         }
     }
     if constexpr (!PACK)
@@ -1434,9 +1530,17 @@ __device__ __forceinline__ double zg_radius(const DInst& in, double z) {
         const int n = in.zt_n;
         if (z <= Z[0]) return A[0];
         if (z >= Z[n - 1]) return A[n - 1];
-        int i = 0;
-        for (int j = 0; j < n - 1; j++)
-            if (Z[j] < z) i = j;  // the segment [Z[i], Z[i+1]] holding z
+        // the segment [Z[i], Z[i+1]] with Z[i] < z <= Z[i+1] (nodes strictly
+        // increasing), by bisection: Z[lo] < z <= Z[hi] throughout
+        int lo = 0, hi = n - 1;
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (Z[mid] < z)
+                lo = mid;
+            else
+                hi = mid;
+        }
+        const int i = lo;
         return A[i] + (A[i + 1] - A[i]) * ((z - Z[i]) / (Z[i + 1] - Z[i]));
     }
     double f = z / in.zaL;
@@ -1837,12 +1941,11 @@ __device__ __forceinline__ void solve_instance(const DInst& in, Smem& sm) {
 #pragma unroll 1
         for (int st = 0; st < 6 && status == LH_OK; st++) {
             double t2 = t + c_nodes[st] * dt;
-            double hf = t2 - t, hb = t - t2;
+            double hf = t2 - t;
             if (zdep && st != 5) zg_refresh(in, sm, t2, rho, kfac);  // stages 5 and 6 share t + dt
             do_prologue<KIND, MO, MC, Smem>(in, sm, Y, K, st, dt, h0, hf);
             if constexpr (KIND == 0)
-                do_rhs<KIND, MO, MC, Smem, EpiStage, true, RAMAN>(in, sm, EpiStage{K[st + 1], in.linop, hb}, rho,
-                                                                  kfac);
+                do_rhs<KIND, MO, MC, Smem, EpiStageFO, true, RAMAN>(in, sm, EpiStageFO{K[st + 1], K[7]}, rho, kfac);
             else
                 do_rhs<KIND, MO, MC, Smem, EpiStageF, true, RAMAN>(in, sm, EpiStageF{K[st + 1], K[7]}, rho, kfac);
             rhs_evals++;

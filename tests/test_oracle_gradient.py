@@ -191,3 +191,38 @@ def test_piecewise_taper_constants_match_mode_at_radius(orc, pkg):
             nfc = nonlinearrhs.norm_mode_average(grid, beta, m.Aeff())
             assert np.linalg.norm(Lz[s] - Lc[s]) / np.linalg.norm(Lc[s]) < 1e-7, z
             assert np.max(np.abs(nf[s] - nfc[s]) / np.abs(nfc[s])) < 1e-12, z
+
+
+def test_curved_callable_taper_within_interpolation_bound(orc, pkg):
+    """A callable radius is sampled on 1025 nodes over the fibre and
+    interpolated linearly between them.  For the concave quadratic
+    a(z) = 150 µm − 400e-6·(z − 0.2)² on L = 0.4 m the chord lies below the
+    curve by at most b = h²·max|a''|/8 = 1.5e-11 m (h = L/1024), reached
+    halfway between nodes.  There the radius in use is a(z) − b to rounding;
+    the oracle's normalisation (∝ 1/a) is off the host mode model's at the
+    exact a(z) by b/a, and its operator by at most 1e-7 (the agreement at an
+    exact radius) + 4·b/a (the operator's sensitivity to the radius); a radius
+    off by 100·b is rejected."""
+    from luna_jl_amd import linops, modes, nonlinearrhs
+    L, n = 0.4, 1025
+    h = L / (n - 1)
+    a_of = lambda z: 150e-6 - 400e-6 * (z - 0.2) ** 2
+    b = h * h * 800e-6 / 8
+    Ew, grid, linop, tr, hd = setup_taper(a_of, 1.2, L)
+    s = grid.sidx
+    for j in (3, 300, 511, 700, 1020):
+        z = (j + 0.5) * h
+        a = a_of(z)
+        assert tr.gradient.radius(z) == pytest.approx(a - b, rel=1e-12)
+        Lz, nf, rho = orc.grad_constants(hd, tr.gradient, z)
+        m = modes.MarcatiliMode(a, "He", 1.2)
+        Lc, beta, b1 = linops.make_const_linop(grid, m, 800e-9)
+        nfc = nonlinearrhs.norm_mode_average(grid, beta, m.Aeff())
+        enf = np.max(np.abs(nf[s] - nfc[s]) / np.abs(nfc[s]))
+        eL = np.linalg.norm(Lz[s] - Lc[s]) / np.linalg.norm(Lc[s])
+        assert enf <= 1.01 * b / a, (z, enf, b / a)
+        assert eL <= 1e-7 + 4 * b / a, (z, eL)
+        mw = modes.MarcatiliMode(a + 100 * b, "He", 1.2)
+        Lw, betaw, _ = linops.make_const_linop(grid, mw, 800e-9)
+        nfw = nonlinearrhs.norm_mode_average(grid, betaw, mw.Aeff())
+        assert np.max(np.abs(nf[s] - nfw[s]) / np.abs(nfw[s])) > 50 * b / a
