@@ -588,7 +588,7 @@ __device__ __forceinline__ void pointwise_field(const DInst& in, Smem& sm, doubl
 }
 
 // Epilogues of a transform: called with v[i] = the RHS of bin
-// k = threadIdx.x + (i0 + i)·LH_T for a chunk of at most 9 of this thread's
+// k = (the thread's index) + (i0 + i)·LH_T for a chunk of at most 9 of this thread's
 // bins (bins k ≥ n hold zero and are not written).
 struct EpiStore {  // out[k] = v
     static constexpr bool FOLDS_OUTFAC = false;
@@ -597,7 +597,7 @@ struct EpiStore {  // out[k] = v
     __device__ __forceinline__ void operator()(const double2 (&v)[NP], int n, int i0) const {
 #pragma unroll
         for (int i = 0; i < NP; i++) {
-            const int k = threadIdx.x + (i0 + i) * LH_T;
+            const int k = fresh_tid() + (i0 + i) * LH_T;
             if (k < n) out[k] = v[i];
         }
     }
@@ -616,12 +616,12 @@ struct EpiStageFactor {
         double2 f[NP];  // the chunk's loads before its first store (out may alias F as far as the compiler knows)
 #pragma unroll
         for (int i = 0; i < NP; i++) {
-            const int k = threadIdx.x + (i0 + i) * LH_T;
+            const int k = fresh_tid() + (i0 + i) * LH_T;
             f[i] = (k < n) ? F[k] : make_double2(0.0, 0.0);
         }
 #pragma unroll
         for (int i = 0; i < NP; i++) {
-            const int k = threadIdx.x + (i0 + i) * LH_T;
+            const int k = fresh_tid() + (i0 + i) * LH_T;
             if (k < n) out[k] = cmul(v[i], f[i]);
         }
     }
@@ -650,21 +650,21 @@ __device__ __forceinline__ void rhs_field(const DInst& in, Smem& sm, Epi epi, do
         fft_lds<MO, false>(sm.buf, sm.tw);
 #pragma unroll
         for (int i = 0; i < NPT; i++) {
-            int k = threadIdx.x + i * LH_T;
+            int k = fresh_tid() + i * LH_T;
             res[i] = (k < N) ? r2c_bin<MO>(sm.buf, sm.tw, k) : make_double2(0.0, 0.0);
         }
     } else {
         double2 of[NPT];  // loads in flight during the forward FFT
 #pragma unroll
         for (int i = 0; i < NPT; i++) {
-            int k = threadIdx.x + i * LH_T;
+            int k = fresh_tid() + i * LH_T;
             of[i] = (k < N) ? in.outfac[k] : make_double2(0.0, 0.0);
         }
         __syncthreads();
         fft_lds<MO, false>(sm.buf, sm.tw);
 #pragma unroll
         for (int i = 0; i < NPT; i++) {
-            int k = threadIdx.x + i * LH_T;
+            int k = fresh_tid() + i * LH_T;
             res[i] = (k < N) ? cmul(r2c_bin<MO>(sm.buf, sm.tw, k), of[i]) : make_double2(0.0, 0.0);
         }
     }
@@ -834,7 +834,7 @@ __device__ __forceinline__ void stepfun_filter_cols(const DInst& in, const doubl
     constexpr int NPT = (N + LH_T - 1) / LH_T;
 #pragma unroll
     for (int i = 0; i < NPT; i++) {
-        int k = threadIdx.x + i * LH_T;
+        int k = fresh_tid() + i * LH_T;
         if (k < N) {
             double2 v = YN[k];
             if (h != 0.0) v = cmul(v, cexp_lh(linop[k], h));
@@ -846,7 +846,7 @@ __device__ __forceinline__ void stepfun_filter_cols(const DInst& in, const doubl
     __syncthreads();
     fft_lds<MC, true>(sm.buf, sm.tw);
     const double inv_nt = in.inv_nt;
-    for (int j = threadIdx.x; j < MC; j += LH_T) {
+    for (int j = fresh_tid(); j < MC; j += LH_T) {
         double2 z = sm.buf[pad(j)];
         double x0 = (inv_nt * z.x) * in.t_win[2 * j];
         double x1 = (inv_nt * z.y) * in.t_win[2 * j + 1];
@@ -862,7 +862,7 @@ __device__ __forceinline__ void stepfun_filter_cols(const DInst& in, const doubl
             double wv[NPT], ev[NPT];
 #pragma unroll
             for (int i = 0; i < NPT; i++) {
-                int k = threadIdx.x + i * LH_T;
+                int k = fresh_tid() + i * LH_T;
                 wv[i] = (k < N) ? in.st_omega[k] : 0.0;
                 ev[i] = (k < N) ? in.st_ew[k] : 0.0;
                 yv[i] = make_double2(0.0, 0.0);
@@ -876,7 +876,7 @@ __device__ __forceinline__ void stepfun_filter_cols(const DInst& in, const doubl
             WinSums ws = {{0.0, 0.0, 0.0, 0.0}};
 #pragma unroll
             for (int i = 0; i < NPT; i++) {
-                int k = threadIdx.x + i * LH_T;
+                int k = fresh_tid() + i * LH_T;
                 double a = yv[i].x * yv[i].x + yv[i].y * yv[i].y;
                 s0 += a;
                 s1 += wv[i] * a;
@@ -884,14 +884,14 @@ __device__ __forceinline__ void stepfun_filter_cols(const DInst& in, const doubl
                 if (k < N) win_add(in, ws, k, a);
                 if (k < N - 1) sm.buf[pad(k)] = make_double2(2.0 * yv[i].x, 2.0 * yv[i].y);
             }
-            for (int k = N - 1 + threadIdx.x; k < 2 * MC; k += LH_T) sm.buf[pad(k)] = make_double2(0.0, 0.0);
+            for (int k = N - 1 + fresh_tid(); k < 2 * MC; k += LH_T) sm.buf[pad(k)] = make_double2(0.0, 0.0);
             stats_time<0, 2 * MC, Smem>(in, sm, s0, s1, se, ws, z, dz, rho, rec);
             return;
         }
     }
 #pragma unroll
     for (int i = 0; i < NPT; i++) {
-        int k = threadIdx.x + i * LH_T;
+        int k = fresh_tid() + i * LH_T;
         if (k < N) YN[k] = r2c_bin<MC>(sm.buf, sm.tw, k);
     }
     __syncthreads();
@@ -1105,7 +1105,7 @@ __device__ __forceinline__ void stepfun_env(const DInst& in, Smem& sm, double2* 
     constexpr int NPT = (NT + LH_T - 1) / LH_T;
 #pragma unroll
     for (int i = 0; i < NPT; i++) {
-        int k = threadIdx.x + i * LH_T;
+        int k = fresh_tid() + i * LH_T;
         if (k < NT) {
             double2 v = YN[k];
             if (h != 0.0) v = cmul(v, cexp_lh(in.linop[k], h));
@@ -1115,7 +1115,7 @@ __device__ __forceinline__ void stepfun_env(const DInst& in, Smem& sm, double2* 
     __syncthreads();
     fft_lds<NT, true>(sm.buf, sm.tw);
     const double inv_nt = in.inv_nt;
-    for (int j = threadIdx.x; j < NT; j += LH_T) {
+    for (int j = fresh_tid(); j < NT; j += LH_T) {
         double2 z = sm.buf[pad(j)];
         double w = in.t_win[j];
         sm.buf[pad(j)] = make_double2((inv_nt * z.x) * w, (inv_nt * z.y) * w);
@@ -1126,7 +1126,7 @@ __device__ __forceinline__ void stepfun_env(const DInst& in, Smem& sm, double2* 
     WinSums ws = {{0.0, 0.0, 0.0, 0.0}};
 #pragma unroll
     for (int i = 0; i < NPT; i++) {
-        int k = threadIdx.x + i * LH_T;
+        int k = fresh_tid() + i * LH_T;
         if (k < NT) {
             double2 v = sm.buf[pad(k)];
             YN[k] = v;
@@ -1896,7 +1896,7 @@ __device__ __forceinline__ void solve_instance(const DInst& in, Smem& sm) {
     double2* const Y0 = in.Y;
     double2* Y = in.Y;
     double2* YN = in.YN;
-    for (int k = threadIdx.x; k < N; k += LH_T) Y[k] = in.Yin[k];
+    for (int k = fresh_tid(); k < N; k += LH_T) Y[k] = in.Yin[k];
     if (threadIdx.x == 0) sm.flag = LH_OK;
     __syncthreads();
 
@@ -1933,7 +1933,7 @@ __device__ __forceinline__ void solve_instance(const DInst& in, Smem& sm) {
             zg_refresh(in, sm, t, rho, kfac);
             if (h0 != 0.0) {
                 double2* K0 = K[0];
-                for (int k = threadIdx.x; k < N; k += LH_T) K0[k] = cmul(K0[k], cexp_lh(in.linop[k], h0));
+                for (int k = fresh_tid(); k < N; k += LH_T) K0[k] = cmul(K0[k], cexp_lh(in.linop[k], h0));
                 __syncthreads();
             }
             h0 = 0.0;
@@ -1964,7 +1964,7 @@ __device__ __forceinline__ void solve_instance(const DInst& in, Smem& sm) {
                 double2 yv[CH], yn[CH], ye[CH];
 #pragma unroll
                 for (int i = 0; i < CH; i++) {
-                    yv[i] = Y[threadIdx.x + (c + i) * LH_T];
+                    yv[i] = Y[fresh_tid() + (c + i) * LH_T];
                     yn[i] = yv[i];
                     ye[i] = make_double2(0.0, 0.0);
                 }
@@ -1975,7 +1975,7 @@ __device__ __forceinline__ void solve_instance(const DInst& in, Smem& sm) {
                     const double2* Kj = K[j];
 #pragma unroll
                     for (int i = 0; i < CH; i++) {
-                        const double2 kv = Kj[threadIdx.x + (c + i) * LH_T];
+                        const double2 kv = Kj[fresh_tid() + (c + i) * LH_T];
                         yn[i] = cadd(yn[i], cscal(cb, kv));
                         ye[i] = cadd(ye[i], cscal(ce, cscal(dt, kv)));
                     }
@@ -1985,14 +1985,14 @@ __device__ __forceinline__ void solve_instance(const DInst& in, Smem& sm) {
                     sy += yv[i].x * yv[i].x + yv[i].y * yv[i].y;
                     syn += yn[i].x * yn[i].x + yn[i].y * yn[i].y;
                     se += ye[i].x * ye[i].x + ye[i].y * ye[i].y;
-                    YN[threadIdx.x + (c + i) * LH_T] = yn[i];
+                    YN[fresh_tid() + (c + i) * LH_T] = yn[i];
                 }
             }
         } else {
             double2 yv[NPT], yn[NPT], ye[NPT];
 #pragma unroll
             for (int i = 0; i < NPT; i++) {
-                int k = threadIdx.x + i * LH_T;
+                int k = fresh_tid() + i * LH_T;
                 yv[i] = (k < N) ? Y[k] : make_double2(0.0, 0.0);
                 yn[i] = yv[i];
                 ye[i] = make_double2(0.0, 0.0);
@@ -2004,7 +2004,7 @@ __device__ __forceinline__ void solve_instance(const DInst& in, Smem& sm) {
                 const double2* Kj = K[j];
 #pragma unroll
                 for (int i = 0; i < NPT; i++) {
-                    int k = threadIdx.x + i * LH_T;
+                    int k = fresh_tid() + i * LH_T;
                     if (k < N) {
                         double2 kv = Kj[k];
                         yn[i] = cadd(yn[i], cscal(cb, kv));
@@ -2014,7 +2014,7 @@ __device__ __forceinline__ void solve_instance(const DInst& in, Smem& sm) {
             }
 #pragma unroll
             for (int i = 0; i < NPT; i++) {
-                int k = threadIdx.x + i * LH_T;
+                int k = fresh_tid() + i * LH_T;
                 if (k < N) {
                     sy += yv[i].x * yv[i].x + yv[i].y * yv[i].y;
                     syn += yn[i].x * yn[i].x + yn[i].y * yn[i].y;
@@ -2063,9 +2063,9 @@ __device__ __forceinline__ void solve_instance(const DInst& in, Smem& sm) {
                 if (in.saves) {
                     double2* out = in.saves + (size_t)saved * N;
                     if (ti == t) {
-                        for (int k = threadIdx.x; k < N; k += LH_T) out[k] = Y[k];
+                        for (int k = fresh_tid(); k < N; k += LH_T) out[k] = Y[k];
                     } else if (ti == tn) {
-                        for (int k = threadIdx.x; k < N; k += LH_T) out[k] = YN[k];
+                        for (int k = fresh_tid(); k < N; k += LH_T) out[k] = YN[k];
                     } else {
                         double sg = (ti - t) / dt;
                         const double sp1 = sg, sp2 = pow(sg, 2.0), sp3 = pow(sg, 3.0), sp4 = pow(sg, 4.0);
@@ -2089,19 +2089,19 @@ __device__ __forceinline__ void solve_instance(const DInst& in, Smem& sm) {
                                     ((sp1 * c_iC[0][j] + sp2 * c_iC[1][j]) + sp3 * c_iC[2][j]) + sp4 * c_iC[3][j];
 #pragma unroll
                                 for (int i = 0; i < CH; i++) {
-                                    const int k = threadIdx.x + (c + i) * LH_T;
+                                    const int k = fresh_tid() + (c + i) * LH_T;
                                     if (k < N) yi[i] = cadd(yi[i], cscal(bj, kj[k]));
                                 }
                             }
 #pragma unroll
                             for (int i = 0; i < CH; i++) {
-                                const int k = threadIdx.x + (c + i) * LH_T;
+                                const int k = fresh_tid() + (c + i) * LH_T;
                                 yv[i] = (k < N) ? Y[k] : make_double2(0.0, 0.0);
                                 Lv[i] = (k < N) ? in.linop[k] : make_double2(0.0, 0.0);
                             }
 #pragma unroll
                             for (int i = 0; i < CH; i++) {
-                                const int k = threadIdx.x + (c + i) * LH_T;
+                                const int k = fresh_tid() + (c + i) * LH_T;
                                 if (k < N) out[k] = cmul(cadd(yv[i], cscal(dt, yi[i])), cexp_lh(Lv[i], h));
                             }
                         }
@@ -2114,7 +2114,7 @@ __device__ __forceinline__ void solve_instance(const DInst& in, Smem& sm) {
                 // its saves are in place; hand over the state, then publish
                 const int slot = (int)(npub & 1);
                 double2* py = in.pub_y + (size_t)slot * N;
-                for (int k = threadIdx.x; k < N; k += LH_T) py[k] = YN[k];
+                for (int k = fresh_tid(); k < N; k += LH_T) py[k] = YN[k];
                 if (threadIdx.x == 0) {
                     double* info = in.pub_info + 4 * slot;
                     info[0] = tn;
@@ -2150,7 +2150,7 @@ __device__ __forceinline__ void solve_instance(const DInst& in, Smem& sm) {
     // The final state s.yn lives in whichever buffer Y points to after the
     // swaps; copy it into the caller-visible buffer Y0.
     if (Y != Y0)
-        for (int k = threadIdx.x; k < N; k += LH_T) Y0[k] = Y[k];
+        for (int k = fresh_tid(); k < N; k += LH_T) Y0[k] = Y[k];
     if (threadIdx.x == 0 && in.stats) {
         lh_stats s;
         s.steps = steps;
