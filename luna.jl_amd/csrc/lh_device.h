@@ -54,6 +54,26 @@ __device__ __forceinline__ double2 cexp_lh_inv(double2 L, double h, double2& inv
     return make_double2(er * c, er * s);
 }
 
+// Global-memory view of a pointer.  Pointers the kernels read out of their
+// parameter structures are generic to the compiler, so accesses through them
+// become flat instructions: those need a 64-bit address per lane and also
+// count on the LDS wait counter (an LDS wait then waits for outstanding HBM
+// traffic too).  Every pointer here is to device (or device-mapped host)
+// memory; the address-space cast lets the compiler emit global_*
+// instructions (the pointer converts back to a plain one where a function
+// takes one, and the compiler still knows where it points).
+#if defined(__HIP_DEVICE_COMPILE__)
+template <class T>
+using gptr = __attribute__((address_space(1))) T*;
+#else  // the host pass only type-checks device code
+template <class T>
+using gptr = T*;
+#endif
+template <class T>
+__device__ __forceinline__ gptr<T> gl(T* p) {
+    return (gptr<T>)p;
+}
+
 // LDS padding: one spare double2 every 8 keeps radix-8 strided accesses
 // conflict-free for 16-byte lanes.
 __device__ __forceinline__ int pad(int i) { return i + (i >> 3); }
@@ -183,8 +203,9 @@ template <int M, int R, int NS, bool INV, int PLAN = 0>
 __device__ __forceinline__ void fft_pass(double2* buf, const TwLds& tw) {
     // plan 0: min(M/8, LH_T) threads (one radix-8 butterfly each where they
     // fit); plan 1: one thread per radix-R butterfly up to LH_T
-    constexpr int TA = (PLAN == 1) ? ((M / R < LH_T) ? M / R : LH_T) : ((M / 8 < LH_T) ? M / 8 : LH_T);
     constexpr int NB = M / R;
+    constexpr int TA0 = (PLAN == 1) ? ((M / R < LH_T) ? M / R : LH_T) : ((M / 8 < LH_T) ? M / 8 : LH_T);
+    constexpr int TA = (TA0 < NB) ? TA0 : NB;  // never more threads than butterflies
     constexpr int BPT = NB / TA;
     static_assert(BPT >= 1 && NB % TA == 0, "FFT pass split");
     constexpr int TSTRIDE = LH_TWN / (NS * R);
@@ -378,13 +399,14 @@ __device__ __forceinline__ double wave_incl_scan(double x) {
     return x;
 }
 
-// Inclusive prefix sum over lanes 0..7 (row 0; the cross-wave step of the
-// workgroup scans, one value per wave).
-__device__ __forceinline__ double lanes8_incl_scan(double x) {
-    static_assert(LH_NW <= 8, "one value per wave in row 0");
+// Inclusive prefix sum over lanes 0..LH_NW−1 (row 0; the cross-wave step of
+// the workgroup scans, one value per wave).
+__device__ __forceinline__ double lanes_incl_scan(double x) {
+    static_assert(LH_NW <= 16, "one value per wave in row 0");
     x += dpp_f64<DPP_ROW_SHR1>(x);
     x += dpp_f64<DPP_ROW_SHR2>(x);
     x += dpp_f64<DPP_ROW_SHR4>(x);
+    if constexpr (LH_NW > 8) x += dpp_f64<DPP_ROW_SHR8>(x);
     return x;
 }
 

@@ -143,7 +143,9 @@ __constant__ double c_iC[4][7] = {
 // other buffers leave 11 KB (1408 samples) for it.
 template <int MO>
 struct PList {
-    static constexpr int CAP = (MO >= 8192) ? 1344 : 4096;
+    // what the largest class's LDS leaves after the buffer, twiddles and the
+    // per-wave reduction and scan slots
+    static constexpr int CAP = (MO >= 8192) ? 1344 - 19 * (LH_NW - 8) : 4096;
 };
 
 template <int MO>
@@ -219,8 +221,8 @@ __device__ __forceinline__ void ppt_rates(const DInst& in, const double* aE, dou
     double4 pc[Q], pm[Q];
 #pragma unroll
     for (int q = 0; q < Q; q++) {
-        pc[q] = in.ppt[idx[q] - 1];
-        pm[q] = in.ppt[idx[q] - 2];
+        pc[q] = gl(in.ppt)[idx[q] - 1];
+        pm[q] = gl(in.ppt)[idx[q] - 2];
     }
 #pragma unroll
     for (int q = 0; q < Q; q++) {
@@ -287,7 +289,7 @@ __device__ __forceinline__ double block_cumtrapz(double* y, double dx, int g0, d
         Ww = S[lane];
         c = (lane > 0) ? (0.5 * (S[2 * LH_NW + lane - 1] + S[LH_NW + lane])) * dx + Ww : Ww;
     }
-    c = lanes8_incl_scan(c);
+    c = lanes_incl_scan(c);
     const double off = (carry + (lane_value(c, wid) - lane_value(Ww, wid))) + ex;
 #pragma unroll
     for (int q = 0; q < QR; q++) y[q] = off + y[q];
@@ -385,6 +387,7 @@ __device__ __forceinline__ void block_cumtrapz2(double* a, const double* l, doub
     c = run2_cat(dpp_run2<DPP_ROW_SHR1>(c), c, dx);
     c = run2_cat(dpp_run2<DPP_ROW_SHR2>(c), c, dx);
     c = run2_cat(dpp_run2<DPP_ROW_SHR4>(c), c, dx);
+    if constexpr (LH_NW > 8) c = run2_cat(dpp_run2<DPP_ROW_SHR8>(c), c, dx);
     const Run2 win = run2_cat(dpp_run2<DPP_ROW_SHR1>(c), jn, dx);  // runs before wave w's first lane
     // runs before this thread: the wave's, then the lanes' before it in the wave
     const Run2 w_in{__builtin_amdgcn_readlane(win.n, __builtin_amdgcn_readfirstlane(wid)), lane_value(win.sj, wid),
@@ -415,7 +418,7 @@ __device__ __forceinline__ double block_exscan1(double v, double* S, double* tot
     if (lane == 63) S[wid] = x;
     __syncthreads();
     const double Ww = (lane < LH_NW) ? S[lane] : 0.0;
-    const double c = lanes8_incl_scan(Ww);
+    const double c = lanes_incl_scan(Ww);
     *total = lane_value(c, LH_NW - 1);
     return (lane_value(c, wid) - lane_value(Ww, wid)) + ex;
 }
@@ -455,8 +458,8 @@ __device__ __forceinline__ void pointwise_field(const DInst& in, Smem& sm, doubl
     const int tid = fresh_tid();
     const double es = in.e_scale;
     const bool kerr = (in.kerr == LH_KERR_FIELD), plasma = in.plasma;
-    const double2* praman = RAMAN ? in.escr + 3 * MO : nullptr;  // P(t) of raman_field, packed pairs
-    const double* tw = in.to_win;
+    const double2* praman = RAMAN ? gl(in.escr) + 3 * MO : nullptr;  // P(t) of raman_field, packed pairs
+    const double* tw = gl(in.to_win);
     const double dt = in.dt_o, pp1 = in.preion_p1, er = in.e_ratio, ip = in.ionpot;
     double cF = 0.0, cJ = 0.0, cP = 0.0;           // carried integrals
     double lastRate = 0.0, lastPhi = 0.0, lastLoss = 0.0;
@@ -598,7 +601,7 @@ struct EpiStore {  // out[k] = v
 #pragma unroll
         for (int i = 0; i < NP; i++) {
             const int k = fresh_tid() + (i0 + i) * LH_T;
-            if (k < n) out[k] = v[i];
+            if (k < n) gl(out)[k] = v[i];
         }
     }
 };
@@ -617,12 +620,12 @@ struct EpiStageFactor {
 #pragma unroll
         for (int i = 0; i < NP; i++) {
             const int k = fresh_tid() + (i0 + i) * LH_T;
-            f[i] = (k < n) ? F[k] : make_double2(0.0, 0.0);
+            f[i] = (k < n) ? static_cast<const double2*>(gl(F))[k] : make_double2(0.0, 0.0);
         }
 #pragma unroll
         for (int i = 0; i < NP; i++) {
             const int k = fresh_tid() + (i0 + i) * LH_T;
-            if (k < n) out[k] = cmul(v[i], f[i]);
+            if (k < n) gl(out)[k] = cmul(v[i], f[i]);
         }
     }
 };
@@ -658,7 +661,7 @@ __device__ __forceinline__ void rhs_field(const DInst& in, Smem& sm, Epi epi, do
 #pragma unroll
         for (int i = 0; i < NPT; i++) {
             int k = fresh_tid() + i * LH_T;
-            of[i] = (k < N) ? in.outfac[k] : make_double2(0.0, 0.0);
+            of[i] = (k < N) ? static_cast<const double2*>(gl(in.outfac))[k] : make_double2(0.0, 0.0);
         }
         __syncthreads();
         fft_lds<MO, false>(sm.buf, sm.tw);
@@ -680,9 +683,9 @@ struct KSet {
                        // the envelope stage's exp(L·(t − t2))
     long long stride;
     int s0, s6;        // slots of k1 and k7 (swapped by FSAL); k2..k6 keep slots 1..5
-    __device__ __forceinline__ double2* operator[](int j) const {
+    __device__ __forceinline__ gptr<double2> operator[](int j) const {
         const int s = (j == 0) ? s0 : (j == 6) ? s6 : j;
-        return base + (long long)s * stride;
+        return gl(base + (long long)s * stride);
     }
     __device__ __forceinline__ void swap06() {  // FSAL: ks[1] .= ks[7] as a pointer swap
         const int t = s0;
@@ -707,7 +710,7 @@ __device__ __forceinline__ void stage_prologue_cols(const double2* __restrict__ 
 template <int MO, class Smem>
 __device__ __forceinline__ void stage_prologue(const DInst& in, Smem& sm, const double2* Y, const KSet& K,
                                                int stage, double dt, double h0, double hf) {
-    stage_prologue_cols<MO, Smem, KSet, true, true>(in.linop, in.sc_t, sm, Y, K, stage, dt, h0, hf, in.outfac);
+    stage_prologue_cols<MO, Smem, KSet, true, true>(gl(in.linop), in.sc_t, sm, Y, K, stage, dt, h0, hf, gl(in.outfac));
 }
 
 // Same on explicit per-column pointers (one column of a radial state).
@@ -804,7 +807,7 @@ struct WinSums {
 __device__ __forceinline__ void win_add(const DInst& in, WinSums& w, int k, double a) {
 #pragma unroll
     for (int q = 0; q < LH_STAT_MAX_WINDOWS; q++)
-        if (q < in.st_nwin) w.s[q] += in.st_win[(size_t)q * in.n_omega + k] * a;
+        if (q < in.st_nwin) w.s[q] += gl(in.st_win)[(size_t)q * in.n_omega + k] * a;
 }
 
 template <int KIND, int NT, class Smem>
@@ -823,7 +826,7 @@ __device__ __forceinline__ void stepfun_filter_cols(const DInst& in, const doubl
 template <int MC, class Smem>
 __device__ __forceinline__ void stepfun_filter(const DInst& in, Smem& sm, double2* YN, double h, double* rec,
                                                double z, double dz, double rho) {
-    stepfun_filter_cols<MC, Smem, true>(in, in.linop, sm, YN, h, rec, z, dz, rho);
+    stepfun_filter_cols<MC, Smem, true>(in, gl(in.linop), sm, YN, h, rec, z, dz, rho);
 }
 
 template <int MC, class Smem, bool STATS>
@@ -838,7 +841,7 @@ __device__ __forceinline__ void stepfun_filter_cols(const DInst& in, const doubl
         if (k < N) {
             double2 v = YN[k];
             if (h != 0.0) v = cmul(v, cexp_lh(linop[k], h));
-            sm.buf[pad(k)] = cscal(in.omega_win[k], v);
+            sm.buf[pad(k)] = cscal(gl(in.omega_win)[k], v);
         }
     }
     __syncthreads();
@@ -848,8 +851,8 @@ __device__ __forceinline__ void stepfun_filter_cols(const DInst& in, const doubl
     const double inv_nt = in.inv_nt;
     for (int j = fresh_tid(); j < MC; j += LH_T) {
         double2 z = sm.buf[pad(j)];
-        double x0 = (inv_nt * z.x) * in.t_win[2 * j];
-        double x1 = (inv_nt * z.y) * in.t_win[2 * j + 1];
+        double x0 = (inv_nt * z.x) * gl(in.t_win)[2 * j];
+        double x1 = (inv_nt * z.y) * gl(in.t_win)[2 * j + 1];
         sm.buf[pad(j)] = make_double2(x0, x1);
     }
     __syncthreads();
@@ -863,8 +866,8 @@ __device__ __forceinline__ void stepfun_filter_cols(const DInst& in, const doubl
 #pragma unroll
             for (int i = 0; i < NPT; i++) {
                 int k = fresh_tid() + i * LH_T;
-                wv[i] = (k < N) ? in.st_omega[k] : 0.0;
-                ev[i] = (k < N) ? in.st_ew[k] : 0.0;
+                wv[i] = (k < N) ? gl(in.st_omega)[k] : 0.0;
+                ev[i] = (k < N) ? gl(in.st_ew)[k] : 0.0;
                 yv[i] = make_double2(0.0, 0.0);
                 if (k < N) {
                     yv[i] = r2c_bin<MC>(sm.buf, sm.tw, k);
@@ -924,7 +927,7 @@ __device__ __forceinline__ void stage_prologue_env(const DInst& in, Smem& sm, co
         int k = tid + i * LH_T;
         bool v = k < N;
         yb[i] = v ? Y[k] : make_double2(0.0, 0.0);
-        L[i] = v ? in.linop[k] : make_double2(0.0, 0.0);
+        L[i] = v ? static_cast<const double2*>(gl(in.linop))[k] : make_double2(0.0, 0.0);
     }
     if (stage == 0 && h0 != 0.0) {
         double2* K0 = K[0];
@@ -994,7 +997,7 @@ __device__ __forceinline__ void rhs_env(const DInst& in, Smem& sm, Epi epi) {
         for (int q = 0; q < PER; q++) {
             double2 z = sm.buf[pad(i0 + q)];
             E[q] = make_double2(z.x * es, z.y * es);
-            in.escr[i0 + q] = E[q];
+            gl(in.escr)[i0 + q] = E[q];
         }
         __syncthreads();  // every thread has read its E before the packing overwrites
 #pragma unroll
@@ -1012,20 +1015,20 @@ __device__ __forceinline__ void rhs_env(const DInst& in, Smem& sm, Epi epi) {
         for (int k = tid; k <= NO / 2; k += LH_T) {
             int m = NO - k;
             if (k == 0) {
-                double2 Y0 = cmul(in.hwdt[0], r2c_bin<NO>(sm.buf, sm.tw, 0));
-                double2 YM = cmul(in.hwdt[NO], r2c_bin<NO>(sm.buf, sm.tw, NO));
+                double2 Y0 = cmul(gl(in.hwdt)[0], r2c_bin<NO>(sm.buf, sm.tw, 0));
+                double2 YM = cmul(gl(in.hwdt)[NO], r2c_bin<NO>(sm.buf, sm.tw, NO));
                 Y0.y = 0.0;
                 YM.y = 0.0;
                 double2 A = cadd(Y0, cconj(YM)), B = csub(Y0, cconj(YM));
                 sm.buf[pad(0)] = cadd(A, cmul_pi(B));
             } else if (k == NO / 2) {
-                double2 Yk = cmul(in.hwdt[k], r2c_bin<NO>(sm.buf, sm.tw, k));
+                double2 Yk = cmul(gl(in.hwdt)[k], r2c_bin<NO>(sm.buf, sm.tw, k));
                 double2 A = cadd(Yk, cconj(Yk));
                 double2 B = cmul(csub(Yk, cconj(Yk)), cconj(tw_get(sm.tw, k * TS)));
                 sm.buf[pad(k)] = cadd(A, cmul_pi(B));
             } else {
-                double2 Yk = cmul(in.hwdt[k], r2c_bin<NO>(sm.buf, sm.tw, k));
-                double2 Ym = cmul(in.hwdt[m], r2c_bin<NO>(sm.buf, sm.tw, m));
+                double2 Yk = cmul(gl(in.hwdt)[k], r2c_bin<NO>(sm.buf, sm.tw, k));
+                double2 Ym = cmul(gl(in.hwdt)[m], r2c_bin<NO>(sm.buf, sm.tw, m));
                 double2 wk = cconj(tw_get(sm.tw, k * TS)), wm = cconj(tw_get(sm.tw, m * TS));
                 double2 Ak = cadd(Yk, cconj(Ym)), Bk = cmul(csub(Yk, cconj(Ym)), wk);
                 double2 Am = cadd(Ym, cconj(Yk)), Bm = cmul(csub(Ym, cconj(Yk)), wm);
@@ -1045,10 +1048,10 @@ __device__ __forceinline__ void rhs_env(const DInst& in, Smem& sm, Epi epi) {
         __syncthreads();
         const double rho = in.density, k34 = in.kerr34;
         const bool kerr = (in.kerr == LH_KERR_ENV);
-        const double* tw = in.to_win;
+        const double* tw = gl(in.to_win);
 #pragma unroll
         for (int q = 0; q < PER; q++) {
-            double2 e = in.escr[i0 + q];
+            double2 e = gl(in.escr)[i0 + q];
             double2 p = make_double2(0.0, 0.0);
             if (kerr) {
                 double s = k34 * (e.x * e.x + e.y * e.y);
@@ -1061,7 +1064,7 @@ __device__ __forceinline__ void rhs_env(const DInst& in, Smem& sm, Epi epi) {
     } else {
         const double k34 = in.kerr34;
         const bool kerr = (in.kerr == LH_KERR_ENV);
-        const double* tw = in.to_win;
+        const double* tw = gl(in.to_win);
 #pragma unroll
         for (int q = 0; q < PER; q++) {
             double2 z = sm.buf[pad(i0 + q)];
@@ -1079,7 +1082,7 @@ __device__ __forceinline__ void rhs_env(const DInst& in, Smem& sm, Epi epi) {
 #pragma unroll
     for (int i = 0; i < NPT; i++) {
         int k = tid + i * LH_T;
-        of[i] = (k < N) ? in.outfac[k] : make_double2(0.0, 0.0);
+        of[i] = (k < N) ? static_cast<const double2*>(gl(in.outfac))[k] : make_double2(0.0, 0.0);
     }
     __syncthreads();
     fft_lds<NO, false>(sm.buf, sm.tw);
@@ -1108,8 +1111,8 @@ __device__ __forceinline__ void stepfun_env(const DInst& in, Smem& sm, double2* 
         int k = fresh_tid() + i * LH_T;
         if (k < NT) {
             double2 v = YN[k];
-            if (h != 0.0) v = cmul(v, cexp_lh(in.linop[k], h));
-            sm.buf[pad(k)] = cscal(in.omega_win[k], v);
+            if (h != 0.0) v = cmul(v, cexp_lh(gl(in.linop)[k], h));
+            sm.buf[pad(k)] = cscal(gl(in.omega_win)[k], v);
         }
     }
     __syncthreads();
@@ -1117,7 +1120,7 @@ __device__ __forceinline__ void stepfun_env(const DInst& in, Smem& sm, double2* 
     const double inv_nt = in.inv_nt;
     for (int j = fresh_tid(); j < NT; j += LH_T) {
         double2 z = sm.buf[pad(j)];
-        double w = in.t_win[j];
+        double w = gl(in.t_win)[j];
         sm.buf[pad(j)] = make_double2((inv_nt * z.x) * w, (inv_nt * z.y) * w);
     }
     __syncthreads();
@@ -1133,8 +1136,8 @@ __device__ __forceinline__ void stepfun_env(const DInst& in, Smem& sm, double2* 
             if (rec) {  // the envelope's analytic field is ifft(Eω): the spectrum stays in place
                 double a = v.x * v.x + v.y * v.y;
                 s0 += a;
-                s1 += in.st_omega[k] * a;
-                se += in.st_ew[k] * a;
+                s1 += gl(in.st_omega)[k] * a;
+                se += gl(in.st_ew)[k] * a;
                 win_add(in, ws, k, a);
             }
         }
@@ -1222,20 +1225,20 @@ __device__ __forceinline__ void raman_spectrum_big(const DInst& in, const Smem& 
     for (int k = tid; k <= NO / 2; k += LH_T) {
         const int m = NO - k;
         if (k == 0) {
-            double2 Y0 = cmul(in.hwdt[0], r2c_bin_big<NO>(W, sm, 0));
-            double2 YM = cmul(in.hwdt[NO], r2c_bin_big<NO>(W, sm, NO));
+            double2 Y0 = cmul(gl(in.hwdt)[0], r2c_bin_big<NO>(W, sm, 0));
+            double2 YM = cmul(gl(in.hwdt)[NO], r2c_bin_big<NO>(W, sm, NO));
             Y0.y = 0.0;
             YM.y = 0.0;
             const double2 A = cadd(Y0, cconj(YM)), B = csub(Y0, cconj(YM));
             W[0] = cadd(A, cmul_pi(B));
         } else if (k == NO / 2) {
-            const double2 Yk = cmul(in.hwdt[k], r2c_bin_big<NO>(W, sm, k));
+            const double2 Yk = cmul(gl(in.hwdt)[k], r2c_bin_big<NO>(W, sm, k));
             const double2 A = cadd(Yk, cconj(Yk));
             const double2 B = cmul(csub(Yk, cconj(Yk)), cconj(tw_frac<2 * NO>(sm.tw, k)));
             W[k] = cadd(A, cmul_pi(B));
         } else {
-            const double2 Yk = cmul(in.hwdt[k], r2c_bin_big<NO>(W, sm, k));
-            const double2 Ym = cmul(in.hwdt[m], r2c_bin_big<NO>(W, sm, m));
+            const double2 Yk = cmul(gl(in.hwdt)[k], r2c_bin_big<NO>(W, sm, k));
+            const double2 Ym = cmul(gl(in.hwdt)[m], r2c_bin_big<NO>(W, sm, m));
             const double2 wk = cconj(tw_frac<2 * NO>(sm.tw, k)), wm = cconj(tw_frac<2 * NO>(sm.tw, m));
             const double2 Ak = cadd(Yk, cconj(Ym)), Bk = cmul(csub(Yk, cconj(Ym)), wk);
             const double2 Am = cadd(Ym, cconj(Yk)), Bm = cmul(csub(Ym, cconj(Yk)), wm);
@@ -1256,9 +1259,9 @@ __device__ __forceinline__ void raman_spectrum_big(const DInst& in, const Smem& 
 template <int MO, class Smem>
 __device__ __forceinline__ void raman_field(const DInst& in, Smem& sm) {
     constexpr int NR = 2 * MO;
-    double2* S = in.escr;
-    double2* W = in.escr + MO;
-    double2* T = in.escr + 3 * MO;
+    double2* S = gl(in.escr);
+    double2* W = gl(in.escr) + MO;
+    double2* T = gl(in.escr) + 3 * MO;
     const int tid = fresh_tid();
     const double es = in.e_scale;
     for (int j = tid; j < MO; j += LH_T) {
@@ -1287,7 +1290,7 @@ __device__ __forceinline__ void stage_prologue_big(const DInst& in, Smem& sm, co
                                                    int stage, double dt, double h0, double hf) {
     constexpr int NPT = NN / LH_T, CH = 8;
     static_assert(NN % (LH_T * CH) == 0, "big-grid chunks");
-    double2* W = in.escr + NN;
+    double2* W = gl(in.escr) + NN;
     const int tid = fresh_tid();
     const double sc_t = in.sc_t;
     double2* K7 = K[7];
@@ -1298,7 +1301,7 @@ __device__ __forceinline__ void stage_prologue_big(const DInst& in, Smem& sm, co
         for (int i = 0; i < CH; i++) {
             const int k = tid + (c + i) * LH_T;
             yb[i] = Y[k];
-            L[i] = in.linop[k];
+            L[i] = gl(in.linop)[k];
         }
         if (stage == 0 && h0 != 0.0) {
             double2* K0 = K[0];
@@ -1332,14 +1335,14 @@ __device__ __forceinline__ void stage_prologue_big(const DInst& in, Smem& sm, co
 template <int NO, class Smem, class Epi>
 __device__ __forceinline__ void rhs_big(const DInst& in, Smem& sm, Epi epi) {
     constexpr int M = NO / 2;
-    double2* E = in.escr;
-    double2* W = in.escr + NO;
-    double2* T = in.escr + 2 * NO;
+    double2* E = gl(in.escr);
+    double2* W = gl(in.escr) + NO;
+    double2* T = gl(in.escr) + 2 * NO;
     const int tid = fresh_tid();
     fft_big<M, true>(W, T, sm);
     const double es = in.e_scale, k34 = in.kerr34, rho = in.density;
     const bool kerr = (in.kerr == LH_KERR_ENV);
-    const double* twin = in.to_win;
+    const double* twin = gl(in.to_win);
     if (in.raman) {
         for (int i = tid; i < NO; i += LH_T) {
             const double2 z = W[i];
@@ -1392,7 +1395,7 @@ __device__ __forceinline__ void rhs_big(const DInst& in, Smem& sm, Epi epi) {
 #pragma unroll
         for (int i = 0; i < CH; i++) {
             const int k = tid + (c + i) * LH_T;
-            res[i] = cmul(W[k], in.outfac[k]);
+            res[i] = cmul(W[k], gl(in.outfac)[k]);
         }
         epi(res, NO, c);
     }
@@ -1401,19 +1404,19 @@ __device__ __forceinline__ void rhs_big(const DInst& in, Smem& sm, Epi epi) {
 // stepfun on the big envelope grid: YN ← fft(twin · ifft(ωwin · YN·exp(L h))).
 template <int NT, class Smem>
 __device__ __forceinline__ void stepfun_big(const DInst& in, Smem& sm, double2* YN, double h) {
-    double2* W = in.escr + NT;
-    double2* T = in.escr + 2 * NT;
+    double2* W = gl(in.escr) + NT;
+    double2* T = gl(in.escr) + 2 * NT;
     const int tid = fresh_tid();
     for (int k = tid; k < NT; k += LH_T) {
         double2 v = YN[k];
-        if (h != 0.0) v = cmul(v, cexp_lh(in.linop[k], h));
-        W[k] = cscal(in.omega_win[k], v);
+        if (h != 0.0) v = cmul(v, cexp_lh(gl(in.linop)[k], h));
+        W[k] = cscal(gl(in.omega_win)[k], v);
     }
     fft_big<NT / 2, true>(W, T, sm);
     const double inv_nt = in.inv_nt;
     for (int j = tid; j < NT; j += LH_T) {
         const double2 z = W[j];
-        const double w = in.t_win[j];
+        const double w = gl(in.t_win)[j];
         W[j] = make_double2((inv_nt * z.x) * w, (inv_nt * z.y) * w);
     }
     fft_big<NT / 2, false>(W, T, sm);
@@ -1481,8 +1484,8 @@ __constant__ double c_fd8[9] = {1.0 / 280, -4.0 / 105, 1.0 / 5, -4.0 / 5, 0.0, 4
 
 // Pressure profile p(z) of Capillary.gradient, src/Capillary.jl:306-340.
 __device__ __forceinline__ double zg_pressure(const DInst& in, double z) {
-    const double* Z = in.zg_Z;
-    const double* P = in.zg_P;
+    const double* Z = gl(in.zg_Z);
+    const double* P = gl(in.zg_P);
     const int n = in.zg_np;
     if (in.zg_two) {
         const double L = Z[1], p0 = P[0], p1 = P[1];
@@ -1521,12 +1524,12 @@ __device__ __forceinline__ double2 zg_neff(const DInst& in, double eco, double2 
 // linop!(out, t2) whenever t2 changes): ρ(z) and β1(z) on thread 0, then per
 // bin the operator -i(ω/c·conj_clamp(neff) - ω β1) (src/LinearOps.jl:337-358,
 // 254) and the output factor sc_f·pre/β·√Aeff·ωwin (β = ω/c·Re neff),
-// written to in.zL / in.zOF.  Returns ρ(z) and the Kerr factor ρ ε0 γ3.
+// written to gl(in.zL) / gl(in.zOF).  Returns ρ(z) and the Kerr factor ρ ε0 γ3.
 // Radius of the taper at z: linear a0 + (a1 - a0)·clamp(z/L, 0, 1), or the
 // piecewise-linear profile through (zt_Z, zt_A), constant outside.
 __device__ __forceinline__ double zg_radius(const DInst& in, double z) {
     if (in.ztaper == 2) {
-        const double *Z = in.zt_Z, *A = in.zt_A;
+        const double *Z = gl(in.zt_Z), *A = gl(in.zt_A);
         const int n = in.zt_n;
         if (z <= Z[0]) return A[0];
         if (z >= Z[n - 1]) return A[n - 1];
@@ -1570,17 +1573,17 @@ __device__ __forceinline__ void zg_refresh(const DInst& in, Smem& sm, double z, 
     const int tid = fresh_tid();
     if (tid == 0) {
         const double p = zg_pressure(in, z);
-        const double r = cspline_uniform(in.zd, in.zd_n, in.zd[0].x, in.zd[in.zd_n - 1].x, p);
+        const double r = cspline_uniform(gl(in.zd), in.zd_n, gl(in.zd)[0].x, gl(in.zd)[in.zd_n - 1].x, p);
         // β1 = Modes.dispersion(mode, 1, ω0, z): β(ω) = ω/c·Re neff(mode, ω; z)
         // with neff's own clamp (src/Capillary.jl:162-186)
         const double a = in.ztaper ? zg_radius(in, z) : 0.0;
         double acc = 0.0;
         for (int j = 0; j < 9; j++) {
-            const double* f = in.zfd + 4 * j;
+            const double* f = gl(in.zfd) + 4 * j;
             const double sc = sqrt(1 + f[1] * r);
             double2 nwj = make_double2(f[2], f[3]);
             if (in.ztaper) {
-                const double* t = in.zfdt + 3 * j;
+                const double* t = gl(in.zfdt) + 3 * j;
                 nwj = zg_nwg_taper(in, t[0], make_double2(t[1], t[2]), a);
             }
             double nr = zg_neff(in, sc * sc, nwj).x;
@@ -1599,10 +1602,10 @@ __device__ __forceinline__ void zg_refresh(const DInst& in, Smem& sm, double z, 
     kfac = rho * LH_EPS0 * in.kerr_gamma3;
     for (int k = tid; k < in.n_omega; k += LH_T) {
         double2 L = make_double2(0.0, 0.0), of = make_double2(in.sc_f, 0.0);
-        if (in.zsidx[k]) {
-            const double w = in.st_omega[k];
-            const double sc = sqrt(1 + in.zgam[k] * rho);  // coren(ω; z)^2
-            const double2 nw = in.ztaper ? zg_nwg_taper(in, in.zU[k], in.zV[k], a) : in.znwg[k];
+        if (gl(in.zsidx)[k]) {
+            const double w = gl(in.st_omega)[k];
+            const double sc = sqrt(1 + gl(in.zgam)[k] * rho);  // coren(ω; z)^2
+            const double2 nw = in.ztaper ? zg_nwg_taper(in, gl(in.zU)[k], gl(in.zV)[k], a) : static_cast<const double2*>(gl(in.znwg))[k];
             const double2 n = zg_neff(in, sc * sc, nw);
             const double ncr = (n.x < 1e-3) ? 1e-3 : n.x;  // conj_clamp
             const double lim = 3000 * LH_CLIGHT / w;
@@ -1611,12 +1614,12 @@ __device__ __forceinline__ void zg_refresh(const DInst& in, Smem& sm, double z, 
             const double zr = wc * ncr - w * b1, zi = wc * nci;
             L = make_double2(zi, -zr);
             const double beta = wc * n.x;
-            const double2 pre = in.zpre[k];
+            const double2 pre = gl(in.zpre)[k];
             const double nfx = pre.x / beta * sqrt_aeff, nfy = pre.y / beta * sqrt_aeff;
-            of = make_double2((in.sc_f * nfx) * in.omega_win[k], (in.sc_f * nfy) * in.omega_win[k]);
+            of = make_double2((in.sc_f * nfx) * gl(in.omega_win)[k], (in.sc_f * nfy) * gl(in.omega_win)[k]);
         }
-        in.zL[k] = L;
-        in.zOF[k] = of;
+        gl(in.zL)[k] = L;
+        gl(in.zOF)[k] = of;
     }
     __syncthreads();
 }
@@ -1654,8 +1657,8 @@ __device__ __forceinline__ void step_stats(const DInst& in, Smem& sm, const doub
         if (k < N) {
             double a = v.x * v.x + v.y * v.y;
             s0 += a;
-            s1 += in.st_omega[k] * a;
-            se += in.st_ew[k] * a;
+            s1 += gl(in.st_omega)[k] * a;
+            se += gl(in.st_ew)[k] * a;
             win_add(in, ws, k, a);
         }
         if constexpr (KIND == 0)  // copyto_fft!: bins 0..Nω-2 doubled, the +fs/2 bin dropped
@@ -1703,7 +1706,7 @@ __device__ __forceinline__ void stats_time(const DInst& in, Smem& sm, double s0,
     // pass 1: P = |Et|² (kept in .x), rate (kept in .y); argmax, max/sum of
     // rate.  Sample j = tid + m·LH_T; rates in batches of QC (spline-table
     // loads issued together).
-    constexpr int MPT = NT / LH_T, QC = 4;
+    constexpr int MPT = NT / LH_T, QC = (MPT < 4) ? MPT : 4;
     static_assert(NT % LH_T == 0 && MPT % QC == 0, "stats sample split");
     const double inv = 1.0 / (double)NT;  // FFTW's ifft normalisation
     const bool ion = (KIND == 0) && in.plasma;
@@ -1813,7 +1816,7 @@ __device__ __forceinline__ void stats_time(const DInst& in, Smem& sm, double s0,
             bL = min(bL, (int)sm.wsum[4 * w + 2]);
             bR = max(bR, (int)sm.wsum[4 * w + 3]);
         }
-        const double* t = in.st_t;
+        const double* t = gl(in.st_t);
         const double nan = __longlong_as_double(0x7ff8000000000000LL);
         // a missing crossing or a neighbour off the grid is the reference's
         // caught exception: NaN
@@ -1892,11 +1895,11 @@ __device__ __forceinline__ void do_stepfun(const DInst& in, Smem& sm, double2* Y
 template <int KIND, int MO, int MC, class Smem, bool ZDEP, bool STREAM, bool RAMAN>
 __device__ __forceinline__ void solve_instance(const DInst& in, Smem& sm) {
     constexpr int N = Sizes<KIND, MO, MC>::N, NPT = Sizes<KIND, MO, MC>::NPT;
-    KSet K{in.K[0], in.kstride, 0, 6};
-    double2* const Y0 = in.Y;
-    double2* Y = in.Y;
-    double2* YN = in.YN;
-    for (int k = fresh_tid(); k < N; k += LH_T) Y[k] = in.Yin[k];
+    KSet K{gl(in.K[0]), in.kstride, 0, 6};
+    double2* const Y0 = gl(in.Y);
+    double2* Y = gl(in.Y);
+    double2* YN = gl(in.YN);
+    for (int k = fresh_tid(); k < N; k += LH_T) Y[k] = gl(in.Yin)[k];
     if (threadIdx.x == 0) sm.flag = LH_OK;
     __syncthreads();
 
@@ -1933,7 +1936,7 @@ __device__ __forceinline__ void solve_instance(const DInst& in, Smem& sm) {
             zg_refresh(in, sm, t, rho, kfac);
             if (h0 != 0.0) {
                 double2* K0 = K[0];
-                for (int k = fresh_tid(); k < N; k += LH_T) K0[k] = cmul(K0[k], cexp_lh(in.linop[k], h0));
+                for (int k = fresh_tid(); k < N; k += LH_T) K0[k] = cmul(K0[k], cexp_lh(gl(in.linop)[k], h0));
                 __syncthreads();
             }
             h0 = 0.0;
@@ -2051,17 +2054,17 @@ __device__ __forceinline__ void solve_instance(const DInst& in, Smem& sm) {
             K.swap06();  // FSAL: ks[1] .= ks[7] (src/RK45.jl:184-186)
             // stepfun, then MemoryOutput's statistics of the filtered state
             // (src/Output.jl:65-67) from the same LDS pass
-            double* rec = (in.steprec && accepted < in.steprec_cap) ? in.steprec + (size_t)accepted * LH_NSTAT
+            double* rec = (gl(in.steprec) && accepted < in.steprec_cap) ? gl(in.steprec) + (size_t)accepted * LH_NSTAT
                                                                       : nullptr;
             do_stepfun<KIND, MO, MC, Smem>(in, sm, YN, tn - t, rec, tn, dtn, rho);
             if (rec && sm.flag != LH_OK) status = sm.flag;
             // GridCondition saves via dense output; ks[1] and ks[7] both hold k7
             const int saved_before = saved;
-            while (saved < in.save_n && in.zgrid[saved] <= tn) {
-                double ti = in.zgrid[saved];
-                if (zdep && in.saves && ti != t && ti != tn) zg_refresh(in, sm, ti, rho, kfac);  // prop!(out, t, ti)
-                if (in.saves) {
-                    double2* out = in.saves + (size_t)saved * N;
+            while (saved < in.save_n && gl(in.zgrid)[saved] <= tn) {
+                double ti = gl(in.zgrid)[saved];
+                if (zdep && gl(in.saves) && ti != t && ti != tn) zg_refresh(in, sm, ti, rho, kfac);  // prop!(out, t, ti)
+                if (gl(in.saves)) {
+                    double2* out = gl(in.saves) + (size_t)saved * N;
                     if (ti == t) {
                         for (int k = fresh_tid(); k < N; k += LH_T) out[k] = Y[k];
                     } else if (ti == tn) {
@@ -2097,7 +2100,7 @@ __device__ __forceinline__ void solve_instance(const DInst& in, Smem& sm) {
                             for (int i = 0; i < CH; i++) {
                                 const int k = fresh_tid() + (c + i) * LH_T;
                                 yv[i] = (k < N) ? Y[k] : make_double2(0.0, 0.0);
-                                Lv[i] = (k < N) ? in.linop[k] : make_double2(0.0, 0.0);
+                                Lv[i] = (k < N) ? static_cast<const double2*>(gl(in.linop))[k] : make_double2(0.0, 0.0);
                             }
 #pragma unroll
                             for (int i = 0; i < CH; i++) {
@@ -2151,7 +2154,7 @@ __device__ __forceinline__ void solve_instance(const DInst& in, Smem& sm) {
     // swaps; copy it into the caller-visible buffer Y0.
     if (Y != Y0)
         for (int k = fresh_tid(); k < N; k += LH_T) Y0[k] = Y[k];
-    if (threadIdx.x == 0 && in.stats) {
+    if (threadIdx.x == 0 && gl(in.stats)) {
         lh_stats s;
         s.steps = steps;
         s.accepted = accepted;
@@ -2184,20 +2187,20 @@ __device__ __forceinline__ void rhs_once(const DInst& in, Smem& sm) {
     double rho = in.density, kfac = in.kerr_fac;
     if (KIND == 0 && in.zg_np > 0) zg_refresh(in, sm, in.z0, rho, kfac);  // the call's z
     for (int k = threadIdx.x; k < N; k += LH_T) {
-        double2 v = cscal(in.sc_t, in.rhs_in[k]);
+        double2 v = cscal(in.sc_t, gl(in.rhs_in)[k]);
         if constexpr (KIND == 0)
             sm.buf[pad(k)] = v;
         else if constexpr (KIND == 1)
             sm.buf[pad(env_slot<MO, MC>(k))] = v;
         else
-            in.escr[MO + k] = v;  // the big grid's working series W
+            gl(in.escr)[MO + k] = v;  // the big grid's working series W
     }
     if constexpr (KIND == 0) {
         for (int k = N + threadIdx.x; k <= MO; k += LH_T) sm.buf[pad(k)] = make_double2(0, 0);
     } else if constexpr (MO > MC) {
         for (int k = MC / 2 + threadIdx.x; k < MO - MC / 2; k += LH_T) sm.buf[pad(k)] = make_double2(0, 0);
     }
-    double2* out = in.rhs_out;
+    double2* out = gl(in.rhs_out);
     if (KIND == 0 && in.raman)
         do_rhs<KIND, MO, MC, Smem, EpiStore, false, true>(in, sm, EpiStore{out}, rho, kfac);
     else
@@ -2228,7 +2231,7 @@ __global__ __launch_bounds__(LH_T) void lh_stats_kernel(const DInst* insts, cons
     const DInst& in = insts[blockIdx.x];
     double rho = in.density, kfac = in.kerr_fac;
     if (KIND == 0 && in.zg_np > 0) zg_refresh(in, sm, z, rho, kfac);
-    step_stats<KIND, MO, MC, SmemT<MO>>(in, sm, in.rhs_in, z, dz, rho, in.steprec);
+    step_stats<KIND, MO, MC, SmemT<MO>>(in, sm, gl(in.rhs_in), z, dz, rho, gl(in.steprec));
     if (threadIdx.x == 0) status[blockIdx.x] = sm.flag;
 }
 

@@ -34,7 +34,7 @@ struct RadK {
     long long stride;
     const int* perm;
     size_t off;
-    __device__ __forceinline__ double2* operator[](int j) const { return base + (long long)perm[j] * stride + off; }
+    __device__ __forceinline__ gptr<double2> operator[](int j) const { return gl(base + (long long)perm[j] * stride + off); }
 };
 
 struct RadArgs {
@@ -69,29 +69,29 @@ template <int MO, int PLAN>
 __global__ __launch_bounds__(LH_T) void rad_stage_a(const RadArgs* __restrict__ ap, const double2* __restrict__ twg,
                                                     int stage) {
     const RadArgs& a = *ap;
-    const RadCtl& ctl = *a.ctl;
+    const RadCtl& ctl = *gl(a.ctl);
     if (ctl.done) return;
     __shared__ SmemT<MO> sm;
     tw_load(sm.tw, twg);
     const int c = blockIdx.x, N = a.n_omega;
     const size_t off = (size_t)N * c;
-    const RadK Kc{a.K[0], a.kstride, ctl.perm, off};
+    const RadK Kc{gl(a.K[0]), a.kstride, ctl.perm, off};
     if (stage < 0) {  // plain RHS of the input column (PreconStepper init / single RHS)
-        const double2* src = (a.rhs_in ? a.rhs_in : a.Y[ctl.ycur]) + off;
+        const double2* src = (gl(a.rhs_in) ? gl(a.rhs_in) : gl(a.Y[ctl.ycur])) + off;
         for (int k = threadIdx.x; k < N; k += LH_T) sm.buf[pad(k)] = cscal(a.sc_t, src[k]);
         for (int k = N + threadIdx.x; k <= MO; k += LH_T) sm.buf[pad(k)] = make_double2(0.0, 0.0);
     } else {
         const double t = ctl.t, dt = ctl.dt;
         const double t2 = t + c_nodes[stage] * dt;
         const double h0 = (stage == 0) ? ctl.h0 : 0.0;
-        stage_prologue_cols<MO, SmemT<MO>, RadK, false>(a.linop + off, a.sc_t, sm, a.Y[ctl.ycur] + off, Kc, stage, dt, h0,
+        stage_prologue_cols<MO, SmemT<MO>, RadK, false>(gl(a.linop) + off, a.sc_t, sm, gl(a.Y[ctl.ycur]) + off, Kc, stage, dt, h0,
                                                  t2 - t);
     }
     __syncthreads();
     c2r_prepack<MO>(sm.buf, sm.tw);
     __syncthreads();
     fft_lds<MO, true, PLAN>(sm.buf, sm.tw);
-    double* out = a.Ek + (size_t)a.n_to * c;
+    double* out = gl(a.Ek) + (size_t)a.n_to * c;
     for (int j = threadIdx.x; j < MO; j += LH_T) {
         double2 z = sm.buf[pad(j)];
         out[2 * j] = z.x;
@@ -111,10 +111,10 @@ __global__ __launch_bounds__(LH_T) void rad_stage_a(const RadArgs* __restrict__ 
 template <int MT>
 __global__ __launch_bounds__(256) void rad_gemm(const RadArgs* __restrict__ ap, int dir) {
     const RadArgs& a = *ap;
-    if (a.ctl->done) return;
-    const double* __restrict__ A = dir == 0 ? a.Ek : a.Er;
-    double* __restrict__ C = dir == 0 ? a.Er : a.Ek;
-    const double* __restrict__ B = a.Mt;
+    if (gl(a.ctl)->done) return;
+    const double* __restrict__ A = dir == 0 ? gl(a.Ek) : gl(a.Er);
+    double* __restrict__ C = dir == 0 ? gl(a.Er) : gl(a.Ek);
+    const double* __restrict__ B = gl(a.Mt);
     const double scale = dir == 0 ? a.gemm_kr : a.gemm_rk;
     const int M = a.n_to, K = a.n_r;  // K = N, a multiple of 64; M a multiple of 32·MT
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -169,20 +169,20 @@ __global__ __launch_bounds__(256) void rad_gemm(const RadArgs* __restrict__ ap, 
 template <int MO>
 __global__ __launch_bounds__(LH_T) void rad_pointwise(const RadArgs* __restrict__ ap) {
     const RadArgs& a = *ap;
-    if (a.ctl->done) return;
+    if (gl(a.ctl)->done) return;
     __shared__ SmemT<MO> sm;
     if (threadIdx.x == 0) sm.flag = LH_OK;
-    double* col = a.Er + (size_t)a.n_to * blockIdx.x;
+    double* col = gl(a.Er) + (size_t)a.n_to * blockIdx.x;
     for (int j = threadIdx.x; j < MO; j += LH_T) sm.buf[pad(j)] = make_double2(col[2 * j], col[2 * j + 1]);
     __syncthreads();
-    pointwise_field<MO, SmemT<MO>>(*a.pw, sm, a.pw->density, a.pw->kerr_fac);
+    pointwise_field<MO, SmemT<MO>>(*gl(a.pw), sm, gl(a.pw)->density, gl(a.pw)->kerr_fac);
     __syncthreads();
     for (int j = threadIdx.x; j < MO; j += LH_T) {
         double2 z = sm.buf[pad(j)];
         col[2 * j] = z.x;
         col[2 * j + 1] = z.y;
     }
-    if (threadIdx.x == 0 && sm.flag != LH_OK) atomicCAS(&a.ctl->status, LH_OK, sm.flag);
+    if (threadIdx.x == 0 && sm.flag != LH_OK) atomicCAS(&gl(a.ctl)->status, LH_OK, sm.flag);
 }
 
 // ---------------------------------------------------------------- E: r2c + epilogue
@@ -190,24 +190,24 @@ template <int MO, int PLAN>
 __global__ __launch_bounds__(LH_T) void rad_stage_e(const RadArgs* __restrict__ ap, const double2* __restrict__ twg,
                                                     int stage) {
     const RadArgs& a = *ap;
-    const RadCtl& ctl = *a.ctl;
+    const RadCtl& ctl = *gl(a.ctl);
     if (ctl.done) return;
     __shared__ SmemT<MO> sm;
     tw_load(sm.tw, twg);
     const int c = blockIdx.x, N = a.n_omega;
     const size_t off = (size_t)N * c;
-    const double* col = a.Ek + (size_t)a.n_to * c;
+    const double* col = gl(a.Ek) + (size_t)a.n_to * c;
     for (int j = threadIdx.x; j < MO; j += LH_T) sm.buf[pad(j)] = make_double2(col[2 * j], col[2 * j + 1]);
     __syncthreads();
     fft_lds<MO, false, PLAN>(sm.buf, sm.tw);
-    const double2* of = a.outfac + off;
+    const double2* of = gl(a.outfac) + off;
     if (stage < 0) {
-        double2* out = (a.rhs_out ? a.rhs_out : a.K[ctl.perm[0]]) + off;
+        double2* out = (gl(a.rhs_out) ? gl(a.rhs_out) : gl(a.K[ctl.perm[0]])) + off;
         for (int k = threadIdx.x; k < N; k += LH_T) out[k] = cmul(r2c_bin<MO>(sm.buf, sm.tw, k), of[k]);
     } else {
         const double t = ctl.t, t2 = t + c_nodes[stage] * ctl.dt, hb = t - t2;
-        double2* out = a.K[ctl.perm[stage + 1]] + off;
-        const double2* L = a.linop + off;
+        double2* out = gl(a.K[ctl.perm[stage + 1]]) + off;
+        const double2* L = gl(a.linop) + off;
         for (int k = threadIdx.x; k < N; k += LH_T)
             out[k] = cmul(cmul(r2c_bin<MO>(sm.buf, sm.tw, k), of[k]), cexp_lh(L[k], hb));
     }
@@ -216,20 +216,20 @@ __global__ __launch_bounds__(LH_T) void rad_stage_e(const RadArgs* __restrict__ 
 // ---------------------------------------------------------------- combination
 __global__ __launch_bounds__(LH_T) void rad_combine(const RadArgs* __restrict__ ap) {
     const RadArgs& a = *ap;
-    const RadCtl& ctl = *a.ctl;
+    const RadCtl& ctl = *gl(a.ctl);
     if (ctl.done) return;
     __shared__ double scratch[3 * LH_NW];
     const int c = blockIdx.x, N = a.n_omega;
     const size_t off = (size_t)N * c;
     const double dt = ctl.dt;
-    const double2* Y = a.Y[ctl.ycur] + off;
-    double2* YN = a.Y[ctl.ycur ^ 1] + off;
+    const double2* Y = gl(a.Y[ctl.ycur]) + off;
+    double2* YN = gl(a.Y[ctl.ycur ^ 1]) + off;
     double sy = 0.0, syn = 0.0, se = 0.0;
     for (int k = threadIdx.x; k < N; k += LH_T) {
         double2 yv = Y[k], yn = yv, ye = make_double2(0.0, 0.0);
         for (int j = 0; j < 7; j++) {
             if (j == 1) continue;
-            double2 kv = a.K[ctl.perm[j]][off + k];
+            double2 kv = gl(a.K[ctl.perm[j]])[off + k];
             yn = cadd(yn, cscal(dt * c_b5[j], kv));
             ye = cadd(ye, cscal(c_b5[j] - c_b4[j], cscal(dt, kv)));
         }
@@ -240,9 +240,9 @@ __global__ __launch_bounds__(LH_T) void rad_combine(const RadArgs* __restrict__ 
     }
     block_sum3(sy, syn, se, scratch);
     if (threadIdx.x == 0) {
-        a.partial[3 * c + 0] = sy;
-        a.partial[3 * c + 1] = syn;
-        a.partial[3 * c + 2] = se;
+        gl(a.partial)[3 * c + 0] = sy;
+        gl(a.partial)[3 * c + 1] = syn;
+        gl(a.partial)[3 * c + 2] = se;
     }
 }
 
@@ -253,13 +253,13 @@ __global__ __launch_bounds__(LH_T) void rad_combine(const RadArgs* __restrict__ 
 // fixed xor tree (deterministic); lane 0 runs the scalar controller.
 __global__ __launch_bounds__(64) void rad_control(const RadArgs* __restrict__ ap) {
     const RadArgs& a = *ap;
-    RadCtl* ctl = a.ctl;
+    RadCtl* ctl = gl(a.ctl);
     if (ctl->done) return;
     double sy = 0.0, syn = 0.0, se = 0.0;
     for (int c = threadIdx.x; c < a.n_r; c += 64) {
-        sy += a.partial[3 * c + 0];
-        syn += a.partial[3 * c + 1];
-        se += a.partial[3 * c + 2];
+        sy += gl(a.partial)[3 * c + 0];
+        syn += gl(a.partial)[3 * c + 1];
+        se += gl(a.partial)[3 * c + 2];
     }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
@@ -306,7 +306,7 @@ __global__ __launch_bounds__(64) void rad_control(const RadArgs* __restrict__ ap
         s.accepted++;
         s.repeated = 0;
         s.save_from = s.saved;
-        while (s.saved < a.save_n && a.zgrid[s.saved] <= s.tn) s.saved++;
+        while (s.saved < a.save_n && gl(a.zgrid)[s.saved] <= s.tn) s.saved++;
         s.save_to = s.saved;
     } else {
         s.rejected++;
@@ -324,28 +324,28 @@ __global__ __launch_bounds__(64) void rad_control(const RadArgs* __restrict__ ap
 template <int MC>
 __global__ __launch_bounds__(LH_T) void rad_stepfun(const RadArgs* __restrict__ ap, const double2* __restrict__ twg) {
     const RadArgs& a = *ap;
-    const RadCtl& ctl = *a.ctl;
+    const RadCtl& ctl = *gl(a.ctl);
     if (ctl.done || !ctl.ok) return;
     __shared__ SmemT<MC> sm;
     tw_load(sm.tw, twg);
     __syncthreads();
     const size_t off = (size_t)a.n_omega * blockIdx.x;
-    stepfun_filter_cols<MC>(*a.pw, a.linop + off, sm, a.Y[ctl.ycur ^ 1] + off, ctl.tn - ctl.t);
+    stepfun_filter_cols<MC>(*gl(a.pw), gl(a.linop) + off, sm, gl(a.Y[ctl.ycur ^ 1]) + off, ctl.tn - ctl.t);
 }
 
 // ---------------------------------------------------------------- saves per column
 __global__ __launch_bounds__(LH_T) void rad_saves(const RadArgs* __restrict__ ap) {
     const RadArgs& a = *ap;
-    const RadCtl& ctl = *a.ctl;
-    if (ctl.done || !ctl.ok || !a.saves) return;
+    const RadCtl& ctl = *gl(a.ctl);
+    if (ctl.done || !ctl.ok || !gl(a.saves)) return;
     const int c = blockIdx.x, N = a.n_omega;
     const size_t off = (size_t)N * c, nst = (size_t)N * a.n_r;
-    const double2* Y = a.Y[ctl.ycur] + off;
-    const double2* YN = a.Y[ctl.ycur ^ 1] + off;
+    const double2* Y = gl(a.Y[ctl.ycur]) + off;
+    const double2* YN = gl(a.Y[ctl.ycur ^ 1]) + off;
     const double t = ctl.t, tn = ctl.tn, dt = ctl.dt;
     for (int sidx = ctl.save_from; sidx < ctl.save_to; sidx++) {
-        const double ti = a.zgrid[sidx];
-        double2* out = a.saves + nst * sidx + off;
+        const double ti = gl(a.zgrid)[sidx];
+        double2* out = gl(a.saves) + nst * sidx + off;
         if (ti == t) {
             for (int k = threadIdx.x; k < N; k += LH_T) out[k] = Y[k];
         } else if (ti == tn) {
@@ -364,10 +364,10 @@ __global__ __launch_bounds__(LH_T) void rad_saves(const RadArgs* __restrict__ ap
                 double2 yi = make_double2(0.0, 0.0);
                 for (int j = 0; j < 7; j++) {
                     if (j == 1) continue;
-                    const double2* kj = a.K[ctl.perm[j == 6 ? 0 : j]] + off;
+                    const double2* kj = gl(a.K[ctl.perm[j == 6 ? 0 : j]]) + off;
                     yi = cadd(yi, cscal(b[j], kj[k]));
                 }
-                out[k] = cmul(cadd(Y[k], cscal(dt, yi)), cexp_lh(a.linop[off + k], h));
+                out[k] = cmul(cadd(Y[k], cscal(dt, yi)), cexp_lh(gl(a.linop)[off + k], h));
             }
         }
     }
@@ -376,7 +376,7 @@ __global__ __launch_bounds__(LH_T) void rad_saves(const RadArgs* __restrict__ ap
 // ---------------------------------------------------------------- step end
 __global__ void rad_step_end(const RadArgs* __restrict__ ap) {
     const RadArgs& a = *ap;
-    RadCtl* ctl = a.ctl;
+    RadCtl* ctl = gl(a.ctl);
     if (threadIdx.x != 0 || ctl->done) return;
     RadCtl s = *ctl;
     if (s.ok) s.ycur ^= 1;  // y ← yn
