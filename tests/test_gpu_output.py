@@ -94,7 +94,19 @@ def test_hdf5_vs_memory(pkg, tmp_path):
     assert np.array_equal(ro["Eω", slice(None), [0, 1, 49]], mem["Eω"][:, [0, 1, 49]])
 
 
-def test_continuing(pkg, tmp_path):
+def test_continuing(pkg, orc, capi, tmp_path):
+    """The reference's "Continuing" test, with one bound changed: the joined
+    file's spectral intensity is held to 2e-7 of the uninterrupted run's
+    (relative L2) instead of 1e-7.  The restart takes a fresh step controller
+    and re-evaluates k1 (src/Luna.jl:374-379), so the two runs' step sequences
+    part after the cache point, and by how much depends on which step the
+    interruption lands on: the oracle itself, interrupted after the first
+    step beyond 2.5, 3.0, 3.05, 3.1 and 3.5 cm, gives 2.5e-8, 2.6e-8, 2.1e-8,
+    1.07e-7 and 4.8e-8 -- and which step that is depends on rounding and on
+    how many saving steps one streamed call carries (the kernel does not wait
+    for the host, so a call can carry several).  What
+    the resume itself must do is pinned exactly: the resumed saves equal the
+    oracle's resumption from the file's cache to the propagation bar."""
     print("case: HDF5Output interrupted at 3 cm and continued", flush=True)
     from luna_jl_amd import interface, output
     Ew0, grid, linop, tr, sf = _continuation_case(pkg)
@@ -112,6 +124,7 @@ def test_continuing(pkg, tmp_path):
     tc = first["meta", "cache", "t"]
     saved_c = first["meta", "cache", "saved"]
     assert 3e-2 < tc < 5e-2 and 0 < saved_c < 51
+    yc, dtc = np.asarray(first["meta", "cache", "y"]), first["meta", "cache", "dt"]
     # run again: the cache is found and the propagation resumes from it
     o = output.HDF5Output(fpath, 0, grid.zmax, 51, sf)
     assert o.saved == saved_c
@@ -124,7 +137,16 @@ def test_continuing(pkg, tmp_path):
     Ewm = mem["Eω"][idx1:idx2 + 1, :]
     Ewh = o["Eω"][idx1:idx2 + 1, :]
     Iw, Iwm = np.abs(Ewh) ** 2, np.abs(Ewm) ** 2
-    assert np.linalg.norm(Iw - Iwm) / np.linalg.norm(Iw) < 1e-7
+    print("continued vs uninterrupted:", np.linalg.norm(Iw - Iwm) / np.linalg.norm(Iw), flush=True)
+    assert np.linalg.norm(Iw - Iwm) / np.linalg.norm(Iw) < 2e-7
+    # the resumption against the oracle's from the same cache
+    ropts = capi.rk_opts(grid.zmax, z0=tc, init_dz=dtc, save_n=51, save_first=saved_c)
+    rst, _, S_r, _, _ = orc.solve(tr.descriptor(linop), yc.reshape(-1), ropts)
+    assert rst == 0
+    s = grid.sidx
+    for k in range(saved_c, 51):
+        a, b = np.abs(o["Eω"][:, k][s]) ** 2, np.abs(S_r[k][s]) ** 2
+        assert np.linalg.norm(a - b) / np.linalg.norm(b) <= 1e-6, k
     assert np.allclose(Ewm, Ewh, atol=1e-4 * np.max(np.abs(Ewm)), rtol=0)
     assert np.allclose(o["stats"]["ω0"], mem.data["stats"]["ω0"], rtol=1e-6)
     assert np.allclose(o["stats"]["energy"], mem.data["stats"]["energy"], rtol=1e-12)
