@@ -44,7 +44,7 @@ def parse():
                         "the GNLSE supercontinuum; ppt: PPT tables; copy: counter calibration")
     p.add_argument("--n-train", type=int, default=4096)
     p.add_argument("--n-cal", type=int, default=512)
-    p.add_argument("--train-steps", type=int, default=8000)
+    p.add_argument("--train-steps", type=int, default=20000)
     p.add_argument("--batch", type=int, default=32)
     p.add_argument("--bound", type=float, default=1e-2)
     p.add_argument("--lr", type=float, default=1e-3)
@@ -541,7 +541,7 @@ def main():
                                        "FETCH_SIZE + WRITE_SIZE of this launch, copy-calibrated)",
                      "note": "FP64 vector ALU work (the solve kernel issues no MFMA); MI355X's FP64 vector "
                              "peak equals its FP64 matrix peak, 78.6 TFLOP/s",
-                     "kernel": "lh_solve_kernel<0,8192,4096,false> (Nto=16384 class)", "kernel_ms": kernel_ms,
+                     "kernel": "lh_solve_kernel<0, 8192, 4096, false, false, false> (Nto=16384 class)", "kernel_ms": kernel_ms,
                      "flop_per_rhs": rhs_flops(16384, 4097),
                      "algorithmic_bytes": sum(st.rhs_evals * 2 * c.params["n_omega"] * 16 for st, c in big)},
         "cpu_baseline": cpu,
