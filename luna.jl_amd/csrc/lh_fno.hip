@@ -381,12 +381,22 @@ __global__ void fno_colsum_part(const float* __restrict__ X, long long rows, int
             red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] + red[3][threadIdx.x];
 }
 
-__global__ void fno_colsum_final(const float* __restrict__ part, int nchunk, int N, float* __restrict__ out) {
-    const int c = blockIdx.x * FNO_T + threadIdx.x;
-    if (c >= N) return;
+// One workgroup per column: thread t adds chunks t, t + 256, ... in order,
+// then a fixed tree over the threads (the same order on every call).
+__global__ __launch_bounds__(FNO_T) void fno_colsum_final(const float* __restrict__ part, int nchunk, int N,
+                                                          float* __restrict__ out) {
+    __shared__ float red[FNO_T];
+    const int c = blockIdx.x, t = threadIdx.x;
     float s = 0.0f;
-    for (int k = 0; k < nchunk; k++) s += part[(long long)k * N + c];
-    out[c] = s;
+    for (int k = t; k < nchunk; k += FNO_T) s += part[(long long)k * N + c];
+    red[t] = s;
+    __syncthreads();
+#pragma unroll
+    for (int w = FNO_T / 2; w > 0; w >>= 1) {
+        if (t < w) red[t] += red[t + w];
+        __syncthreads();
+    }
+    if (t == 0) out[c] = red[0];
 }
 
 // AdamW on the FP32 master copy; refreshes the BF16 copy.
@@ -534,8 +544,9 @@ static int fno_gemm_launch(lh_fno* f, GemmArgs g, int ta, int tb) {
     const long long tiles = (long long)g.batch * ((g.M + 63) / 64) * ((g.N + 63) / 64);
     int ksplit = 1;
     if (tiles < 256 && g.K >= 1024 && !g.bias) {
-        long long want = 1024 / (tiles > 0 ? tiles : 1);
-        long long maxk = (g.K + 511) / 512;
+        // at least ~512 workgroups (two per CU) in k-chunks of at least 128
+        long long want = 512 / (tiles > 0 ? tiles : 1);
+        long long maxk = (g.K + 127) / 128;
         ksplit = (int)(want < maxk ? want : maxk);
         if (ksplit < 1) ksplit = 1;
         while (ksplit > 1 && (long long)g.batch * ksplit * g.M * g.N > f->work_cap) ksplit--;
@@ -584,8 +595,7 @@ static int colsum(lh_fno* f, const float* X, long long rows, int N, float* out) 
     const int nchunk = (int)((rows + FNO_CS_ROWS - 1) / FNO_CS_ROWS);
     if ((long long)nchunk * N > f->part_cap) return LH_E_ARG;
     hipLaunchKernelGGL(fno_colsum_part, dim3((N + 63) / 64, nchunk), dim3(FNO_T), 0, f->stream, X, rows, N, f->part);
-    hipLaunchKernelGGL(fno_colsum_final, dim3((N + FNO_T - 1) / FNO_T), dim3(FNO_T), 0, f->stream, f->part, nchunk,
-                       N, out);
+    hipLaunchKernelGGL(fno_colsum_final, dim3(N), dim3(FNO_T), 0, f->stream, f->part, nchunk, N, out);
     FNOCK(hipGetLastError());
     return LH_OK;
 }
