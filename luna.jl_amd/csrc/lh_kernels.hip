@@ -2382,6 +2382,8 @@ struct lh_ctx {
     int saves_cap;
     double* zgrid;
     int zgrid_cap;
+    double* zgrid_h;  // host copy of the save grid in zgrid (an unchanged grid is not uploaded again)
+    int zgrid_hn;
     lh_stats* dstats;
     DInst* dinst;
     double2* rhs_in;
@@ -2737,6 +2739,7 @@ static int ensure_solver_buffers(lh_ctx* c, int save_n, bool want_saves) {
     if (zn > c->zgrid_cap) {
         if (c->zgrid) HIPCK(hipFree(c->zgrid));
         c->zgrid = nullptr;
+        c->zgrid_hn = 0;
         HIPCK(hipMalloc(&c->zgrid, sizeof(double) * (size_t)zn));
         c->zgrid_cap = zn;
     }
@@ -2791,8 +2794,22 @@ static int prepare_instance(lh_ctx* c, const lh_rk_opts* o, bool want_saves, DIn
     if (st) return st;
     std::vector<double> g;
     save_grid(o, g);
-    HIPCK(hipMemcpyAsync(c->zgrid, g.data(), sizeof(double) * g.size(), hipMemcpyHostToDevice, s));
-    HIPCK(hipStreamSynchronize(s));  // g is a local buffer
+    // the kernels only read the grid: a sweep repeated with the same options
+    // skips the upload (and its synchronisation) for every instance
+    const bool same = c->zgrid_h && c->zgrid_hn == (int)g.size() &&
+                      memcmp(c->zgrid_h, g.data(), sizeof(double) * g.size()) == 0;
+    if (!same) {
+        HIPCK(hipMemcpyAsync(c->zgrid, g.data(), sizeof(double) * g.size(), hipMemcpyHostToDevice, s));
+        HIPCK(hipStreamSynchronize(s));  // g is a local buffer
+        double* h = (double*)realloc(c->zgrid_h, sizeof(double) * g.size());
+        if (h) {
+            memcpy(h, g.data(), sizeof(double) * g.size());
+            c->zgrid_h = h;
+            c->zgrid_hn = (int)g.size();
+        } else {
+            c->zgrid_hn = 0;
+        }
+    }
     *in = base_inst(c);
     in->Yin = c->Yin;
     in->Y = c->Y;
@@ -3022,6 +3039,7 @@ int lh_destroy(lh_ctx* c) {
     for (void* p : rptrs)
         if (p) hipFree(p);
     if (c->rgraph) hipGraphExecDestroy(c->rgraph);
+    free(c->zgrid_h);
     if (c->stream) hipStreamDestroy(c->stream);
     free(c);
     return LH_OK;
@@ -3350,6 +3368,22 @@ int lh_get_saves(lh_ctx* c, double* saves, double* zsaves) {
     return LH_OK;
 }
 
+// One stream per (device, size class) for the batched launches, created on
+// first use and kept for the process (default priority).
+static std::mutex g_cls_mu;
+static hipStream_t g_cls_stream[64][LH_NCLASS];
+
+static int class_stream(int dev, int c, hipStream_t* out) {
+    if (dev < 0 || dev >= 64) return LH_E_ARG;
+    std::lock_guard<std::mutex> lk(g_cls_mu);
+    if (!g_cls_stream[dev][c] && hipStreamCreate(&g_cls_stream[dev][c]) != hipSuccess) {
+        g_cls_stream[dev][c] = nullptr;
+        return LH_E_HIP;
+    }
+    *out = g_cls_stream[dev][c];
+    return LH_OK;
+}
+
 // The resident batch with options per instance: op[i] for instance i.
 static int batch_resident(lh_ctx* const* ctxs, int n, const lh_rk_opts* const* op, int want_saves, lh_stats* stats) {
     if (!ctxs || n <= 0 || !op || !stats) return LH_E_ARG;
@@ -3395,7 +3429,7 @@ static int batch_resident(lh_ctx* const* ctxs, int n, const lh_rk_opts* const* o
     if (!st && hipEventRecord(ready, s0) != hipSuccess) st = LH_E_HIP;
     for (int c = 0; c < LH_NCLASS && !st; c++) {
         if (!count[c]) continue;
-        if (hipStreamCreate(&cs[c]) != hipSuccess || hipEventCreateWithFlags(&done[c], hipEventDisableTiming) != hipSuccess ||
+        if (class_stream(dev, c, &cs[c]) != LH_OK || hipEventCreateWithFlags(&done[c], hipEventDisableTiming) != hipSuccess ||
             hipStreamWaitEvent(cs[c], ready, 0) != hipSuccess) {
             st = LH_E_HIP;
             break;
@@ -3445,7 +3479,6 @@ static int batch_resident(lh_ctx* const* ctxs, int n, const lh_rk_opts* const* o
         }
         if (t0[c]) (void)hipEventDestroy(t0[c]);
         if (t1[c]) (void)hipEventDestroy(t1[c]);
-        if (cs[c]) (void)hipStreamDestroy(cs[c]);
         if (done[c]) (void)hipEventDestroy(done[c]);
     }
     if (ready) (void)hipEventDestroy(ready);
