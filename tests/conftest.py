@@ -34,6 +34,14 @@ def orc(capi):
 
 
 @pytest.fixture(scope="session")
+def refvals():
+    """Values the reference's own tests hold (tests/golden/reference_values.json)."""
+    import json
+    with open(os.path.join(ROOT, "tests", "golden", "reference_values.json")) as f:
+        return {k: v["value"] for k, v in json.load(f).items()}
+
+
+@pytest.fixture(scope="session")
 def cfg0(pkg):
     from luna_jl_amd import configs
     return configs.capillary_instance()
