@@ -45,8 +45,10 @@ enum {
                               step (the cached-continuation test,
                               test/test_output.jl:203-276); resume from the
                               returned state, z, dz and saved count          */
-    LH_E_TOO_LARGE = 10    /* the problem does not fit the fixed working
+    LH_E_TOO_LARGE = 10,   /* the problem does not fit the fixed working
                               memory of lh_solve_noalloc                     */
+    LH_E_HOST_WAIT = 11    /* lh_solve_stream with every_step: the host did
+                              not take a published step within a minute      */
 };
 
 enum { LH_GRID_REAL = 0, LH_GRID_ENV = 1 };           /* Grid.RealGrid / Grid.EnvGrid */
@@ -112,7 +114,11 @@ typedef struct lh_rk_opts {
                                  cache continues GridCondition at this index, as
                                  HDF5Output's `saved` (src/Output.jl:170-176,
                                  473-487; Luna.run src/Luna.jl:374-379) (0)      */
-    int32_t reserved;         /* 0                                               */
+    int32_t every_step;       /* lh_solve_stream: 1 hands every accepted step to
+                                 the callback, as the reference calls its output
+                                 at each (src/Luna.jl:366-372, src/Output.jl:
+                                 65-81), the solve waiting for the host to take
+                                 each; 0: only steps that save, never waiting (0) */
     double zstop;             /* > 0: end with LH_E_STOPPED after the first
                                  accepted step with tn > zstop (its window filter
                                  and saves done); 0: run to zmax                 */
@@ -165,18 +171,24 @@ int lh_solve(lh_ctx *ctx, double *Ew_inout, const lh_rk_opts *opts,
 /* Per-step output while the device solve runs: the output callable of
  * Luna.run's stepfun, output(y, z, dz, interpolant) (src/Luna.jl:366-372,
  * src/Output.jl:65-81, 262-302), for outputs other than a fixed save buffer
- * (HDF5Output, user callbacks).  The kernel publishes every accepted step that
- * writes a GridCondition save; the host hands each publication to cb on the
- * calling thread while the kernel goes on.  One call carries the saves written
- * since the previous call, the state, z and dz of the latest saving step
- * (HDF5Output's cache: t, dt, y, saved) and the statistics records of the
- * steps since the previous call.  Pointers are valid during the call only. */
+ * (HDF5Output, user callbacks, statistics evaluated on the host).  The kernel
+ * publishes every accepted step that writes a GridCondition save (with
+ * lh_rk_opts.every_step, every accepted step); the host hands each publication
+ * to cb on the calling thread while the kernel goes on.  One call carries the
+ * saves written since the previous call, the state, z and dz of the latest
+ * published step (HDF5Output's cache: t, dt, y, saved) and the statistics
+ * records of the steps since the previous call.  Without every_step a
+ * publication the host has not taken before the next one is folded into it
+ * (its saves and records are still handed over); with every_step each is
+ * handed over on its own, the kernel waiting (up to a minute, then
+ * LH_E_HOST_WAIT) while the host is two publications behind.  Pointers are
+ * valid during the call only. */
 typedef struct lh_step_out {
     int32_t first_save;       /* index of the first save in this call           */
-    int32_t n_saves;          /* saves in this call (>= 1)                      */
+    int32_t n_saves;          /* saves in this call (>= 1; >= 0 with every_step) */
     const double *saves;      /* complex [n_saves][n_omega*n_r]                 */
     const double *zsaves;     /* [n_saves] their positions                      */
-    double z;                 /* tn of the latest saving step                   */
+    double z;                 /* tn of the latest published step                */
     double dz;                /* dtn after it                                   */
     const double *y;          /* complex [n_omega*n_r]: the filtered state at z */
     int64_t accepted;         /* accepted steps up to that step                 */
@@ -187,13 +199,15 @@ typedef struct lh_step_out {
 
 /* Return nonzero to stop the run (as an error thrown by the reference's
  * output or stepfun ends the propagation): nothing more is handed over, the
- * kernel ends after its next saving step with LH_E_STOPPED. */
+ * solve ends after its next published step with LH_E_STOPPED. */
 typedef int (*lh_output_cb)(void *user, const lh_step_out *out);
 
 /* lh_solve with the saves and records streamed to cb instead of filled into
  * buffers; same trajectory, bit for bit.  Mode-averaged contexts with a
- * constant fill (LH_E_ARG for radial contexts and z-dependent fills, whose
- * outputs are filled from lh_solve's buffers after the run). */
+ * constant fill, and radial contexts (whose steps the host replays one at a
+ * time here, copying each published state out of device memory); LH_E_ARG for
+ * z-dependent fills, whose outputs are filled from lh_solve's buffers after
+ * the run. */
 int lh_solve_stream(lh_ctx *ctx, double *Ew_inout, const lh_rk_opts *opts, lh_output_cb cb, void *user,
                     lh_stats *stats);
 

@@ -10,7 +10,7 @@ import os
 import numpy as np
 
 (LH_OK, LH_E_ARG, LH_E_HIP, LH_E_PPT_RANGE, LH_E_REPEAT_LIMIT, LH_E_NONFINITE, LH_E_NOMEM, LH_E_NODEVICE,
- LH_E_MAXSTEPS, LH_E_STOPPED, LH_E_TOO_LARGE) = range(11)
+ LH_E_MAXSTEPS, LH_E_STOPPED, LH_E_TOO_LARGE, LH_E_HOST_WAIT) = range(12)
 LH_NOALLOC_SCRATCH_BYTES = 1536 * 1024
 LH_GRID_REAL, LH_GRID_ENV = 0, 1
 LH_TRANS_MODEAVG, LH_TRANS_RADIAL = 0, 1
@@ -46,7 +46,7 @@ class LhRkOpts(C.Structure):
         ("z0", C.c_double), ("zmax", C.c_double), ("rtol", C.c_double), ("atol", C.c_double),
         ("safety", C.c_double), ("max_dz", C.c_double), ("min_dz", C.c_double),
         ("init_dz", C.c_double), ("repeat_limit", C.c_int32), ("save_n", C.c_int32),
-        ("save_z0", C.c_double), ("save_first", C.c_int32), ("reserved", C.c_int32),
+        ("save_z0", C.c_double), ("save_first", C.c_int32), ("every_step", C.c_int32),
         ("zstop", C.c_double), ("max_steps", C.c_int64),
     ]
 
@@ -148,16 +148,20 @@ class DescHolder:
 
 
 def rk_opts(zmax, z0=0.0, rtol=1e-6, atol=1e-10, safety=0.9, max_dz=None, min_dz=0.0,
-            init_dz=1e-4, repeat_limit=10, save_n=201, save_z0=0.0, save_first=0, zstop=0.0, max_steps=0):
+            init_dz=1e-4, repeat_limit=10, save_n=201, save_z0=0.0, save_first=0, zstop=0.0, max_steps=0,
+            every_step=False):
     """``Luna.run`` defaults, ``src/Luna.jl:358-362``; the save grid
     ``range(save_z0, zmax, save_n)`` is MemoryOutput's/HDF5Output's
     ``GridCondition(0, zmax, saveN)``; ``save_first``/``zstop`` resume and stop
-    a run (HDF5Output's cache, ``src/Output.jl:473-487``)."""
+    a run (HDF5Output's cache, ``src/Output.jl:473-487``); ``every_step``:
+    ``lh_solve_stream`` hands over every accepted step, not only those that
+    save."""
     o = LhRkOpts()
     o.z0, o.zmax, o.rtol, o.atol, o.safety = z0, zmax, rtol, atol, safety
     o.max_dz = zmax / 2 if max_dz is None else max_dz
     o.min_dz, o.init_dz, o.repeat_limit, o.save_n = min_dz, init_dz, repeat_limit, save_n
     o.save_z0, o.save_first, o.zstop, o.max_steps = save_z0, save_first, zstop, max_steps
+    o.every_step = 1 if every_step else 0
     return o
 
 

@@ -115,6 +115,11 @@ struct DInst {
     double* pub_info;        // [2][4]
     double2* pub_y;          // [2][n_state]
     const int* pub_stop;     // set by the host: end with LH_E_STOPPED at the next publication
+    // every-step mode (pub_wait > 0): every accepted step publishes, and
+    // publication k waits until the host has taken k - 2 (pub_ack, written by
+    // the host), for at most pub_wait wall-clock ticks
+    const unsigned long long* pub_ack;
+    long long pub_wait;
 };
 
 // Dormand–Prince tableau, src/dopri.jl:1-28 (rational literals evaluated in double).
@@ -2112,7 +2117,28 @@ __device__ __forceinline__ void solve_instance(const DInst& in, Smem& sm) {
                 }
                 saved++;
             }
-            if (STREAM && saved > saved_before) {
+            bool late = false;
+            if (STREAM && in.pub_wait > 0 && npub >= 2) {
+                // every-step mode: slot npub & 1 is free once the host has
+                // taken publication npub - 2
+                if (threadIdx.x == 0) {
+                    const long long w0 = wall_clock64();
+                    while (__hip_atomic_load(in.pub_ack, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM) + 1 <
+                           (unsigned long long)npub) {
+                        if (wall_clock64() - w0 > in.pub_wait) {
+                            sm.flag = LH_E_HOST_WAIT;
+                            break;
+                        }
+                        __builtin_amdgcn_s_sleep(32);
+                    }
+                }
+                __syncthreads();
+                if (sm.flag == LH_E_HOST_WAIT) {
+                    status = LH_E_HOST_WAIT;
+                    late = true;
+                }
+            }
+            if (STREAM && !late && (saved > saved_before || in.pub_wait > 0)) {
                 // the output call of this step (src/Output.jl:65-81, 262-302):
                 // its saves are in place; hand over the state, then publish
                 const int slot = (int)(npub & 1);
@@ -2767,7 +2793,7 @@ static void save_grid(const lh_rk_opts* o, std::vector<double>& g) {
 
 static bool opts_ok(const lh_rk_opts* o) {
     return o->save_n >= 0 && o->repeat_limit >= 0 && o->rtol > 0 && o->zmax >= o->z0 && o->save_first >= 0 &&
-           o->save_first <= o->save_n && o->max_steps >= 0 && o->zstop >= 0;
+           o->save_first <= o->save_n && o->max_steps >= 0 && o->zstop >= 0 && (o->every_step == 0 || o->every_step == 1);
 }
 
 static void inst_options(DInst* in, const lh_rk_opts* o) {
@@ -2913,10 +2939,24 @@ static RadArgs rad_args(const lh_ctx* c, int slot) {
     return a;
 }
 
+// Per-step output of a radial run (lh_solve_stream): the steps are replayed
+// one at a time and each accepted step that saves (every accepted step with
+// every) is handed to cb with the saves since the previous call and the state,
+// copied out of device memory into page-locked host buffers.
+struct RadHook {
+    lh_output_cb cb;
+    void* user;
+    bool every;
+    double2* y_h;          // [n_state]
+    double2* sv_h;         // [save_n][n_state]
+    const double* zgrid;   // [save_n] host save grid
+};
+
 // Full propagation of one radial context on stream s (buffers and save grid
 // prepared by prepare_instance).  The per-step launch sequence is captured
 // once per context as a graph and replayed until the control block says done.
-static int rad_solve(lh_ctx* c, const lh_rk_opts* o, bool want_saves, hipStream_t s, lh_stats* out) {
+static int rad_solve(lh_ctx* c, const lh_rk_opts* o, bool want_saves, hipStream_t s, lh_stats* out,
+                     const RadHook* hook = nullptr) {
     const double2* tw;
     int st = twiddles(c->device, &tw);
     if (st) return st;
@@ -2970,13 +3010,49 @@ static int rad_solve(lh_ctx* c, const lh_rk_opts* o, bool want_saves, hipStream_
             return LH_E_HIP;
         }
     }
-    // replay in chunks; steps after completion return at once
-    int chunk = 4;
+    // replay in chunks (one step at a time with a hook); steps after
+    // completion return at once
+    int chunk = hook ? 1 : 4;
+    long long acc_done = 0;
+    int saves_done = o->save_first;
+    bool stop_req = false;
+    const size_t NS = (size_t)c->n_state;
     while (!h.done) {
         for (int i = 0; i < chunk; i++) HIPCK(hipGraphLaunch(c->rgraph, s));
         HIPCK(hipMemcpyAsync(&h, c->rctl, sizeof(h), hipMemcpyDeviceToHost, s));
         HIPCK(hipStreamSynchronize(s));
-        if (chunk < 16) chunk *= 2;
+        if (!hook) {
+            if (chunk < 16) chunk *= 2;
+            continue;
+        }
+        if (h.accepted == acc_done) continue;
+        acc_done = h.accepted;
+        if (!stop_req && (hook->every || h.saved > saves_done)) {
+            const int nsv = h.saved - saves_done;
+            if (nsv > 0)
+                HIPCK(hipMemcpyAsync(hook->sv_h + (size_t)saves_done * NS, c->saves + (size_t)saves_done * NS,
+                                     sizeof(double2) * NS * (size_t)nsv, hipMemcpyDeviceToHost, s));
+            HIPCK(hipMemcpyAsync(hook->y_h, h.ycur ? c->YN : c->Y, sizeof(double2) * NS, hipMemcpyDeviceToHost, s));
+            HIPCK(hipStreamSynchronize(s));
+            lh_step_out so;
+            memset(&so, 0, sizeof(so));
+            so.first_save = saves_done;
+            so.n_saves = nsv;
+            so.saves = (const double*)(hook->sv_h + (size_t)saves_done * NS);
+            so.zsaves = hook->zgrid + saves_done;
+            so.z = h.tn;
+            so.dz = h.dtn;
+            so.y = (const double*)hook->y_h;
+            so.accepted = h.accepted;
+            if (hook->cb(hook->user, &so) != 0) stop_req = true;
+        }
+        saves_done = h.saved;
+        if (stop_req && !h.done) {
+            // nothing more is handed over; the run ends here
+            h.status = LH_E_STOPPED;
+            h.done = 1;
+            HIPCK(hipMemcpy(c->rctl, &h, sizeof(h), hipMemcpyHostToDevice));
+        }
     }
     if (h.ycur == 1)  // the final state s.yn lives in YN
         HIPCK(hipMemcpyAsync(c->Y, c->YN, sizeof(double2) * (size_t)c->n_state, hipMemcpyDeviceToDevice, s));
@@ -3007,6 +3083,7 @@ const char* lh_strerror(int status) {
         case LH_E_MAXSTEPS: return "reached the cap on step attempts";
         case LH_E_STOPPED: return "stopped before the end of the propagation (zstop)";
         case LH_E_TOO_LARGE: return "problem exceeds the fixed working memory of lh_solve_noalloc";
+        case LH_E_HOST_WAIT: return "the host did not take a published step within the wait limit";
         default: return "unknown status";
     }
 }
@@ -3217,18 +3294,57 @@ int lh_solve(lh_ctx* c, double* Ew, const lh_rk_opts* o, double* saves, double* 
     return sp->status;
 }
 
+// lh_solve_stream of a radial context: the host replays the steps (RadHook).
+static int rad_solve_stream(lh_ctx* c, double* Ew, const lh_rk_opts* o, lh_output_cb cb, void* user,
+                            lh_stats* stats) {
+    const size_t NS = (size_t)c->n_state;
+    const size_t nsave = (size_t)(o->save_n > 0 ? o->save_n : 0);
+    double2* y_h = nullptr;
+    double2* sv_h = nullptr;
+    HIPCK(hipHostMalloc((void**)&y_h, sizeof(double2) * NS, 0));
+    if (nsave && hipHostMalloc((void**)&sv_h, sizeof(double2) * NS * nsave, 0) != hipSuccess) {
+        (void)hipHostFree(y_h);
+        return LH_E_NOMEM;
+    }
+    std::vector<double> zg;
+    save_grid(o, zg);
+    DInst in;
+    int st = prepare_instance(c, o, true, &in, c->dstats, c->stream);
+    RadHook hk{cb, user, o->every_step != 0, y_h, sv_h, zg.data()};
+    if (!st) st = rad_solve(c, o, true, c->stream, stats, &hk);
+    if (!st && hipMemcpy(Ew, c->Y, sizeof(double2) * NS, hipMemcpyDeviceToHost) != hipSuccess) st = LH_E_HIP;
+    (void)hipHostFree(y_h);
+    if (sv_h) (void)hipHostFree(sv_h);
+    if (st) return st;
+    c->last_status = stats->status;
+    c->have_result = 1;
+    c->result_save_n = 0;  // the saves went to the callback
+    c->result_nrec = 0;
+    c->have_records = 1;
+    return stats->status;
+}
+
+// Wall-clock ticks of the every-step wait (one minute).
+static long long host_wait_ticks(int dev) {
+    int khz = 0;
+    if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess || khz <= 0) khz = 100000;
+    return (long long)khz * 1000LL * 60LL;
+}
+
 int lh_solve_stream(lh_ctx* c, double* Ew, const lh_rk_opts* o, lh_output_cb cb, void* user, lh_stats* stats) {
     if (!c || !Ew || !o || !cb || !stats || !opts_ok(o)) return LH_E_ARG;
     const int cls = ctx_class(c);
-    if (cls < 0 || cls == LH_CLASS_RADIAL || c->zg_on) return LH_E_ARG;
+    if (cls < 0 || c->zg_on) return LH_E_ARG;
     int st = lh_set_state(c, Ew);
     if (st) return st;
+    if (cls == LH_CLASS_RADIAL) return rad_solve_stream(c, Ew, o, cb, user, stats);
     const double2* tw;
     st = twiddles(c->device, &tw);
     if (st) return st;
-    // one page-locked, device-mapped, coherent block: publication counter and
-    // stop flag, slot info, two state slots, every save and its position, the
-    // statistics records
+    // one page-locked, device-mapped, coherent block: publication counter,
+    // stop flag and (every-step mode) the host's count of publications taken,
+    // slot info, two state slots, every save and its position, the statistics
+    // records
     const size_t NS = (size_t)c->n_state;
     const size_t nsave = (size_t)(o->save_n > 0 ? o->save_n : 0);
     const size_t ncap = (size_t)(c->st_on ? c->st_cap : 0);
@@ -3244,6 +3360,8 @@ int lh_solve_stream(lh_ctx* c, double* Ew, const lh_rk_opts* o, lh_output_cb cb,
     memset(hb, 0, off_y);
     unsigned long long* pub = (unsigned long long*)hb;
     int* stop = (int*)(hb + 64);
+    unsigned long long* ack = (unsigned long long*)(hb + 128);
+    const bool every = o->every_step != 0;
     const double* info = (const double*)(hb + off_info);
     const double2* ys = (const double2*)(hb + off_y);
     const double2* sv = (const double2*)(hb + off_sv);
@@ -3259,6 +3377,8 @@ int lh_solve_stream(lh_ctx* c, double* Ew, const lh_rk_opts* o, lh_output_cb cb,
         in.pub_stop = (const int*)(db + 64);
         in.pub_info = (double*)(db + off_info);
         in.pub_y = (double2*)(db + off_y);
+        in.pub_ack = (const unsigned long long*)(db + 128);
+        in.pub_wait = every ? host_wait_ticks(c->device) : 0;
         if (hipMemcpyAsync(c->dinst, &in, sizeof(DInst), hipMemcpyHostToDevice, c->stream) != hipSuccess) st = LH_E_HIP;
     }
     if (!st) {
@@ -3268,8 +3388,10 @@ int lh_solve_stream(lh_ctx* c, double* Ew, const lh_rk_opts* o, lh_output_cb cb,
     // Drain the publications while the kernel runs.  A publication's state
     // slot is rewritten two publications later, so a copy of it counts only
     // if the counter has not moved on meanwhile; saves and records have slots
-    // of their own.  After the callback asks to stop, nothing more is handed
-    // over (the run ends at its next publication).
+    // of their own.  In every-step mode the kernel does not rewrite a slot
+    // before the host has taken it: each publication is handed over in turn
+    // and then counted in ack.  After the callback asks to stop, nothing more
+    // is handed over (the run ends at its next publication).
     std::vector<double2> ycopy(NS);
     unsigned long long done = 0;
     int saves_done = o->save_first;
@@ -3282,7 +3404,35 @@ int lh_solve_stream(lh_ctx* c, double* Ew, const lh_rk_opts* o, lh_output_cb cb,
             break;
         }
         unsigned long long p = __atomic_load_n(pub, __ATOMIC_ACQUIRE);
-        if (p > done && !stop_req) {
+        while (every && p > done) {
+            const int slot = (int)(done & 1);  // publication done + 1
+            const int nsaved = (int)info[4 * slot + 2];
+            const long long nacc = (long long)info[4 * slot + 3];
+            const long long nrec = ncap ? (nacc < (long long)ncap ? nacc : (long long)ncap) : 0;
+            if (!stop_req) {
+                lh_step_out out;
+                out.first_save = saves_done;
+                out.n_saves = nsaved - saves_done;
+                out.saves = (const double*)(sv + (size_t)saves_done * NS);
+                out.zsaves = zg.data() + saves_done;
+                out.z = info[4 * slot];
+                out.dz = info[4 * slot + 1];
+                out.y = (const double*)(ys + (size_t)slot * NS);
+                out.accepted = nacc;
+                out.first_record = (int32_t)recs_done;
+                out.n_records = (int32_t)(nrec - recs_done);
+                out.records = recs + (size_t)recs_done * LH_NSTAT;
+                if (cb(user, &out) != 0) {
+                    stop_req = true;
+                    __atomic_store_n(stop, 1, __ATOMIC_RELEASE);
+                }
+            }
+            done++;
+            saves_done = nsaved;
+            recs_done = nrec;
+            __atomic_store_n(ack, done, __ATOMIC_RELEASE);
+        }
+        if (!every && p > done && !stop_req) {
             double z = 0, dz = 0, fsaved = 0, facc = 0;
             for (;;) {
                 const int slot = (int)((p - 1) & 1);

@@ -86,6 +86,22 @@ def energy_t(grid, Et):
     return simpson_even(dt, It)
 
 
+def energyfuncs(grid, q=None):
+    """``Fields.energyfuncs(grid[, q])`` (``src/Fields.jl:510-565``): the pair
+    (energy_t(Et), energy_ω(Eω)); with a QDHT q the radial forms, Et and Eω of
+    shape (N, Nr) in real and reciprocal space."""
+    dw = grid.omega[1] - grid.omega[0]
+    if q is None:
+        if grid.kind == "real":
+            pre = 2 * math.pi / grid.omega[-1] ** 2
+            return (lambda Et: energy_t(grid, Et)), (lambda Ew: pre * simpson_even(dw, np.abs(Ew) ** 2))
+        pre = 2 * math.pi * dw / (len(grid.omega) * dw) ** 2
+        return (lambda Et: energy_t(grid, Et)), (lambda Ew: pre * np.sum(np.abs(Ew) ** 2))
+    if grid.kind != "real":
+        raise NotImplementedError("radial energy functions of an envelope grid are not provided")
+    return (lambda Et: energy_t_radial(grid, q, Et)), (lambda Ew: energy_omega_radial(grid, q, Ew))
+
+
 # --------------------------------------------------------------------------
 # Radially symmetric inputs on a QDHT grid (field grid only)
 # --------------------------------------------------------------------------

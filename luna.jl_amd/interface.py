@@ -25,6 +25,7 @@ from .nonlinear import (CombinedRamanResponse, Kerr_env, Kerr_field, PlasmaCumtr
                         RamanPolarField, RamanRespNormedSingleDampedOscillator)
 from .nonlinearrhs import TransModeAvg, TransRadial, norm_mode_average, norm_mode_average_gnlse
 from .output import HDF5Output, MemoryOutput  # noqa: F401 (re-exported: Output.MemoryOutput)
+from .stats import StepRecords
 
 log = logging.getLogger("luna_jl_amd")
 
@@ -42,7 +43,9 @@ def run(Ew, grid, linop, transform, FT=None, output=None, min_dz=0.0, max_dz=Non
     GPU.  A cached propagation in the output (``HDF5Output``) is resumed from
     its cache; the grid and simulation type are written to the output; the
     device solve fills a ``MemoryOutput`` from its save buffer and streams
-    every saving step to any other output (``lh_solve_stream``)."""
+    every saving step to any other output (``lh_solve_stream``).  Statistics
+    evaluated on the host (``collect_stats``, ``Stats.default``'s userfuns)
+    stream every accepted step and are evaluated on its filtered state."""
     if output is None:
         output = MemoryOutput(0, grid.zmax, 201)
     save_first = 0
@@ -53,17 +56,21 @@ def run(Ew, grid, linop, transform, FT=None, output=None, min_dz=0.0, max_dz=Non
         save_first = output.saved
     output(grid.to_dict(), group="grid")
     output(simtype(grid, transform, linop), group="simulation_type")
+    statsfun = output.statsfun
+    host = statsfun is not None and getattr(statsfun, "has_host", False)
+    zdep = getattr(transform, "gradient", None) is not None
+    if host and zdep:
+        raise NotImplementedError("statistics evaluated on the host are not provided with a z-dependent fill")
     opts = _capi.rk_opts(grid.zmax, z0=z0, rtol=rtol, atol=atol, safety=safety,
                          max_dz=grid.zmax / 2 if max_dz is None else max_dz, min_dz=min_dz,
-                         init_dz=init_dz, save_n=output.saveN, save_z0=output.tmin, save_first=save_first)
+                         init_dz=init_dz, save_n=output.saveN, save_z0=output.tmin, save_first=save_first,
+                         every_step=host)
     ctx = transform.context(linop)
-    statsfun = output.statsfun
-    ctx.set_step_stats(statsfun)
+    ctx.set_step_stats(statsfun if hasattr(statsfun, "desc") else None)
     shape = (grid.n_omega,) + ((transform.n_r,) if transform.n_r > 1 else ())
-    if (isinstance(output, MemoryOutput) or transform.kind == _capi.LH_TRANS_RADIAL
-            or getattr(transform, "gradient", None) is not None):
+    if not host and (isinstance(output, MemoryOutput) or zdep):
         # the whole run in the device save buffer, handed over at the end
-        # (radial runs and z-dependent fills are not streamed)
+        # (z-dependent fills are not streamed)
         Ew_out, saves, zs, stats = ctx.solve(Ew, opts)
         _capi.check(stats.status, "Luna.run")
         rec = ctx.step_stats() if statsfun is not None else np.zeros((0, _capi.LH_NSTAT))
@@ -75,12 +82,15 @@ def run(Ew, grid, linop, transform, FT=None, output=None, min_dz=0.0, max_dz=Non
     else:
         def on_step(first, saves, zsv, z, dz, y, records):
             saves = saves.reshape(shape + (saves.shape[-1],), order="F")
-            return output.on_step(first, saves, zsv, z, dz, y.reshape(shape, order="F"), records)
+            y = y.reshape(shape, order="F")
+            if host:  # one accepted step per call
+                records = StepRecords(records, [statsfun.host_step(y, z, dz)])
+            return output.on_step(first, saves, zsv, z, dz, y, records)
 
         Ew_out, stats = ctx.solve_stream(Ew, opts, on_step)
         if stats.status != _capi.LH_E_STOPPED:
             _capi.check(stats.status, "Luna.run")
-    if statsfun is not None and stats.accepted > statsfun.capacity:
+    if hasattr(statsfun, "desc") and stats.accepted > statsfun.capacity:
         warnings.warn(f"statistics kept for the first {statsfun.capacity} of {stats.accepted} steps "
                       "(raise the capacity of stats.default)", RuntimeWarning)
     output("solver", stats.as_dict(), meta=True, force=True)

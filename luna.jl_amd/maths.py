@@ -128,3 +128,37 @@ def hilbert(x, axis=0):
     xf[1:n1] *= 2
     xf[n1:] = 0
     return np.moveaxis(np.fft.ifft(xf, axis=0), 0, axis)
+
+
+def fwhm(x, y, minmax="min"):
+    """``Maths.fwhm(x, y; method=:linear, minmax)`` (``src/Maths.jl:128-214``)
+    at level ½·max(y): the crossings nearest the (first) maximum for "min",
+    the outermost for "max", each interpolated linearly between the samples
+    either side; NaN where a crossing or its neighbour is missing (the
+    reference's try/catch)."""
+    x = np.asarray(x, dtype=float)
+    y = np.asarray(y, dtype=float)
+    if minmax not in ("min", "max"):
+        raise ValueError("minmax has to be min or max")
+    i0 = int(np.argmax(y))
+    val = 0.5 * y[i0]
+    xm = x[i0]
+    n = len(x)
+
+    def xing(i1, i2):  # linterpx: x where the line through (x1, y1), (x2, y2) reaches val
+        slope = (y[i2] - y[i1]) / (x[i2] - x[i1])
+        return (val - (y[i1] - slope * x[i1])) / slope
+
+    if minmax == "min":
+        lo = np.flatnonzero((x < xm) & (y < val))
+        hi = np.flatnonzero((x > xm) & (y < val))
+        if len(lo) == 0 or len(hi) == 0:
+            return math.nan
+        li, ri = lo[-1], hi[0]
+        return abs(xing(ri - 1, ri) - xing(li, li + 1))
+    lo = np.flatnonzero((x < xm) & (y > val))
+    hi = np.flatnonzero((x > xm) & (y > val))
+    if len(lo) == 0 or len(hi) == 0 or lo[0] < 1 or hi[-1] + 1 >= n:
+        return math.nan
+    li, ri = lo[0], hi[-1]
+    return abs(xing(ri, ri + 1) - xing(li - 1, li))

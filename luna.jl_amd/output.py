@@ -234,6 +234,8 @@ class _OutputBase:
     def _stats_names(self):
         if self.statsfun is None:
             return []
+        if hasattr(self.statsfun, "names"):
+            return self.statsfun.names()
         return list(self.statsfun.to_dict(np.zeros((0, _nstat()))).keys())
 
     def check_cache(self, y, t, dt):

@@ -91,11 +91,12 @@ class Context:
 
     def solve_stream(self, Ew0, opts, on_step):
         """``lh_solve_stream``: the device solve with every accepted step that
-        saves handed to ``on_step(first_save, saves, zs, z, dz, y, records)``
-        while the kernel runs (saves: (n_state, n) column per save; y: the
-        filtered state at z; records: [n][LH_NSTAT] statistics of the steps
-        since the previous call).  ``on_step`` returning True stops the run.
-        An exception raised by ``on_step`` stops it too and is re-raised."""
+        saves (every accepted step with ``opts.every_step``) handed to
+        ``on_step(first_save, saves, zs, z, dz, y, records)`` while the kernel
+        runs (saves: (n_state, n) column per save; y: the filtered state at z;
+        records: [n][LH_NSTAT] statistics of the steps since the previous
+        call).  ``on_step`` returning True stops the run.  An exception raised
+        by ``on_step`` stops it too and is re-raised."""
         Ew = np.ascontiguousarray(np.asarray(Ew0, dtype=np.complex128).reshape(-1, order="F")).copy()
         ns = self.n_state
         failure = []
@@ -103,9 +104,13 @@ class Context:
         def cb(user, out):
             try:
                 o = out.contents
-                sv = np.ctypeslib.as_array(o.saves, shape=(o.n_saves * ns * 2,)).view(np.complex128)
-                saves = sv.reshape(o.n_saves, ns).T.copy()
-                zs = np.ctypeslib.as_array(o.zsaves, shape=(o.n_saves,)).copy()
+                if o.n_saves > 0:
+                    sv = np.ctypeslib.as_array(o.saves, shape=(o.n_saves * ns * 2,)).view(np.complex128)
+                    saves = sv.reshape(o.n_saves, ns).T.copy()
+                    zs = np.ctypeslib.as_array(o.zsaves, shape=(o.n_saves,)).copy()
+                else:  # an accepted step that saves nothing (every_step)
+                    saves = np.zeros((ns, 0), dtype=np.complex128)
+                    zs = np.zeros(0)
                 y = np.ctypeslib.as_array(o.y, shape=(2 * ns,)).view(np.complex128).copy()
                 if o.n_records > 0:
                     rec = np.ctypeslib.as_array(o.records, shape=(o.n_records * _capi.LH_NSTAT,))

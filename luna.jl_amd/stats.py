@@ -27,8 +27,17 @@ mode-averaged ones with Aeff replaced by the mode's P/I(0) (``aeff_onaxis``).
 ``pressure`` (``:335-339``, PhysData.pressure of the density) is filled on the
 host from each record's density with the gas model, and ``zdw`` (``zdw_linop``,
 ``:571-582``: ``Modes.zdw``, a root search on the mode's dispersion) from the
-mode: once for a constant fill, at each record's z for a gradient fill.  The
-``userfuns`` option is not provided.
+mode: once for a constant fill, at each record's z for a gradient fill.
+
+``userfuns`` (functions f(d, Eω, Et, z, dz) that add entries to d, ``:523-529``)
+and ``collect_stats`` (``:476-493``) with any functions — the statistics of a
+radial propagation, which ``Stats.default`` does not cover — run on the host:
+the solve then hands every accepted step's filtered state to the host
+(``lh_solve_stream`` with ``every_step``), which evaluates them on it and on its
+analytic field (``plan_analytic``, ``:423-470``).  Host restatements of the
+reference's statistics functions for that use: ``omega0``, ``energy``,
+``energy_window``, ``energy_lambda``, ``peakpower``, ``fwhm_t``, ``density``,
+``pressure``, ``zdz``.
 """
 import math
 
@@ -36,6 +45,7 @@ import numpy as np
 
 from . import _capi
 from . import physdata as pd
+from .maths import fwhm as _fwhm
 from .nonlinear import PlasmaCumtrapz
 
 
@@ -58,7 +68,7 @@ def planck_taper(x, left0, left1, right1, right0):
     return out
 
 
-def energy_window(grid, lambda_lims, winwidth=0.0):
+def energy_window_spec(grid, lambda_lims, winwidth=0.0):
     """(label, window over grid.omega) of ``energy_λ(grid, energyfun_ω, λlims;
     winwidth)``, ``src/Stats.jl:52-61``."""
     lam = np.asarray(lambda_lims, dtype=float)
@@ -96,10 +106,15 @@ class StatsFun:
     and the description the propagation kernel uses to record the same values
     at every accepted step (``desc()``)."""
 
-    def __init__(self, grid, transform, capacity=16384, gas=None, zdw=None, windows=None, aeff=None):
+    def __init__(self, grid, transform, capacity=16384, gas=None, zdw=None, windows=None, aeff=None,
+                 userfuns=()):
         if getattr(transform, "kind", None) != _capi.LH_TRANS_MODEAVG:
-            raise NotImplementedError("per-step statistics are provided for mode-averaged transforms")
+            raise NotImplementedError("Stats.default is provided for mode-averaged transforms "
+                                      "(radial propagations: collect_stats)")
         self.grid = grid
+        self.userfuns = list(userfuns)
+        self.has_host = bool(self.userfuns)  # evaluated on the host at every accepted step
+        self._ukeys = []
         self.transform = transform
         self.capacity = int(capacity)
         self.energy_w = np.ascontiguousarray(energy_weights(grid))
@@ -112,7 +127,7 @@ class StatsFun:
         self.windows = []
         for w in (windows or ()):
             lims, ww = (w[0], w[1]) if np.ndim(w[0]) == 1 else (w, 0.0)
-            self.windows.append(energy_window(grid, lims, ww))
+            self.windows.append(energy_window_spec(grid, lims, ww))
         if len(self.windows) > _capi.LH_STAT_MAX_WINDOWS:
             raise ValueError(f"at most {_capi.LH_STAT_MAX_WINDOWS} energy windows")
         self.window_w = (np.ascontiguousarray([self.energy_w * win ** 2 for _, win in self.windows])
@@ -135,9 +150,29 @@ class StatsFun:
             keys = [k for k in keys if k not in ("electrondensity", "peak_ionisation_rate")]
         return keys
 
+    def names(self):
+        keys = list(self.to_dict(np.zeros((0, _capi.LH_NSTAT))).keys())
+        return keys + [k for k in self._ukeys if k not in keys]
+
+    def host_step(self, Ew, z, dz):
+        """The userfuns' values at one accepted step."""
+        d = {}
+        if self.userfuns:
+            Et = analytic(self.grid, Ew)
+            for uf in self.userfuns:
+                uf(d, Ew, Et, z, dz)
+            if not self._ukeys:
+                self._ukeys = list(d)
+        return d
+
     def to_dict(self, records):
-        """Records [n][LH_NSTAT] → {name: array over steps} (plus ``pressure``
-        from the recorded density when a gas is given)."""
+        """Records [n][LH_NSTAT] (or a ``StepRecords``) → {name: array over
+        steps} (plus ``pressure`` from the recorded density when a gas is
+        given, and the userfuns' values, which replace a device value of the
+        same name as a later function does in the reference)."""
+        host = ()
+        if isinstance(records, StepRecords):
+            host, records = records.host, records.device
         records = np.asarray(records, dtype=float).reshape(-1, _capi.LH_NSTAT)
         out = {k: records[:, _capi.STAT_KEYS.index(k)].copy() for k in self.keys()}
         for q, (label, _) in enumerate(self.windows):
@@ -148,21 +183,22 @@ class StatsFun:
             out["zdw"] = np.array([self.zdw(z) for z in out["z"]])
         elif self.zdw is not None:
             out["zdw"] = np.full(len(records), float(self.zdw))
+        out.update(_stack(host))
         return out
 
     def __call__(self, Ew, z, dz):
         ctx = self.transform.context()
         ctx.set_step_stats(self)
         rec = ctx.stats_eval(Ew, z, dz)
-        return {k: v[0] for k, v in self.to_dict(rec[None, :]).items()}
+        d = {k: v[0] for k, v in self.to_dict(rec[None, :]).items()}
+        d.update(self.host_step(np.asarray(Ew), z, dz))
+        return d
 
 
 def default(grid, Ew, mode, linop, transform, *, windows=None, gas=None, onaxis=False, userfuns=(),
             capacity=16384):
     """``Stats.default(grid, Eω, mode, linop, transform; gas, windows, onaxis,
     userfuns)``, ``src/Stats.jl:492-532``, for a mode-averaged transform."""
-    if userfuns:
-        raise NotImplementedError("userfuns statistics are not provided")
     aeff = None
     if onaxis:
         if mode is None or not hasattr(mode, "aeff_onaxis"):
@@ -181,4 +217,177 @@ def default(grid, Ew, mode, linop, transform, *, windows=None, gas=None, onaxis=
                                                    loss=mode.loss))
         else:
             zdw = mode_zdw(mode)
-    return StatsFun(grid, transform, capacity=capacity, gas=gas, zdw=zdw, windows=windows, aeff=aeff)
+    return StatsFun(grid, transform, capacity=capacity, gas=gas, zdw=zdw, windows=windows, aeff=aeff,
+                    userfuns=userfuns)
+
+
+# --------------------------------------------------------------------------
+# Statistics evaluated on the host at every accepted step
+# --------------------------------------------------------------------------
+class StepRecords:
+    """The statistics one output call receives: the device records
+    [n][LH_NSTAT] of the steps since the previous call and one dict of host
+    values per step (every-step streaming: one step per call)."""
+
+    def __init__(self, device=None, host=()):
+        self.device = (np.zeros((0, _capi.LH_NSTAT)) if device is None
+                       else np.asarray(device, dtype=float).reshape(-1, _capi.LH_NSTAT))
+        self.host = list(host)
+
+    def __len__(self):
+        return max(len(self.device), len(self.host))
+
+
+def _stack(host):
+    """Per-step dicts → {name: values over steps (first axis)}."""
+    if not host:
+        return {}
+    return {k: np.array([h[k] for h in host]) for k in host[0]}
+
+
+def analytic(grid, Ew):
+    """``plan_analytic(grid, Eω)``'s transform (``src/Stats.jl:423-470``): the
+    analytic time-domain field.  Envelope grid: the inverse FFT of Eω along its
+    first axis.  Field grid: the Nω rFFT samples doubled into the first Nω - 1
+    bins of a 2(Nω - 1)-point buffer (the +fs/2 sample dropped,
+    ``copyto_fft!``), then the inverse FFT."""
+    Ew = np.asarray(Ew)
+    if grid.kind != "real":
+        return np.fft.ifft(Ew, axis=0)
+    n = Ew.shape[0] - 1
+    buf = np.zeros((2 * n,) + Ew.shape[1:], dtype=np.complex128)
+    buf[:n] = 2 * Ew[:n]
+    return np.fft.ifft(buf, axis=0)
+
+
+def zdz(d, Ew, Et, z, dz):
+    """``zdz!`` (``src/Stats.jl:411-414``)."""
+    d["z"] = z
+    d["dz"] = dz
+
+
+def omega0(grid):
+    """``ω0(grid)`` (``src/Stats.jl:18-29``): the first moment of |Eω|² over ω,
+    one value per column of a 2-D field."""
+    w = np.asarray(grid.omega, dtype=float)
+
+    def addstat(d, Ew, Et, z, dz):
+        P = np.abs(Ew) ** 2
+        m = np.tensordot(w, P, axes=(0, 0)) / np.sum(P, axis=0)
+        d["ω0"] = float(m) if np.ndim(m) == 0 else m
+    return addstat
+
+
+def energy(grid, energyfun_w):
+    """``energy(grid, energyfun_ω)`` (``src/Stats.jl:35-44``)."""
+    def addstat(d, Ew, Et, z, dz):
+        if np.ndim(Ew) > 1:
+            d["energy"] = np.array([energyfun_w(Ew[:, i]) for i in range(Ew.shape[1])])
+        else:
+            d["energy"] = energyfun_w(Ew)
+    return addstat
+
+
+def energy_window(grid, energyfun_w, window, label):
+    """``energy_window(grid, energyfun_ω, window; label)`` (``src/Stats.jl:69-79``)."""
+    key = "energy_" + label
+    window = np.asarray(window)
+
+    def addstat(d, Ew, Et, z, dz):
+        if np.ndim(Ew) > 1:
+            d[key] = np.array([energyfun_w(Ew[:, i] * window) for i in range(Ew.shape[1])])
+        else:
+            d[key] = energyfun_w(Ew * window)
+    return addstat
+
+
+def energy_lambda(grid, energyfun_w, lambda_lims, label=None, winwidth=0.0):
+    """``energy_λ(grid, energyfun_ω, λlims; label, winwidth)`` (``src/Stats.jl:52-61``)."""
+    lab, win = energy_window_spec(grid, lambda_lims, winwidth)
+    return energy_window(grid, energyfun_w, win, lab if label is None else label)
+
+
+def peakpower(grid):
+    """``peakpower(grid)`` (``src/Stats.jl:86-99``): max |Et|² (per column of a
+    2-D field, and ``peakpower_allmodes``, the maximum over t of the sum over
+    columns)."""
+    def addstat(d, Ew, Et, z, dz):
+        P = np.abs(Et) ** 2
+        if np.ndim(Et) > 1:
+            d["peakpower"] = np.max(P, axis=0)
+            d["peakpower_allmodes"] = float(np.max(np.sum(P, axis=1)))
+        else:
+            d["peakpower"] = float(np.max(P))
+    return addstat
+
+
+def fwhm_t(grid):
+    """``fwhm_t(grid)`` (``src/Stats.jl:182-199``): FWHM of |Et|² at its
+    narrowest (``min``) and widest (``max``) crossings, per column of a 2-D
+    field (and of the column sum, ``_allmodes``)."""
+    t = np.asarray(grid.t, dtype=float)
+
+    def addstat(d, Ew, Et, z, dz):
+        P = np.abs(Et) ** 2
+        if np.ndim(Et) > 1:
+            Ps = np.sum(P, axis=1)
+            d["fwhm_t_min"] = np.array([_fwhm(t, P[:, i]) for i in range(P.shape[1])])
+            d["fwhm_t_max"] = np.array([_fwhm(t, P[:, i], "max") for i in range(P.shape[1])])
+            d["fwhm_t_min_allmodes"] = _fwhm(t, Ps)
+            d["fwhm_t_max_allmodes"] = _fwhm(t, Ps, "max")
+        else:
+            d["fwhm_t_min"] = _fwhm(t, P)
+            d["fwhm_t_max"] = _fwhm(t, P, "max")
+    return addstat
+
+
+def density(dfun):
+    """``density(dfun)`` (``src/Stats.jl:323-327``)."""
+    def addstat(d, Ew, Et, z, dz):
+        d["density"] = dfun(z)
+    return addstat
+
+
+def pressure(dfun, gas):
+    """``pressure(dfun, gas)`` (``src/Stats.jl:335-339``)."""
+    def addstat(d, Ew, Et, z, dz):
+        d["pressure"] = pd.pressure(gas, dfun(z))
+    return addstat
+
+
+class HostStats:
+    """``collect_stats(grid, Eω, funcs...)`` (``src/Stats.jl:476-493``): every
+    function f(d, Eω, Et, z, dz) at every accepted step, on the host, with
+    ``zdz`` added when missing."""
+
+    has_host = True
+
+    def __init__(self, grid, Ew, funcs):
+        self.grid = grid
+        self.funcs = list(funcs)
+        if zdz not in self.funcs:
+            self.funcs.append(zdz)
+        self._keys = []
+
+    def host_step(self, Ew, z, dz):
+        d = {}
+        Et = analytic(self.grid, Ew)
+        for f in self.funcs:
+            f(d, Ew, Et, z, dz)
+        if not self._keys:
+            self._keys = list(d)
+        return d
+
+    def __call__(self, Ew, z, dz):
+        return self.host_step(np.asarray(Ew), z, dz)
+
+    def names(self):
+        return list(self._keys)
+
+    def to_dict(self, records):
+        return _stack(records.host if isinstance(records, StepRecords) else ())
+
+
+def collect_stats(grid, Ew, *funcs):
+    """``Stats.collect_stats(grid, Eω, funcs...)``."""
+    return HostStats(grid, Ew, funcs)

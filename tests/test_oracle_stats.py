@@ -162,8 +162,8 @@ def test_energy_windows(small, orc, capi):
     w0 = pd.wlfreq(800e-9)
     sig = 4 * math.log(2) / tau / (2 * math.sqrt(2 * math.log(2)))
     gk = np.exp(-((w - w0) ** 2) / (2 * sig ** 2))
-    _, rect = stats.energy_window(g, (pd.wlfreq(w0 + sig), pd.wlfreq(w0 - sig)))
-    _, tap = stats.energy_window(g, (pd.wlfreq(w0 + sig), pd.wlfreq(w0 - sig)), winwidth=sig)
+    _, rect = stats.energy_window_spec(g, (pd.wlfreq(w0 + sig), pd.wlfreq(w0 - sig)))
+    _, tap = stats.energy_window_spec(g, (pd.wlfreq(w0 + sig), pd.wlfreq(w0 - sig)), winwidth=sig)
     st, rec = orc.step_stats(small.holder, t, aeff, small.Ew0, windows=[rect, tap])
     r = dict(zip(capi.STAT_KEYS, rec))
     assert 0.6 < r["window0"] / r["energy"] < 0.75  # ≈ erf(1/√2) on the bins
