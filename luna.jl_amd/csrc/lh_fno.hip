@@ -240,17 +240,33 @@ __global__ __launch_bounds__(FNO_T) void fno_gemm32(GemmArgs g) {
             }
 }
 
+// The k-slices of 32 outputs per workgroup: eight groups of threads take the
+// slices part, part + 8, ... of each output in order (independent loads, so
+// many are in flight), and the eight group sums are added in a fixed tree.
+constexpr int RED_OUT = 32, RED_PART = FNO_T / RED_OUT;
+static_assert(RED_PART == 8, "the reduction tree below adds eight group sums");
 __global__ __launch_bounds__(FNO_T) void fno_splitk_reduce(GemmArgs g) {
+    __shared__ float red[RED_PART][RED_OUT + 1];
     const long long MN = (long long)g.M * g.N;
-    const long long idx = (long long)blockIdx.x * FNO_T + threadIdx.x;
-    if (idx >= MN * g.batch) return;
-    const int b = (int)(idx / MN);
+    const int ol = threadIdx.x % RED_OUT, part = threadIdx.x / RED_OUT;
+    const long long idx = (long long)blockIdx.x * RED_OUT + ol;
+    const bool live = idx < MN * g.batch;
+    const int b = live ? (int)(idx / MN) : 0;
     const long long rem = idx - (long long)b * MN;
-    const int m = (int)(rem / g.N), n = (int)(rem - (long long)m * g.N);
     float s = 0.0f;
-    for (int ks = 0; ks < g.ksplit; ks++) s += g.work[((long long)(b * g.ksplit + ks) * g.M + m) * g.N + n];
+    if (live) {
+        const float* w = g.work + (long long)b * g.ksplit * MN + rem;
+#pragma unroll 8
+        for (int ks = part; ks < g.ksplit; ks += RED_PART) s += w[(long long)ks * MN];
+    }
+    red[part][ol] = s;
+    __syncthreads();
+    if (part != 0 || !live) return;
+    const float t = ((red[0][ol] + red[1][ol]) + (red[2][ol] + red[3][ol])) +
+                    ((red[4][ol] + red[5][ol]) + (red[6][ol] + red[7][ol]));
+    const int m = (int)(rem / g.N), n = (int)(rem - (long long)m * g.N);
     const long long ci = (long long)b * g.sc_b + (long long)m * g.sc_m + (long long)n * g.sc_n;
-    float o = g.alpha * s;
+    float o = g.alpha * t;
     if (g.beta != 0.0f) o += g.beta * g.C[ci];
     if (g.bias) o += g.bias[n];
     g.C[ci] = o;
@@ -572,7 +588,7 @@ static int fno_gemm_launch(lh_fno* f, GemmArgs g, int ta, int tb) {
         hipLaunchKernelGGL((fno_gemm<__bf16, __bf16>), grid, dim3(FNO_T), 0, s, g);
     if (ksplit > 1) {
         const long long n = (long long)g.batch * g.M * g.N;
-        hipLaunchKernelGGL(fno_splitk_reduce, dim3(nblk(n)), dim3(FNO_T), 0, s, g);
+        hipLaunchKernelGGL(fno_splitk_reduce, dim3((unsigned)((n + RED_OUT - 1) / RED_OUT)), dim3(FNO_T), 0, s, g);
     }
     FNOCK(hipGetLastError());
     return LH_OK;
