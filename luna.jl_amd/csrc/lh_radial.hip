@@ -429,7 +429,7 @@ static void rad_launch_step(const RadArgs* dargs, const double2* tw, int n_to, i
 }
 
 // Oversampled lengths with compiled radial kernels (Nt = Nto/2, Nω = Nt/2+1).
-static bool rad_size_ok(int n_to) { return n_to == 2048 || n_to == 4096 || n_to == 8192; }
+static bool rad_size_ok(int n_to) { return n_to == 2048 || n_to == 4096 || n_to == 8192 || n_to == 16384; }
 
 // One RHS (stage < 0: of rhs_in or the current state) or one step, under plan p.
 template <int MO, int MC, int PLAN, int MT>
@@ -462,6 +462,7 @@ static void rad_run_plan(const RadPlan& p, bool step, const RadArgs* a, const do
 static void rad_dispatch(const RadPlan& p, bool step, int n_to, int n_r, const RadArgs* a, const double2* tw,
                          int stage, hipStream_t s) {
     switch (n_to) {
+        case 16384: rad_run_plan<8192, 4096>(p, step, a, tw, n_to, n_r, stage, s); break;
         case 8192: rad_run_plan<4096, 2048>(p, step, a, tw, n_to, n_r, stage, s); break;
         case 4096: rad_run_plan<2048, 1024>(p, step, a, tw, n_to, n_r, stage, s); break;
         default: rad_run_plan<1024, 512>(p, step, a, tw, n_to, n_r, stage, s); break;

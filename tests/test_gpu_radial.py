@@ -1,8 +1,8 @@
 """GPU parity of the radial transform (TransRadial, src/NonlinearRHS.jl:466-474)
 and its propagation (Luna.run, src/Luna.jl:358-391) against the oracle.
 
-Single RHS: relative L2 ≤ 1e-12 at every compiled size, including the full
-Nr = 256 × Nto = 8192 workload (the FP64 MFMA contraction against the oracle's
+Single RHS: relative L2 ≤ 1e-12 at every compiled size (Nto = 2048 to
+16384), including the full Nr = 256 × Nto = 8192 workload (the FP64 MFMA contraction against the oracle's
 plain triple loop).  Propagations: relative spectral-energy error ≤ 1e-6 at
 rtol = 1e-6, and the reference's paraxial-beam check (test/test_radial.jl).
 """
@@ -59,7 +59,7 @@ def test_radial_rhs_near_focus(cfg3, ctx3, orc):
     assert err <= 1e-12
 
 
-@pytest.mark.parametrize("trange,N", [(0.3e-12, 64), (0.5e-12, 128)])
+@pytest.mark.parametrize("trange,N", [(0.3e-12, 64), (0.5e-12, 128), (2e-12, 64)])
 def test_radial_rhs_small_sizes(orc, trange, N):
     print(f"case: radial rhs trange={trange} N={N}", flush=True)
     from luna_jl_amd import configs, runtime
@@ -103,6 +103,22 @@ def test_radial_nonlinear_propagation_vs_oracle(orc, capi):
     for k in range(11):
         assert spec_err(sv[k], sv_r[k]) <= 1e-6, k
     assert np.array_equal(zs, zs_r)
+
+
+def test_radial_16384_propagation_vs_oracle(orc, capi):
+    # the largest radial class: Nto = 16384 (a 2 ps window), Nr = 64, 2 cm before the focus
+    print("case: radial 2 cm N=64 Nto=16384", flush=True)
+    from luna_jl_amd import configs, runtime
+    inst = configs.radial_instance(flength=0.02, energy=50e-6, propz=-0.01, R=1.5e-3, N=64, trange=2e-12)
+    assert inst.grid.n_to == 16384
+    ctx = runtime.Context(inst.holder)
+    opts = capi.rk_opts(inst.zmax, save_n=5)
+    Ew, sv, zs, stats = ctx.solve(inst.Ew0, opts)
+    st, Ew_r, sv_r, zs_r, stats_r = orc.solve(inst.holder, inst.Ew0, opts)
+    assert stats.status == 0 and st == 0
+    assert (stats.steps, stats.accepted) == (stats_r.steps, stats_r.accepted)
+    for k in range(5):
+        assert spec_err(sv[k], sv_r[k]) <= 1e-6, k
 
 
 def test_radial_full_workload(cfg3, ctx3, capi):
