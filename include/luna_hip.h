@@ -47,8 +47,9 @@ enum {
                               returned state, z, dz and saved count          */
     LH_E_TOO_LARGE = 10,   /* the problem does not fit the fixed working
                               memory of lh_solve_noalloc                     */
-    LH_E_HOST_WAIT = 11    /* lh_solve_stream with every_step: the host did
-                              not take a published step within a minute      */
+    LH_E_HOST_WAIT = 11    /* lh_solve_stream with every_step: the host fell
+                              behind the device by more than the ring of
+                              per-step slots (the run was stopped)            */
 };
 
 enum { LH_GRID_REAL = 0, LH_GRID_ENV = 1 };           /* Grid.RealGrid / Grid.EnvGrid */
@@ -117,8 +118,7 @@ typedef struct lh_rk_opts {
     int32_t every_step;       /* lh_solve_stream: 1 hands every accepted step to
                                  the callback, as the reference calls its output
                                  at each (src/Luna.jl:366-372, src/Output.jl:
-                                 65-81), the solve waiting for the host to take
-                                 each; 0: only steps that save, never waiting (0) */
+                                 65-81); 0: only steps that save (0)            */
     double zstop;             /* > 0: end with LH_E_STOPPED after the first
                                  accepted step with tn > zstop (its window filter
                                  and saves done); 0: run to zmax                 */
@@ -180,9 +180,10 @@ int lh_solve(lh_ctx *ctx, double *Ew_inout, const lh_rk_opts *opts,
  * records of the steps since the previous call.  Without every_step a
  * publication the host has not taken before the next one is folded into it
  * (its saves and records are still handed over); with every_step each is
- * handed over on its own, the kernel waiting (up to a minute, then
- * LH_E_HOST_WAIT) while the host is two publications behind.  Pointers are
- * valid during the call only. */
+ * handed over on its own from a ring of page-locked slots (64 MiB, 64 to 4096
+ * states) that the kernel fills without waiting: a host that falls behind by
+ * the whole ring stops the run with LH_E_HOST_WAIT.  Pointers are valid
+ * during the call only. */
 typedef struct lh_step_out {
     int32_t first_save;       /* index of the first save in this call           */
     int32_t n_saves;          /* saves in this call (>= 1; >= 0 with every_step) */

@@ -115,11 +115,10 @@ struct DInst {
     double* pub_info;        // [2][4]
     double2* pub_y;          // [2][n_state]
     const int* pub_stop;     // set by the host: end with LH_E_STOPPED at the next publication
-    // every-step mode (pub_wait > 0): every accepted step publishes, and
-    // publication k waits until the host has taken k - 2 (pub_ack, written by
-    // the host), for at most pub_wait wall-clock ticks
-    const unsigned long long* pub_ack;
-    long long pub_wait;
+    // every-step mode (pub_ring > 0): every accepted step publishes, into
+    // slot k % pub_ring of pub_ring slots (pub_info and pub_y hold that many);
+    // the kernel never waits, the host checks that it has not been lapped
+    int pub_ring;
 };
 
 // Dormand–Prince tableau, src/dopri.jl:1-28 (rational literals evaluated in double).
@@ -2117,31 +2116,10 @@ __device__ __forceinline__ void solve_instance(const DInst& in, Smem& sm) {
                 }
                 saved++;
             }
-            bool late = false;
-            if (STREAM && in.pub_wait > 0 && npub >= 2) {
-                // every-step mode: slot npub & 1 is free once the host has
-                // taken publication npub - 2
-                if (threadIdx.x == 0) {
-                    const long long w0 = wall_clock64();
-                    while (__hip_atomic_load(in.pub_ack, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM) + 1 <
-                           (unsigned long long)npub) {
-                        if (wall_clock64() - w0 > in.pub_wait) {
-                            sm.flag = LH_E_HOST_WAIT;
-                            break;
-                        }
-                        __builtin_amdgcn_s_sleep(32);
-                    }
-                }
-                __syncthreads();
-                if (sm.flag == LH_E_HOST_WAIT) {
-                    status = LH_E_HOST_WAIT;
-                    late = true;
-                }
-            }
-            if (STREAM && !late && (saved > saved_before || in.pub_wait > 0)) {
+            if (STREAM && (saved > saved_before || in.pub_ring > 0)) {
                 // the output call of this step (src/Output.jl:65-81, 262-302):
                 // its saves are in place; hand over the state, then publish
-                const int slot = (int)(npub & 1);
+                const int slot = (in.pub_ring > 0) ? (int)(npub % in.pub_ring) : (int)(npub & 1);
                 double2* py = in.pub_y + (size_t)slot * N;
                 for (int k = fresh_tid(); k < N; k += LH_T) py[k] = YN[k];
                 if (threadIdx.x == 0) {
@@ -3083,7 +3061,7 @@ const char* lh_strerror(int status) {
         case LH_E_MAXSTEPS: return "reached the cap on step attempts";
         case LH_E_STOPPED: return "stopped before the end of the propagation (zstop)";
         case LH_E_TOO_LARGE: return "problem exceeds the fixed working memory of lh_solve_noalloc";
-        case LH_E_HOST_WAIT: return "the host did not take a published step within the wait limit";
+        case LH_E_HOST_WAIT: return "the host fell behind the per-step output by more than its ring of slots";
         default: return "unknown status";
     }
 }
@@ -3324,12 +3302,9 @@ static int rad_solve_stream(lh_ctx* c, double* Ew, const lh_rk_opts* o, lh_outpu
     return stats->status;
 }
 
-// Wall-clock ticks of the every-step wait (one minute).
-static long long host_wait_ticks(int dev) {
-    int khz = 0;
-    if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess || khz <= 0) khz = 100000;
-    return (long long)khz * 1000LL * 60LL;
-}
+// Page-locked bytes of the every-step ring of state slots (clamped to 64..4096
+// slots): 1023 slots of a 4097-bin state.
+static constexpr size_t LH_RING_BYTES = (size_t)64 << 20;
 
 int lh_solve_stream(lh_ctx* c, double* Ew, const lh_rk_opts* o, lh_output_cb cb, void* user, lh_stats* stats) {
     if (!c || !Ew || !o || !cb || !stats || !opts_ok(o)) return LH_E_ARG;
@@ -3341,15 +3316,21 @@ int lh_solve_stream(lh_ctx* c, double* Ew, const lh_rk_opts* o, lh_output_cb cb,
     const double2* tw;
     st = twiddles(c->device, &tw);
     if (st) return st;
-    // one page-locked, device-mapped, coherent block: publication counter,
-    // stop flag and (every-step mode) the host's count of publications taken,
-    // slot info, two state slots, every save and its position, the statistics
-    // records
+    // one page-locked, device-mapped, coherent block: publication counter and
+    // stop flag, slot info and state slots (two, or the every-step ring), every
+    // save and its position, the statistics records
     const size_t NS = (size_t)c->n_state;
     const size_t nsave = (size_t)(o->save_n > 0 ? o->save_n : 0);
     const size_t ncap = (size_t)(c->st_on ? c->st_cap : 0);
-    const size_t off_info = 256, off_y = 512, off_sv = off_y + 2 * NS * sizeof(double2),
-                 off_rec = off_sv + nsave * NS * sizeof(double2), total = off_rec + ncap * LH_NSTAT * sizeof(double);
+    const bool every = o->every_step != 0;
+    size_t nslot = 2;
+    if (every) {
+        nslot = LH_RING_BYTES / (NS * sizeof(double2));
+        nslot = nslot < 64 ? 64 : nslot > 4096 ? 4096 : nslot;
+    }
+    const size_t off_info = 256, off_y = (off_info + nslot * 4 * sizeof(double) + 255) / 256 * 256,
+                 off_sv = off_y + nslot * NS * sizeof(double2), off_rec = off_sv + nsave * NS * sizeof(double2),
+                 total = off_rec + ncap * LH_NSTAT * sizeof(double);
     unsigned char* hb = nullptr;
     HIPCK(hipHostMalloc((void**)&hb, total, hipHostMallocMapped | hipHostMallocCoherent));
     unsigned char* db = nullptr;
@@ -3357,11 +3338,9 @@ int lh_solve_stream(lh_ctx* c, double* Ew, const lh_rk_opts* o, lh_output_cb cb,
         (void)hipHostFree(hb);
         return LH_E_HIP;
     }
-    memset(hb, 0, off_y);
+    memset(hb, 0, 256);
     unsigned long long* pub = (unsigned long long*)hb;
     int* stop = (int*)(hb + 64);
-    unsigned long long* ack = (unsigned long long*)(hb + 128);
-    const bool every = o->every_step != 0;
     const double* info = (const double*)(hb + off_info);
     const double2* ys = (const double2*)(hb + off_y);
     const double2* sv = (const double2*)(hb + off_sv);
@@ -3377,8 +3356,7 @@ int lh_solve_stream(lh_ctx* c, double* Ew, const lh_rk_opts* o, lh_output_cb cb,
         in.pub_stop = (const int*)(db + 64);
         in.pub_info = (double*)(db + off_info);
         in.pub_y = (double2*)(db + off_y);
-        in.pub_ack = (const unsigned long long*)(db + 128);
-        in.pub_wait = every ? host_wait_ticks(c->device) : 0;
+        in.pub_ring = every ? (int)nslot : 0;
         if (hipMemcpyAsync(c->dinst, &in, sizeof(DInst), hipMemcpyHostToDevice, c->stream) != hipSuccess) st = LH_E_HIP;
     }
     if (!st) {
@@ -3388,15 +3366,18 @@ int lh_solve_stream(lh_ctx* c, double* Ew, const lh_rk_opts* o, lh_output_cb cb,
     // Drain the publications while the kernel runs.  A publication's state
     // slot is rewritten two publications later, so a copy of it counts only
     // if the counter has not moved on meanwhile; saves and records have slots
-    // of their own.  In every-step mode the kernel does not rewrite a slot
-    // before the host has taken it: each publication is handed over in turn
-    // and then counted in ack.  After the callback asks to stop, nothing more
-    // is handed over (the run ends at its next publication).
+    // of their own.  In every-step mode each publication is handed over in
+    // turn from the ring, which the kernel rewrites nslot publications later:
+    // the copy of publication q counts if the counter has not passed
+    // q + nslot − 2 after it (the kernel writes publication j into its slot
+    // only once j publications are out), else the host has fallen behind and
+    // the run ends with LH_E_HOST_WAIT.  After the callback asks to stop,
+    // nothing more is handed over (the run ends at its next publication).
     std::vector<double2> ycopy(NS);
     unsigned long long done = 0;
     int saves_done = o->save_first;
     long long recs_done = 0;
-    bool stop_req = false;
+    bool stop_req = false, lapped = false;
     while (!st) {
         const hipError_t q = hipStreamQuery(c->stream);
         if (q != hipSuccess && q != hipErrorNotReady) {
@@ -3404,10 +3385,18 @@ int lh_solve_stream(lh_ctx* c, double* Ew, const lh_rk_opts* o, lh_output_cb cb,
             break;
         }
         unsigned long long p = __atomic_load_n(pub, __ATOMIC_ACQUIRE);
-        while (every && p > done) {
-            const int slot = (int)(done & 1);  // publication done + 1
+        while (every && p > done && !lapped) {
+            const unsigned long long q = done + 1;
+            const int slot = (int)((q - 1) % nslot);
+            const double z = info[4 * slot], dz = info[4 * slot + 1];
             const int nsaved = (int)info[4 * slot + 2];
             const long long nacc = (long long)info[4 * slot + 3];
+            memcpy(ycopy.data(), ys + (size_t)slot * NS, NS * sizeof(double2));
+            if (__atomic_load_n(pub, __ATOMIC_ACQUIRE) > q + nslot - 2) {
+                lapped = true;
+                __atomic_store_n(stop, 1, __ATOMIC_RELEASE);
+                break;
+            }
             const long long nrec = ncap ? (nacc < (long long)ncap ? nacc : (long long)ncap) : 0;
             if (!stop_req) {
                 lh_step_out out;
@@ -3415,9 +3404,9 @@ int lh_solve_stream(lh_ctx* c, double* Ew, const lh_rk_opts* o, lh_output_cb cb,
                 out.n_saves = nsaved - saves_done;
                 out.saves = (const double*)(sv + (size_t)saves_done * NS);
                 out.zsaves = zg.data() + saves_done;
-                out.z = info[4 * slot];
-                out.dz = info[4 * slot + 1];
-                out.y = (const double*)(ys + (size_t)slot * NS);
+                out.z = z;
+                out.dz = dz;
+                out.y = (const double*)ycopy.data();
                 out.accepted = nacc;
                 out.first_record = (int32_t)recs_done;
                 out.n_records = (int32_t)(nrec - recs_done);
@@ -3427,10 +3416,9 @@ int lh_solve_stream(lh_ctx* c, double* Ew, const lh_rk_opts* o, lh_output_cb cb,
                     __atomic_store_n(stop, 1, __ATOMIC_RELEASE);
                 }
             }
-            done++;
+            done = q;
             saves_done = nsaved;
             recs_done = nrec;
-            __atomic_store_n(ack, done, __ATOMIC_RELEASE);
         }
         if (!every && p > done && !stop_req) {
             double z = 0, dz = 0, fsaved = 0, facc = 0;
@@ -3474,6 +3462,7 @@ int lh_solve_stream(lh_ctx* c, double* Ew, const lh_rk_opts* o, lh_output_cb cb,
     }
     if (!st) {
         if (hipMemcpy(stats, c->dstats, sizeof(lh_stats), hipMemcpyDeviceToHost) != hipSuccess) st = LH_E_HIP;
+        if (!st && lapped) stats->status = LH_E_HOST_WAIT;
     } else {
         (void)hipStreamSynchronize(c->stream);  // never free a block the kernel may still write
     }
