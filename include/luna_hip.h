@@ -374,8 +374,8 @@ int lh_get_step_stats(lh_ctx *ctx, double *rec, int capacity, int *n);
 /* Measurement hook (no counterpart in the reference): HIP-event durations of
  * the persistent launches of the last batch, one per grid-size class
  * (field grid Nto = 16384, 8192, 4096; envelope grid Nto = 8192, 4096; all
- * radial contexts together; envelope grid Nto = 16384), with the number of
- * instances in each.  nclass must be >= 7. */
+ * radial contexts together; envelope grid Nto = 16384; field grid Nto =
+ * 32768), with the number of instances in each.  nclass must be >= 8. */
 int lh_last_launch_times(double *ms, int *n_inst, int nclass);
 
 /* Page-locked host memory for the states and saves a sweep copies back (no

@@ -246,7 +246,7 @@ EXPORTED = ["lh_strerror", "lh_device_count", "lh_create", "lh_destroy", "lh_rhs
             "lh_fno_create", "lh_fno_destroy", "lh_fno_param_count", "lh_fno_set_params",
             "lh_fno_get_params", "lh_fno_forward", "lh_fno_loss_grad", "lh_fno_set_data", "lh_fno_train"]
 
-CLASS_NAMES = (16384, 8192, 4096, "env8192", "env4096", "radial", "env16384")
+CLASS_NAMES = (16384, 8192, 4096, "env8192", "env4096", "radial", "env16384", 32768)
 
 
 def check(status, where=""):

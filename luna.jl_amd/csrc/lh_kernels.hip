@@ -449,17 +449,31 @@ __device__ __forceinline__ double field_at(const Smem& sm, int g, double e_scale
     return ((g & 1) ? v.y : v.x) * e_scale;
 }
 
+// The integrals and scan slot pointwise_field carries from round to round
+// (kept across the LDS windows of the series staged through HBM).
+struct PwCarry {
+    double cF, cJ, cP, lastRate, lastPhi, lastLoss, Fprev, fprev;
+    int sl;
+};
+
 // Pointwise field responses on the packed real series in LDS (in place):
 // E = c2r/Nto/field_div; P = ρε0γ3 E³ + ρ·Plasma(E); P *= towin.
 // The series is processed in rounds of QR samples per thread (contiguous
 // chunks), with the three cumtrapz integrals carried from round to round;
 // each thread's samples are touched only by that thread.
-template <int MO, class Smem, bool RAMAN = false>
-__device__ __forceinline__ void pointwise_field(const DInst& in, Smem& sm, double rho, double kfac) {
+// WIN: LDS holds the window of 2·LDS_N samples from sample `base` on (the
+// series of the HBM-staged grid); rounds r0 .. r0 + rounds in the window run,
+// with the carries taken from and left in *cw.
+template <int MO, class Smem, bool RAMAN = false, bool WIN = false, int LDS_N = MO>
+__device__ __forceinline__ void pointwise_field(const DInst& in, Smem& sm, double rho, double kfac,
+                                                PwCarry* cw = nullptr, int r0 = 0, int base = 0) {
     constexpr int QT = 2 * MO / LH_T;          // samples per thread over all rounds
     constexpr int QR = (QT > 16) ? 16 : QT;    // samples per thread per round
     constexpr int NR = QT / QR;                // rounds
+    constexpr int RW = WIN ? (2 * LDS_N) / (LH_T * QR) : NR;  // rounds of this call
+    static_assert(!WIN || (2 * LDS_N) % (LH_T * QR) == 0, "whole rounds per LDS window");
     const int tid = fresh_tid();
+    const int lb = WIN ? base : 0;             // sample index of LDS sample 0
     const double es = in.e_scale;
     const bool kerr = (in.kerr == LH_KERR_FIELD), plasma = in.plasma;
     const double2* praman = RAMAN ? gl(in.escr) + 3 * MO : nullptr;  // P(t) of raman_field, packed pairs
@@ -469,8 +483,14 @@ __device__ __forceinline__ void pointwise_field(const DInst& in, Smem& sm, doubl
     double lastRate = 0.0, lastPhi = 0.0, lastLoss = 0.0;
     double Fprev = 0.0, fprev = pp1 - 1.0;         // exp(-0) = 1
     int sl = 0;                                    // scan slot of the next workgroup scan
+    if constexpr (WIN) {
+        cF = cw->cF, cJ = cw->cJ, cP = cw->cP;
+        lastRate = cw->lastRate, lastPhi = cw->lastPhi, lastLoss = cw->lastLoss;
+        Fprev = cw->Fprev, fprev = cw->fprev;
+        sl = cw->sl;
+    }
 #pragma unroll 1
-    for (int r = 0; r < NR; r++) {
+    for (int r = r0; r < r0 + RW; r++) {
         const int g0 = r * (LH_T * QR) + QR * tid;
         double a[QR];
         if (plasma) {
@@ -479,7 +499,7 @@ __device__ __forceinline__ void pointwise_field(const DInst& in, Smem& sm, doubl
             constexpr int QC = (QR < 4) ? QR : 4;
             constexpr int PCAP = PList<MO>::CAP;
 #pragma unroll
-            for (int q = 0; q < QR; q++) jf[q] = fabs(field_at(sm, g0 + q, es));
+            for (int q = 0; q < QR; q++) jf[q] = fabs(field_at(sm, g0 + q - lb, es));
             if constexpr (MO <= 4096) {
                 // samples inside the table's range get a rate; above it (or NaN) raise the flag
                 unsigned mask = 0;
@@ -538,7 +558,7 @@ __device__ __forceinline__ void pointwise_field(const DInst& in, Smem& sm, doubl
                     for (int q = 0; q < QR; q++) a[q] = 0.0;
                 } else if (hi - lo < PCAP) {
                     for (int g = lo + tid; g <= hi; g += LH_T) {
-                        double ae = fabs(field_at(sm, g, es)), rr = 0.0;
+                        double ae = fabs(field_at(sm, g - lb, es)), rr = 0.0;
                         if (!(ae < in.ppt_emin)) ppt_rates<1>(in, &ae, &rr, bad);
                         sm.plist[g - lo] = rr;
                     }
@@ -560,7 +580,7 @@ __device__ __forceinline__ void pointwise_field(const DInst& in, Smem& sm, doubl
             sl ^= 1;
 #pragma unroll
             for (int q = 0; q < QR; q++) {
-                double E = field_at(sm, g0 + q, es);
+                double E = field_at(sm, g0 + q - lb, es);
                 double F = a[q];
                 double f = (F == Fprev) ? fprev : pp1 - exp(-F);  // F is flat outside the pulse
                 Fprev = F;
@@ -576,7 +596,7 @@ __device__ __forceinline__ void pointwise_field(const DInst& in, Smem& sm, doubl
 #pragma unroll
         for (int s2 = 0; s2 < QR / 2; s2++) {
             const int g = g0 + 2 * s2;
-            double2 v = sm.buf[pad(g >> 1)];
+            double2 v = sm.buf[pad((g - lb) >> 1)];
             double E0 = v.x * es, E1 = v.y * es;
             double p0 = kerr ? kfac * (E0 * E0 * E0) : 0.0;
             double p1 = kerr ? kfac * (E1 * E1 * E1) : 0.0;
@@ -589,8 +609,14 @@ __device__ __forceinline__ void pointwise_field(const DInst& in, Smem& sm, doubl
                 p0 = p0 + (rho * E0) * pr.x;
                 p1 = p1 + (rho * E1) * pr.y;
             }
-            sm.buf[pad(g >> 1)] = make_double2(p0 * tw[g], p1 * tw[g + 1]);
+            sm.buf[pad((g - lb) >> 1)] = make_double2(p0 * tw[g], p1 * tw[g + 1]);
         }
+    }
+    if constexpr (WIN) {
+        cw->cF = cF, cw->cJ = cJ, cw->cP = cP;
+        cw->lastRate = lastRate, cw->lastPhi = lastPhi, cw->lastLoss = lastLoss;
+        cw->Fprev = Fprev, cw->fprev = fprev;
+        cw->sl = sl;
     }
 }
 
@@ -1428,6 +1454,151 @@ __device__ __forceinline__ void stepfun_big(const DInst& in, Smem& sm, double2* 
     __syncthreads();
 }
 
+// ==========================================================================
+// Field grid of Nto = 4·LDS points (KIND 3): the c2r/r2c transforms of the
+// Nto reals are MO = Nto/2-point complex transforms staged through HBM
+// (fft_big), the pointwise responses run over the series in two LDS windows.
+// HBM work area escr = [W | T], MO + MO/2 complex: W the packed series,
+// T the half-length stash of fft_big.
+// ==========================================================================
+// c2r pre-pack of bin k into Z in HBM for a spectrum zero above M/2 (as
+// c2r_pack_bin, with the twiddles of 2M points from tw_frac).
+template <int M>
+__device__ __forceinline__ void c2r_pack_bin_big(double2* Z, const TwLds& tw, int k, double2 Xk) {
+    const double2 Xm = make_double2(0.0, 0.0);
+    if (k == 0) {
+        Xk.y = 0.0;
+        double2 A = cadd(Xk, cconj(Xm));
+        double2 B = csub(Xk, cconj(Xm));
+        Z[0] = cadd(A, cmul_pi(B));
+    } else if (k == M / 2) {
+        double2 A = cadd(Xk, cconj(Xk));
+        double2 B = cmul(csub(Xk, cconj(Xk)), cconj(tw_frac<2 * M>(tw, k)));
+        Z[k] = cadd(A, cmul_pi(B));
+    } else {
+        const int m = M - k;
+        double2 wk = cconj(tw_frac<2 * M>(tw, k)), wm = cconj(tw_frac<2 * M>(tw, m));
+        double2 Ak = cadd(Xk, cconj(Xm));
+        double2 Bk = cmul(csub(Xk, cconj(Xm)), wk);
+        double2 Am = cadd(Xm, cconj(Xk));
+        double2 Bm = cmul(csub(Xm, cconj(Xk)), wm);
+        Z[k] = cadd(Ak, cmul_pi(Bk));
+        Z[m] = cadd(Am, cmul_pi(Bm));
+    }
+}
+
+// Stage prologue of the big field grid (stage_prologue with FOLD, in chunks of
+// 8 bins per thread): the c2r pre-pack of sc_t·(y + Δz Σ B k)·exp(L hf) into
+// W, outfac·exp(−L hf) into slot 7; k1 first propagated by h0.
+template <int MO, class Smem>
+__device__ __forceinline__ void stage_prologue_bigfield(const DInst& in, Smem& sm, const double2* Y, const KSet& K,
+                                                        int stage, double dt, double h0, double hf) {
+    constexpr int N = MO / 2 + 1, CH = 8, NC = (N + LH_T * CH - 1) / (LH_T * CH);
+    double2* W = gl(in.escr);
+    const double2* linop = gl(in.linop);
+    const double2* outfac = gl(in.outfac);
+    const int tid = fresh_tid();
+    const double sc_t = in.sc_t;
+    double2* K7 = K[7];
+#pragma unroll 1
+    for (int c = 0; c < NC; c++) {
+        double2 yb[CH], L[CH];
+#pragma unroll
+        for (int i = 0; i < CH; i++) {
+            const int k = tid + (c * CH + i) * LH_T;
+            yb[i] = (k < N) ? Y[k] : make_double2(0.0, 0.0);
+            L[i] = (k < N) ? linop[k] : make_double2(0.0, 0.0);
+        }
+        if (stage == 0 && h0 != 0.0) {
+            double2* K0 = K[0];
+            double2 kv[CH];
+#pragma unroll
+            for (int i = 0; i < CH; i++) {
+                const int k = tid + (c * CH + i) * LH_T;
+                kv[i] = (k < N) ? K0[k] : make_double2(0.0, 0.0);
+            }
+#pragma unroll
+            for (int i = 0; i < CH; i++) {
+                const int k = tid + (c * CH + i) * LH_T;
+                if (k < N) K0[k] = cmul(kv[i], cexp_lh(L[i], h0));
+            }
+        }
+#pragma unroll 1
+        for (int j = 0; j <= stage; j++) {
+            const double cj = dt * c_B[stage][j];
+            if (cj == 0.0) continue;
+            const double2* Kj = K[j];
+#pragma unroll
+            for (int i = 0; i < CH; i++) {
+                const int k = tid + (c * CH + i) * LH_T;
+                if (k < N) yb[i] = cadd(yb[i], cscal(cj, Kj[k]));
+            }
+        }
+        double2 ofv[CH];  // every load before the first store to slot 7
+#pragma unroll
+        for (int i = 0; i < CH; i++) {
+            const int k = tid + (c * CH + i) * LH_T;
+            ofv[i] = (k < N) ? outfac[k] : make_double2(0.0, 0.0);
+        }
+#pragma unroll
+        for (int i = 0; i < CH; i++) {
+            const int k = tid + (c * CH + i) * LH_T;
+            if (k < N) {
+                double2 yp = yb[i], inv = make_double2(1.0, 0.0);
+                if (hf != 0.0) yp = cmul(yb[i], cexp_lh_inv(L[i], hf, inv));
+                c2r_pack_bin_big<MO>(W, sm.tw, k, cscal(sc_t, yp));
+                K7[k] = cmul(ofv[i], inv);
+            }
+        }
+    }
+}
+
+// TransModeAvg on the big field grid: W holds the c2r pre-pack of sc_t·Eω on
+// entry.  Inverse transform, the pointwise responses over the series in two
+// LDS windows (carrying the plasma integrals), forward transform, r2c
+// post-step; the epilogue gets bin k < Nω (times outfac unless it folds it).
+template <int MO, class Smem, class Epi>
+__device__ __forceinline__ void rhs_bigfield(const DInst& in, Smem& sm, Epi epi, double rho, double kfac) {
+    constexpr int H = MO / 2;  // complex pairs per LDS window
+    double2* W = gl(in.escr);
+    double2* T = gl(in.escr) + MO;
+    const int tid = fresh_tid();
+    fft_big<H, true>(W, T, sm);
+    PwCarry cr;
+    cr.cF = cr.cJ = cr.cP = 0.0;
+    cr.lastRate = cr.lastPhi = cr.lastLoss = 0.0;
+    cr.Fprev = 0.0;
+    cr.fprev = in.preion_p1 - 1.0;
+    cr.sl = 0;
+#pragma unroll 1
+    for (int h = 0; h < 2; h++) {
+        __syncthreads();
+        for (int j = tid; j < H; j += LH_T) sm.buf[pad(j)] = W[h * H + j];
+        __syncthreads();
+        pointwise_field<MO, Smem, false, true, H>(in, sm, rho, kfac, &cr, 2 * h, 2 * H * h);
+        __syncthreads();
+        for (int j = tid; j < H; j += LH_T) W[h * H + j] = sm.buf[pad(j)];
+    }
+    fft_big<H, false>(W, T, sm);
+    constexpr int N = MO / 2 + 1, CH = 8, NC = (N + LH_T * CH - 1) / (LH_T * CH);
+    const double2* outfac = gl(in.outfac);
+#pragma unroll 1
+    for (int c = 0; c < NC; c++) {
+        double2 res[CH];
+#pragma unroll
+        for (int i = 0; i < CH; i++) {
+            const int k = tid + (c * CH + i) * LH_T;
+            res[i] = make_double2(0.0, 0.0);
+            if (k < N) {
+                res[i] = r2c_bin_big<MO>(W, sm, k);
+                if constexpr (!Epi::FOLDS_OUTFAC) res[i] = cmul(res[i], outfac[k]);
+            }
+        }
+        epi(res, N, c * CH);
+    }
+    __syncthreads();
+}
+
 // hω·δt on the big grid: h zero-padded to 2·NO reals, packed into NO complex
 // points in W, transformed, post-stepped into hwdt[0..NO].  One workgroup.
 template <int NO>
@@ -1472,9 +1643,9 @@ __global__ __launch_bounds__(LH_T) void lh_raman_hw_kernel(const double* __restr
 // KIND 2: envelope grid with the transforms staged through HBM, MO = MC = Nt.
 template <int KIND, int MO, int MC>
 struct Sizes {
-    static constexpr int N = (KIND == 0) ? Bins<MO>::N : MC;
+    static constexpr int N = (KIND == 0 || KIND == 3) ? Bins<MO>::N : MC;
     static constexpr int NPT = (N + LH_T - 1) / LH_T;
-    static constexpr int LDS = (KIND == 2) ? MO / 2 : MO;  // LDS transform length (SmemT)
+    static constexpr int LDS = (KIND == 2 || KIND == 3) ? MO / 2 : MO;  // LDS transform length (SmemT)
 };
 
 // ==========================================================================
@@ -1865,8 +2036,10 @@ __device__ __forceinline__ void do_prologue(const DInst& in, Smem& sm, const dou
         stage_prologue<MO, Smem>(in, sm, Y, K, stage, dt, h0, hf);
     else if constexpr (KIND == 1)
         stage_prologue_env<MO, MC, Smem>(in, sm, Y, K, stage, dt, h0, hf);
-    else
+    else if constexpr (KIND == 2)
         stage_prologue_big<MC, Smem>(in, sm, Y, K, stage, dt, h0, hf);
+    else
+        stage_prologue_bigfield<MO, Smem>(in, sm, Y, K, stage, dt, h0, hf);
 }
 
 // PACKED: the field-grid buffer already holds the c2r pre-pack (the stage
@@ -1879,8 +2052,10 @@ __device__ __forceinline__ void do_rhs(const DInst& in, Smem& sm, Epi epi, doubl
         rhs_field<MO, Smem, Epi, PACKED, RAMAN>(in, sm, epi, rho, kfac);
     else if constexpr (KIND == 1)
         rhs_env<MO, MC, Smem>(in, sm, epi);
-    else
+    else if constexpr (KIND == 2)
         rhs_big<MO, Smem>(in, sm, epi);
+    else
+        rhs_bigfield<MO, Smem>(in, sm, epi, rho, kfac);  // W holds the pre-pack in both entries
 }
 
 template <int KIND, int MO, int MC, class Smem>
@@ -1890,8 +2065,10 @@ __device__ __forceinline__ void do_stepfun(const DInst& in, Smem& sm, double2* Y
         stepfun_filter<MC, Smem>(in, sm, YN, h, rec, z, dz, rho);
     else if constexpr (KIND == 1)
         stepfun_env<MC, Smem>(in, sm, YN, h, rec, z, dz, rho);
-    else
+    else if constexpr (KIND == 2)
         stepfun_big<MC, Smem>(in, sm, YN, h);  // no statistics on this grid
+    else
+        stepfun_filter_cols<MC, Smem, false>(in, gl(in.linop), sm, YN, h);  // coarse grid in LDS; no statistics
 }
 
 // ZDEP: compiled with the z-dependent fill (the launch picks it only for a
@@ -1951,7 +2128,7 @@ __device__ __forceinline__ void solve_instance(const DInst& in, Smem& sm) {
             double hf = t2 - t;
             if (zdep && st != 5) zg_refresh(in, sm, t2, rho, kfac);  // stages 5 and 6 share t + dt
             do_prologue<KIND, MO, MC, Smem>(in, sm, Y, K, st, dt, h0, hf);
-            if constexpr (KIND == 0)
+            if constexpr (KIND == 0 || KIND == 3)
                 do_rhs<KIND, MO, MC, Smem, EpiStageFO, true, RAMAN>(in, sm, EpiStageFO{K[st + 1], K[7]}, rho, kfac);
             else
                 do_rhs<KIND, MO, MC, Smem, EpiStageF, true, RAMAN>(in, sm, EpiStageF{K[st + 1], K[7]}, rho, kfac);
@@ -1964,14 +2141,15 @@ __device__ __forceinline__ void solve_instance(const DInst& in, Smem& sm) {
         // yn = y + Δz Σ b5 k ; yerr = Σ (Δz k) e ; weaknorm (src/RK45.jl:170-181, 343-354)
         double sy = 0.0, syn = 0.0, se = 0.0;
         if constexpr (NPT > 16) {
-            // the big envelope grid: the same sums over chunks of 8 bins
+            // the big grids: the same sums over chunks of 8 bins
             constexpr int CH = 8;
 #pragma unroll 1
             for (int c = 0; c < NPT; c += CH) {
                 double2 yv[CH], yn[CH], ye[CH];
 #pragma unroll
                 for (int i = 0; i < CH; i++) {
-                    yv[i] = Y[fresh_tid() + (c + i) * LH_T];
+                    const int k = fresh_tid() + (c + i) * LH_T;
+                    yv[i] = (k < N) ? Y[k] : make_double2(0.0, 0.0);
                     yn[i] = yv[i];
                     ye[i] = make_double2(0.0, 0.0);
                 }
@@ -1982,17 +2160,19 @@ __device__ __forceinline__ void solve_instance(const DInst& in, Smem& sm) {
                     const double2* Kj = K[j];
 #pragma unroll
                     for (int i = 0; i < CH; i++) {
-                        const double2 kv = Kj[fresh_tid() + (c + i) * LH_T];
+                        const int k = fresh_tid() + (c + i) * LH_T;
+                        const double2 kv = (k < N) ? Kj[k] : make_double2(0.0, 0.0);
                         yn[i] = cadd(yn[i], cscal(cb, kv));
                         ye[i] = cadd(ye[i], cscal(ce, cscal(dt, kv)));
                     }
                 }
 #pragma unroll
                 for (int i = 0; i < CH; i++) {
+                    const int k = fresh_tid() + (c + i) * LH_T;
                     sy += yv[i].x * yv[i].x + yv[i].y * yv[i].y;
                     syn += yn[i].x * yn[i].x + yn[i].y * yn[i].y;
                     se += ye[i].x * ye[i].x + ye[i].y * ye[i].y;
-                    YN[fresh_tid() + (c + i) * LH_T] = yn[i];
+                    if (k < N) YN[k] = yn[i];
                 }
             }
         } else {
@@ -2080,8 +2260,7 @@ __device__ __forceinline__ void solve_instance(const DInst& in, Smem& sm) {
                         // this thread's bins in registers (chunks of at most 9),
                         // one k-vector at a time, every load of a chunk before
                         // its first store to out
-                        constexpr int CH = (NPT > 9) ? 8 : NPT;
-                        static_assert(NPT % CH == 0, "save chunks");
+                        constexpr int CH = (NPT > 9) ? 8 : NPT;  // every access below checks k < N
 #pragma unroll 1
                         for (int c = 0; c < NPT; c += CH) {
                             double2 yi[CH], yv[CH], Lv[CH];
@@ -2196,12 +2375,14 @@ __device__ __forceinline__ void rhs_once(const DInst& in, Smem& sm) {
             sm.buf[pad(k)] = v;
         else if constexpr (KIND == 1)
             sm.buf[pad(env_slot<MO, MC>(k))] = v;
-        else
+        else if constexpr (KIND == 2)
             gl(in.escr)[MO + k] = v;  // the big grid's working series W
+        else
+            c2r_pack_bin_big<MO>(gl(in.escr), sm.tw, k, v);  // the big field grid's pre-pack in W
     }
     if constexpr (KIND == 0) {
         for (int k = N + threadIdx.x; k <= MO; k += LH_T) sm.buf[pad(k)] = make_double2(0, 0);
-    } else if constexpr (MO > MC) {
+    } else if constexpr (KIND == 1 && MO > MC) {
         for (int k = MC / 2 + threadIdx.x; k < MO - MC / 2; k += LH_T) sm.buf[pad(k)] = make_double2(0, 0);
     }
     double2* out = gl(in.rhs_out);
@@ -2248,11 +2429,13 @@ enum {
     LH_CLASS_ENV4096 = 4,  // envelope grid, Nto = Nt = 4096
     LH_CLASS_RADIAL = 5,   // radial transform (solved context by context)
     LH_CLASS_ENV16384 = 6, // envelope grid, Nto = Nt = 16384, transforms staged through HBM
-    LH_NCLASS = 7
+    LH_CLASS_32768 = 7,    // field grid, Nto = 32768, Nt = 16384, transforms staged through HBM
+    LH_NCLASS = 8
 };
 
 static int size_class_of(int grid_kind, int n_to, int n_t) {
     if (grid_kind == LH_GRID_REAL) {
+        if (n_to == 32768 && n_t == 16384) return LH_CLASS_32768;
         if (n_to == 16384 && n_t == 8192) return LH_CLASS_16384;
         if (n_to == 8192 && n_t == 4096) return LH_CLASS_8192;
         if (n_to == 4096 && n_t == 2048) return LH_CLASS_4096;
@@ -2309,6 +2492,7 @@ static void launch_solve(int cls, int n, const DInst* dinst, const double2* tw, 
         case LH_CLASS_4096: launch_solve_k<0, 2048, 1024>(zdep, raman, n, dinst, tw, s); break;
         case LH_CLASS_ENV8192: launch_solve_k<1, 8192, 8192>(false, false, n, dinst, tw, s); break;
         case LH_CLASS_ENV16384: launch_solve_k<2, 16384, 16384>(false, false, n, dinst, tw, s); break;
+        case LH_CLASS_32768: launch_solve_k<3, 16384, 8192>(false, false, n, dinst, tw, s); break;
         default: launch_solve_k<1, 4096, 4096>(false, false, n, dinst, tw, s); break;
     }
 }
@@ -2320,6 +2504,7 @@ static void launch_rhs(int cls, const DInst* dinst, const double2* tw, int* stat
         case LH_CLASS_4096: hipLaunchKernelGGL((lh_rhs_kernel<0, 2048, 1024>), dim3(1), dim3(LH_T), 0, s, dinst, tw, status); break;
         case LH_CLASS_ENV8192: hipLaunchKernelGGL((lh_rhs_kernel<1, 8192, 8192>), dim3(1), dim3(LH_T), 0, s, dinst, tw, status); break;
         case LH_CLASS_ENV16384: hipLaunchKernelGGL((lh_rhs_kernel<2, 16384, 16384>), dim3(1), dim3(LH_T), 0, s, dinst, tw, status); break;
+        case LH_CLASS_32768: hipLaunchKernelGGL((lh_rhs_kernel<3, 16384, 8192>), dim3(1), dim3(LH_T), 0, s, dinst, tw, status); break;
         default: hipLaunchKernelGGL((lh_rhs_kernel<1, 4096, 4096>), dim3(1), dim3(LH_T), 0, s, dinst, tw, status); break;
     }
 }
@@ -3154,12 +3339,18 @@ int lh_create(lh_ctx** out, const lh_desc* d) {
         st = shared_upload(c->device, &c->ppt, p.data(), p.size());
     }
     const bool big_env = (ctx_class(c) == LH_CLASS_ENV16384);
+    const bool big_field = (ctx_class(c) == LH_CLASS_32768);
     const bool field_raman = (d->grid_kind == LH_GRID_REAL && d->transform == LH_TRANS_MODEAVG && d->raman);
-    if (!st && (big_env || d->raman)) {
+    if (!st && big_field && d->raman) st = LH_E_ARG;  // no Raman convolution on the staged field grid
+    if (!st && (big_env || big_field || d->raman)) {
         // envelope Raman: E(t) stash of Nto points; the big envelope grid:
         // [E(t) | W | T], Nto + Nto + Nto/2 points, with or without Raman;
-        // field-grid Raman: [series | W | P], Nto/2 + Nto + Nto/2 points
-        c->escr_n = big_env ? (size_t)d->n_to * 5 / 2 : field_raman ? (size_t)d->n_to * 2 : (size_t)d->n_to;
+        // field-grid Raman: [series | W | P], Nto/2 + Nto + Nto/2 points;
+        // the big field grid: [W | T], Nto/2 + Nto/4 points
+        c->escr_n = big_env     ? (size_t)d->n_to * 5 / 2
+                    : big_field ? (size_t)d->n_to * 3 / 4
+                    : field_raman ? (size_t)d->n_to * 2
+                                  : (size_t)d->n_to;
         if (hipMalloc(&c->escr, sizeof(double2) * c->escr_n) != hipSuccess) st = LH_E_NOMEM;
     }
     if (!st && d->raman) {
@@ -3309,7 +3500,7 @@ static constexpr size_t LH_RING_BYTES = (size_t)64 << 20;
 int lh_solve_stream(lh_ctx* c, double* Ew, const lh_rk_opts* o, lh_output_cb cb, void* user, lh_stats* stats) {
     if (!c || !Ew || !o || !cb || !stats || !opts_ok(o)) return LH_E_ARG;
     const int cls = ctx_class(c);
-    if (cls < 0 || c->zg_on) return LH_E_ARG;
+    if (cls < 0 || cls == LH_CLASS_32768 || c->zg_on) return LH_E_ARG;
     int st = lh_set_state(c, Ew);
     if (st) return st;
     if (cls == LH_CLASS_RADIAL) return rad_solve_stream(c, Ew, o, cb, user, stats);
@@ -3669,7 +3860,8 @@ int lh_set_step_stats(lh_ctx* c, const lh_stats_desc* sd) {
         c->st_on = 0;
         return LH_OK;
     }
-    if (c->d.transform != LH_TRANS_MODEAVG || ctx_class(c) == LH_CLASS_ENV16384) return LH_E_ARG;
+    if (c->d.transform != LH_TRANS_MODEAVG || ctx_class(c) == LH_CLASS_ENV16384 || ctx_class(c) == LH_CLASS_32768)
+        return LH_E_ARG;
     HIPCK(hipSetDevice(c->device));
     if (sd->capacity < 0 || !sd->energy_w || !sd->t || !(sd->aeff > 0) || !c->omega) return LH_E_ARG;
     if (sd->n_windows < 0 || sd->n_windows > LH_STAT_MAX_WINDOWS || (sd->n_windows > 0 && !sd->window_w))
@@ -3734,7 +3926,8 @@ int lh_set_gradient(lh_ctx* c, const lh_gradient_desc* g) {
     }
     const lh_desc& d = c->d;
     // (a Raman response depends on the density; with ρ(z) it would change along z)
-    if (d.transform != LH_TRANS_MODEAVG || d.grid_kind != LH_GRID_REAL || !c->omega || !c->zsidx || d.raman)
+    if (d.transform != LH_TRANS_MODEAVG || d.grid_kind != LH_GRID_REAL || !c->omega || !c->zsidx || d.raman ||
+        ctx_class(c) == LH_CLASS_32768)
         return LH_E_ARG;
     if (g->n_points < 2 || !g->Z || !g->P || g->dens_n < 4 || !g->dens_p || !g->dens_rho || !g->dens_d ||
         !g->gamma || !g->nwg || !g->pre || !g->fd || !(g->fd_h > 0) || !(g->sqrt_aeff > 0) ||

@@ -165,11 +165,12 @@ def test_fno_arg_checks(capi):
 
 def test_grids_beyond_largest_classes_rejected(capi, pkg):
     """An envelope grid of Nt = 32768 (trange 26 ps over 450-8000 nm) and a
-    field grid of Nto = 32768 lie above the largest compiled classes
-    (envelope Nt = 16384, field Nto = 16384); lh_create refuses them with
-    LH_E_ARG before touching a device.  The Nt = 16384 envelope grid of the
-    soliton self-frequency-shift case (test/test_gnlse.jl:33-56) is a
-    supported class."""
+    field grid of Nto = 65536 (trange 4 ps over 150 nm-4 μm) lie above the
+    largest compiled classes (envelope Nt = 16384, field Nto = 32768);
+    lh_create refuses them with LH_E_ARG before touching a device.  The Nt =
+    16384 envelope grid of the soliton self-frequency-shift case
+    (test/test_gnlse.jl:33-56) and the Nto = 32768 field grid of a 2 ps
+    window are supported classes."""
     from luna_jl_amd import interface
     gamma, beta2, tau0, fr = 0.1, -1e-26, 10e-15, 0.18
     P0 = abs(beta2) / ((1 - fr) * gamma * tau0 ** 2)
@@ -184,9 +185,9 @@ def test_grids_beyond_largest_classes_rejected(capi, pkg):
     h = ctypes.c_void_p()
     assert capi.lib().lh_create(ctypes.byref(h), sizes[26e-12][1].ref) == capi.LH_E_ARG
     Ew, grid, linop, tr, _, _ = interface.prop_capillary_args(
-        125e-6, 1.0, "He", 1.0, lambda0=800e-9, lambda_lims=(150e-9, 4e-6), trange=2e-12,
+        125e-6, 1.0, "He", 1.0, lambda0=800e-9, lambda_lims=(150e-9, 4e-6), trange=4e-12,
         tau_fwhm=10e-15, energy=120e-6, shotnoise=False, ppt_N=2 ** 10)
-    assert grid.n_to == 32768
+    assert grid.n_to == 65536
     assert capi.lib().lh_create(ctypes.byref(h), tr.descriptor(linop).ref) == capi.LH_E_ARG
 
 
