@@ -2478,6 +2478,7 @@ static void launch_solve_stream(int cls, bool raman, const DInst* dinst, const d
         case LH_CLASS_4096: hipLaunchKernelGGL((lh_solve_kernel<0, 2048, 1024, false, true>), dim3(1), dim3(LH_T), 0, s, dinst, tw); break;
         case LH_CLASS_ENV8192: hipLaunchKernelGGL((lh_solve_kernel<1, 8192, 8192, false, true>), dim3(1), dim3(LH_T), 0, s, dinst, tw); break;
         case LH_CLASS_ENV16384: hipLaunchKernelGGL((lh_solve_kernel<2, 16384, 16384, false, true>), dim3(1), dim3(LH_T), 0, s, dinst, tw); break;
+        case LH_CLASS_32768: hipLaunchKernelGGL((lh_solve_kernel<3, 16384, 8192, false, true>), dim3(1), dim3(LH_T), 0, s, dinst, tw); break;
         default: hipLaunchKernelGGL((lh_solve_kernel<1, 4096, 4096, false, true>), dim3(1), dim3(LH_T), 0, s, dinst, tw); break;
     }
 }
@@ -3500,7 +3501,7 @@ static constexpr size_t LH_RING_BYTES = (size_t)64 << 20;
 int lh_solve_stream(lh_ctx* c, double* Ew, const lh_rk_opts* o, lh_output_cb cb, void* user, lh_stats* stats) {
     if (!c || !Ew || !o || !cb || !stats || !opts_ok(o)) return LH_E_ARG;
     const int cls = ctx_class(c);
-    if (cls < 0 || cls == LH_CLASS_32768 || c->zg_on) return LH_E_ARG;
+    if (cls < 0 || c->zg_on) return LH_E_ARG;
     int st = lh_set_state(c, Ew);
     if (st) return st;
     if (cls == LH_CLASS_RADIAL) return rad_solve_stream(c, Ew, o, cb, user, stats);

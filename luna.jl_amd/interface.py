@@ -59,14 +59,25 @@ def run(Ew, grid, linop, transform, FT=None, output=None, min_dz=0.0, max_dz=Non
     statsfun = output.statsfun
     host = statsfun is not None and getattr(statsfun, "has_host", False)
     zdep = getattr(transform, "gradient", None) is not None
+    ctx = transform.context(linop)
+    host_records = False  # the device records replaced by the same records computed on the host
+    if hasattr(statsfun, "desc"):
+        try:
+            ctx.set_step_stats(statsfun)
+        except _capi.LunaHipError as e:
+            # the grids staged through HBM record no statistics on the device
+            if e.status != _capi.LH_E_ARG or zdep:
+                raise
+            ctx.set_step_stats(None)
+            host = host_records = True
+    else:
+        ctx.set_step_stats(None)
     if host and zdep:
         raise NotImplementedError("statistics evaluated on the host are not provided with a z-dependent fill")
     opts = _capi.rk_opts(grid.zmax, z0=z0, rtol=rtol, atol=atol, safety=safety,
                          max_dz=grid.zmax / 2 if max_dz is None else max_dz, min_dz=min_dz,
                          init_dz=init_dz, save_n=output.saveN, save_z0=output.tmin, save_first=save_first,
                          every_step=host)
-    ctx = transform.context(linop)
-    ctx.set_step_stats(statsfun if hasattr(statsfun, "desc") else None)
     shape = (grid.n_omega,) + ((transform.n_r,) if transform.n_r > 1 else ())
     if not host and (isinstance(output, MemoryOutput) or zdep):
         # the whole run in the device save buffer, handed over at the end
@@ -84,13 +95,15 @@ def run(Ew, grid, linop, transform, FT=None, output=None, min_dz=0.0, max_dz=Non
             saves = saves.reshape(shape + (saves.shape[-1],), order="F")
             y = y.reshape(shape, order="F")
             if host:  # one accepted step per call
+                if host_records:
+                    records = statsfun.host_record(y, z, dz)[None, :]
                 records = StepRecords(records, [statsfun.host_step(y, z, dz)])
             return output.on_step(first, saves, zsv, z, dz, y, records)
 
         Ew_out, stats = ctx.solve_stream(Ew, opts, on_step)
         if stats.status != _capi.LH_E_STOPPED:
             _capi.check(stats.status, "Luna.run")
-    if hasattr(statsfun, "desc") and stats.accepted > statsfun.capacity:
+    if hasattr(statsfun, "desc") and not host_records and stats.accepted > statsfun.capacity:
         warnings.warn(f"statistics kept for the first {statsfun.capacity} of {stats.accepted} steps "
                       "(raise the capacity of stats.default)", RuntimeWarning)
     output("solver", stats.as_dict(), meta=True, force=True)

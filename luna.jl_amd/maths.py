@@ -83,6 +83,23 @@ def cspline_eval(x, y, D, x0):
             + (2 * (y[i - 1] - y[i]) + D[i - 1] + D[i]) * (t * t * t))
 
 
+def cspline_eval_vec(x, y, D, x0):
+    """``cspline_eval`` at an array of points (the same index rule and cubic)."""
+    x = np.asarray(x, dtype=float)
+    x0 = np.asarray(x0, dtype=float)
+    n = len(x)
+    xmin, xmax = x[0], x[-1]
+    with np.errstate(invalid="ignore"):
+        r = np.ceil((x0 - xmin) / (xmax - xmin) * (n - 1))
+    r = np.where(np.isfinite(r), r, 0.0)
+    i1 = np.where(x0 <= xmin, 2, np.where(x0 >= xmax, n, r.astype(np.int64) + 1))
+    i = np.clip(i1, 2, n) - 1  # 0-based index of the upper node
+    xl, xu, yl, yu, Dl, Du = x[i - 1], x[i], y[i - 1], y[i], D[i - 1], D[i]
+    t = (x0 - xl) / (xu - xl)
+    v = yl + Dl * t + (3 * (yu - yl) - 2 * Dl - Du) * (t * t) + (2 * (yl - yu) + Dl + Du) * (t * t * t)
+    return np.where(x0 == xu, yu, np.where(x0 == xl, yl, v))
+
+
 def converge_series(f, x0, n0=0, rtol=1e-6, maxiter=10000):
     """Brute-force series limit, ``Maths.jl:721-738``."""
     n = n0

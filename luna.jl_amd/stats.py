@@ -45,6 +45,7 @@ import numpy as np
 
 from . import _capi
 from . import physdata as pd
+from .maths import cspline_eval_vec
 from .maths import fwhm as _fwhm
 from .nonlinear import PlasmaCumtrapz
 
@@ -149,6 +150,43 @@ class StatsFun:
         if not self.plasma:
             keys = [k for k in keys if k not in ("electrondensity", "peak_ionisation_rate")]
         return keys
+
+    def host_record(self, Ew, z, dz):
+        """The record the propagation kernel writes (the same values and
+        formulas), evaluated on the host: for the grids whose transforms are
+        staged through HBM, where the kernel records no statistics."""
+        K = _capi.STAT_KEYS.index
+        g = self.grid
+        Ew = np.asarray(Ew).reshape(-1)
+        y = np.abs(Ew) ** 2
+        rec = np.full(_capi.LH_NSTAT, np.nan)
+        rec[K("z")], rec[K("dz")] = z, dz
+        rec[K("ω0")] = np.sum(np.asarray(g.omega, dtype=float) * y) / np.sum(y)
+        rec[K("energy")] = np.sum(self.energy_w * y)
+        for q in range(len(self.windows)):
+            rec[K(f"window{q}")] = np.sum(self.window_w[q] * y)
+        Et = analytic(g, Ew)
+        P = np.abs(Et) ** 2
+        pmax = float(np.max(P))
+        rec[K("peakpower")] = pmax
+        rec[K("peakintensity")] = pmax / self.aeff
+        rec[K("fwhm_t_min")] = _fwhm(self.t, P)
+        rec[K("fwhm_t_max")] = _fwhm(self.t, P, "max")
+        rho = self.transform.density
+        rec[K("density")] = rho
+        if self.plasma:
+            # electrondensity (src/Stats.jl:219-241): the PPT rate of the real
+            # field in V/m, its cumulative integral, 1 − exp(−∫rate) at the end
+            E = np.abs(np.real(Et) / math.sqrt(pd.eps_0 * pd.c * self.aeff / 2))
+            spl = next(r.ratefunc for r in self.transform.responses if isinstance(r, PlasmaCumtrapz))
+            if not np.all(E <= spl.emax):
+                raise RuntimeError("field strength exceeds maximum for PPT ionisation rate")
+            rate = np.where(E < spl.emin, 0.0, np.exp(cspline_eval_vec(spl.x, spl.y, spl.d, E)))
+            dt = float(self.t[1] - self.t[0])
+            F = np.sum(0.5 * (rate[1:] + rate[:-1])) * dt
+            rec[K("electrondensity")] = (1 - math.exp(-F)) * rho
+            rec[K("peak_ionisation_rate")] = float(np.max(rate))
+        return rec
 
     def names(self):
         keys = list(self.to_dict(np.zeros((0, _capi.LH_NSTAT))).keys())

@@ -160,3 +160,27 @@ def test_env16384_scg_short(pkg, orc, capi):
     assert stats.status == 0 and stats.steps == stats_r.steps
     for k in range(11):
         assert spec_err(sv[k], sv_r[k], g.sidx) <= 1e-8
+
+
+def test_prop_gnlse_16384_statistics(pkg, capi):
+    """prop_gnlse on the 12.5 ps grid (Nt = 16384, the reference's
+    supercontinuum example) with its default statistics: evaluated on the host
+    from every accepted step (the kernel of this grid records none); the saves
+    equal lh_solve's."""
+    print("case: prop_gnlse 1 cm on the Nt = 16384 grid, statistics on the host", flush=True)
+    from luna_jl_amd import interface, runtime
+    betas = [0.0, 0.0, -11.830e-27, 8.1038e-41, -9.5205e-56, 2.0737e-70, -5.3943e-85,
+             1.3486e-99, -2.5495e-114, 3.0524e-129, -1.7140e-144]
+    Ew0, grid, linop, tr, _, out = interface.prop_gnlse_args(
+        0.11, 0.01, betas, lambda0=835e-9, lambda_lims=(400e-9, 2400e-9), trange=12.5e-12, tau_fwhm=50e-15,
+        power=1e4, pulseshape="sech", shotnoise=False, raman=True, shock=True, saveN=6)
+    assert len(grid.t) == 16384
+    _, st = interface.run(Ew0, grid, linop, tr, None, out)
+    assert st.status == 0
+    s = out.data["stats"]
+    assert len(s["z"]) == st.accepted and np.all(np.diff(s["z"]) > 0)
+    assert np.all(np.isfinite(s["peakpower"])) and s["peakpower"][0] > 0
+    ctx = runtime.Context(tr.descriptor(linop))
+    _, sv, _, st2 = ctx.solve(Ew0, capi.rk_opts(grid.zmax, save_n=6))
+    assert st2.accepted == st.accepted
+    assert np.array_equal(out.data["Eω"], sv.T)

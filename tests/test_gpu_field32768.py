@@ -8,8 +8,9 @@ windows with the plasma integrals carried between them.
   oracle.
 * A 5 cm propagation (Luna.run, src/Luna.jl:358-391): the same step counts as
   the oracle and ≤ 1e-6 relative spectral-energy error at every save.
-* Per-step statistics, streaming and z-dependent fills are refused on this
-  grid (LH_E_ARG).
+* Per-step statistics on the device and z-dependent fills are refused on this
+  grid (LH_E_ARG); prop_capillary's statistics are then evaluated on the host
+  from every accepted step, streamed, with the trajectory unchanged.
 """
 import numpy as np
 import pytest
@@ -68,5 +69,26 @@ def test_refusals_32768(big, capi):
     ctx = runtime.Context(big.holder)
     with pytest.raises(capi.LunaHipError):
         ctx.set_step_stats(stats.StatsFun(big.grid, big.transform, capacity=64))
-    with pytest.raises(capi.LunaHipError):
-        ctx.solve_stream(big.Ew0, capi.rk_opts(big.zmax, save_n=2), lambda *a: False)
+
+
+def test_prop_capillary_32768_statistics(pkg, capi):
+    """prop_capillary on the 2 ps grid with its default statistics: the kernel
+    records none on this grid, so Luna.run streams every accepted step and
+    evaluates the same record on the host; the saves equal lh_solve's."""
+    print("case: prop_capillary 5 cm on the Nto = 32768 grid, statistics on the host", flush=True)
+    from luna_jl_amd import interface, runtime
+    Ew0, grid, linop, tr, _, out = interface.prop_capillary_args(
+        125e-6, 0.05, "He", 2.0, lambda0=800e-9, lambda_lims=(150e-9, 4e-6), trange=2e-12, tau_fwhm=10e-15,
+        energy=150e-6, shotnoise=False, saveN=6)
+    assert grid.n_to == 32768
+    _, st = interface.run(Ew0, grid, linop, tr, None, out)
+    assert st.status == 0
+    s = out.data["stats"]
+    assert len(s["z"]) == st.accepted and np.all(np.diff(s["z"]) > 0) and s["z"][-1] == st.z
+    assert s["electrondensity"][-1] > 0 and np.all(np.isfinite(s["fwhm_t_min"]))
+    e0 = out.statsfun.host_record(Ew0, 0.0, 0.0)[capi.STAT_KEYS.index("energy")]
+    assert abs(s["energy"][0] - e0) <= 1e-2 * e0
+    ctx = runtime.Context(tr.descriptor(linop))
+    _, sv, _, st2 = ctx.solve(Ew0, capi.rk_opts(grid.zmax, save_n=6))
+    assert st2.accepted == st.accepted
+    assert np.array_equal(out.data["Eω"], sv.T)

@@ -183,3 +183,39 @@ def test_collect_stats_in_hdf5_output(small, tmp_path):
                   rec)
     assert np.allclose(o["stats", "z"], [0.2, 0.4, 0.6])
     assert o["stats", "ω0"].shape == (3,)
+
+
+@pytest.mark.parametrize("evolved", [False, True])
+def test_host_record_matches_oracle(small, orc, capi, evolved):
+    """StatsFun.host_record (the grids whose kernels record no statistics)
+    against the oracle's record of the same state, energy windows included."""
+    from luna_jl_amd import stats
+    g = small.grid
+    Ew = _evolved(small, orc, capi) if evolved else small.Ew0
+    sf = stats.StatsFun(g, small.transform, capacity=16, windows=[(700e-9, 900e-9), ((300e-9, 500e-9), 1e14)])
+    st, ref = orc.step_stats(small.holder, g.t, small.transform.aeff, Ew, 0.4, 2e-3,
+                             windows=[w for _, w in sf.windows])
+    assert st == 0
+    rec = sf.host_record(Ew, 0.4, 2e-3)
+    for k, name in enumerate(capi.STAT_KEYS):
+        if np.isnan(ref[k]):
+            assert np.isnan(rec[k]), name
+        else:
+            tol = 1e-10 if name in ("electrondensity", "peak_ionisation_rate") else 1e-12
+            assert rec[k] == pytest.approx(ref[k], rel=tol, abs=0), name
+    assert rec[capi.STAT_KEYS.index("electrondensity")] > 0
+
+
+def test_host_record_envelope(pkg, orc, capi):
+    from luna_jl_amd import configs, stats
+    inst = configs.gnlse_instance(flength=0.02, shotnoise=False)
+    Ew = _evolved(inst, orc, capi, z=0.02)
+    sf = stats.StatsFun(inst.grid, inst.transform, capacity=16)
+    st, ref = orc.step_stats(inst.holder, inst.grid.t, inst.transform.aeff, Ew, 0.02, 1e-4)
+    assert st == 0
+    rec = sf.host_record(Ew, 0.02, 1e-4)
+    for k, name in enumerate(capi.STAT_KEYS):
+        if np.isnan(ref[k]):
+            assert np.isnan(rec[k]), name
+        else:
+            assert rec[k] == pytest.approx(ref[k], rel=1e-12, abs=0), name
