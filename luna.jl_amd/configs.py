@@ -12,12 +12,12 @@ import numpy as np
 
 from . import physdata as pd
 from .fields import GaussGaussField
-from .grid import RealGrid
+from .grid import EnvGrid, RealGrid
 from .hankel import QDHT
 from .interface import prop_capillary_args, prop_gnlse_args, setup_radial
 from .ionisation import ppt_spline_cached
 from .linops import make_const_linop_radial
-from .nonlinear import Kerr_field, PlasmaCumtrapz
+from .nonlinear import Kerr_env, Kerr_field, PlasmaCumtrapz
 from .nonlinearrhs import const_norm_radial
 
 
@@ -95,6 +95,28 @@ def radial_instance(gas="Ar", pressure=0.5, flength=0.5, lambda0=800e-9, tau_fwh
     normfun = const_norm_radial(grid, q, nfun)
     inputs = GaussGaussField(lambda0, tau_fwhm, energy, w0, propz=propz)
     Ewk, transform = setup_radial(grid, q, rho, normfun, resp, inputs)
+    holder = transform.descriptor(linop)
+    return Instance(holder=holder, Ew0=np.asfortranarray(Ewk), zmax=flength, grid=grid, linop=linop,
+                    transform=transform, sidx=grid.sidx.copy(),
+                    params=dict(energy=energy, pressure=pressure, trange=trange, flength=flength, R=R, N=N,
+                                n_to=grid.n_to, n_omega=grid.n_omega, n_r=N, q=q))
+
+
+def radial_env_instance(gas="Ar", pressure=1.2, flength=0.6, lambda0=800e-9, tau_fwhm=20e-15, energy=2e-9,
+                        w0=40e-6, propz=-0.3, R=4e-3, N=128, trange=1e-12, lambda_lims=(400e-9, 2000e-9)):
+    """Radially symmetric propagation on the envelope grid (TransRadial on an
+    EnvGrid, Kerr_env only), the reference's free-space envelope example
+    (examples/low_level_interface/freespace/radial_env.jl: Ar at 1.2 bar, 20 fs
+    at 800 nm, 2 nJ, 40 μm waist 0.3 m after the input, R = 4 mm) with a 1 ps
+    window (Nt = Nto = 1024) and N = 128 radial points by default."""
+    grid = EnvGrid(flength, lambda0, lambda_lims, trange)
+    q = QDHT(R, N, dim=2)
+    rho = pd.density(gas, pressure)
+    nfun = pd.ref_index_fun(gas, pressure)
+    linop, beta1 = make_const_linop_radial(grid, q, nfun)
+    normfun = const_norm_radial(grid, q, nfun)
+    inputs = GaussGaussField(lambda0, tau_fwhm, energy, w0, propz=propz)
+    Ewk, transform = setup_radial(grid, q, rho, normfun, [Kerr_env(pd.gamma3_gas(gas))], inputs)
     holder = transform.descriptor(linop)
     return Instance(holder=holder, Ew0=np.asfortranarray(Ewk), zmax=flength, grid=grid, linop=linop,
                     transform=transform, sidx=grid.sidx.copy(),

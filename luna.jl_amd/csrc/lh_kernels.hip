@@ -2944,10 +2944,14 @@ static int ctx_class(const lh_ctx* c);
 
 static bool supported(const lh_desc* d) {
     if (d->transform == LH_TRANS_RADIAL) {
-        // field grid, Nr a multiple of the 64-wide contraction tile
-        return d->grid_kind == LH_GRID_REAL && d->n_r >= 64 && d->n_r <= 4096 && d->n_r % 64 == 0 &&
-               rad_size_ok(d->n_to) && d->n_t == d->n_to / 2 && d->n_omega == d->n_t / 2 + 1 && !d->raman &&
-               d->kerr != LH_KERR_ENV && d->qdht != nullptr;
+        // Nr a multiple of the 64-wide contraction tile; field grid with Kerr and
+        // PPT plasma, or envelope grid (Nt = Nto) with Kerr_env
+        if (d->n_r < 64 || d->n_r > 4096 || d->n_r % 64 != 0 || d->raman || d->qdht == nullptr) return false;
+        if (d->grid_kind == LH_GRID_ENV)
+            return rad_env_size_ok(d->n_to) && d->n_t == d->n_to && d->n_omega == d->n_t && !d->plasma &&
+                   d->kerr != LH_KERR_FIELD;
+        return d->grid_kind == LH_GRID_REAL && rad_size_ok(d->n_to) && d->n_t == d->n_to / 2 &&
+               d->n_omega == d->n_t / 2 + 1 && d->kerr != LH_KERR_ENV;
     }
     if (d->transform != LH_TRANS_MODEAVG || d->n_r != 1) return false;
     if (size_class_of(d->grid_kind, d->n_to, d->n_t) < 0) return false;
@@ -3212,8 +3216,9 @@ static int rad_setup(lh_ctx* c, const lh_desc* d) {
         for (size_t n = 0; n < nr; n++) mt[k * nr + n] = d->qdht[n * nr + k];
     int st = upload(&c->Mt, mt.data(), mt.size());
     if (st) return st;
-    HIPCK(hipMalloc(&c->Ek, sizeof(double) * nto * nr));
-    HIPCK(hipMalloc(&c->Er, sizeof(double) * nto * nr));
+    const size_t col = (d->grid_kind == LH_GRID_ENV) ? 2 * nto : nto;  // complex samples on the envelope grid
+    HIPCK(hipMalloc(&c->Ek, sizeof(double) * col * nr));
+    HIPCK(hipMalloc(&c->Er, sizeof(double) * col * nr));
     HIPCK(hipMalloc(&c->partial, sizeof(double) * 3 * nr));
     HIPCK(hipMalloc(&c->rargs, sizeof(RadArgs) * 2));
     HIPCK(hipMalloc(&c->rctl, sizeof(RadCtl) * 2));
@@ -3242,6 +3247,7 @@ static RadArgs rad_args(const lh_ctx* c, int slot) {
     a.n_t = d.n_t;
     a.n_to = d.n_to;
     a.n_r = d.n_r;
+    a.col = (d.grid_kind == LH_GRID_ENV) ? 2 * d.n_to : d.n_to;
     a.sc_t = scale_to_time(d);
     a.gemm_kr = d.qdht_inv / (double)d.n_to;  // ldiv!(Eto, QDHT, Eto) with irfft's 1/Nto
     a.gemm_rk = d.qdht_fwd;                   // mul!(Pto, QDHT, Pto)
@@ -3302,7 +3308,7 @@ static int rad_solve(lh_ctx* c, const lh_rk_opts* o, bool want_saves, hipStream_
     HIPCK(hipMemcpy(c->rargs, &a, sizeof(a), hipMemcpyHostToDevice));
     HIPCK(hipMemcpy(c->rctl, &h, sizeof(h), hipMemcpyHostToDevice));
     HIPCK(hipMemcpyAsync(c->Y, c->Yin, sizeof(double2) * (size_t)c->n_state, hipMemcpyDeviceToDevice, s));
-    rad_rhs_dispatch(c->rplan, n_to, n_r, c->rargs, tw, -1, s);  // PreconStepper init: k1 = fbar(y0)
+    rad_rhs_dispatch(c->rplan, c->d.grid_kind == LH_GRID_ENV, n_to, n_r, c->rargs, tw, -1, s);  // PreconStepper init: k1 = fbar(y0)
     HIPCK(hipGetLastError());
     HIPCK(hipMemcpyAsync(&h, c->rctl, sizeof(h), hipMemcpyDeviceToHost, s));
     HIPCK(hipStreamSynchronize(s));
@@ -3313,7 +3319,7 @@ static int rad_solve(lh_ctx* c, const lh_rk_opts* o, bool want_saves, hipStream_
     if (!h.done && !c->rgraph) {
         hipGraph_t g = nullptr;
         HIPCK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-        rad_step_dispatch(c->rplan, n_to, n_r, c->rargs, tw, s);
+        rad_step_dispatch(c->rplan, c->d.grid_kind == LH_GRID_ENV, n_to, n_r, c->rargs, tw, s);
         hipError_t e1 = hipGetLastError();
         hipError_t e2 = hipStreamEndCapture(s, &g);
         if (e1 != hipSuccess || e2 != hipSuccess || !g) {
@@ -3569,7 +3575,7 @@ int lh_rhs_device(lh_ctx* c, void* nl_dev, const void* Ew_dev, double z, void* s
         rad_ctl_init(&h, nullptr);
         HIPCK(hipMemcpyAsync(c->rargs + 1, &a, sizeof(a), hipMemcpyHostToDevice, s));
         HIPCK(hipMemcpyAsync(c->rctl + 1, &h, sizeof(h), hipMemcpyHostToDevice, s));
-        rad_rhs_dispatch(c->rplan, c->d.n_to, c->d.n_r, c->rargs + 1, tw, -1, s);
+        rad_rhs_dispatch(c->rplan, c->d.grid_kind == LH_GRID_ENV, c->d.n_to, c->d.n_r, c->rargs + 1, tw, -1, s);
     } else {
         DInst in = base_inst(c);
         in.rhs_in = (const double2*)Ew_dev;
@@ -4270,7 +4276,7 @@ int lh_autotune(lh_ctx* c, const double* Ew, int reps, lh_plan* tried, int cap, 
         double refn = 0.0;
         for (int i = 0; i < np && !st; i++) {
             const RadPlan p = plans[i];
-            rad_rhs_dispatch(p, c->d.n_to, c->d.n_r, c->rargs + 1, tw, -1, s);
+            rad_rhs_dispatch(p, c->d.grid_kind == LH_GRID_ENV, c->d.n_to, c->d.n_r, c->rargs + 1, tw, -1, s);
             if (hipGetLastError() != hipSuccess || hipStreamSynchronize(s) != hipSuccess) { st = LH_E_HIP; break; }
             if (hipMemcpy(i == 0 ? ref.data() : out.data(), dout, sizeof(double2) * n, hipMemcpyDeviceToHost) !=
                 hipSuccess) { st = LH_E_HIP; break; }
@@ -4285,7 +4291,7 @@ int lh_autotune(lh_ctx* c, const double* Ew, int reps, lh_plan* tried, int cap, 
             }
             const bool agrees = !(dn > 1e-24 * refn);
             (void)hipEventRecord(e0, s);
-            for (int r = 0; r < reps; r++) rad_rhs_dispatch(p, c->d.n_to, c->d.n_r, c->rargs + 1, tw, -1, s);
+            for (int r = 0; r < reps; r++) rad_rhs_dispatch(p, c->d.grid_kind == LH_GRID_ENV, c->d.n_to, c->d.n_r, c->rargs + 1, tw, -1, s);
             (void)hipEventRecord(e1, s);
             float ms = 0.0f;
             if (hipGetLastError() != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||

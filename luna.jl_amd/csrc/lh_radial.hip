@@ -39,6 +39,7 @@ struct RadK {
 
 struct RadArgs {
     int n_omega, n_t, n_to, n_r;
+    int col;  // doubles per column of Ek/Er: Nto real samples (field grid), 2·Nto (envelope)
     double sc_t, gemm_kr, gemm_rk;
     const double2* linop;   // [Nr][Nω]
     const double2* outfac;  // [Nr][Nω]
@@ -91,7 +92,7 @@ __global__ __launch_bounds__(LH_T) void rad_stage_a(const RadArgs* __restrict__ 
     c2r_prepack<MO>(sm.buf, sm.tw);
     __syncthreads();
     fft_lds<MO, true, PLAN>(sm.buf, sm.tw);
-    double* out = gl(a.Ek) + (size_t)a.n_to * c;
+    double* out = gl(a.Ek) + (size_t)a.col * c;
     for (int j = threadIdx.x; j < MO; j += LH_T) {
         double2 z = sm.buf[pad(j)];
         out[2 * j] = z.x;
@@ -116,7 +117,7 @@ __global__ __launch_bounds__(256) void rad_gemm(const RadArgs* __restrict__ ap, 
     double* __restrict__ C = dir == 0 ? gl(a.Er) : gl(a.Ek);
     const double* __restrict__ B = gl(a.Mt);
     const double scale = dir == 0 ? a.gemm_kr : a.gemm_rk;
-    const int M = a.n_to, K = a.n_r;  // K = N, a multiple of 64; M a multiple of 32·MT
+    const int M = a.col, K = a.n_r;  // K = N, a multiple of 64; M a multiple of 32·MT
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int n0 = blockIdx.y * 64 + (wave & 1) * 32;
     const int m0 = blockIdx.x * (32 * MT) + (wave >> 1) * (16 * MT);
@@ -172,7 +173,7 @@ __global__ __launch_bounds__(LH_T) void rad_pointwise(const RadArgs* __restrict_
     if (gl(a.ctl)->done) return;
     __shared__ SmemT<MO> sm;
     if (threadIdx.x == 0) sm.flag = LH_OK;
-    double* col = gl(a.Er) + (size_t)a.n_to * blockIdx.x;
+    double* col = gl(a.Er) + (size_t)a.col * blockIdx.x;
     for (int j = threadIdx.x; j < MO; j += LH_T) sm.buf[pad(j)] = make_double2(col[2 * j], col[2 * j + 1]);
     __syncthreads();
     pointwise_field<MO, SmemT<MO>>(*gl(a.pw), sm, gl(a.pw)->density, gl(a.pw)->kerr_fac);
@@ -196,7 +197,7 @@ __global__ __launch_bounds__(LH_T) void rad_stage_e(const RadArgs* __restrict__ 
     tw_load(sm.tw, twg);
     const int c = blockIdx.x, N = a.n_omega;
     const size_t off = (size_t)N * c;
-    const double* col = gl(a.Ek) + (size_t)a.n_to * c;
+    const double* col = gl(a.Ek) + (size_t)a.col * c;
     for (int j = threadIdx.x; j < MO; j += LH_T) sm.buf[pad(j)] = make_double2(col[2 * j], col[2 * j + 1]);
     __syncthreads();
     fft_lds<MO, false, PLAN>(sm.buf, sm.tw);
@@ -211,6 +212,135 @@ __global__ __launch_bounds__(LH_T) void rad_stage_e(const RadArgs* __restrict__ 
         for (int k = threadIdx.x; k < N; k += LH_T)
             out[k] = cmul(cmul(r2c_bin<MO>(sm.buf, sm.tw, k), of[k]), cexp_lh(L[k], hb));
     }
+}
+
+// ---------------------------------------------------------------- envelope grid
+// TransRadial on an EnvGrid (src/NonlinearRHS.jl:457-459, 466-474): Nω = Nt
+// bins per column, one complex transform of NO = Nto points; a column of Ek/Er
+// holds Nto complex samples (2·Nto doubles), on which the real QDHT GEMMs act
+// as on 2·Nto rows.  State bin k sits at slot k (k < Nt/2) or k + Nto − Nt of
+// the oversampled spectrum (copy_scale_both!, src/NonlinearRHS.jl:89-95).
+__device__ __forceinline__ int rad_env_slot(int k, int n, int no) { return (k < n / 2) ? k : k + (no - n); }
+
+template <int NO>
+__global__ __launch_bounds__(LH_T) void rad_stage_a_env(const RadArgs* __restrict__ ap,
+                                                        const double2* __restrict__ twg, int stage) {
+    const RadArgs& a = *ap;
+    const RadCtl& ctl = *gl(a.ctl);
+    if (ctl.done) return;
+    __shared__ SmemT<NO> sm;
+    tw_load(sm.tw, twg);
+    const int c = blockIdx.x, N = a.n_omega;
+    const size_t off = (size_t)N * c;
+    for (int j = threadIdx.x; j < NO; j += LH_T) sm.buf[pad(j)] = make_double2(0.0, 0.0);
+    __syncthreads();
+    if (stage < 0) {  // plain RHS of the input column
+        const double2* src = (gl(a.rhs_in) ? gl(a.rhs_in) : gl(a.Y[ctl.ycur])) + off;
+        for (int k = threadIdx.x; k < N; k += LH_T) sm.buf[pad(rad_env_slot(k, N, NO))] = cscal(a.sc_t, src[k]);
+    } else {
+        // yb = y + dt·Σ B[stage][j]·k_j, then exp(L·(t2 − t)); at stage 0 k1 is
+        // first carried to t by exp(L·h0) (src/RK45.jl:209-223, 271-281)
+        const RadK Kc{gl(a.K[0]), a.kstride, ctl.perm, off};
+        const double dt = ctl.dt, hf = c_nodes[stage] * dt;
+        const double h0 = (stage == 0) ? ctl.h0 : 0.0;
+        const double2* Y = gl(a.Y[ctl.ycur]) + off;
+        const double2* L = gl(a.linop) + off;
+        for (int k = threadIdx.x; k < N; k += LH_T) {
+            const double2 Lk = L[k];
+            if (h0 != 0.0) Kc[0][k] = cmul(Kc[0][k], cexp_lh(Lk, h0));
+            double2 yb = Y[k];
+            for (int j = 0; j <= stage; j++) {
+                const double cj = dt * c_B[stage][j];
+                if (cj != 0.0) yb = cadd(yb, cscal(cj, Kc[j][k]));
+            }
+            if (hf != 0.0) yb = cmul(yb, cexp_lh(Lk, hf));
+            sm.buf[pad(rad_env_slot(k, N, NO))] = cscal(a.sc_t, yb);
+        }
+    }
+    __syncthreads();
+    fft_lds<NO, true>(sm.buf, sm.tw);
+    double2* out = reinterpret_cast<double2*>(gl(a.Ek) + (size_t)a.col * c);
+    for (int j = threadIdx.x; j < NO; j += LH_T) out[j] = sm.buf[pad(j)];
+}
+
+// Kerr_env (src/Nonlinear.jl:47-49): P = ¾ρε0γ3|E|²E, ×towin, in place.
+__global__ __launch_bounds__(LH_T) void rad_pointwise_env(const RadArgs* __restrict__ ap) {
+    const RadArgs& a = *ap;
+    if (gl(a.ctl)->done) return;
+    const DInst& in = *gl(a.pw);
+    double2* col = reinterpret_cast<double2*>(gl(a.Er) + (size_t)a.col * blockIdx.x);
+    const double k34 = in.kerr34;
+    const bool kerr = (in.kerr == LH_KERR_ENV);
+    const double* tw = gl(in.to_win);
+    for (int j = threadIdx.x; j < a.n_to; j += LH_T) {
+        const double2 e = col[j];
+        double2 p = make_double2(0.0, 0.0);
+        if (kerr) {
+            const double s = k34 * (e.x * e.x + e.y * e.y);
+            p = make_double2(s * e.x, s * e.y);
+        }
+        col[j] = make_double2(p.x * tw[j], p.y * tw[j]);
+    }
+}
+
+template <int NO>
+__global__ __launch_bounds__(LH_T) void rad_stage_e_env(const RadArgs* __restrict__ ap,
+                                                        const double2* __restrict__ twg, int stage) {
+    const RadArgs& a = *ap;
+    const RadCtl& ctl = *gl(a.ctl);
+    if (ctl.done) return;
+    __shared__ SmemT<NO> sm;
+    tw_load(sm.tw, twg);
+    const int c = blockIdx.x, N = a.n_omega;
+    const size_t off = (size_t)N * c;
+    const double2* col = reinterpret_cast<const double2*>(gl(a.Ek) + (size_t)a.col * c);
+    for (int j = threadIdx.x; j < NO; j += LH_T) sm.buf[pad(j)] = col[j];
+    __syncthreads();
+    fft_lds<NO, false>(sm.buf, sm.tw);
+    const double2* of = gl(a.outfac) + off;
+    if (stage < 0) {
+        double2* out = (gl(a.rhs_out) ? gl(a.rhs_out) : gl(a.K[ctl.perm[0]])) + off;
+        for (int k = threadIdx.x; k < N; k += LH_T) out[k] = cmul(sm.buf[pad(rad_env_slot(k, N, NO))], of[k]);
+    } else {
+        const double t = ctl.t, t2 = t + c_nodes[stage] * ctl.dt, hb = t - t2;
+        double2* out = gl(a.K[ctl.perm[stage + 1]]) + off;
+        const double2* L = gl(a.linop) + off;
+        for (int k = threadIdx.x; k < N; k += LH_T)
+            out[k] = cmul(cmul(sm.buf[pad(rad_env_slot(k, N, NO))], of[k]), cexp_lh(L[k], hb));
+    }
+}
+
+// stepfun per column on the envelope grid (src/Luna.jl:366-370 with a c2c
+// FT): YN ← fft(twin · ifft(ωwin · YN·exp(L h))), NT = Nt points.
+template <int NT>
+__global__ __launch_bounds__(LH_T) void rad_stepfun_env(const RadArgs* __restrict__ ap,
+                                                        const double2* __restrict__ twg) {
+    const RadArgs& a = *ap;
+    const RadCtl& ctl = *gl(a.ctl);
+    if (ctl.done || !ctl.ok) return;
+    __shared__ SmemT<NT> sm;
+    tw_load(sm.tw, twg);
+    const DInst& in = *gl(a.pw);
+    const size_t off = (size_t)a.n_omega * blockIdx.x;
+    double2* YN = gl(a.Y[ctl.ycur ^ 1]) + off;
+    const double2* L = gl(a.linop) + off;
+    const double h = ctl.tn - ctl.t;
+    for (int k = threadIdx.x; k < NT; k += LH_T) {
+        double2 v = YN[k];
+        if (h != 0.0) v = cmul(v, cexp_lh(L[k], h));
+        sm.buf[pad(k)] = cscal(gl(in.omega_win)[k], v);
+    }
+    __syncthreads();
+    fft_lds<NT, true>(sm.buf, sm.tw);
+    const double inv_nt = in.inv_nt;
+    for (int j = threadIdx.x; j < NT; j += LH_T) {
+        const double2 z = sm.buf[pad(j)];
+        const double w = gl(in.t_win)[j];
+        sm.buf[pad(j)] = make_double2((inv_nt * z.x) * w, (inv_nt * z.y) * w);
+    }
+    __syncthreads();
+    fft_lds<NT, false>(sm.buf, sm.tw);
+    for (int k = threadIdx.x; k < NT; k += LH_T) YN[k] = sm.buf[pad(k)];
 }
 
 // ---------------------------------------------------------------- combination
@@ -428,8 +558,45 @@ static void rad_launch_step(const RadArgs* dargs, const double2* tw, int n_to, i
     hipLaunchKernelGGL(rad_step_end, dim3(1), dim3(64), 0, s, dargs);
 }
 
-// Oversampled lengths with compiled radial kernels (Nt = Nto/2, Nω = Nt/2+1).
+// Envelope grid, Nt = Nto = NO: the GEMMs run over 2·Nto rows.
+template <int NO, int MT>
+static void rad_launch_rhs_env(const RadArgs* dargs, const double2* tw, int n_r, int stage, hipStream_t s) {
+    dim3 g(2 * NO / (32 * MT), n_r / 64);
+    hipLaunchKernelGGL((rad_stage_a_env<NO>), dim3(n_r), dim3(LH_T), 0, s, dargs, tw, stage);
+    hipLaunchKernelGGL((rad_gemm<MT>), g, dim3(256), 0, s, dargs, 0);
+    hipLaunchKernelGGL(rad_pointwise_env, dim3(n_r), dim3(LH_T), 0, s, dargs);
+    hipLaunchKernelGGL((rad_gemm<MT>), g, dim3(256), 0, s, dargs, 1);
+    hipLaunchKernelGGL((rad_stage_e_env<NO>), dim3(n_r), dim3(LH_T), 0, s, dargs, tw, stage);
+}
+
+template <int NO, int MT>
+static void rad_run_env(bool step, const RadArgs* a, const double2* tw, int n_r, int stage, hipStream_t s) {
+    if (!step) {
+        rad_launch_rhs_env<NO, MT>(a, tw, n_r, stage, s);
+        return;
+    }
+    for (int st = 0; st < 6; st++) rad_launch_rhs_env<NO, MT>(a, tw, n_r, st, s);
+    hipLaunchKernelGGL(rad_combine, dim3(n_r), dim3(LH_T), 0, s, a);
+    hipLaunchKernelGGL(rad_control, dim3(1), dim3(64), 0, s, a);
+    hipLaunchKernelGGL((rad_stepfun_env<NO>), dim3(n_r), dim3(LH_T), 0, s, a, tw);
+    hipLaunchKernelGGL(rad_saves, dim3(n_r), dim3(LH_T), 0, s, a);
+    hipLaunchKernelGGL(rad_step_end, dim3(1), dim3(64), 0, s, a);
+}
+
+template <int NO>
+static void rad_run_env_mt(const RadPlan& p, bool step, const RadArgs* a, const double2* tw, int n_r, int stage,
+                           hipStream_t s) {
+    switch (p.mt) {
+        case 1: rad_run_env<NO, 1>(step, a, tw, n_r, stage, s); break;
+        case 4: rad_run_env<NO, 4>(step, a, tw, n_r, stage, s); break;
+        default: rad_run_env<NO, 2>(step, a, tw, n_r, stage, s); break;
+    }
+}
+
+// Oversampled lengths with compiled radial kernels (field grid: Nt = Nto/2,
+// Nω = Nt/2+1; envelope grid: Nω = Nt = Nto).
 static bool rad_size_ok(int n_to) { return n_to == 2048 || n_to == 4096 || n_to == 8192 || n_to == 16384; }
+static bool rad_env_size_ok(int n_to) { return n_to == 1024 || n_to == 2048 || n_to == 4096 || n_to == 8192; }
 
 // One RHS (stage < 0: of rhs_in or the current state) or one step, under plan p.
 template <int MO, int MC, int PLAN, int MT>
@@ -459,8 +626,17 @@ static void rad_run_plan(const RadPlan& p, bool step, const RadArgs* a, const do
         rad_run_mt<MO, MC, 0>(p, step, a, tw, n_to, n_r, stage, s);
 }
 
-static void rad_dispatch(const RadPlan& p, bool step, int n_to, int n_r, const RadArgs* a, const double2* tw,
-                         int stage, hipStream_t s) {
+static void rad_dispatch(const RadPlan& p, bool step, bool env, int n_to, int n_r, const RadArgs* a,
+                         const double2* tw, int stage, hipStream_t s) {
+    if (env) {  // the radix plan of the column transforms is the default one
+        switch (n_to) {
+            case 8192: rad_run_env_mt<8192>(p, step, a, tw, n_r, stage, s); break;
+            case 4096: rad_run_env_mt<4096>(p, step, a, tw, n_r, stage, s); break;
+            case 2048: rad_run_env_mt<2048>(p, step, a, tw, n_r, stage, s); break;
+            default: rad_run_env_mt<1024>(p, step, a, tw, n_r, stage, s); break;
+        }
+        return;
+    }
     switch (n_to) {
         case 16384: rad_run_plan<8192, 4096>(p, step, a, tw, n_to, n_r, stage, s); break;
         case 8192: rad_run_plan<4096, 2048>(p, step, a, tw, n_to, n_r, stage, s); break;
@@ -469,12 +645,12 @@ static void rad_dispatch(const RadPlan& p, bool step, int n_to, int n_r, const R
     }
 }
 
-static void rad_rhs_dispatch(const RadPlan& p, int n_to, int n_r, const RadArgs* a, const double2* tw, int stage,
-                             hipStream_t s) {
-    rad_dispatch(p, false, n_to, n_r, a, tw, stage, s);
+static void rad_rhs_dispatch(const RadPlan& p, bool env, int n_to, int n_r, const RadArgs* a, const double2* tw,
+                             int stage, hipStream_t s) {
+    rad_dispatch(p, false, env, n_to, n_r, a, tw, stage, s);
 }
 
-static void rad_step_dispatch(const RadPlan& p, int n_to, int n_r, const RadArgs* a, const double2* tw,
+static void rad_step_dispatch(const RadPlan& p, bool env, int n_to, int n_r, const RadArgs* a, const double2* tw,
                               hipStream_t s) {
-    rad_dispatch(p, true, n_to, n_r, a, tw, -1, s);
+    rad_dispatch(p, true, env, n_to, n_r, a, tw, -1, s);
 }

@@ -98,7 +98,11 @@ def energyfuncs(grid, q=None):
         pre = 2 * math.pi * dw / (len(grid.omega) * dw) ** 2
         return (lambda Et: energy_t(grid, Et)), (lambda Ew: pre * np.sum(np.abs(Ew) ** 2))
     if grid.kind != "real":
-        raise NotImplementedError("radial energy functions of an envelope grid are not provided")
+        # Fields.jl:556-570: |Et|² of the envelope, Σ|Eω|² over ω
+        dw = grid.omega[1] - grid.omega[0]
+        pre = 2 * math.pi * pd.c * pd.eps_0 / 2 * 2 * math.pi * dw / (len(grid.omega) * dw) ** 2
+        return (lambda Et: energy_t_radial(grid, q, Et)), \
+            (lambda Ew: pre * q.integrateK(np.sum(np.abs(Ew) ** 2, axis=0)))
     return (lambda Et: energy_t_radial(grid, q, Et)), (lambda Ew: energy_omega_radial(grid, q, Ew))
 
 
@@ -111,9 +115,10 @@ def gauss_sigma(x, sigma):
 
 
 def energy_t_radial(grid, q, Et):
-    """``energyfuncs(grid::RealGrid, q::QDHT)[1]``, ``Fields.jl:536-541``; Et is (Nt, Nr)."""
+    """``energyfuncs(grid, q::QDHT)[1]``, ``Fields.jl:536-541`` (field grid: of
+    the analytic signal) and ``:556-559`` (envelope grid); Et is (Nt, Nr)."""
     dt = grid.t[1] - grid.t[0]
-    Eta = hilbert(Et, axis=0)
+    Eta = hilbert(Et, axis=0) if grid.kind == "real" else Et
     tintg = simpson_even(dt, np.abs(Eta) ** 2)
     return 2 * math.pi * pd.c * pd.eps_0 / 2 * q.integrateR(tintg)
 
@@ -148,17 +153,21 @@ class GaussGaussField:
         self.phi, self.tau0, self.m, self.propz = phi, tau0, m, propz
 
     def make_Etr(self, grid, q):
-        if grid.kind != "real":
-            raise NotImplementedError("radial inputs are built on the field grid")
+        """``make_Etr``, ``Fields.jl:366-376``: cos(ω0 t + ϕ) on the field grid,
+        exp(i(ϕ + Δω t)) with Δω = ω(λ0) − ω0 on the envelope grid."""
         t = grid.t - self.tau0
-        w0 = pd.wlfreq(self.lam0)
         I = gauss(t, self.tau_fwhm, power=2 * self.m)[:, None] * gauss_sigma(q.r, self.w0 / 2)[None, :]
+        if grid.kind != "real":
+            dw = pd.wlfreq(self.lam0) - grid.omega0
+            return np.sqrt(I) * np.exp(1j * (self.phi + dw * t))[:, None]
+        w0 = pd.wlfreq(self.lam0)
         return np.sqrt(I) * np.cos(w0 * t + self.phi)[:, None]
 
     def __call__(self, grid, q):
         Etr = self.make_Etr(grid, q)
         Etr = Etr * (math.sqrt(self.energy) / math.sqrt(energy_t_radial(grid, q, Etr)))
-        Ewk = q.forward(np.fft.rfft(Etr, axis=0))
+        ft = np.fft.rfft if grid.kind == "real" else np.fft.fft  # FFTW's forward plans (unnormalised)
+        Ewk = q.forward(ft(Etr, axis=0))
         if self.propz != 0.0:
             prop_radial(Ewk, self.propz, grid, q)
         return Ewk

@@ -111,8 +111,9 @@ def run(Ew, grid, linop, transform, FT=None, output=None, min_dz=0.0, max_dz=Non
 
 
 def setup_radial(grid, q, densityfun, normfun, responses, inputs):
-    """``Luna.setup(grid::RealGrid, q::QDHT, densityfun, normfun, responses, inputs)``,
-    ``Luna.jl:239-258``: returns ``(Eωk, transform)``; inputs are summed in (ω, k)."""
+    """``Luna.setup(grid, q::QDHT, densityfun, normfun, responses, inputs)``,
+    ``Luna.jl:239-279`` (field and envelope grids): returns ``(Eωk,
+    transform)``; inputs are summed in (ω, k)."""
     inputs = inputs if isinstance(inputs, (list, tuple)) else [inputs]
     Ewk = np.zeros((grid.n_omega, q.N), dtype=np.complex128)
     for f in inputs:

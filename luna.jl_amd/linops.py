@@ -32,7 +32,9 @@ def make_const_linop(grid, mode, ref_lambda):
 
 
 def make_const_linop_radial(grid, q, nfun_lambda, beta1=None):
-    """``make_const_linop(grid::RealGrid, q::QDHT, nfun)``, ``LinearOps.jl:130-144, 173-184``."""
+    """``make_const_linop(grid, q::QDHT, nfun)`` for the field grid
+    (``LinearOps.jl:130-144, 173-184``) and the envelope grid (``:146-164,
+    211-225``)."""
     omega = grid.omega
     n = np.zeros_like(omega)
     s = grid.sidx
@@ -41,6 +43,11 @@ def make_const_linop_radial(grid, q, nfun_lambda, beta1=None):
         from .modes import derivative1
         beta1 = derivative1(lambda w: w / pd.c * float(np.real(nfun_lambda(pd.wlfreq(w)))),
                             pd.wlfreq(grid.ref_lambda))
+    # envelope grid (LinearOps.jl:146-164, 211-225): the same fill over the
+    # absolute ω, less -iβref with βref = ω0/c·n(λ0) (thg = false)
+    bref = 0.0
+    if grid.kind != "real":
+        bref = grid.omega0 / pd.c * float(np.real(nfun_lambda(2 * np.pi * pd.c / grid.omega0)))
     k2 = (n * omega / pd.c) ** 2
     kr2 = q.k ** 2
     out = np.empty((len(omega), q.N), dtype=np.complex128, order="F")
@@ -50,5 +57,7 @@ def make_const_linop_radial(grid, q, nfun_lambda, beta1=None):
         col = np.empty(len(omega), dtype=np.complex128)
         col[neg] = -1j * (-beta1 * omega[neg]) - np.minimum(np.sqrt(np.abs(bsq[neg])), 200)
         col[~neg] = -1j * (np.sqrt(bsq[~neg]) - beta1 * omega[~neg])
+        if bref != 0.0:
+            col -= -1j * bref
         out[:, ir] = col
     return out, beta1

@@ -335,6 +335,26 @@ static void responses_env(const lh_desc *d, rhs_ws *w, const cplx *E, cplx *Pt) 
 /* ------------------------------------------------------------------------ */
 /* Transforms                                                                */
 /* ------------------------------------------------------------------------ */
+/* X[:, i] = scale * sum_j T[i, j] X[:, j] for complex columns of nto samples
+ * (Hankel.jl's ldiv!/mul! on a ComplexF64 array: the real matrix applied to
+ * real and imaginary parts), through tmp; sum in increasing j as for the
+ * field grid. */
+static void qdht_cols_c(const lh_desc *d, cplx *X, cplx *tmp, double scale) {
+    int nto = d->n_to, nr = d->n_r;
+    for (int i = 0; i < nr; i++)
+        for (int t = 0; t < nto; t++) {
+            double sr = 0.0, si = 0.0;
+            for (int j = 0; j < nr; j++) {
+                double m = d->qdht[(size_t)i * nr + j];
+                cplx x = X[(size_t)j * nto + t];
+                sr += m * creal(x);
+                si += m * cimag(x);
+            }
+            tmp[(size_t)i * nto + t] = (sr * scale) + I * (si * scale);
+        }
+    memcpy(X, tmp, sizeof(cplx) * (size_t)nto * nr);
+}
+
 static int rhs_eval(const lh_desc *d, rhs_ws *w, cplx *nl, const cplx *Ew) {
     int N = d->n_omega, nto = d->n_to, nr = d->n_r;
     int st = LH_OK;
@@ -404,9 +424,18 @@ static int rhs_eval(const lh_desc *d, rhs_ws *w, cplx *nl, const cplx *Ew) {
             for (int i = 0; i < nto; i++) Et[i] = inv_n * Et[i];
             if (d->transform == LH_TRANS_MODEAVG)
                 for (int i = 0; i < nto; i++) Et[i] = Et[i] / d->field_div;
-            responses_env(d, w, Et, w->Ptc + (size_t)c * nto);
+        }
+        /* TransRadial on the envelope grid (src/NonlinearRHS.jl:457-459, 466-474):
+         * the same real QDHT matrix acts on the real and imaginary parts */
+        if (d->transform == LH_TRANS_RADIAL) qdht_cols_c(d, w->Etc, w->Ptc, d->qdht_inv);
+        for (int c = 0; c < nr; c++) {
+            responses_env(d, w, w->Etc + (size_t)c * nto, w->Ptc + (size_t)c * nto);
             cplx *Pt = w->Ptc + (size_t)c * nto;
             for (int i = 0; i < nto; i++) Pt[i] = Pt[i] * d->to_win[i];
+        }
+        if (d->transform == LH_TRANS_RADIAL) qdht_cols_c(d, w->Ptc, w->Etc, d->qdht_fwd);
+        for (int c = 0; c < nr; c++) {
+            cplx *Pt = w->Ptc + (size_t)c * nto;
             fft_c2c(nto, Pt, -1);
             cplx *dst = nl + (size_t)c * N;
             for (int i = 0; i < h; i++) dst[i] = sc_f * Pt[i];
