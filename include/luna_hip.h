@@ -147,7 +147,7 @@ int lh_device_count(int *count);
  * HIP device.  Replaces TransModeAvg(...) / TransRadial(...) construction,
  * src/NonlinearRHS.jl:340-350, 428-460, and Luna.setup src/Luna.jl:115-152,239-279.
  * Radial: field grid (Nto 2048-16384, Kerr_field/PPT plasma) or envelope grid
- * (Nt = Nto 1024-8192, Kerr_env), Nr a multiple of 64 up to 4096; other
+ * (Nt = Nto 256-8192, Kerr_env), Nr a multiple of 64 up to 4096; other
  * configurations give LH_E_ARG. */
 int lh_create(lh_ctx **ctx, const lh_desc *desc);
 int lh_destroy(lh_ctx *ctx);

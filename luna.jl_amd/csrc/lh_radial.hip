@@ -596,7 +596,7 @@ static void rad_run_env_mt(const RadPlan& p, bool step, const RadArgs* a, const 
 // Oversampled lengths with compiled radial kernels (field grid: Nt = Nto/2,
 // Nω = Nt/2+1; envelope grid: Nω = Nt = Nto).
 static bool rad_size_ok(int n_to) { return n_to == 2048 || n_to == 4096 || n_to == 8192 || n_to == 16384; }
-static bool rad_env_size_ok(int n_to) { return n_to == 1024 || n_to == 2048 || n_to == 4096 || n_to == 8192; }
+static bool rad_env_size_ok(int n_to) { return n_to >= 256 && n_to <= 8192 && (n_to & (n_to - 1)) == 0; }
 
 // One RHS (stage < 0: of rhs_in or the current state) or one step, under plan p.
 template <int MO, int MC, int PLAN, int MT>
@@ -633,7 +633,9 @@ static void rad_dispatch(const RadPlan& p, bool step, bool env, int n_to, int n_
             case 8192: rad_run_env_mt<8192>(p, step, a, tw, n_r, stage, s); break;
             case 4096: rad_run_env_mt<4096>(p, step, a, tw, n_r, stage, s); break;
             case 2048: rad_run_env_mt<2048>(p, step, a, tw, n_r, stage, s); break;
-            default: rad_run_env_mt<1024>(p, step, a, tw, n_r, stage, s); break;
+            case 1024: rad_run_env_mt<1024>(p, step, a, tw, n_r, stage, s); break;
+            case 512: rad_run_env_mt<512>(p, step, a, tw, n_r, stage, s); break;
+            default: rad_run_env_mt<256>(p, step, a, tw, n_r, stage, s); break;
         }
         return;
     }

@@ -4,8 +4,11 @@ test_oracle_radial_env.py), on the reference's free-space envelope example
 (examples/low_level_interface/freespace/radial_env.jl: Ar at 1.2 bar, 20 fs at
 800 nm, 40 μm waist 0.3 m after the input) with a 1 ps window.
 
-* One RHS: relative L2 ≤ 1e-12 at Nto = 1024 (Nr = 128) and 2048 (Nr = 64),
-  at the input and at the beam 1 cm before its focus (20 μJ: Kerr active).
+* One RHS: relative L2 ≤ 1e-12 at Nto = 1024 (Nr = 128), 2048 (Nr = 64) and
+  the example's own grid (0.2 ps: Nt = Nto = 256, Nr = 512), at the input and
+  at the beam 1 cm before its focus (20 μJ: Kerr active).
+* The example's grid propagated over 0.4 m at 20 μJ with Nr = 64: the
+  oracle's step counts, final state within 1e-6.
 * Propagation over 0.4 m at 20 μJ (19 % away from the linear propagation):
   the oracle's step counts, final state and saves within 1e-6 relative
   spectral-energy error.
@@ -26,7 +29,7 @@ def spec_err(a, b):
     return np.linalg.norm(pa - pb) / np.linalg.norm(pb)
 
 
-@pytest.mark.parametrize("trange,N", [(1e-12, 128), (2e-12, 64)])
+@pytest.mark.parametrize("trange,N", [(1e-12, 128), (2e-12, 64), (0.2e-12, 512)])
 def test_radial_env_rhs(pkg, orc, trange, N):
     from luna_jl_amd import configs, fields, runtime
     inst = configs.radial_env_instance(N=N, trange=trange, energy=20e-6)
@@ -78,3 +81,16 @@ def test_radial_env_stream(prop_case, capi):
     assert stb.status == 0 and stb.accepted == sta.accepted
     assert np.array_equal(Ea, Eb)
     assert np.array_equal(np.concatenate(got, axis=1).T, sva)
+
+
+def test_radial_env_example_grid_propagation(pkg, orc, capi):
+    print("case: radial envelope propagation Nto=256 Nr=64, 0.4 m", flush=True)
+    from luna_jl_amd import configs, runtime
+    inst = configs.radial_env_instance(N=64, trange=0.2e-12, energy=20e-6, flength=0.4)
+    assert inst.grid.n_to == 256
+    opts = capi.rk_opts(0.4, save_n=3)
+    st, E1, _, _, st_o = orc.solve(inst.holder, inst.Ew0, opts, want_saves=False)
+    assert st == 0
+    Eg, _, _, stg = runtime.Context(inst.holder).solve(inst.Ew0, opts, want_saves=False)
+    assert stg.status == 0 and (stg.steps, stg.accepted, stg.rejected) == (st_o.steps, st_o.accepted, st_o.rejected)
+    assert spec_err(Eg, E1) <= 1e-6
