@@ -216,3 +216,22 @@ def test_header_compiles_on_its_own(header, lang):
                               "-I", os.path.join(ROOT, "include"), "-o", os.path.join(td, "h"), src],
                              capture_output=True, text=True)
         assert res.returncode == 0, res.stderr
+
+
+def test_radial_envelope_limits(capi, pkg):
+    """Envelope-grid radial contexts: Nt = Nto from 256 to 8192 points and Nr a
+    multiple of 64; a window of 0.1 ps (Nt = 128) or Nr = 100 is refused with
+    LH_E_ARG before a device is touched (a supported descriptor gets as far as
+    the device query)."""
+    from luna_jl_amd import configs
+    h = ctypes.c_void_p()
+    small = configs.radial_env_instance(N=64, trange=0.1e-12)
+    assert small.grid.n_t == 128
+    assert capi.lib().lh_create(ctypes.byref(h), small.holder.ref) == capi.LH_E_ARG
+    odd = configs.radial_env_instance(N=100, trange=0.2e-12)
+    assert capi.lib().lh_create(ctypes.byref(h), odd.holder.ref) == capi.LH_E_ARG
+    ok = configs.radial_env_instance(N=64, trange=0.2e-12)
+    st = capi.lib().lh_create(ctypes.byref(h), ok.holder.ref)
+    if st == capi.LH_OK:
+        capi.lib().lh_destroy(h)
+    assert st != capi.LH_E_ARG
