@@ -45,7 +45,10 @@ def run(Ew, grid, linop, transform, FT=None, output=None, min_dz=0.0, max_dz=Non
     device solve fills a ``MemoryOutput`` from its save buffer and streams
     every saving step to any other output (``lh_solve_stream``).  Statistics
     evaluated on the host (``collect_stats``, ``Stats.default``'s userfuns)
-    stream every accepted step and are evaluated on its filtered state."""
+    stream every accepted step and are evaluated on its filtered state.
+    Device statistics that outgrow ``capacity`` in a ``MemoryOutput`` run are
+    recomputed by a second solve with room for every accepted step; other
+    outputs keep the first ``capacity`` records and warn."""
     if output is None:
         output = MemoryOutput(0, grid.zmax, 201)
     save_first = 0
@@ -83,6 +86,12 @@ def run(Ew, grid, linop, transform, FT=None, output=None, min_dz=0.0, max_dz=Non
         # the whole run in the device save buffer, handed over at the end
         # (z-dependent fills are not streamed)
         Ew_out, saves, zs, stats = ctx.solve(Ew, opts)
+        if hasattr(statsfun, "desc") and stats.status == _capi.LH_OK and stats.accepted > statsfun.capacity:
+            # the reference's statistics arrays grow with the run: solve again
+            # (the same trajectory) with a record for every accepted step
+            statsfun.capacity = int(stats.accepted)
+            ctx.set_step_stats(statsfun)
+            Ew_out, saves, zs, stats = ctx.solve(Ew, opts)
         _capi.check(stats.status, "Luna.run")
         rec = ctx.step_stats() if statsfun is not None else np.zeros((0, _capi.LH_NSTAT))
         n = output.saveN - save_first

@@ -143,3 +143,26 @@ def test_radial_run_collect_stats(rad, capi):
     plain = MemoryOutput(0, g.zmax, 11)
     interface.run(rad.Ew0, g, rad.linop, rad.transform, None, plain)
     assert np.array_equal(out.data["Eω"], plain.data["Eω"])
+
+
+def test_run_grows_statistics(small, pkg):
+    """Luna.run keeps a record of every accepted step when the run outgrows
+    the device capacity (the reference grows its arrays, src/Stats.jl:495-532):
+    the records and saves equal those of a run with room for all of them."""
+    print("case: Luna.run statistics beyond the capacity", flush=True)
+    from luna_jl_amd import interface, stats
+    from luna_jl_amd.output import MemoryOutput
+    g = small.grid
+    sf = stats.default(g, small.Ew0, None, small.linop, small.transform, capacity=8)
+    out = MemoryOutput(0, g.zmax, 21, sf)
+    _, st = interface.run(small.Ew0, g, small.linop, small.transform, None, out)
+    ref = MemoryOutput(0, g.zmax, 21, stats.default(g, small.Ew0, None, small.linop, small.transform,
+                                                    capacity=4096))
+    _, st_r = interface.run(small.Ew0, g, small.linop, small.transform, None, ref)
+    assert st.accepted == st_r.accepted > 8
+    assert sf.capacity == st.accepted
+    s, r = out.data["stats"], ref.data["stats"]
+    assert len(s["energy"]) == st.accepted
+    for k in ("z", "energy", "peakpower", "ω0"):
+        assert np.array_equal(s[k], r[k], equal_nan=True), k
+    assert np.array_equal(out.data["Eω"], ref.data["Eω"])
